@@ -1,0 +1,340 @@
+"""Initial conditions for the benchmark workloads (host-side numpy).
+
+States are float64 arrays of shape (8, ny, nx) / (8, nz, ny, nx) / (8, nx),
+ghosts included (4 per side on every active axis), variable order
+ro, mx, my, mz, bx, by, bz, en (common/mhd_class.hpp:34-40), x fastest
+(ss = nx*(ny*k+j)+i).  Face-centred B is built from a vector potential with
+the reference's 4th-order difference, as 2D/SERIAL/loop/mhd2d_init_.cpp:39-51
+and 3D/MPI/loop/mhd3d_init_.cpp:36-75 do, so the discrete div B is zero to
+round-off.  Coordinates are cell-centred: x_i = (i - 4 + 1/2) dx + xmin
+(1D/SERIAL/shock/mhd1d_paras.cpp:24).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NG = 4  # ghost cells per side (mhd2d_class.hpp:10)
+
+
+@dataclass
+class Problem:
+    """A state plus the scheme/grid settings it is meant to run with."""
+    state: np.ndarray
+    dx: float
+    dy: float = 1.0
+    dz: float = 1.0
+    gam: float = 5.0 / 3.0
+    cfl: float = 0.4
+    rmn: int = 1
+    odr: int = 4
+    r_k: int = 3
+    ctw: int = 1
+    dn: np.ndarray = field(default_factory=lambda: np.zeros((3, 8), dtype=int))
+    nsteps: int = 1
+    name: str = ""
+
+    @property
+    def ndim(self):
+        return self.state.ndim - 1
+
+    @property
+    def interior(self):
+        return tuple(n - 2 * NG for n in self.state.shape[1:])
+
+    def copy(self):
+        p = Problem(**{k: getattr(self, k) for k in self.__dataclass_fields__})
+        p.state = self.state.copy()
+        p.dn = np.array(self.dn)
+        return p
+
+
+def _df4(fm1, f0, f1, f2):
+    """cal_df_4th (common_interp.h:21-24) on arrays."""
+    return (27.0 * (f1 - f0) - (f2 - fm1)) / 24.0
+
+
+def _fcen4(fm1, f0, f1, f2):
+    """cal_fcen_4th (common_interp.h:10-13) on arrays."""
+    return 0.0625 * (9.0 * (f0 + f1) - (fm1 + f2))
+
+
+def energy(ro, vx, vy, vz, cx, cy, cz, pr, gam):
+    """mhd_energy (common/mhd_func.c:18-28)."""
+    v2 = vx * vx + vy * vy + vz * vz
+    b2 = cx * cx + cy * cy + cz * cz
+    return pr / (gam - 1) + 0.5 * (ro * v2 + b2)
+
+
+def periodic_fill(a, axes):
+    """Periodic ghost copy (bc2d/bc3d dn=0) on the given array axes."""
+    for ax in axes:
+        n = a.shape[ax]
+        sl = [slice(None)] * a.ndim
+        src = [slice(None)] * a.ndim
+        sl[ax] = slice(0, NG)
+        src[ax] = slice(n - 2 * NG, n - NG)
+        a[tuple(sl)] = a[tuple(src)]
+        sl[ax] = slice(n - NG, n)
+        src[ax] = slice(NG, 2 * NG)
+        a[tuple(sl)] = a[tuple(src)]
+
+
+# ---------------------------------------------------------------- 1D
+def brio_wu(nx: int = 1024, gam: float = 2.0) -> Problem:
+    """Brio & Wu shock tube on [-0.5, 0.5] (1D/SERIAL/shock/mhd1d_init_.cpp:21-30,
+    mhd1d_paras.cpp:6-25): open boundaries (dn=+2) for every variable."""
+    n = nx + 2 * NG
+    dx = 1.0 / nx
+    x = (np.arange(n) - NG + 0.5) * dx - 0.5
+    left = x <= 0
+    ro = np.where(left, 1.0, 0.125)
+    pr = np.where(left, 1.0, 0.1)
+    by = np.where(left, 1.0, -1.0)
+    bx = np.full(n, 0.75)
+    z = np.zeros(n)
+    st = np.zeros((8, n))
+    st[0] = ro
+    st[1] = ro * z
+    st[2] = ro * z
+    st[3] = ro * z
+    st[4] = bx
+    st[5] = by
+    st[6] = z
+    st[7] = energy(ro, z, z, z, bx, by, z, pr, gam)
+    dn = np.zeros((3, 8), dtype=int)
+    dn[0, :] = 2
+    return Problem(state=st, dx=dx, gam=gam, cfl=0.4, rmn=1, odr=4, r_k=3, dn=dn, name="brio_wu_%d" % nx)
+
+
+# ---------------------------------------------------------------- 2D
+def _modes2d(rng, nmodes, kmax):
+    ks = [(kx, ky) for kx in range(-kmax, kmax + 1) for ky in range(-kmax, kmax + 1)
+          if 0 < kx * kx + ky * ky <= kmax * kmax]
+    idx = rng.choice(len(ks), size=nmodes, replace=False)
+    amp = rng.uniform(0.0, 1.0, size=nmodes)
+    ph = rng.uniform(0.0, 2.0 * math.pi, size=nmodes)
+    return [ks[i] for i in idx], amp, ph
+
+
+def _potential2d(x, y, modes, amp, ph):
+    k, a, p = modes, amp, ph
+    out = np.zeros(np.broadcast(x, y).shape)
+    for (kx, ky), aa, pp in zip(k, a, p):
+        kk = math.sqrt(kx * kx + ky * ky)
+        out += (aa / kk) * np.cos(kx * x + ky * y + pp)
+    return out
+
+
+def turbulence2d(n: int = 1024, seed: int = 2, nmodes: int = 16, kmax: int = 4, gam: float = 5.0 / 3.0,
+                 vrms: float = 1.0, brms: float = 1.0, rho: float | None = None, pres: float | None = None,
+                 ny: int | None = None) -> Problem:
+    """Decaying 2D MHD turbulence on [0, 2pi]^2, periodic.
+
+    rho = gam^2, p = gam (c_s = 1) by default.  Velocity = curl(psi z) and
+    B = curl(Az z) with psi, Az sums of `nmodes` random Fourier modes
+    0 < |k| <= kmax (numpy PCG64(seed)); rescaled to interior rms|v| = vrms and
+    rms|B| = brms (cell-centred B from fcen).  Az lives at cell corners."""
+    nx = n
+    ny = n if ny is None else ny
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = 2.0 * math.pi
+    dx, dy = L / nx, L / ny
+    Nx, Ny = nx + 2 * NG, ny + 2 * NG
+    xc = (np.arange(Nx) - NG + 0.5) * dx
+    yc = (np.arange(Ny) - NG + 0.5) * dy
+    X, Y = np.meshgrid(xc, yc)              # (Ny, Nx) cell centres
+    XC, YC = X - 0.5 * dx, Y - 0.5 * dy      # corners (i-1/2, j-1/2)
+    # velocity from a stream function (analytic derivatives)
+    km, am, pm = _modes2d(rng, nmodes, kmax)
+    vx = np.zeros_like(X)
+    vy = np.zeros_like(X)
+    for (kx, ky), aa, pp in zip(km, am, pm):
+        kk = math.sqrt(kx * kx + ky * ky)
+        s = -(aa / kk) * np.sin(kx * X + ky * Y + pp)
+        vx += ky * s     # d psi / dy
+        vy -= kx * s     # -d psi / dx
+    # vector potential at corners -> face B (mhd2d_init_.cpp:48-51 pattern)
+    ka, aa_, pa = _modes2d(rng, nmodes, kmax)
+    az = _potential2d(XC, YC, ka, aa_, pa)
+    bx = np.zeros_like(X)
+    by = np.zeros_like(X)
+    bx[1:-2, :] = _df4(az[0:-3, :], az[1:-2, :], az[2:-1, :], az[3:, :]) / dy
+    by[:, 1:-2] = -_df4(az[:, 0:-3], az[:, 1:-2], az[:, 2:-1], az[:, 3:]) / dx
+    periodic_fill(bx, (0, 1))
+    periodic_fill(by, (0, 1))
+    I = (slice(NG, Ny - NG), slice(NG, Nx - NG))
+    cx = np.zeros_like(X)
+    cy = np.zeros_like(X)
+    cx[:, 1:-2] = _fcen4(bx[:, 0:-3], bx[:, 1:-2], bx[:, 2:-1], bx[:, 3:])
+    cy[1:-2, :] = _fcen4(by[0:-3, :], by[1:-2, :], by[2:-1, :], by[3:, :])
+    sv = vrms / math.sqrt(np.mean(vx[I] ** 2 + vy[I] ** 2))
+    sb = brms / math.sqrt(np.mean(cx[I] ** 2 + cy[I] ** 2))
+    vx *= sv
+    vy *= sv
+    bx *= sb
+    by *= sb
+    cx *= sb
+    cy *= sb
+    ro = np.full_like(X, gam * gam if rho is None else rho)
+    pr = np.full_like(X, gam if pres is None else pres)
+    z = np.zeros_like(X)
+    periodic_fill(cx, (0, 1))
+    periodic_fill(cy, (0, 1))
+    st = np.zeros((8, Ny, Nx))
+    st[0] = ro
+    st[1] = ro * vx
+    st[2] = ro * vy
+    st[3] = ro * z
+    st[4] = bx
+    st[5] = by
+    st[6] = z
+    st[7] = energy(ro, vx, vy, z, cx, cy, z, pr, gam)
+    return Problem(state=st, dx=dx, dy=dy, gam=gam, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1,
+                   name="turb2d_%dx%d" % (nx, ny))
+
+
+def kelvin_helmholtz(n: int = 2048, mach: float = 0.01, seed: int = 3, gam: float = 5.0 / 3.0,
+                     width: float = 0.02, b0: float = 0.05) -> Problem:
+    """Low-Mach Kelvin-Helmholtz on [0,1]^2, periodic, MLAU (RMN=3).
+
+    rho = 1, p = 1/gam (c_s = 1), Bx = b0.  The shear layer
+    vx = M/2 tanh((y-1/2)/w) is mirrored for periodicity as
+    vx = M/2 [tanh(s/w) - tanh((s-1/2)/w) - tanh((s+1/2)/w)], s = y - 1/2,
+    i.e. layers at y = 1/2 and y = 0 (= 1).  White-noise vy of amplitude
+    1e-3 M (uniform in [-1, 1], PCG64(seed)) where |y - 1/2| < 0.05."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = n + 2 * NG
+    d = 1.0 / n
+    c = (np.arange(N) - NG + 0.5) * d
+    X, Y = np.meshgrid(c, c)
+    s = Y - 0.5
+    vx = 0.5 * mach * (np.tanh(s / width) - np.tanh((s - 0.5) / width) - np.tanh((s + 0.5) / width))
+    noise = rng.uniform(-1.0, 1.0, size=(n, n))
+    vy = np.zeros_like(X)
+    vy[NG:-NG, NG:-NG] = 1e-3 * mach * noise
+    vy[np.abs(s) >= 0.05] = 0.0
+    periodic_fill(vy, (0, 1))
+    ro = np.ones_like(X)
+    pr = np.full_like(X, 1.0 / gam)
+    bx = np.full_like(X, b0)
+    z = np.zeros_like(X)
+    st = np.zeros((8, N, N))
+    st[0] = ro
+    st[1] = ro * vx
+    st[2] = ro * vy
+    st[3] = ro * z
+    st[4] = bx
+    st[5] = z
+    st[6] = z
+    st[7] = energy(ro, vx, vy, z, bx, z, z, pr, gam)
+    return Problem(state=st, dx=d, dy=d, gam=gam, cfl=0.4, rmn=3, odr=4, r_k=3, ctw=1, name="khi_%d" % n)
+
+
+# ---------------------------------------------------------------- 3D
+def _modes3d(rng, nmodes, kmax):
+    ks = [(a, b, c) for a in range(-kmax, kmax + 1) for b in range(-kmax, kmax + 1) for c in range(-kmax, kmax + 1)
+          if 0 < a * a + b * b + c * c <= kmax * kmax]
+    idx = rng.choice(len(ks), size=nmodes, replace=False)
+    amp = rng.uniform(0.0, 1.0, size=(nmodes, 3))
+    ph = rng.uniform(0.0, 2.0 * math.pi, size=(nmodes, 3))
+    return [ks[i] for i in idx], amp, ph
+
+
+def _vec_modes(k, amp, ph, X, Y, Z):
+    """Sum of modes a_m cos(k.x + p_m) for each of 3 components."""
+    out = [np.zeros(X.shape) for _ in range(3)]
+    for (kx, ky, kz), a, p in zip(k, amp, ph):
+        arg = kx * X + ky * Y + kz * Z
+        kk = math.sqrt(kx * kx + ky * ky + kz * kz)
+        for c in range(3):
+            out[c] += (a[c] / kk) * np.cos(arg + p[c])
+    return out
+
+
+def turbulence3d(n: int = 512, seed: int = 4, nmodes: int = 32, kmax: int = 4, gam: float = 5.0 / 3.0,
+                 vrms: float = 1.0, brms: float = 0.5, bz0: float = 0.5, dtype_chunks: int = 0) -> Problem:
+    """3D MHD turbulence on [0, 2pi]^3, periodic, rho = 1, p = 1/gam.
+
+    Velocity = curl(psi) and delta-B = curl(A) with psi, A sums of `nmodes`
+    random modes 0 < |k| <= kmax (PCG64(seed)); A lives at cell edges
+    (Ax at (i, j-1/2, k-1/2), Ay at (i-1/2, j, k-1/2), Az at (i-1/2, j-1/2, k))
+    and face B is its 4th-order discrete curl, as in
+    3D/MPI/loop/mhd3d_init_.cpp:36-75; plus a mean Bz = bz0."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = 2.0 * math.pi
+    d = L / n
+    N = n + 2 * NG
+    c = (np.arange(N) - NG + 0.5) * d
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    h = 0.5 * d
+    # velocity: analytic curl of psi at cell centres
+    kv, av, pv = _modes3d(rng, nmodes, kmax)
+    vx = np.zeros(X.shape)
+    vy = np.zeros(X.shape)
+    vz = np.zeros(X.shape)
+    for (kx, ky, kz), a, p in zip(kv, av, pv):
+        arg = kx * X + ky * Y + kz * Z
+        kk = math.sqrt(kx * kx + ky * ky + kz * kz)
+        s = [-(a[q] / kk) * np.sin(arg + p[q]) for q in range(3)]  # d/dx of cos = -k sin
+        vx += ky * s[2] - kz * s[1]
+        vy += kz * s[0] - kx * s[2]
+        vz += kx * s[1] - ky * s[0]
+    del s
+    ka, aa, pa = _modes3d(rng, nmodes, kmax)
+    Ax = _vec_modes(ka, aa, pa, X, Y - h, Z - h)[0]
+    Ay = _vec_modes(ka, aa, pa, X - h, Y, Z - h)[1]
+    Az = _vec_modes(ka, aa, pa, X - h, Y - h, Z)[2]
+    del X, Y, Z
+    bx = np.zeros((N, N, N))
+    by = np.zeros((N, N, N))
+    bz = np.zeros((N, N, N))
+    # bx(i-1/2,j,k) = dAz/dy - dAy/dz
+    bx[1:-2, 1:-2, :] = (_df4(Az[1:-2, 0:-3, :], Az[1:-2, 1:-2, :], Az[1:-2, 2:-1, :], Az[1:-2, 3:, :]) / d
+                         - _df4(Ay[0:-3, 1:-2, :], Ay[1:-2, 1:-2, :], Ay[2:-1, 1:-2, :], Ay[3:, 1:-2, :]) / d)
+    # by(i,j-1/2,k) = dAx/dz - dAz/dx
+    by[1:-2, :, 1:-2] = (_df4(Ax[0:-3, :, 1:-2], Ax[1:-2, :, 1:-2], Ax[2:-1, :, 1:-2], Ax[3:, :, 1:-2]) / d
+                         - _df4(Az[1:-2, :, 0:-3], Az[1:-2, :, 1:-2], Az[1:-2, :, 2:-1], Az[1:-2, :, 3:]) / d)
+    # bz(i,j,k-1/2) = dAy/dx - dAx/dy
+    bz[:, 1:-2, 1:-2] = (_df4(Ay[:, 1:-2, 0:-3], Ay[:, 1:-2, 1:-2], Ay[:, 1:-2, 2:-1], Ay[:, 1:-2, 3:]) / d
+                         - _df4(Ax[:, 0:-3, 1:-2], Ax[:, 1:-2, 1:-2], Ax[:, 2:-1, 1:-2], Ax[:, 3:, 1:-2]) / d)
+    del Ax, Ay, Az
+    for b in (bx, by, bz):
+        periodic_fill(b, (0, 1, 2))
+    I = (slice(NG, N - NG),) * 3
+    cx = _fcen4(bx[..., 0:-3], bx[..., 1:-2], bx[..., 2:-1], bx[..., 3:])[I[0], I[1], NG - 1:N - NG - 1]
+    cy = _fcen4(by[:, 0:-3, :], by[:, 1:-2, :], by[:, 2:-1, :], by[:, 3:, :])[I[0], NG - 1:N - NG - 1, I[2]]
+    cz = _fcen4(bz[0:-3], bz[1:-2], bz[2:-1], bz[3:])[NG - 1:N - NG - 1, I[1], I[2]]
+    sv = vrms / math.sqrt(np.mean(vx[I] ** 2 + vy[I] ** 2 + vz[I] ** 2))
+    sb = brms / math.sqrt(np.mean(cx ** 2 + cy ** 2 + cz ** 2))
+    del cx, cy, cz
+    for v in (vx, vy, vz):
+        v *= sv
+    for b in (bx, by, bz):
+        b *= sb
+    bz += bz0
+    ro = np.ones((N, N, N))
+    pr = np.full((N, N, N), 1.0 / gam)
+    st = np.empty((8, N, N, N))
+    st[0] = ro
+    st[1] = vx
+    st[2] = vy
+    st[3] = vz
+    st[4] = bx
+    st[5] = by
+    st[6] = bz
+    # energy with cell-centred B (fcen of the face values)
+    CX = np.zeros((N, N, N))
+    CY = np.zeros((N, N, N))
+    CZ = np.zeros((N, N, N))
+    CX[..., 1:-2] = _fcen4(bx[..., 0:-3], bx[..., 1:-2], bx[..., 2:-1], bx[..., 3:])
+    CY[:, 1:-2, :] = _fcen4(by[:, 0:-3, :], by[:, 1:-2, :], by[:, 2:-1, :], by[:, 3:, :])
+    CZ[1:-2] = _fcen4(bz[0:-3], bz[1:-2], bz[2:-1], bz[3:])
+    for q in (CX, CY, CZ):
+        periodic_fill(q, (0, 1, 2))
+    st[7] = energy(ro, vx, vy, vz, CX, CY, CZ, pr, gam)
+    del CX, CY, CZ
+    return Problem(state=st, dx=d, dy=d, dz=d, gam=gam, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1,
+                   name="turb3d_%d" % n)

@@ -1,0 +1,215 @@
+"""Write golden vectors from the reference's own code (run in the build
+container, where /root/reference exists).
+
+    python tests/golden/make_golden.py
+
+Every expected output below is produced by the reference's sources compiled
+from /root/reference by oracle/build.py (common/*.c, the 1D/SERIAL/shock and
+2D/SERIAL/OTvortex drivers, g++ -O2 -fopenmp as Makefile.inc:2).  Inputs are
+seeded numpy draws; both inputs and outputs are stored, so the fixtures are
+self-contained data and need neither the reference nor this script at test
+time.  Files:
+
+  unit_vectors.npz  per-function vectors: interpolation (interpol/l_interp
+                    tables, ODR 1..4), fcen/df, mhd_lrstate (ODR 1..4),
+                    calc_flux_{roe,hlld,lhlld,mlau}, mhd_cuct_weight
+  steps_1d.npz      Brio-Wu 1024 (config 1), all four solvers
+  steps_2d.npz      2D decaying-turbulence states after N steps: every
+                    RMN x ODR pair, R_K 1/2, CTW 0, four boundary families
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import build  # noqa: E402
+from qasmhd_amd import problems  # noqa: E402
+
+RNG = np.random.Generator(np.random.PCG64(20261202))
+
+
+# ------------------------------------------------------------------ units
+def interp_inputs(n=400):
+    f = RNG.normal(size=(n, 5))
+    f[:40] = RNG.normal(size=(40, 1)) + 0.0 * f[:40]          # constant stencils
+    f[40:80] = (RNG.normal(size=(40, 1)) + RNG.normal(size=(40, 1)) * np.arange(5))  # linear
+    f[80:120] = np.where(np.arange(5) < 2, 1.0, 0.1) * RNG.uniform(0.5, 2, size=(40, 1))  # steps
+    f[120:140] = 0.0
+    f[140:160] *= 1e-12
+    f[160:180] *= 1e6
+    f[180:200] = RNG.normal(size=(20, 5)) ** 2  # positive rough
+    return f
+
+
+def lrstate_inputs(n=600, gam=5.0 / 3.0):
+    """7 stencils (ro, vn, vt, vu, bt, bu, pr) of 5 points + bn."""
+    ro = np.exp(RNG.normal(scale=0.4, size=(n, 5)))
+    v = RNG.normal(size=(n, 3, 5)) * RNG.choice([0.3, 1.0, 3.0], size=(n, 1, 1))
+    b = RNG.normal(size=(n, 2, 5))
+    pr = np.exp(RNG.normal(scale=0.4, size=(n, 5)))
+    bn = RNG.normal(size=n)
+    # exact-zero flags (mhd_eigen.c:575-587): zero stencils
+    v[0:60, 0] = 0.0
+    v[60:120, 1:] = 0.0
+    b[120:180] = 0.0
+    # near-vacuum -> positivity fallback (mhd_func.c:150-160)
+    ro[180:260, 1] = 1e-6
+    pr[180:260, 3] = 1e-8
+    ro[260:300] = np.where(np.arange(5) < 2, 1.0, 1e-5)
+    # smooth stencils
+    x = np.arange(5)[None, :]
+    ro[300:360] = 1.0 + 0.1 * np.sin(0.3 * x + RNG.uniform(0, 6, size=(60, 1)))
+    st = np.zeros((n, 7, 5))
+    st[:, 0] = ro
+    st[:, 1:4] = v
+    st[:, 4:6] = b
+    st[:, 6] = pr
+    return st, bn
+
+
+def flux_inputs(n=800):
+    def state(m):
+        q = np.zeros((m, 7))
+        q[:, 0] = np.exp(RNG.normal(scale=0.5, size=m))
+        q[:, 1:4] = RNG.normal(size=(m, 3)) * RNG.choice([0.01, 0.5, 2.0, 6.0], size=(m, 1))
+        q[:, 4:6] = RNG.normal(size=(m, 2))
+        q[:, 6] = np.exp(RNG.normal(scale=0.5, size=m))
+        return q
+    L, R = state(n), state(n)
+    bn = RNG.normal(size=n)
+    dv = RNG.normal(size=(n, 2)) * 0.5
+    bn[0:80] = 0.0                              # HLLC-type limit
+    R[80:160] = L[80:160]                       # consistency F(U,U)
+    L[160:200, 4:6] = 0.0
+    R[160:200, 4:6] = 0.0                       # pure hydro
+    R[200:260] = L[200:260] * (1 + 1e-9 * RNG.normal(size=(60, 7)))  # near-degenerate
+    L[260:300, 1] = 10.0
+    R[260:300, 1] = 10.0                        # supersonic to the right
+    dv[300:340] = -np.abs(dv[300:340]) * 5.0    # compressive sensors
+    return L, R, bn, dv
+
+
+def cuct_inputs(n=300):
+    c = np.zeros((n, 12))
+    c[:, 0:4] = np.exp(RNG.normal(scale=0.3, size=(n, 4)))
+    c[:, 4:12] = RNG.normal(size=(n, 8))
+    b = RNG.normal(size=(n, 4))
+    c[0:30, 4:12] = 0.0
+    b[0:30] = 0.0                               # eps-dominated corner
+    return c, b
+
+
+def make_units(path):
+    out = {}
+    f5 = interp_inputs()
+    out["interp_in"] = f5
+    for kind in (0, 1, 2, 3, 10, 11, 12, 13):
+        out["interp_%d" % kind] = build.run_unit("interp", kind, f5)
+    f4 = RNG.normal(size=(200, 4))
+    f4[:20] = np.arange(4)[None, :] * RNG.normal(size=(20, 1)) + RNG.normal(size=(20, 1))
+    out["f4_in"] = f4
+    for odr in (1, 2, 3, 4):
+        out["fcen_%d" % odr] = build.run_unit("fcen", odr, f4)[:, 0]
+        out["df_%d" % odr] = build.run_unit("df", odr, f4)[:, 0]
+    gam = 5.0 / 3.0
+    st, bn = lrstate_inputs(gam=gam)
+    out["lrstate_st"], out["lrstate_bn"], out["lrstate_gam"] = st, bn, np.array(gam)
+    inp = np.concatenate([st.reshape(len(bn), 35), bn[:, None]], axis=1)
+    for odr in (1, 2, 3, 4):
+        r = build.run_unit("lrstate", odr, inp, gam)
+        out["lrstate_%d_vl" % odr], out["lrstate_%d_vr" % odr] = r[:, :7], r[:, 7:]
+    L, R, bnf, dv = flux_inputs()
+    out["flux_L"], out["flux_R"], out["flux_bn"], out["flux_dv"] = L, R, bnf, dv
+    for gname, g in (("g53", 5.0 / 3.0), ("g2", 2.0), ("g14", 1.4)):
+        out["flux_gam_%s" % gname] = np.array(g)
+        inp = np.concatenate([L, R, bnf[:, None], dv], axis=1)
+        for rmn in (0, 1, 2, 3):
+            out["flux_%d_%s" % (rmn, gname)] = build.run_unit("flux", rmn, inp, g)
+    c, b = cuct_inputs()
+    out["cuct_c"], out["cuct_b"] = c, b
+    out["cuct"] = build.run_unit("cuct", 0, np.concatenate([c, b], axis=1))[:, 0]
+    np.savez_compressed(path, **out)
+
+
+# ------------------------------------------------------------------ steps
+def make_steps_1d(path):
+    out = {}
+    p = problems.brio_wu(1024)
+    out["in"] = p.state
+    out["dx"], out["gam"], out["dn"] = np.array(p.dx), np.array(p.gam), p.dn
+    from oracle import pyoracle as po
+    cfg = po.make_cfg(p.state.shape[1:], p.dx, gam=p.gam, dn=p.dn)
+    dt0 = po.cfl_dt(cfg, p.state.copy())
+    out["dt0"] = np.array(dt0)
+    # config 1: setdt(1) then exec_(0) -> fixed dt, nrec = (int)(0.1/dt + 0.5) steps
+    nsteps = int(0.1 / dt0 + 0.5)
+    out["nsteps"] = np.array(nsteps)
+    for rmn in (0, 1, 2, 3):
+        b = build.build_ref(1, 1024, 1, rmn, 4, 3, 1, 0.4)
+        st, hist = build.run_ref(b, p.state, p.dx, p.gam, nsteps if rmn == 1 else 200, 0, dt=dt0,
+                                 dnx=list(p.dn[0]))
+        out["out_r%d" % rmn] = st
+        out["steps_r%d" % rmn] = np.array(nsteps if rmn == 1 else 200)
+    np.savez_compressed(path, **out)
+
+
+CASES_2D = []
+for _rmn in (0, 1, 2, 3):
+    for _odr in (1, 2, 3, 4):
+        CASES_2D.append(dict(name="r%do%d" % (_rmn, _odr), rmn=_rmn, odr=_odr, r_k=3, ctw=1,
+                             dnx=[0] * 8, dny=[0] * 8))
+CASES_2D += [
+    dict(name="rk1", rmn=1, odr=4, r_k=1, ctw=1, dnx=[0] * 8, dny=[0] * 8),
+    dict(name="rk2", rmn=1, odr=4, r_k=2, ctw=1, dnx=[0] * 8, dny=[0] * 8),
+    dict(name="ctw0", rmn=3, odr=4, r_k=3, ctw=0, dnx=[0] * 8, dny=[0] * 8),
+    dict(name="bcA", rmn=1, odr=4, r_k=3, ctw=1, dnx=[2] * 8, dny=[1, 1, -1, 1, 1, -1, 1, 1]),
+    dict(name="bcB", rmn=3, odr=4, r_k=3, ctw=1, dnx=[1, -1, 1, -2, -1, 1, -2, 1], dny=[0] * 8),
+    dict(name="bcC", rmn=2, odr=4, r_k=3, ctw=1, dnx=[0] * 8, dny=[2, 2, 2, -2, 2, 2, 2, 2]),
+    dict(name="bcD", rmn=0, odr=3, r_k=3, ctw=1, dnx=[1, -1, 1, 1, -1, 1, 1, 1], dny=[1, 1, -1, 1, 1, -1, 1, 1]),
+]
+N2D = 16
+NSTEPS_2D = 6
+
+
+def make_steps_2d(path):
+    out = {}
+    p = problems.turbulence2d(n=N2D, seed=11, vrms=1.0)
+    out["in"] = p.state
+    out["dx"], out["dy"], out["gam"] = np.array(p.dx), np.array(p.dy), np.array(p.gam)
+    out["nsteps"] = np.array(NSTEPS_2D)
+    names = []
+    for c in CASES_2D:
+        b = build.build_ref(2, N2D, N2D, c["rmn"], c["odr"], c["r_k"], c["ctw"], 0.4)
+        st, hist = build.run_ref(b, p.state, p.dx, p.gam, NSTEPS_2D, 1, dnx=c["dnx"], dny=c["dny"], dy=p.dy)
+        assert np.isfinite(st).all(), c
+        out["out_" + c["name"]] = st
+        out["dt_" + c["name"]] = hist
+        out["cfg_" + c["name"]] = np.array([c["rmn"], c["odr"], c["r_k"], c["ctw"]] + c["dnx"] + c["dny"])
+        names.append(c["name"])
+    out["names"] = np.array(names)
+    # one longer run: 64^2, 40 steps, HLLD
+    p64 = problems.turbulence2d(n=64, seed=5)
+    b = build.build_ref(2, 64, 64, 1, 4, 3, 1, 0.4)
+    st, hist = build.run_ref(b, p64.state, p64.dx, p64.gam, 40, 1, dy=p64.dy)
+    out["long_in"], out["long_out"], out["long_dt"] = p64.state, st, hist
+    np.savez_compressed(path, **out)
+
+
+def main():
+    if not build.reference_present():
+        raise SystemExit("the reference sources are needed to write golden vectors")
+    make_units(os.path.join(HERE, "unit_vectors.npz"))
+    make_steps_1d(os.path.join(HERE, "steps_1d.npz"))
+    make_steps_2d(os.path.join(HERE, "steps_2d.npz"))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()
