@@ -1,0 +1,122 @@
+/*
+ * qasmhd.h -- C ABI of the MI355X (gfx950) ideal-MHD step of qasMHD.
+ *
+ * The reference has no FFI: its hot path sits behind the protected driver
+ * methods ideal(double dt), bound() and setdt(int) of MHD1D / MHD2D / MHD3D
+ * (1D/SERIAL/shock/mhd1d_class.hpp:35-36, 2D/SERIAL/OTvortex/mhd2d_class.hpp:
+ * 43-45, 3D/MPI/OTvortex/mhd3d_class.hpp:58-60), reached only from exec_()
+ * (mhd2d_class.cpp:35-61).  A driver that swaps this library in overrides
+ * those three methods with the calls below; its field
+ * layout (common/mhd_class.hpp:32-42) and solver switches (RMN, ODR, R_K,
+ * CTW of mymacros.hpp, dnxs/dnys/dnzs and stxs/stys/stzs) carry over as the
+ * runtime fields of qas_grid / qas_scheme.
+ *
+ * Conventions
+ *  - State = 8 SoA double arrays val[0..7] = {ro, mx, my, mz, bx, by, bz, en}
+ *    of nx*ny*nz cells each, ghosts included, index ss = nx*(ny*k+j)+i;
+ *    bx at the (i-1/2) face, by at (j-1/2), bz at (k-1/2).
+ *  - Every call returns 0 on success or a negative QAS_E* code; the message
+ *    of the last failure is qas_last_error().  Nothing throws across the ABI.
+ *  - One host thread per context.  Calls are ordered on the context's HIP
+ *    stream; qas_download and qas_cfl_dt synchronise, the others are
+ *    asynchronous.
+ */
+#ifndef QASMHD_H
+#define QASMHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QAS_OK 0
+#define QAS_E_ARG (-1)      /* invalid argument / unsupported configuration */
+#define QAS_E_HIP (-2)      /* HIP runtime error (message carries the HIP name) */
+#define QAS_E_NOMEM (-3)    /* device allocation failed */
+#define QAS_E_STATE (-4)    /* call out of order (e.g. step before upload) */
+
+typedef struct qas_ctx qas_ctx;
+
+/* Grid: cells per axis INCLUDE the ghosts, as nx/ny/nz of mhd*_class.hpp.
+ * ndim 1: ny = nz = 1; ndim 2: nz = 1.  xoff/yoff/zoff must be 4 on the
+ * active axes (mhd2d_class.hpp:10) and 0 on the others. */
+typedef struct {
+  int ndim;
+  int nx, ny, nz;
+  int xoff, yoff, zoff;
+  double dx, dy, dz;
+} qas_grid;
+
+/* Scheme: the reference's compile-time switches as runtime fields. */
+typedef struct {
+  double gam;       /* specific-heat ratio (MHD::gam, mhd_class.hpp:32)       */
+  double cfl;       /* CFL number (mymacros.hpp CFL)                         */
+  int rmn;          /* Riemann solver: 0 Roe, 1 HLLD, 2 LHLLD, 3 MLAU (RMN)  */
+  int odr;          /* spatial order 1..4 (ODR)                              */
+  int r_k;          /* SSP-RK stages 1..3 (R_K)                              */
+  int ctw;          /* CUCT upwind weighting on/off (CTW)                    */
+  int dn[3][8];     /* boundary flags dnxs/dnys/dnzs: 0 periodic, +-1
+                       Neumann/Dirichlet, +-2 open/zero-fix                 */
+  int st[3][8];     /* staggering flags stxs/stys/stzs (bx:x, by:y, bz:z = 1) */
+  int precision;    /* 0 = FP64 (bit-identical to the reference),
+                       1 = mixed (FP32 reconstruction and fluxes; FP64 state,
+                           RK stage sum and pressure recovery)               */
+} qas_scheme;
+
+/* Fill a scheme with the reference defaults: gam 5/3, cfl 0.4, HLLD, ODR 4,
+ * R_K 3, CTW 1, periodic, stxs[4] = stys[5] = stzs[6] = 1, FP64. */
+void qas_scheme_default(qas_scheme *s);
+
+/* Allocate device state for one domain on the current HIP device.
+ * Replaces MHD1D/2D/3D's constructor-side arrays (mhd2d_class.cpp:116-164)
+ * on the device. */
+int qas_create(qas_ctx **out, const qas_grid *g, const qas_scheme *s);
+void qas_destroy(qas_ctx *c);
+const char *qas_last_error(const qas_ctx *c);
+
+/* Host <-> device copies of the state (val[] order above). */
+int qas_upload(qas_ctx *c, const double *const val[8]);
+int qas_download(qas_ctx *c, double *const val[8]);
+/* Same for one contiguous host block of 8*nd doubles (variable-major). */
+int qas_upload_packed(qas_ctx *c, const double *state);
+int qas_download_packed(qas_ctx *c, double *state);
+/* Device-to-device copies: `state` is a device pointer to 8*nd doubles. */
+int qas_upload_device(qas_ctx *c, const double *state);
+int qas_download_device(qas_ctx *c, double *state);
+
+/* bound(val): ghost fill of the 8 variables -- replaces MHD2D::bound /
+ * bc1d / bc2d / bc3d (common/common_func.c:34-278) and, on one device, the
+ * MPI halo exchange mpi_sdrv3d + mpi_[xyz]bc3d (mpi/common_mpi.c:198-525). */
+int qas_bound(qas_ctx *c);
+
+/* ideal(dt): one SSP-RK step including the per-stage bound -- replaces
+ * MHD1D/2D/3D::ideal (1D/SERIAL/shock/mhd1d_solve.cpp:3-124,
+ * 2D/SERIAL/OTvortex/mhd2d_solve.cpp:3-400, 3D/MPI/OTvortex/mhd3d_solve.cpp:
+ * 3-729). */
+int qas_step(qas_ctx *c, double dt);
+
+/* setdt(1): CFL time step of the current state (vmax floored at 1.0,
+ * 2nd-order cell-centred B) -- replaces MHD*::setdt (mhd2d_class.cpp:11-33,
+ * mhd1d_class.cpp:11-30, mhd3d_class.cpp:15-46).  Synchronises. */
+int qas_cfl_dt(qas_ctx *c, double *dt_out);
+
+/* exec_-style loop on the device (mhd2d_class.cpp:44-57 without I/O):
+ * nsteps x {bound; ideal(dt); if flg and n even: dt = setdt}, with dt kept
+ * in device memory between steps (no host synchronisation).  On entry dt is
+ * *dt_inout if *dt_inout > 0, else setdt(1) of the current state; on exit
+ * *dt_inout holds the current dt (this synchronises once).  dt_hist, if not
+ * NULL, receives the dt used by each step.  The step counter n continues
+ * across calls (n % 2 is taken on the running count). */
+int qas_run(qas_ctx *c, int nsteps, int flg, double *dt_inout, double *dt_hist);
+/* Running step count used by qas_run's n % 2 schedule; reset to 0 here. */
+int qas_reset_counter(qas_ctx *c);
+
+/* Block until all work queued on the context has finished. */
+int qas_sync(qas_ctx *c);
+
+/* Device memory held by the context (bytes). */
+long long qas_device_bytes(const qas_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,80 @@
+"""Build libqasmhd_amd.so (HIP, gfx950) in-tree: qasmhd_amd/_lib/.
+
+    python -m qasmhd_amd.build        # or __graft_entry__.build()
+
+Every .hip file under csrc/ is compiled with hipcc for gfx950 and linked
+into one shared library exporting the C ABI of include/qasmhd.h.  Flags:
+  -ffp-contract=off      no FMA contraction: the FP64 path keeps the
+                         reference's rounding sequence (bit-identical results)
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                         no packed FP32 instructions are generated
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "_lib")
+OBJDIR = os.path.join(HERE, "_lib", "obj")
+LIBNAME = "libqasmhd_amd.so"
+ARCH = os.environ.get("QAS_OFFLOAD_ARCH", "gfx950")
+
+COMMON_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-Wno-unused-command-line-argument", "-Wno-unused-result"]
+# per-source extra flags (the mixed-precision unit may contract FP32 arithmetic)
+EXTRA = {}
+
+
+def lib_path() -> str:
+    return os.path.join(LIBDIR, LIBNAME)
+
+
+def _hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if c and (os.path.sep not in c or os.path.exists(c)):
+            return c
+    return "hipcc"
+
+
+def sources():
+    return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(OBJDIR, src[:-4] + ".o")
+    flags = list(COMMON_FLAGS) + EXTRA.get(src, ["-ffp-contract=off"])
+    cmd = [_hipcc()] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
+    return obj
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> str:
+    """Compile every HIP source and link the shared library.  Always
+    recompiles (this is the build check)."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    out = lib_path()
+    tmp = out + ".tmp%d" % os.getpid()
+    cmd = [_hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, out)
+    if verbose:
+        print(out)
+    return out
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,202 @@
+"""ctypes binding of libqasmhd_amd (include/qasmhd.h) and a small Python
+driver over it.
+
+The library must have been built (``python -m qasmhd_amd.build``); there is
+no fallback: if the shared library is missing, or no HIP device is present,
+every entry point raises.
+
+States are float64 numpy arrays of shape (8, [nz,] [ny,] nx), ghosts
+included, variable order ro, mx, my, mz, bx, by, bz, en.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libqasmhd_amd.so")
+
+QAS_OK = 0
+
+
+class QasGrid(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xoff", C.c_int), ("yoff", C.c_int), ("zoff", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class QasScheme(C.Structure):
+    _fields_ = [("gam", C.c_double), ("cfl", C.c_double), ("rmn", C.c_int), ("odr", C.c_int),
+                ("r_k", C.c_int), ("ctw", C.c_int), ("dn", (C.c_int * 8) * 3), ("st", (C.c_int * 8) * 3),
+                ("precision", C.c_int)]
+
+
+EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", "qas_upload", "qas_download",
+           "qas_upload_packed", "qas_download_packed", "qas_upload_device", "qas_download_device", "qas_bound",
+           "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_sync", "qas_device_bytes"]
+
+_lib = None
+
+
+class QasError(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load the HIP library (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise QasError("libqasmhd_amd.so is not built (%s); run `python -m qasmhd_amd.build`" % p)
+    l = C.CDLL(p)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    l.qas_scheme_default.argtypes = [C.POINTER(QasScheme)]
+    l.qas_scheme_default.restype = None
+    l.qas_create.argtypes = [C.POINTER(vp), C.POINTER(QasGrid), C.POINTER(QasScheme)]
+    l.qas_destroy.argtypes = [vp]
+    l.qas_destroy.restype = None
+    l.qas_last_error.argtypes = [vp]
+    l.qas_last_error.restype = C.c_char_p
+    for f in ("qas_upload", "qas_download"):
+        getattr(l, f).argtypes = [vp, C.POINTER(dp)]
+    for f in ("qas_upload_packed", "qas_download_packed"):
+        getattr(l, f).argtypes = [vp, dp]
+    for f in ("qas_upload_device", "qas_download_device"):
+        getattr(l, f).argtypes = [vp, vp]
+    l.qas_bound.argtypes = [vp]
+    l.qas_step.argtypes = [vp, C.c_double]
+    l.qas_cfl_dt.argtypes = [vp, dp]
+    l.qas_run.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    l.qas_reset_counter.argtypes = [vp]
+    l.qas_sync.argtypes = [vp]
+    l.qas_device_bytes.argtypes = [vp]
+    l.qas_device_bytes.restype = C.c_longlong
+    _lib = l
+    return l
+
+
+def exported_symbols(path: str | None = None):
+    """Names from EXPORTS that the library resolves (no device calls)."""
+    l = C.CDLL(path or LIB_PATH)
+    return [n for n in EXPORTS if hasattr(l, n)]
+
+
+def make_scheme(gam=5.0 / 3.0, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1, dn=None, st=None, precision=0):
+    s = QasScheme()
+    load().qas_scheme_default(C.byref(s))
+    s.gam, s.cfl, s.rmn, s.odr, s.r_k, s.ctw, s.precision = gam, cfl, rmn, odr, r_k, ctw, precision
+    if dn is not None:
+        dn = np.asarray(dn)
+        for a in range(3):
+            for m in range(8):
+                s.dn[a][m] = int(dn[a][m])
+    if st is not None:
+        st = np.asarray(st)
+        for a in range(3):
+            for m in range(8):
+                s.st[a][m] = int(st[a][m])
+    return s
+
+
+def make_grid(shape, dx, dy=1.0, dz=1.0):
+    """shape = per-variable array shape (nx,), (ny, nx) or (nz, ny, nx)."""
+    g = QasGrid()
+    g.ndim = len(shape)
+    dims = list(reversed(shape)) + [1] * (3 - len(shape))
+    g.nx, g.ny, g.nz = dims
+    g.xoff = 4
+    g.yoff = 4 if g.ndim >= 2 else 0
+    g.zoff = 4 if g.ndim >= 3 else 0
+    g.dx, g.dy, g.dz = dx, dy, dz
+    return g
+
+
+class Solver:
+    """One device-resident domain (owns a qas_ctx)."""
+
+    def __init__(self, shape, dx, dy=1.0, dz=1.0, **scheme):
+        self.lib = load()
+        self.shape = tuple(shape)
+        self.grid = make_grid(shape, dx, dy, dz)
+        self.scheme = make_scheme(**scheme)
+        self.ctx = C.c_void_p()
+        r = self.lib.qas_create(C.byref(self.ctx), C.byref(self.grid), C.byref(self.scheme))
+        if r != QAS_OK:
+            msg = self.lib.qas_last_error(self.ctx).decode() if self.ctx else "qas_create failed"
+            self.lib.qas_destroy(self.ctx)
+            self.ctx = None
+            raise QasError("qas_create: %s (%d)" % (msg, r))
+
+    def _check(self, r, what):
+        if r != QAS_OK:
+            raise QasError("%s: %s (%d)" % (what, self.lib.qas_last_error(self.ctx).decode(), r))
+
+    def close(self):
+        if self.ctx:
+            self.lib.qas_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, state):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert st.shape == (8,) + self.shape, (st.shape, self.shape)
+        self._check(self.lib.qas_upload_packed(self.ctx, st.ctypes.data_as(C.POINTER(C.c_double))), "qas_upload")
+
+    def download(self):
+        out = np.empty((8,) + self.shape)
+        self._check(self.lib.qas_download_packed(self.ctx, out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "qas_download")
+        return out
+
+    def bound(self):
+        self._check(self.lib.qas_bound(self.ctx), "qas_bound")
+
+    def step(self, dt):
+        self._check(self.lib.qas_step(self.ctx, float(dt)), "qas_step")
+
+    def cfl_dt(self):
+        d = C.c_double(0.0)
+        self._check(self.lib.qas_cfl_dt(self.ctx, C.byref(d)), "qas_cfl_dt")
+        return d.value
+
+    def run(self, nsteps, flg=1, dt=None, history=True):
+        """exec_-style loop on the device; returns (dt_history, dt_now)."""
+        d = C.c_double(dt if dt is not None else 0.0)
+        hist = np.zeros(max(nsteps, 1))
+        hp = hist.ctypes.data_as(C.POINTER(C.c_double)) if history else None
+        self._check(self.lib.qas_run(self.ctx, int(nsteps), int(flg), C.byref(d), hp), "qas_run")
+        return hist[:nsteps], d.value
+
+    def reset_counter(self):
+        self._check(self.lib.qas_reset_counter(self.ctx), "qas_reset_counter")
+
+    def sync(self):
+        self._check(self.lib.qas_sync(self.ctx), "qas_sync")
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.qas_device_bytes(self.ctx))
+
+
+def solver_for(problem, **over):
+    """Solver configured from a problems.Problem (scheme fields overridable)."""
+    kw = dict(gam=problem.gam, cfl=problem.cfl, rmn=problem.rmn, odr=problem.odr, r_k=problem.r_k,
+              ctw=problem.ctw, dn=problem.dn)
+    kw.update(over)
+    return Solver(problem.state.shape[1:], problem.dx, problem.dy, problem.dz, **kw)

@@ -1,0 +1,376 @@
+// qas_capi.hip -- extern "C" entry points of libqasmhd_amd (include/qasmhd.h).
+//
+// Owns the device state of one domain: three state buffers A, B, C for the
+// RK stages (stage k reads one and writes the next; the step base U^n is A
+// itself, read and rewritten element-wise by the last stage, so no U^n
+// snapshot copy is made as cpy_array does in mhd2d_solve.cpp:38-41), the
+// primitive scratch, the sweep accumulators and face EMF parts, and dt in
+// device memory so that the exec_ loop runs without host round trips.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qas_internal.hpp"
+
+namespace qas {
+hipError_t launch_record_dt(qas_ctx* c, double* slot);
+}
+
+using namespace qas;
+
+static int fail(qas_ctx* c, int code, const std::string& msg)
+{
+  if (c) c->err = msg;
+  return code;
+}
+
+static int hipfail(qas_ctx* c, hipError_t e, const char* where)
+{
+  return fail(c, QAS_E_HIP, std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+
+#define QAS_CHECK(c, expr)                              \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hipfail((c), _e, #expr); \
+  } while (0)
+
+extern "C" void qas_scheme_default(qas_scheme* s)
+{
+  std::memset(s, 0, sizeof(*s));
+  s->gam = 5.0 / 3.0;
+  s->cfl = 0.4;
+  s->rmn = 1;
+  s->odr = 4;
+  s->r_k = 3;
+  s->ctw = 1;
+  s->st[0][4] = 1;
+  s->st[1][5] = 1;
+  s->st[2][6] = 1;
+  s->precision = 0;
+}
+
+static std::string validate(const qas_grid* g, const qas_scheme* s)
+{
+  if (!g || !s) return "null grid or scheme";
+  if (g->ndim < 1 || g->ndim > 3) return "ndim must be 1, 2 or 3";
+  const int n[3] = {g->nx, g->ny, g->nz};
+  const int off[3] = {g->xoff, g->yoff, g->zoff};
+  const double d[3] = {g->dx, g->dy, g->dz};
+  for (int a = 0; a < 3; a++) {
+    if (a < g->ndim) {
+      if (off[a] != NG) return "ghost width must be 4 on active axes";
+      if (n[a] < 2 * NG + 8) return "each active axis needs at least 8 interior cells";
+      if (!(d[a] > 0.0)) return "grid spacing must be positive";
+    } else {
+      if (n[a] != 1 || off[a] != 0) return "inactive axes must have n = 1 and no ghosts";
+    }
+  }
+  if ((long long)g->nx * g->ny * g->nz > (1LL << 40)) return "grid too large";
+  if (s->rmn < 0 || s->rmn > 3) return "rmn must be 0..3";
+  if (s->odr < 1 || s->odr > 4) return "odr must be 1..4";
+  if (s->r_k < 1 || s->r_k > 3) return "r_k must be 1..3";
+  if (s->ctw != 0 && s->ctw != 1) return "ctw must be 0 or 1";
+  if (s->precision != 0 && s->precision != 1) return "precision must be 0 (FP64) or 1 (mixed)";
+  if (!(s->gam > 1.0)) return "gam must exceed 1";
+  for (int a = 0; a < 3; a++)
+    for (int m = 0; m < 8; m++) {
+      const int dn = s->dn[a][m], st = s->st[a][m];
+      if (dn < -2 || dn > 2) return "boundary flags must be in -2..2";
+      if (st != 0 && st != 1) return "staggering flags must be 0 or 1";
+    }
+  // one device domain: an axis is periodic for every variable or for none
+  // (the reference's MPI bound applies the periodic copy of dnxs[0] to all
+  // variables, common_mpi.c:214-269)
+  for (int a = 0; a < g->ndim; a++) {
+    const bool p0 = s->dn[a][0] == 0;
+    for (int m = 1; m < 8; m++)
+      if ((s->dn[a][m] == 0) != p0) return "an axis must be periodic for all variables or for none";
+  }
+  if (g->ndim == 1 && s->precision != 0) return "the 1D path is FP64 only";
+  return "";
+}
+
+extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
+{
+  if (!out) return QAS_E_ARG;
+  *out = nullptr;
+  qas_ctx* c = new qas_ctx();
+  std::string v = validate(g, s);
+  if (!v.empty()) {
+    c->err = v;
+    *out = c;  // return the context so qas_last_error can report
+    return QAS_E_ARG;
+  }
+  c->g = *g;
+  c->s = *s;
+  Geo& geo = c->geo;
+  geo.ndim = g->ndim;
+  geo.nx = g->nx;
+  geo.ny = g->ny;
+  geo.nz = g->nz;
+  geo.sy = g->nx;
+  geo.sz = (long long)g->nx * g->ny;
+  geo.nd = (long long)g->nx * g->ny * g->nz;
+  geo.dx = g->dx;
+  geo.dy = g->dy;
+  geo.dz = g->dz;
+  for (int a = 0; a < 3; a++)
+    for (int m = 0; m < 8; m++) {
+      c->bc.dn[a][m] = (signed char)s->dn[a][m];
+      c->bc.st[a][m] = (signed char)s->st[a][m];
+    }
+  *out = c;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return hipfail(c, e, "hipGetDevice");
+  QAS_CHECK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // one slab: 24 state arrays + scratch, each padded to 512 bytes
+  const long long nd = geo.nd;
+  const size_t stride = (size_t)((nd + 63) / 64 * 64);
+  const int nprim = g->ndim == 3 ? 7 : (g->ndim == 2 ? 6 : 4);
+  const int nacc = g->ndim == 1 ? 0 : 6;
+  const int nemf = g->ndim == 3 ? 12 : (g->ndim == 2 ? 8 : 0);
+  const int nedge = g->ndim == 3 ? 3 : (g->ndim == 2 ? 1 : 0);
+  const size_t narr = 24 + nprim + nacc + nemf + nedge;
+  c->slab_bytes = narr * stride * sizeof(double) + 4096;
+  e = hipMalloc(&c->slab, c->slab_bytes);
+  if (e != hipSuccess) return fail(c, QAS_E_NOMEM, std::string("hipMalloc of the domain failed: ") + hipGetErrorName(e));
+  double* p = (double*)c->slab;
+  for (int b = 0; b < 3; b++)
+    for (int m = 0; m < 8; m++) {
+      c->U[b][m] = p;
+      p += stride;
+    }
+  c->vx = p; p += stride;
+  c->vy = p; p += stride;
+  c->vz = p; p += stride;
+  c->pr = p; p += stride;
+  if (g->ndim >= 2) { c->cx = p; p += stride; c->cy = p; p += stride; }
+  if (g->ndim == 3) { c->cz = p; p += stride; }
+  for (int k = 0; k < nacc; k++) { c->acc[k] = p; p += stride; }
+  for (int k = 0; k < nemf; k++) { c->emf[k] = p; p += stride; }
+  if (g->ndim == 2) { c->ez = p; p += stride; }
+  if (g->ndim == 3) { c->ex = p; p += stride; c->ey = p; p += stride; c->ez = p; p += stride; }
+  // small device words: dt, vmax accumulator
+  char* tail = (char*)p;
+  c->d_dt = (double*)tail;
+  c->d_vmax = (unsigned long long*)(tail + 64);
+  QAS_CHECK(c, hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
+  const double one = 1.0;
+  QAS_CHECK(c, hipMemcpyAsync(c->d_vmax, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  QAS_CHECK(c, hipHostMalloc((void**)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  return QAS_OK;
+}
+
+extern "C" void qas_destroy(qas_ctx* c)
+{
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->slab) hipFree(c->slab);
+  if (c->d_hist) hipFree(c->d_hist);
+  if (c->h_pinned) hipHostFree(c->h_pinned);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* qas_last_error(const qas_ctx* c)
+{
+  return c ? c->err.c_str() : "null context";
+}
+
+static bool ready(qas_ctx* c) { return c && c->slab; }
+
+extern "C" int qas_upload(qas_ctx* c, const double* const val[8])
+{
+  if (!ready(c) || !val) return fail(c, QAS_E_ARG, "qas_upload: null context or arrays");
+  const size_t bytes = (size_t)c->geo.nd * sizeof(double);
+  for (int m = 0; m < 8; m++) {
+    if (!val[m]) return fail(c, QAS_E_ARG, "qas_upload: null array");
+    QAS_CHECK(c, hipMemcpyAsync(c->U[c->cur][m], val[m], bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->bounded = false;
+  return QAS_OK;
+}
+
+extern "C" int qas_upload_packed(qas_ctx* c, const double* state)
+{
+  if (!ready(c) || !state) return fail(c, QAS_E_ARG, "qas_upload_packed: null argument");
+  const double* v[8];
+  for (int m = 0; m < 8; m++) v[m] = state + (size_t)m * c->geo.nd;
+  return qas_upload(c, v);
+}
+
+extern "C" int qas_upload_device(qas_ctx* c, const double* state)
+{
+  if (!ready(c) || !state) return fail(c, QAS_E_ARG, "qas_upload_device: null argument");
+  const size_t bytes = (size_t)c->geo.nd * sizeof(double);
+  for (int m = 0; m < 8; m++)
+    QAS_CHECK(c, hipMemcpyAsync(c->U[c->cur][m], state + (size_t)m * c->geo.nd, bytes, hipMemcpyDeviceToDevice,
+                                c->stream));
+  c->uploaded = true;
+  c->bounded = false;
+  return QAS_OK;
+}
+
+extern "C" int qas_download(qas_ctx* c, double* const val[8])
+{
+  if (!ready(c) || !val) return fail(c, QAS_E_ARG, "qas_download: null context or arrays");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_download: no state uploaded");
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  const size_t bytes = (size_t)c->geo.nd * sizeof(double);
+  for (int m = 0; m < 8; m++) {
+    if (!val[m]) return fail(c, QAS_E_ARG, "qas_download: null array");
+    QAS_CHECK(c, hipMemcpy(val[m], c->U[c->cur][m], bytes, hipMemcpyDeviceToHost));
+  }
+  return QAS_OK;
+}
+
+extern "C" int qas_download_packed(qas_ctx* c, double* state)
+{
+  if (!ready(c) || !state) return fail(c, QAS_E_ARG, "qas_download_packed: null argument");
+  double* v[8];
+  for (int m = 0; m < 8; m++) v[m] = state + (size_t)m * c->geo.nd;
+  return qas_download(c, v);
+}
+
+extern "C" int qas_download_device(qas_ctx* c, double* state)
+{
+  if (!ready(c) || !state) return fail(c, QAS_E_ARG, "qas_download_device: null argument");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_download_device: no state uploaded");
+  const size_t bytes = (size_t)c->geo.nd * sizeof(double);
+  for (int m = 0; m < 8; m++)
+    QAS_CHECK(c, hipMemcpyAsync(state + (size_t)m * c->geo.nd, c->U[c->cur][m], bytes, hipMemcpyDeviceToDevice,
+                                c->stream));
+  return QAS_OK;
+}
+
+extern "C" int qas_bound(qas_ctx* c)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_bound: null context");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_bound: no state uploaded");
+  QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+  c->bounded = true;
+  return QAS_OK;
+}
+
+// rk_fac of mhd2d_solve.cpp:7
+static void rk_coef(int r_k, int rk, double* a, double* b)
+{
+  if (rk == 0) { *a = 0.0; *b = 1.0; }
+  else if (rk == 1) { *a = 0.5 + (r_k - 2) * 0.25; *b = 0.5 - (r_k - 2) * 0.25; }
+  else { *a = 1. / 3.; *b = 2. / 3.; }
+}
+
+// ideal(dt) with dt already in device memory
+static int do_step(qas_ctx* c)
+{
+  const int R = c->s.r_k;
+  const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
+  int seq_in[3], seq_out[3];
+  if (R == 1) { seq_in[0] = A; seq_out[0] = B; }
+  else if (R == 2) { seq_in[0] = A; seq_out[0] = B; seq_in[1] = B; seq_out[1] = A; }
+  else { seq_in[0] = A; seq_out[0] = B; seq_in[1] = B; seq_out[1] = C; seq_in[2] = C; seq_out[2] = A; }
+  for (int rk = 0; rk < R; rk++) {
+    StagePtrs p;
+    std::memset(&p, 0, sizeof(p));
+    for (int m = 0; m < 8; m++) {
+      p.in[m] = c->U[seq_in[rk]][m];
+      p.out[m] = c->U[seq_out[rk]][m];
+      p.u0[m] = c->U[A][m];
+    }
+    p.vx = c->vx; p.vy = c->vy; p.vz = c->vz; p.pr = c->pr;
+    p.cx = c->cx; p.cy = c->cy; p.cz = c->cz;
+    for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
+    for (int k = 0; k < 12; k++) p.emf[k] = c->emf[k];
+    p.ex = c->ex; p.ey = c->ey; p.ez = c->ez;
+    p.dt = c->d_dt;
+    rk_coef(R, rk, &p.a, &p.b);
+    hipError_t e;
+    if (c->g.ndim == 1) e = launch_stage1d(c, p);
+    else if (c->g.ndim == 2) e = launch_stage2d(c, p);
+    else e = launch_stage3d(c, p);
+    if (e != hipSuccess) return hipfail(c, e, "stage launch");
+    e = launch_bound(c, c->U[seq_out[rk]], 8);
+    if (e != hipSuccess) return hipfail(c, e, "bound launch");
+  }
+  c->cur = seq_out[R - 1];
+  c->bounded = true;
+  return QAS_OK;
+}
+
+extern "C" int qas_step(qas_ctx* c, double dt)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_step: null context");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_step: no state uploaded");
+  if (!(dt > 0.0)) return fail(c, QAS_E_ARG, "qas_step: dt must be positive");
+  QAS_CHECK(c, launch_set_dt(c, dt));
+  return do_step(c);
+}
+
+extern "C" int qas_cfl_dt(qas_ctx* c, double* dt_out)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_cfl_dt: null context");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_cfl_dt: no state uploaded");
+  QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
+  QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
+  QAS_CHECK(c, hipMemcpyAsync(c->h_pinned, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  if (dt_out) *dt_out = c->h_pinned[0];
+  return QAS_OK;
+}
+
+extern "C" int qas_run(qas_ctx* c, int nsteps, int flg, double* dt_inout, double* dt_hist)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_run: null context");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_run: no state uploaded");
+  if (nsteps < 0 || !dt_inout) return fail(c, QAS_E_ARG, "qas_run: bad arguments");
+  if (dt_hist && nsteps > c->hist_cap) {
+    if (c->d_hist) QAS_CHECK(c, hipFree(c->d_hist));
+    c->d_hist = nullptr;
+    QAS_CHECK(c, hipMalloc(&c->d_hist, sizeof(double) * (size_t)nsteps));
+    c->hist_cap = nsteps;
+  }
+  if (*dt_inout > 0.0) {
+    QAS_CHECK(c, launch_set_dt(c, *dt_inout));
+  } else {
+    QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
+    QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
+  }
+  for (int n = 0; n < nsteps; n++) {
+    c->nstep++;
+    if (dt_hist) QAS_CHECK(c, launch_record_dt(c, c->d_hist + n));
+    if (!c->bounded) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+    int r = do_step(c);
+    if (r) return r;
+    if (flg && (c->nstep % 2 == 0)) {
+      QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
+      QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
+    }
+  }
+  QAS_CHECK(c, hipMemcpyAsync(c->h_pinned, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  *dt_inout = c->h_pinned[0];
+  if (dt_hist && nsteps > 0)
+    QAS_CHECK(c, hipMemcpy(dt_hist, c->d_hist, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost));
+  return QAS_OK;
+}
+
+extern "C" int qas_reset_counter(qas_ctx* c)
+{
+  if (!c) return QAS_E_ARG;
+  c->nstep = 0;
+  return QAS_OK;
+}
+
+extern "C" int qas_sync(qas_ctx* c)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_sync: null context");
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  return QAS_OK;
+}
+
+extern "C" long long qas_device_bytes(const qas_ctx* c) { return c ? (long long)c->slab_bytes : 0; }

@@ -1,0 +1,120 @@
+// qas_internal.hpp -- device context and kernel-launch interface shared by
+// the HIP translation units of libqasmhd_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+
+#include "../../include/qasmhd.h"
+
+namespace qas {
+
+constexpr int NG = 4;       // ghost layers (mhd2d_class.hpp:10)
+constexpr int SEG = 60;     // faces owned by one wavefront segment (64 lanes - 4 halo)
+constexpr int WAVE = 64;
+
+// Boundary flags as kernel arguments (dnxs/dnys/dnzs, stxs/stys/stzs).
+struct BcFlags {
+  signed char dn[3][8];
+  signed char st[3][8];
+};
+
+// Geometry as kernel arguments.
+struct Geo {
+  int ndim, nx, ny, nz;
+  long long sy, sz, nd;  // strides of y and z, cells per array
+  double dx, dy, dz;
+};
+
+// Pointers of one stage, passed by value.
+struct StagePtrs {
+  const double* in[8];   // stage input state (ghosts filled)
+  double* out[8];        // stage output state
+  const double* u0[8];   // state at the start of the step (RK base)
+  double *vx, *vy, *vz, *pr, *cx, *cy, *cz;  // primitive scratch
+  double* acc[6];        // partial flux divergence of the sweeps so far
+  double* emf[12];       // face parts of the B fluxes: [sweep][comp][upwind/central]
+  double *ex, *ey, *ez;  // edge EMFs
+  const double* dt;      // device dt
+  double a, b;           // RK stage coefficients (rk_fac)
+};
+
+// Ghost-index resolution: bound() writes ghost cells as (factor x) a kept
+// cell of the same array.  Applying x then y then z (bc2d/bc3d order) to a
+// cell gives value = kz*(ky*(kx*f[src])) with the sources below.
+struct Src {
+  int i;
+  double k;
+  bool mul;
+};
+
+__device__ __forceinline__ Src resolve(int i, int n, int st, int dn)
+{
+  Src r{i, 1.0, false};
+  if (dn == 0) {  // periodic: plain copy (common_func.c:67-74)
+    if (i < NG) r.i = n - 2 * NG + i;
+    else if (i >= n - NG) r.i = i - (n - 2 * NG);
+  } else if (dn == 1 || dn == -1) {  // mirror x dn (common_func.c:75-86)
+    if (i < NG) {
+      r.i = (2 * NG - 1 + st) - i;
+      r.k = (double)dn;
+      r.mul = true;
+    } else if (i >= n - NG + st) {
+      r.i = (n - 2 * NG + st) + (n - 1 - i);
+      r.k = (double)dn;
+      r.mul = true;
+    }
+  } else if (dn == 2 || dn == -2) {  // open / zero-fix (common_func.c:87-98)
+    if (i < NG) {
+      r.i = NG;
+      r.k = 0.25 * (double)(2 + dn);
+      r.mul = true;
+    } else if (i >= n - NG + st) {
+      r.i = n - 1 - NG + st;
+      r.k = 0.25 * (double)(2 + dn);
+      r.mul = true;
+    }
+  }
+  return r;
+}
+
+__device__ __forceinline__ double apply(const Src& s, double v) { return s.mul ? s.k * v : v; }
+
+}  // namespace qas
+
+struct qas_ctx {
+  qas_grid g;
+  qas_scheme s;
+  qas::Geo geo;
+  qas::BcFlags bc;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* slab = nullptr;
+  size_t slab_bytes = 0;
+  double* U[3][8] = {};   // U[cur] holds the current state
+  int cur = 0;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr, *pr = nullptr, *cx = nullptr, *cy = nullptr,
+         *cz = nullptr;
+  double* acc[6] = {};
+  double* emf[12] = {};
+  double *ex = nullptr, *ey = nullptr, *ez = nullptr;
+  double* d_dt = nullptr;             // current dt
+  double* d_hist = nullptr;           // dt history of qas_run
+  int hist_cap = 0;
+  unsigned long long* d_vmax = nullptr;
+  double* h_pinned = nullptr;         // small pinned staging
+  bool uploaded = false;
+  bool bounded = false;
+  long long nstep = 0;                // running step counter (exec_'s n)
+  std::string err;
+};
+
+// ---- launchers (defined in the per-dimension translation units) ----
+namespace qas {
+hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar);
+hipError_t launch_cfl(qas_ctx* c, const double* const U[8]);
+hipError_t launch_cfl_finalize(qas_ctx* c, double* hist_slot);
+hipError_t launch_set_dt(qas_ctx* c, double dt);
+hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
+hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p);
+hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
+}  // namespace qas

@@ -1,0 +1,342 @@
+// qas_kernels2d.hip -- one SSP-RK stage of the 2D ideal-MHD step on gfx950.
+//
+// Replaces the stage body of MHD2D::ideal (2D/SERIAL/OTvortex/
+// mhd2d_solve.cpp:44-386).  A stage is five launches:
+//   prim2d   cell-centred B (fcen, ghosts by the bound() index map) and
+//            primitives at every cell                       (:51-89)
+//   sweep<X> characteristic WCNS + Riemann flux per x face, for every row;
+//            writes the x flux divergence (times -dt/dx) of the interior
+//            and the upwind/central parts of the By flux    (:127-199)
+//   sweep<Y> same along y; adds the y divergence and applies the RK stage
+//            sum to ro, mx, my, mz, bz, en                  (:234-306, 341-358)
+//   emf2d    CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
+//   ct2d     staggered bx, by update                        (:359-380)
+// followed by the bound() kernel (:384-385).
+// A sweep assigns one 64-lane wavefront to 64 consecutive cells of a line;
+// lane l reconstructs cell f0-2+l, neighbour states and fluxes move between
+// lanes by cross-lane shuffles, and the wavefront owns the 60 faces/cells
+// [f0, f0+60).  Everything runs in FP64 with the reference's operation
+// order (the library is built with -ffp-contract=off).
+#include "qas_device.hpp"
+#include "qas_internal.hpp"
+
+namespace qas {
+
+// ------------------------------------------------------------------ prim
+template <int ODR>
+__global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, double gam)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx);
+  const int j = (int)(t / g.nx);
+  const long long nx = g.nx;
+  const double* bx = p.in[4];
+  const double* by = p.in[5];
+  // cx: fcen along x on i in [1, nx-3], then bound with (st 0, dnxs[4], dnys[4])
+  double cx, cy;
+  {
+    const Src sx = resolve(i, g.nx, 0, bc.dn[0][4]);
+    const Src sy = resolve(j, g.ny, 0, bc.dn[1][4]);
+    const long long q = nx * sy.i + sx.i;
+    double v = fcen<ODR>(bx[q - 1], bx[q], bx[q + 1], bx[q + 2]);
+    cx = apply(sy, apply(sx, v));
+  }
+  {
+    const Src sx = resolve(i, g.nx, 0, bc.dn[0][5]);
+    const Src sy = resolve(j, g.ny, 0, bc.dn[1][5]);
+    const long long q = nx * sy.i + sx.i;
+    double v = fcen<ODR>(by[q - nx], by[q], by[q + nx], by[q + 2 * nx]);
+    cy = apply(sy, apply(sx, v));
+  }
+  const double ro = p.in[0][t];
+  const double iro = 1.0 / ro;
+  const double vx = p.in[1][t] * iro;
+  const double vy = p.in[2][t] * iro;
+  const double vz = p.in[3][t] * iro;
+  const double cz = p.in[6][t];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double b2 = cx * cx + cy * cy + cz * cz;
+  p.pr[t] = (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2));
+  p.vx[t] = vx;
+  p.vy[t] = vy;
+  p.vz[t] = vz;
+  p.cx[t] = cx;
+  p.cy[t] = cy;
+}
+
+// shock sensor dv of mhd2d_solve.cpp:105-125 at (i, j) after its bound():
+// DIR 0 -> dvx (flags -dnxs[1], dnys[1]); DIR 1 -> dvy (dnxs[2], -dnys[2])
+template <int DIR>
+__device__ __forceinline__ double dv_at(const double* v, int i, int j, const Geo& g, const BcFlags& bc)
+{
+  const int dnx = DIR == 0 ? -bc.dn[0][1] : bc.dn[0][2];
+  const int dny = DIR == 0 ? bc.dn[1][1] : -bc.dn[1][2];
+  const Src sx = resolve(i, g.nx, 0, dnx);
+  const Src sy = resolve(j, g.ny, 0, dny);
+  const long long q = (long long)g.nx * sy.i + sx.i;
+  const long long o = DIR == 0 ? 1 : g.nx;
+  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
+  return apply(sy, apply(sx, r));
+}
+
+__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, WAVE); }
+__device__ __forceinline__ double shfl_dn(double v, int d) { return __shfl_down(v, d, WAVE); }
+
+// ------------------------------------------------------------------ sweeps
+// DIR 0: x faces for every row j; DIR 1: y faces for every column i.
+template <int DIR, int RMN, int ODR>
+__global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+{
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int nline = DIR == 0 ? g.ny : g.nx;
+  if (wave >= nseg * nline) return;  // whole wavefront exits together
+  const int line = wave / nseg;
+  const int seg = wave - line * nseg;
+  const int n = DIR == 0 ? g.nx : g.ny;       // points along the sweep
+  const long long o = DIR == 0 ? 1 : g.sy;    // stride along the sweep
+  const long long base = DIR == 0 ? g.sy * line : (long long)line;
+  const int f0 = 3 + seg * SEG;
+  const int q = f0 - 2 + lane;                // this lane's cell
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const long long sc = base + o * qc;
+
+  // primitive stencil, cyclic order (vn, vt, vu), (bt, bu)
+  const double* vn = DIR == 0 ? p.vx : p.vy;
+  const double* vt = DIR == 0 ? p.vy : p.vz;
+  const double* vu = DIR == 0 ? p.vz : p.vx;
+  const double* bt = DIR == 0 ? p.cy : p.in[6];
+  const double* bu = DIR == 0 ? p.in[6] : p.cx;
+  const double* cn = DIR == 0 ? p.cx : p.cy;
+  const double* bf = DIR == 0 ? p.in[4] : p.in[5];  // staggered normal B
+  double s[7][5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long long a = sc + (m - 2) * o;
+    s[0][m] = p.in[0][a];
+    s[1][m] = vn[a];
+    s[2][m] = vt[a];
+    s[3][m] = vu[a];
+    s[4][m] = bt[a];
+    s[5][m] = bu[a];
+    s[6][m] = p.pr[a];
+  }
+  double vl[7], vr[7];
+  lrstate<ODR>(s, cn[sc], gam, vl, vr);
+
+  // B-flux face states (mhd_lr_fb): b_t v_n - b_n v_t with the tangential
+  // component that pairs with this sweep (x: By, y: Bx)
+  double ql, qr;
+  {
+    const double* bb = DIR == 0 ? p.cy : p.cx;
+    const double* vv = DIR == 0 ? p.vy : p.vx;
+    double pp[5], qq[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      pp[m] = bb[a] * vn[a];
+      qq[m] = vv[a];
+    }
+    lr_fb<ODR>(pp, qq, bf[sc], bf[sc + o], ql, qr);
+  }
+
+  // flux at face q (between q-1 and q): left state from lane-1
+  double L[7], R[7], F[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    L[k] = shfl_up1(vl[k]);
+    R[k] = vr[k];
+  }
+  const double qlf = shfl_up1(ql);
+  const long long sf = base + o * (q < 3 ? 3 : (q > n - 3 ? n - 3 : q));  // face index (clamped)
+  const double bnf = 0.5 * (bf[sf] + bf[sf]);
+  double dv0 = 0.0, dv1 = 0.0;
+  if constexpr (RMN >= 2) {
+    const int qq = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
+    dv0 = vn[sf] - vn[sf - o];
+    if (DIR == 0) {
+      dv1 = tmin(dv_at<1>(p.vy, qq - 1, line, g, bc), dv_at<1>(p.vy, qq, line, g, bc));
+    } else {
+      dv1 = tmin(dv_at<0>(p.vx, line, qq - 1, g, bc), dv_at<0>(p.vx, line, qq, g, bc));
+    }
+  }
+  riemann<RMN>(L, R, bnf, gam, dv0, dv1, F);
+  const double fc = 0.5 * (qlf + qr);
+  // split of the B flux: x sweep -> By flux (F[4]); y sweep -> Bx flux (F[5])
+  const double fbu = (DIR == 0 ? F[4] : F[5]) - fc;
+
+  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
+  if (own) {
+    const long long fi = base + o * q;
+    p.emf[DIR * 4 + 0][fi] = fbu;
+    p.emf[DIR * 4 + 1][fi] = fc;
+  }
+
+  // flux divergence of ro, mx, my, mz, bz, en at cell q: faces q-1 .. q+2
+  // x: (F0, F1, F2, F3, F5, F6); y: (F0, F3, F1, F2, F4, F6)
+  double fv[6];
+  if (DIR == 0) {
+    fv[0] = F[0]; fv[1] = F[1]; fv[2] = F[2]; fv[3] = F[3]; fv[4] = F[5]; fv[5] = F[6];
+  } else {
+    fv[0] = F[0]; fv[1] = F[3]; fv[2] = F[1]; fv[3] = F[2]; fv[4] = F[4]; fv[5] = F[6];
+  }
+  double du[6];
+#pragma unroll
+  for (int v = 0; v < 6; v++) {
+    const double fm1 = shfl_up1(fv[v]);
+    const double f1 = shfl_dn(fv[v], 1);
+    const double f2 = shfl_dn(fv[v], 2);
+    du[v] = dfc<ODR>(fm1, fv[v], f1, f2);
+  }
+  const int tl = line;
+  const int ntl = DIR == 0 ? g.ny : g.nx;
+  const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (tl >= NG) && (tl <= ntl - NG - 1);
+  if (!upd) return;
+  const long long c = base + o * q;
+  const double dtv = *p.dt;
+  if (DIR == 0) {
+    const double dtdx = dtv / g.dx;
+#pragma unroll
+    for (int v = 0; v < 6; v++) p.acc[v][c] = -dtdx * du[v];
+  } else {
+    const double dtdy = dtv / g.dy;
+    const int vid[6] = {0, 1, 2, 3, 6, 7};
+#pragma unroll
+    for (int v = 0; v < 6; v++) {
+      const int m = vid[v];
+      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (p.acc[v][c] - dtdy * du[v]), p.a, p.b);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ EMF
+template <int ODR>
+__global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
+{
+  const int ni = g.nx - 5, nj = g.ny - 5;   // corners [3, n-3]
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj) return;
+  const int i = 3 + (int)(t % ni);
+  const int j = 3 + (int)(t / ni);
+  const long long nx = g.nx;
+  const long long ss = nx * j + i;
+  double ct = 0.5;
+  if (ctw) {
+    const double* ro = p.in[0];
+    ct = cuct_weight(ro[ss - 1 - nx], ro[ss - nx], ro[ss - 1], ro[ss], p.vx[ss - 1 - nx], p.vx[ss - nx],
+                     p.vx[ss - 1], p.vx[ss], p.vy[ss - 1 - nx], p.vy[ss - nx], p.vy[ss - 1], p.vy[ss],
+                     p.in[4][ss - nx], p.in[4][ss], p.in[5][ss - 1], p.in[5][ss]);
+  }
+  double ez = 0.0;
+  {  // from the x sweep: By flux interpolated along y to (i-1/2, j-1/2)
+    const double* fu = p.emf[0];
+    const double* fcn = p.emf[1];
+    double a[5], b[5], ul0, ur0, ul1, ur1, dum;
+#pragma unroll
+    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 3) * nx]; b[m] = fcn[ss + (m - 3) * nx]; }
+    lin5<ODR>(a, ul0, dum);
+    lin5<ODR>(b, ul1, dum);
+#pragma unroll
+    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 2) * nx]; b[m] = fcn[ss + (m - 2) * nx]; }
+    lin5<ODR>(a, dum, ur0);
+    lin5<ODR>(b, dum, ur1);
+    ez += -0.5 * ((ul0 + ur0) + (1.0 - ct) * (ul1 + ur1));
+  }
+  {  // from the y sweep: Bx flux interpolated along x
+    const double* fu = p.emf[4];
+    const double* fcn = p.emf[5];
+    double a[5], b[5], ul0, ur0, ul1, ur1, dum;
+#pragma unroll
+    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 3)]; b[m] = fcn[ss + (m - 3)]; }
+    lin5<ODR>(a, ul0, dum);
+    lin5<ODR>(b, ul1, dum);
+#pragma unroll
+    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 2)]; b[m] = fcn[ss + (m - 2)]; }
+    lin5<ODR>(a, dum, ur0);
+    lin5<ODR>(b, dum, ur1);
+    ez += +0.5 * ((ul0 + ur0) + ct * (ul1 + ur1));
+  }
+  p.ez[ss] = ez;
+}
+
+// ------------------------------------------------------------------ CT
+template <int ODR>
+__global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
+{
+  // faces i, j in [4, n-4]: bx needs j <= ny-5, by needs i <= nx-5
+  const int ni = g.nx - 7, nj = g.ny - 7;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj) return;
+  const int i = 4 + (int)(t % ni);
+  const int j = 4 + (int)(t / ni);
+  const long long nx = g.nx;
+  const long long ss = nx * j + i;
+  const double dtv = *p.dt;
+  const double* ez = p.ez;
+  if (j <= g.ny - 5) {
+    const double dtdy = dtv / g.dy;
+    const double du = dfc<ODR>(ez[ss - nx], ez[ss], ez[ss + nx], ez[ss + 2 * nx]);
+    p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du, p.a, p.b);
+  }
+  if (i <= g.nx - 5) {
+    const double mdtdx = -(dtv / g.dx);  // mhd_updt2d_ctb is called with -dtdx
+    const double du = dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
+    p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -mdtdx * du, p.a, p.b);
+  }
+}
+
+// ------------------------------------------------------------------ launch
+typedef void (*sweep_fn)(StagePtrs, Geo, BcFlags, double, int);
+
+template <int DIR>
+static sweep_fn pick_sweep(int rmn, int odr)
+{
+#define QAS_S(R, O) \
+  if (rmn == R && odr == O) return k_sweep2d<DIR, R, O>;
+  QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
+  QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
+  QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
+  QAS_S(3, 1) QAS_S(3, 2) QAS_S(3, 3) QAS_S(3, 4)
+#undef QAS_S
+  return nullptr;
+}
+
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
+{
+  const Geo& g = c->geo;
+  const int odr = c->s.odr;
+  const double gam = c->s.gam;
+  hipStream_t st = c->stream;
+  // 1) cell-centred B and primitives
+  switch (odr) {
+    case 1: case 2: hipLaunchKernelGGL(k_prim2d<2>, nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
+    default: hipLaunchKernelGGL(k_prim2d<4>, nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
+  }
+  // 2) x sweep, 3) y sweep
+  const int nsx = (g.nx - 5 + SEG - 1) / SEG;
+  const int nsy = (g.ny - 5 + SEG - 1) / SEG;
+  sweep_fn fx = pick_sweep<0>(c->s.rmn, odr);
+  sweep_fn fy = pick_sweep<1>(c->s.rmn, odr);
+  if (!fx || !fy) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx);
+  hipLaunchKernelGGL(fy, nblk((long long)nsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsy);
+  // 4) corner EMF, 5) CT
+  const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
+  const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
+  switch (odr) {
+    case 1: hipLaunchKernelGGL(k_emf2d<1>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            hipLaunchKernelGGL(k_ct2d<1>, nblk(nc, 256), 256, 0, st, p, g); break;
+    case 2: hipLaunchKernelGGL(k_emf2d<2>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            hipLaunchKernelGGL(k_ct2d<2>, nblk(nc, 256), 256, 0, st, p, g); break;
+    case 3: hipLaunchKernelGGL(k_emf2d<3>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            hipLaunchKernelGGL(k_ct2d<3>, nblk(nc, 256), 256, 0, st, p, g); break;
+    default: hipLaunchKernelGGL(k_emf2d<4>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+             hipLaunchKernelGGL(k_ct2d<4>, nblk(nc, 256), 256, 0, st, p, g); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qas

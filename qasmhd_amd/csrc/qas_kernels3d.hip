@@ -1,0 +1,374 @@
+// qas_kernels3d.hip -- one SSP-RK stage of the 3D ideal-MHD step on gfx950.
+//
+// Replaces the stage body of MHD3D::ideal (3D/MPI/OTvortex/mhd3d_solve.cpp:
+// 42-715) on one device-resident domain:
+//   prim3d    cell-centred B (fcen along x, y, z; ghosts by the bound()
+//             index map) and primitives at every cell        (:49-98)
+//   sweep<d>  for d = x, y, z: characteristic WCNS + Riemann flux on every
+//             line; partial flux divergence of ro, mx, my, mz, en
+//             accumulated in the reference's order
+//             ((-dtdx Dx) - dtdy Dy) - dtdz Dz, the z sweep applying the RK
+//             stage sum; upwind/central parts of both tangential B fluxes
+//             stored per face                                (:166-258, 330-422, 494-586, 658-676)
+//   emf3d     Ex, Ey, Ez at cell edges from the face parts (CUCT)
+//                                                            (:100-136, 259-328, 423-492, 587-656)
+//   ct3d      staggered bx, by, bz update                    (:677-709)
+// followed by the bound() kernel (:713-714).
+#include "qas_device.hpp"
+#include "qas_internal.hpp"
+
+namespace qas {
+
+template <int ODR>
+__global__ __launch_bounds__(256) void k_prim3d(StagePtrs p, Geo g, BcFlags bc, double gam)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx);
+  const int j = (int)((t / g.nx) % g.ny);
+  const int k = (int)(t / g.sz);
+  double c3[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double* b = p.in[4 + a];
+    const Src sx = resolve(i, g.nx, 0, bc.dn[0][4 + a]);
+    const Src sy = resolve(j, g.ny, 0, bc.dn[1][4 + a]);
+    const Src sz = resolve(k, g.nz, 0, bc.dn[2][4 + a]);
+    const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
+    const long long o = a == 0 ? 1 : (a == 1 ? g.sy : g.sz);
+    const double v = fcen<ODR>(b[q - o], b[q], b[q + o], b[q + 2 * o]);
+    c3[a] = apply(sz, apply(sy, apply(sx, v)));
+  }
+  const double ro = p.in[0][t];
+  const double iro = 1.0 / ro;
+  const double vx = p.in[1][t] * iro;
+  const double vy = p.in[2][t] * iro;
+  const double vz = p.in[3][t] * iro;
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double b2 = c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2];
+  p.pr[t] = (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2));
+  p.vx[t] = vx;
+  p.vy[t] = vy;
+  p.vz[t] = vz;
+  p.cx[t] = c3[0];
+  p.cy[t] = c3[1];
+  p.cz[t] = c3[2];
+}
+
+// dv of axis A (0 x, 1 y, 2 z) at (i, j, k) after its bound()
+// (mhd3d_solve.cpp:138-164): flags (-dnxs[1], dnys[1], dnzs[1]) for dvx,
+// (dnxs[2], -dnys[2], dnzs[2]) for dvy, (dnxs[3], dnys[3], -dnzs[3]) for dvz.
+template <int A>
+__device__ __forceinline__ double dv3(const double* v, int i, int j, int k, const Geo& g, const BcFlags& bc)
+{
+  const int dnx = A == 0 ? -bc.dn[0][1] : bc.dn[0][1 + A];
+  const int dny = A == 1 ? -bc.dn[1][2] : bc.dn[1][1 + A];
+  const int dnz = A == 2 ? -bc.dn[2][3] : bc.dn[2][1 + A];
+  const Src sx = resolve(i, g.nx, 0, dnx);
+  const Src sy = resolve(j, g.ny, 0, dny);
+  const Src sz = resolve(k, g.nz, 0, dnz);
+  const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
+  const long long o = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
+  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
+  return apply(sz, apply(sy, apply(sx, r)));
+}
+
+template <int A>
+__device__ __forceinline__ double dv3_face(const StagePtrs& p, int i, int j, int k, const Geo& g,
+                                           const BcFlags& bc)
+{
+  const double* v = A == 0 ? p.vx : (A == 1 ? p.vy : p.vz);
+  return dv3<A>(v, i, j, k, g, bc);
+}
+
+// ------------------------------------------------------------------ sweeps
+template <int DIR, int RMN, int ODR>
+__global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+{
+  constexpr int TA = (DIR + 1) % 3;  // first transverse axis (t)
+  constexpr int TB = (DIR + 2) % 3;  // second transverse axis (u)
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  const long long strd[3] = {1, g.sy, g.sz};
+  // lines: x -> (j, k); y -> (i, k); z -> (i, j); the faster line index is
+  // the lower axis for coalescing
+  constexpr int LA = DIR == 0 ? 1 : 0;  // faster transverse axis
+  constexpr int LB = DIR == 2 ? 1 : 2;  // slower transverse axis
+  const int nla = dims[LA], nlb = dims[LB];
+  const long long nline = (long long)nla * nlb;
+  if ((long long)wave >= nline * nseg) return;
+  const long long line = wave / nseg;
+  const int seg = (int)(wave - line * nseg);
+  const int la = (int)(line % nla), lb = (int)(line / nla);
+  const int n = dims[DIR];
+  const long long o = strd[DIR];
+  const long long base = strd[LA] * la + strd[LB] * lb;
+  const int f0 = 3 + seg * SEG;
+  const int q = f0 - 2 + lane;
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const long long sc = base + o * qc;
+
+  const double* vax[3] = {p.vx, p.vy, p.vz};
+  const double* cax[3] = {p.cx, p.cy, p.cz};
+  const double* vn = vax[DIR];
+  const double* vt = vax[TA];
+  const double* vu = vax[TB];
+  const double* bt = cax[TA];
+  const double* bu = cax[TB];
+  const double* bf = p.in[4 + DIR];
+  double s[7][5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long long a = sc + (m - 2) * o;
+    s[0][m] = p.in[0][a];
+    s[1][m] = vn[a];
+    s[2][m] = vt[a];
+    s[3][m] = vu[a];
+    s[4][m] = bt[a];
+    s[5][m] = bu[a];
+    s[6][m] = p.pr[a];
+  }
+  double vl[7], vr[7];
+  lrstate<ODR>(s, cax[DIR][sc], gam, vl, vr);
+
+  // B-flux face states for both tangential components (mhd_lr_fb)
+  double qlt, qrt, qlu, qru;
+  {
+    double pp[5], qq[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      pp[m] = bt[a] * vn[a];
+      qq[m] = vt[a];
+    }
+    lr_fb<ODR>(pp, qq, bf[sc], bf[sc + o], qlt, qrt);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      pp[m] = bu[a] * vn[a];
+      qq[m] = vu[a];
+    }
+    lr_fb<ODR>(pp, qq, bf[sc], bf[sc + o], qlu, qru);
+  }
+
+  double L[7], F[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
+  const double qlt_f = __shfl_up(qlt, 1, WAVE);
+  const double qlu_f = __shfl_up(qlu, 1, WAVE);
+  const int qf = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
+  const long long sf = base + o * qf;
+  const double bnf = 0.5 * (bf[sf] + bf[sf]);
+  double dv0 = 0.0, dv1 = 0.0;
+  if constexpr (RMN >= 2) {
+    dv0 = vn[sf] - vn[sf - o];
+    int c0[3], c1[3];
+    c0[LA] = la; c0[LB] = lb; c0[DIR] = qf - 1;
+    c1[LA] = la; c1[LB] = lb; c1[DIR] = qf;
+    const double a0 = dv3_face<TA>(p, c0[0], c0[1], c0[2], g, bc);
+    const double a1 = dv3_face<TA>(p, c1[0], c1[1], c1[2], g, bc);
+    const double b0 = dv3_face<TB>(p, c0[0], c0[1], c0[2], g, bc);
+    const double b1 = dv3_face<TB>(p, c1[0], c1[1], c1[2], g, bc);
+    dv1 = tmin(tmin(a0, a1), tmin(b0, b1));
+  }
+  riemann<RMN>(L, vr, bnf, gam, dv0, dv1, F);
+  const double fct = 0.5 * (qlt_f + qrt);
+  const double fcu = 0.5 * (qlu_f + qru);
+  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
+  if (own) {
+    const long long fi = base + o * q;
+    p.emf[DIR * 4 + 0][fi] = F[4] - fct;  // B_t flux, upwind part
+    p.emf[DIR * 4 + 1][fi] = fct;         // B_t flux, central part
+    p.emf[DIR * 4 + 2][fi] = F[5] - fcu;  // B_u flux, upwind part
+    p.emf[DIR * 4 + 3][fi] = fcu;         // B_u flux, central part
+  }
+  // divergence of ro, mx, my, mz, en: (ro, mn, mt, mu) permuted back
+  double fv[5];
+  fv[0] = F[0];
+  fv[1 + DIR] = F[1];
+  fv[1 + TA] = F[2];
+  fv[1 + TB] = F[3];
+  fv[4] = F[6];
+  double du[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    const double fm1 = __shfl_up(fv[v], 1, WAVE);
+    const double f1 = __shfl_down(fv[v], 1, WAVE);
+    const double f2 = __shfl_down(fv[v], 2, WAVE);
+    du[v] = dfc<ODR>(fm1, fv[v], f1, f2);
+  }
+  const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (la >= NG) && (la <= nla - NG - 1) && (lb >= NG) &&
+                   (lb <= nlb - NG - 1);
+  if (!upd) return;
+  const long long c = base + o * q;
+  const double dtv = *p.dt;
+  const int vid[5] = {0, 1, 2, 3, 7};
+  if (DIR == 0) {
+    const double dtdx = dtv / g.dx;
+#pragma unroll
+    for (int v = 0; v < 5; v++) p.acc[v][c] = -dtdx * du[v];
+  } else if (DIR == 1) {
+    const double dtdy = dtv / g.dy;
+#pragma unroll
+    for (int v = 0; v < 5; v++) p.acc[v][c] = p.acc[v][c] - dtdy * du[v];
+  } else {
+    const double dtdz = dtv / g.dz;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const int m = vid[v];
+      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], p.acc[v][c] - dtdz * du[v], p.a, p.b);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ EMF
+// One contribution: interpolate the face parts (upwind fu, central fc) of a
+// sweep along axis o to the edge and combine with weight w:
+//   sgn * ((uL + uR) + w (cL + cR))
+template <int ODR>
+__device__ __forceinline__ double emf_part(const double* fu, const double* fc, long long ss, long long o, double w,
+                                           double sgn)
+{
+  double a[5], b[5], ul0, ur0, ul1, ur1, dum;
+#pragma unroll
+  for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 3) * o]; b[m] = fc[ss + (m - 3) * o]; }
+  lin5<ODR>(a, ul0, dum);
+  lin5<ODR>(b, ul1, dum);
+#pragma unroll
+  for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 2) * o]; b[m] = fc[ss + (m - 2) * o]; }
+  lin5<ODR>(a, dum, ur0);
+  lin5<ODR>(b, dum, ur1);
+  return sgn * ((ul0 + ur0) + w * (ul1 + ur1));
+}
+
+template <int ODR>
+__global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
+{
+  // edges on [3, n-3]^3
+  const int ni = g.nx - 5, nj = g.ny - 5, nk = g.nz - 5;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = 3 + (int)(t % ni);
+  const int j = 3 + (int)((t / ni) % nj);
+  const int k = 3 + (int)(t / ((long long)ni * nj));
+  const long long sy = g.sy, sz = g.sz;
+  const long long ss = sz * k + sy * j + i;
+  const double* ro = p.in[0];
+  const double *bx = p.in[4], *by = p.in[5], *bz = p.in[6];
+  const double *vx = p.vx, *vy = p.vy, *vz = p.vz;
+  double ctz = 0.5, ctx = 0.5, cty = 0.5;
+  if (ctw) {
+    // x-y plane for Ez: strides (1, ny)
+    ctz = cuct_weight(ro[ss - 1 - sy], ro[ss - sy], ro[ss - 1], ro[ss], vx[ss - 1 - sy], vx[ss - sy], vx[ss - 1],
+                      vx[ss], vy[ss - 1 - sy], vy[ss - sy], vy[ss - 1], vy[ss], bx[ss - sy], bx[ss], by[ss - 1],
+                      by[ss]);
+    // y-z plane for Ex: strides (sy, sz)
+    ctx = cuct_weight(ro[ss - sy - sz], ro[ss - sz], ro[ss - sy], ro[ss], vy[ss - sy - sz], vy[ss - sz], vy[ss - sy],
+                      vy[ss], vz[ss - sy - sz], vz[ss - sz], vz[ss - sy], vz[ss], by[ss - sz], by[ss], bz[ss - sy],
+                      bz[ss]);
+    // z-x plane for Ey: strides (sz, 1)
+    cty = cuct_weight(ro[ss - sz - 1], ro[ss - 1], ro[ss - sz], ro[ss], vz[ss - sz - 1], vz[ss - 1], vz[ss - sz],
+                      vz[ss], vx[ss - sz - 1], vx[ss - 1], vx[ss - sz], vx[ss], bz[ss - 1], bz[ss], bx[ss - sz],
+                      bx[ss]);
+  }
+  // Ez: x sweep B_y (t) along y, then y sweep B_x (u) along x
+  double ez = 0.0;
+  ez += emf_part<ODR>(p.emf[0], p.emf[1], ss, sy, 1.0 - ctz, -0.5);
+  ez += emf_part<ODR>(p.emf[6], p.emf[7], ss, 1, ctz, +0.5);
+  // Ey: x sweep B_z (u) along z, then z sweep B_x (t) along x
+  double ey = 0.0;
+  ey += emf_part<ODR>(p.emf[2], p.emf[3], ss, sz, cty, +0.5);
+  ey += emf_part<ODR>(p.emf[8], p.emf[9], ss, 1, 1.0 - cty, -0.5);
+  // Ex: y sweep B_z (t) along z, then z sweep B_y (u) along y
+  double ex = 0.0;
+  ex += emf_part<ODR>(p.emf[4], p.emf[5], ss, sz, 1.0 - ctx, -0.5);
+  ex += emf_part<ODR>(p.emf[10], p.emf[11], ss, sy, ctx, +0.5);
+  p.ex[ss] = ex;
+  p.ey[ss] = ey;
+  p.ez[ss] = ez;
+}
+
+// ------------------------------------------------------------------ CT
+template <int ODR>
+__global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
+{
+  const int ni = g.nx - 7, nj = g.ny - 7, nk = g.nz - 7;  // [4, n-4]
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int i = 4 + (int)(t % ni);
+  const int j = 4 + (int)((t / ni) % nj);
+  const int k = 4 + (int)(t / ((long long)ni * nj));
+  const long long sy = g.sy, sz = g.sz;
+  const long long ss = sz * k + sy * j + i;
+  const double dtv = *p.dt;
+  const double dtdx = dtv / g.dx, dtdy = dtv / g.dy, dtdz = dtv / g.dz;
+  const double *ex = p.ex, *ey = p.ey, *ez = p.ez;
+  const bool ji = j <= g.ny - 5, ki = k <= g.nz - 5, ii = i <= g.nx - 5;
+  if (ji && ki) {  // bx: one extra face in x
+    const double du0 = +dfc<ODR>(ez[ss - sy], ez[ss], ez[ss + sy], ez[ss + 2 * sy]);
+    const double du1 = -dfc<ODR>(ey[ss - sz], ey[ss], ey[ss + sz], ey[ss + 2 * sz]);
+    p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du0 - dtdz * du1, p.a, p.b);
+  }
+  if (ii && ki) {  // by: one extra face in y
+    const double du0 = +dfc<ODR>(ex[ss - sz], ex[ss], ex[ss + sz], ex[ss + 2 * sz]);
+    const double du1 = -dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
+    p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -dtdz * du0 - dtdx * du1, p.a, p.b);
+  }
+  if (ii && ji) {  // bz: one extra face in z
+    const double du0 = +dfc<ODR>(ey[ss - 1], ey[ss], ey[ss + 1], ey[ss + 2]);
+    const double du1 = -dfc<ODR>(ex[ss - sy], ex[ss], ex[ss + sy], ex[ss + 2 * sy]);
+    p.out[6][ss] = rk_upd(p.in[6][ss], p.u0[6][ss], -dtdx * du0 - dtdy * du1, p.a, p.b);
+  }
+}
+
+// ------------------------------------------------------------------ launch
+typedef void (*sweep3_fn)(StagePtrs, Geo, BcFlags, double, int);
+
+template <int DIR>
+static sweep3_fn pick3(int rmn, int odr)
+{
+#define QAS_S(R, O) \
+  if (rmn == R && odr == O) return k_sweep3d<DIR, R, O>;
+  QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
+  QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
+  QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
+  QAS_S(3, 1) QAS_S(3, 2) QAS_S(3, 3) QAS_S(3, 4)
+#undef QAS_S
+  return nullptr;
+}
+
+static inline unsigned nblk3(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
+{
+  const Geo& g = c->geo;
+  const int odr = c->s.odr;
+  const double gam = c->s.gam;
+  hipStream_t st = c->stream;
+  if (odr <= 2) hipLaunchKernelGGL(k_prim3d<2>, nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  else hipLaunchKernelGGL(k_prim3d<4>, nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  sweep3_fn f[3] = {pick3<0>(c->s.rmn, odr), pick3<1>(c->s.rmn, odr), pick3<2>(c->s.rmn, odr)};
+  for (int d = 0; d < 3; d++) {
+    if (!f[d]) return hipErrorInvalidValue;
+    const int nseg = (dims[d] - 5 + SEG - 1) / SEG;
+    const long long nline = g.nd / dims[d];
+    hipLaunchKernelGGL(f[d], nblk3(nline * nseg * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nseg);
+  }
+  const long long ne = (long long)(g.nx - 5) * (g.ny - 5) * (g.nz - 5);
+  const long long nc = (long long)(g.nx - 7) * (g.ny - 7) * (g.nz - 7);
+  switch (odr) {
+    case 1: hipLaunchKernelGGL(k_emf3d<1>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            hipLaunchKernelGGL(k_ct3d<1>, nblk3(nc, 256), 256, 0, st, p, g); break;
+    case 2: hipLaunchKernelGGL(k_emf3d<2>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            hipLaunchKernelGGL(k_ct3d<2>, nblk3(nc, 256), 256, 0, st, p, g); break;
+    case 3: hipLaunchKernelGGL(k_emf3d<3>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            hipLaunchKernelGGL(k_ct3d<3>, nblk3(nc, 256), 256, 0, st, p, g); break;
+    default: hipLaunchKernelGGL(k_emf3d<4>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+             hipLaunchKernelGGL(k_ct3d<4>, nblk3(nc, 256), 256, 0, st, p, g); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qas

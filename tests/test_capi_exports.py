@@ -1,0 +1,32 @@
+"""The C-ABI library loads and exports every symbol include/qasmhd.h
+declares (no device calls)."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "qasmhd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(qas_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_core_entry_points():
+    d = declared()
+    for n in ("qas_create", "qas_destroy", "qas_upload", "qas_download", "qas_bound", "qas_step", "qas_cfl_dt",
+              "qas_run", "qas_last_error"):
+        assert n in d
+
+
+def test_library_exports_declared_symbols():
+    from qasmhd_amd import capi
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("libqasmhd_amd.so not built here")
+    import ctypes
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(capi.EXPORTS) <= set(declared())

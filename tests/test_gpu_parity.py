@@ -1,0 +1,166 @@
+"""HIP path vs the reference (golden fixtures) and the oracle, through the C
+ABI.  FP64 parity is bit-exact: array equality of float64 values."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+S1 = np.load(os.path.join(GOLDEN, "steps_1d.npz"))
+S2 = np.load(os.path.join(GOLDEN, "steps_2d.npz"))
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def qas():
+    from qasmhd_amd import capi
+    capi.load()
+    return capi
+
+
+@pytest.mark.parametrize("name", [str(n) for n in S2["names"]])
+def test_2d_golden(qas, name):
+    c = S2["cfg_" + name]
+    rmn, odr, r_k, ctw = (int(v) for v in c[:4])
+    dn = np.zeros((3, 8), dtype=int)
+    dn[0], dn[1] = c[4:12], c[12:20]
+    st = S2["in"]
+    _say("2d golden", name, st.shape)
+    with qas.Solver(st.shape[1:], float(S2["dx"]), float(S2["dy"]), gam=float(S2["gam"]), rmn=rmn, odr=odr,
+                    r_k=r_k, ctw=ctw, dn=dn) as s:
+        s.upload(st)
+        hist, _ = s.run(int(S2["nsteps"]), flg=1)
+        out = s.download()
+    assert np.array_equal(hist, S2["dt_" + name])
+    assert np.array_equal(out, S2["out_" + name])
+
+
+def test_2d_golden_long(qas):
+    st = S2["long_in"]
+    n = st.shape[-1] - 8
+    d = 2 * np.pi / n
+    _say("2d golden long", st.shape)
+    with qas.Solver(st.shape[1:], d, d) as s:
+        s.upload(st)
+        hist, _ = s.run(40, flg=1)
+        out = s.download()
+    assert np.array_equal(hist, S2["long_dt"])
+    assert np.array_equal(out, S2["long_out"])
+
+
+@pytest.mark.parametrize("rmn", [0, 1, 2, 3])
+def test_1d_brio_wu_golden(qas, rmn):
+    st = S1["in"]
+    _say("1d golden", rmn)
+    with qas.Solver(st.shape[1:], float(S1["dx"]), gam=float(S1["gam"]), rmn=rmn, dn=S1["dn"]) as s:
+        s.upload(st)
+        dt0 = s.cfl_dt()
+        assert dt0 == float(S1["dt0"])
+        s.run(int(S1["steps_r%d" % rmn]), flg=0, dt=dt0, history=False)
+        out = s.download()
+    assert np.array_equal(out, S1["out_r%d" % rmn])
+
+
+@pytest.mark.parametrize("seed,n,ny,rmn,odr", [(1, 40, 24, 1, 4), (2, 24, 48, 3, 4), (3, 32, 32, 2, 3),
+                                               (4, 20, 36, 0, 2), (5, 64, 64, 1, 4)])
+def test_2d_vs_oracle(qas, oracle, seed, n, ny, rmn, odr):
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=n, ny=ny, seed=seed, vrms=1.2)
+    dn = np.zeros((3, 8), dtype=int)
+    if seed % 2 == 0:
+        dn[0] = [2] * 8
+        dn[1] = [1, 1, -1, 1, 1, -1, 1, 1]
+    _say("2d vs oracle", seed, n, ny, rmn, odr)
+    cfg = oracle.make_cfg(p.state.shape[1:], p.dx, p.dy, gam=p.gam, rmn=rmn, odr=odr, dn=dn)
+    ref = p.state.copy()
+    h_ref, dt_ref = oracle.run(cfg, ref, 10, 1)
+    with qas.Solver(p.state.shape[1:], p.dx, p.dy, gam=p.gam, rmn=rmn, odr=odr, dn=dn) as s:
+        s.upload(p.state)
+        hist, dt_now = s.run(10, flg=1)
+        out = s.download()
+    assert np.array_equal(hist, h_ref) and dt_now == dt_ref
+    assert np.array_equal(out, ref)
+
+
+def test_step_and_bound_entry_points(qas, oracle):
+    """qas_bound / qas_step / qas_cfl_dt individually == bound / ideal / setdt."""
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=24, seed=9)
+    cfg = oracle.make_cfg(p.state.shape[1:], p.dx, p.dy, gam=p.gam)
+    ref = p.state.copy()
+    ref[:, :4, :] = 123.0  # stale ghosts: bound must rewrite them
+    st = ref.copy()
+    oracle.bound(cfg, ref)
+    dt = oracle.cfl_dt(cfg, ref)
+    oracle.step(cfg, ref, dt)
+    with qas.Solver(p.state.shape[1:], p.dx, p.dy, gam=p.gam) as s:
+        s.upload(st)
+        s.bound()
+        b = s.download()
+        d = s.cfl_dt()
+        s.step(d)
+        out = s.download()
+    assert d == dt
+    st2 = st.copy()
+    oracle.bound(cfg, st2)
+    assert np.array_equal(b, st2)
+    assert np.array_equal(out, ref)
+
+
+def test_2d_full_size_invariants(qas):
+    """1024^2 turbulence (the benchmark workload), 20 steps: periodic totals
+    of ro, mx, my, mz, bz, en conserved to round-off, and the 4th-order
+    corner divergence of the face B stays at round-off (CT)."""
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=1024, seed=2)
+    _say("2d 1024^2 invariants")
+
+    def divb(st):
+        bx, by = st[4], st[5]
+        d = p.dx
+        dbx = (27.0 * (bx[4:-4, 5:-3] - bx[4:-4, 4:-4]) - (bx[4:-4, 6:-2] - bx[4:-4, 3:-5])) / 24.0 / d
+        dby = (27.0 * (by[5:-3, 4:-4] - by[4:-4, 4:-4]) - (by[6:-2, 4:-4] - by[3:-5, 4:-4])) / 24.0 / d
+        return np.abs(dbx + dby).max()
+
+    st0 = p.state
+    tot0 = st0[:, 4:-4, 4:-4].sum(axis=(1, 2))
+    d0 = divb(st0)
+    with qas.Solver(st0.shape[1:], p.dx, p.dy, gam=p.gam) as s:
+        s.upload(st0)
+        s.run(20, flg=1, history=False)
+        out = s.download()
+    assert np.isfinite(out).all()
+    tot = out[:, 4:-4, 4:-4].sum(axis=(1, 2))
+    scale = np.abs(st0[:, 4:-4, 4:-4]).sum(axis=(1, 2)) + 1.0
+    for m in (0, 1, 2, 3, 6, 7):
+        assert abs(tot[m] - tot0[m]) / scale[m] < 1e-12, m
+    bmax = np.abs(out[4:6]).max()
+    assert divb(out) < 1e-11 * bmax / p.dx + 10 * d0
+
+
+@pytest.mark.parametrize("rmn,odr,bc", [(1, 4, "periodic"), (3, 4, "mixed"), (2, 3, "periodic"), (0, 4, "mixed")])
+def test_3d_vs_oracle(qas, oracle, rmn, odr, bc):
+    from qasmhd_amd import problems
+    p = problems.turbulence3d(n=16, seed=7, nmodes=12)
+    dn = np.zeros((3, 8), dtype=int)
+    if bc == "mixed":
+        dn[0] = [2] * 8
+        dn[2] = [1, 1, 1, -1, 1, 1, -1, 1]
+    _say("3d vs oracle", rmn, odr, bc)
+    cfg = oracle.make_cfg(p.state.shape[1:], p.dx, p.dy, p.dz, gam=p.gam, rmn=rmn, odr=odr, dn=dn)
+    ref = p.state.copy()
+    h_ref, dt_ref = oracle.run(cfg, ref, 4, 1)
+    with qas.Solver(p.state.shape[1:], p.dx, p.dy, p.dz, gam=p.gam, rmn=rmn, odr=odr, dn=dn) as s:
+        s.upload(p.state)
+        hist, dt_now = s.run(4, flg=1)
+        out = s.download()
+    assert np.array_equal(hist, h_ref) and dt_now == dt_ref
+    assert np.array_equal(out, ref)

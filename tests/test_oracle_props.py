@@ -1,0 +1,207 @@
+"""Properties that pin the oracle where no reference vector exists (and
+re-check what the golden vectors cover from the mathematics side):
+
+* interpolants exact on polynomials of their degree; WCNS-4 equal to the
+  linear 4th-order interpolant when the smoothness indicators are equal;
+* F(U, U) equals the physical MHD flux for every Riemann solver;
+* 3D step: a z-invariant state reproduces the 2D reference run (golden
+  vectors written by the reference's own 2D driver) to round-off growth;
+  the cyclic relabelling x->y->z commutes with a step; periodic totals are
+  conserved; the 4th-order div B stays at round-off; a uniform state stays
+  uniform.
+The 3D driver of the reference needs mpi.h and cannot be built here, so these
+are the 3D oracle's pins.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+RNG = np.random.Generator(np.random.PCG64(7))
+
+
+def test_linear_interp_polynomial_exact(oracle):
+    # cell values of a degree-k polynomial at x = -2..2 (unit cells) -> faces +-1/2
+    x = np.arange(-2, 3, dtype=float)
+    # point-value exactness: flr2 (-1,5,2)/6 is linear-exact, flr3 quadratic,
+    # flr4 (9,-56,234,144,-11)/320 cubic (moments 1, 1/2, 1/4, 1/8)
+    for odr, deg in ((1, 0), (2, 1), (3, 2), (4, 3)):
+        for _ in range(20):
+            c = RNG.normal(size=deg + 1)
+            f = np.polyval(c, x)[None, :]
+            fl, fr = oracle.interp(9 + odr, f)
+            assert abs(fl[0] - np.polyval(c, 0.5)) < 1e-12 * (1 + np.abs(c).sum())
+            assert abs(fr[0] - np.polyval(c, -0.5)) < 1e-12 * (1 + np.abs(c).sum())
+
+
+def test_wcns4_equals_linear_on_linear_data(oracle):
+    x = np.arange(-2, 3, dtype=float)
+    f = (RNG.normal(size=(50, 1)) + RNG.normal(size=(50, 1)) * x[None, :])
+    wl, wr = oracle.interp(3, f)
+    ll, lr = oracle.interp(13, f)
+    assert np.allclose(wl, ll, rtol=1e-13, atol=1e-13) and np.allclose(wr, lr, rtol=1e-13, atol=1e-13)
+    const = np.full((5, 5), 2.5)
+    cl, cr = oracle.interp(3, const)
+    assert np.allclose(cl, 2.5, rtol=1e-15) and np.allclose(cr, 2.5, rtol=1e-15)
+
+
+def test_fcen_df_cubic_exact(oracle):
+    # faces at -3/2, -1/2, 1/2, 3/2 -> centre 0
+    xf = np.array([-1.5, -0.5, 0.5, 1.5])
+    for _ in range(20):
+        c = RNG.normal(size=4)
+        f = np.polyval(c, xf)[None, :]
+        assert abs(oracle.fcen(4, f)[0] - np.polyval(c, 0.0)) < 1e-12
+        assert abs(oracle.df(4, f)[0] - np.polyval(np.polyder(c), 0.0)) < 1e-12
+
+
+def phys_flux(q, bn, gam):
+    ro, vn, vt, vu, bt, bu, pr = q.T
+    pml = 0.5 * (bt * bt + bu * bu)
+    en = pr / (gam - 1) + pml + 0.5 * ro * (vn * vn + vt * vt + vu * vu)
+    return np.stack([ro * vn, ro * vn * vn + pr + pml - 0.5 * bn * bn, ro * vn * vt - bn * bt, ro * vn * vu - bn * bu,
+                     bt * vn - bn * vt, bu * vn - bn * vu, (en + pr + pml) * vn - bn * (vt * bt + vu * bu)], axis=1)
+
+
+@pytest.mark.parametrize("rmn", [0, 1, 2, 3])
+def test_flux_consistency(oracle, rmn):
+    n = 200
+    q = np.zeros((n, 7))
+    q[:, 0] = np.exp(RNG.normal(scale=0.3, size=n))
+    q[:, 1:4] = RNG.normal(size=(n, 3))
+    q[:, 4:6] = RNG.normal(size=(n, 2))
+    q[:, 6] = np.exp(RNG.normal(scale=0.3, size=n))
+    bn = RNG.normal(size=n)
+    dv = np.zeros((n, 2))
+    gam = 5.0 / 3.0
+    f = oracle.flux(rmn, q, q, bn, gam, dv)
+    ref = phys_flux(q, bn, gam)
+    assert np.allclose(f, ref, rtol=1e-10, atol=1e-10)
+
+
+# ---------------------------------------------------------------- 3D
+def _cfg3(oracle, st, d, **kw):
+    return oracle.make_cfg(st.shape[1:], d, d, d, **kw)
+
+
+def test_3d_z_invariant_reduces_to_2d_reference(oracle):
+    """Extrude the 2D golden input along z (vz = bz = 0): every z plane of the
+    3D run must match the reference's own 2D run to round-off growth."""
+    S2 = np.load(os.path.join(GOLDEN, "steps_2d.npz"))
+    st2 = S2["in"].copy()
+    st2[3] = 0.0   # mz
+    st2[6] = 0.0   # bz
+    # energy consistent with the zeroed fields is not needed: the 2D run
+    # below starts from the same modified state
+    nz = 16
+    st3 = np.repeat(st2[:, None, :, :], nz, axis=1).copy()
+    d = float(S2["dx"])
+    from oracle import build
+    if build.reference_present():
+        b = build.build_ref(2, 16, 16, 1, 4, 3, 1, 0.4)
+        ref2, _ = build.run_ref(b, st2, d, float(S2["gam"]), 4, 1, dy=float(S2["dy"]))
+    else:
+        cfg2 = oracle.make_cfg(st2.shape[1:], d, float(S2["dy"]), gam=float(S2["gam"]))
+        ref2 = st2.copy()
+        oracle.run(cfg2, ref2, 4, 1)
+    cfg3 = oracle.make_cfg(st3.shape[1:], d, float(S2["dy"]), d, gam=float(S2["gam"]))
+    oracle.run(cfg3, st3, 4, 1)
+    for k in range(4, nz - 4):
+        for m in (0, 1, 2, 4, 5, 7):
+            scale = np.abs(ref2[m]).max() + 1e-300
+            assert np.abs(st3[m, k] - ref2[m]).max() < 1e-11 * scale, (k, m)
+        assert np.abs(st3[3, k]).max() < 1e-13 and np.abs(st3[6, k]).max() < 1e-13
+
+
+def _rotate(st):
+    """Cyclic relabelling x->y->z of a state (8, nz, ny, nx): new x = old z,
+    new y = old x, new z = old y, vectors relabelled likewise."""
+    # new[k', j', i'] = old[j', i', k']  (k' along new z = old y, ...)
+    def r(a):
+        return np.ascontiguousarray(np.transpose(a, (1, 2, 0)))
+    out = np.empty_like(st)
+    out[0] = r(st[0])
+    out[1], out[2], out[3] = r(st[3]), r(st[1]), r(st[2])
+    out[4], out[5], out[6] = r(st[6]), r(st[4]), r(st[5])
+    out[7] = r(st[7])
+    return out
+
+
+def test_rotate_helper_is_consistent():
+    a = np.arange(8 * 5 * 5 * 5, dtype=float).reshape(8, 5, 5, 5)
+    b = _rotate(_rotate(_rotate(a)))
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("rmn", [1, 3])
+def test_3d_cyclic_symmetry(oracle, rmn):
+    from qasmhd_amd import problems
+    p = problems.turbulence3d(n=12, seed=3, nmodes=10)
+    d = p.dx
+    a = p.state.copy()
+    b = _rotate(p.state)
+    cfg = _cfg3(oracle, a, d, gam=p.gam, rmn=rmn)
+    dt = 0.3 * oracle.cfl_dt(cfg, a.copy())
+    for _ in range(3):
+        oracle.bound(cfg, a)
+        oracle.step(cfg, a, dt)
+        oracle.bound(cfg, b)
+        oracle.step(cfg, b, dt)
+    ra = _rotate(a)
+    for m in range(8):
+        scale = np.abs(ra[m]).max() + 1e-300
+        assert np.abs(ra[m] - b[m]).max() < 1e-12 * scale, m
+
+
+def test_3d_conservation_and_divb(oracle):
+    from qasmhd_amd import problems
+    p = problems.turbulence3d(n=16, seed=5, nmodes=12)
+    st = p.state.copy()
+    d = p.dx
+    cfg = _cfg3(oracle, st, d, gam=p.gam)
+    I = (slice(4, -4),) * 3
+
+    def divb(s):
+        bx, by, bz = s[4], s[5], s[6]
+        def D(f, ax):
+            sl = lambda o: tuple(slice(4 + (o if a == ax else 0), s.shape[a + 1] - 4 + (o if a == ax else 0))
+                                 for a in range(3))
+            return (27.0 * (f[sl(1)] - f[sl(0)]) - (f[sl(2)] - f[sl(-1)])) / 24.0 / d
+        return np.abs(D(bx, 2) + D(by, 1) + D(bz, 0)).max()
+
+    t0 = st[(slice(None),) + I].sum(axis=(1, 2, 3))
+    d0 = divb(st)
+    oracle.run(cfg, st, 4, 1)
+    t1 = st[(slice(None),) + I].sum(axis=(1, 2, 3))
+    scale = np.abs(p.state[(slice(None),) + I]).sum(axis=(1, 2, 3)) + 1
+    for m in range(8):
+        assert abs(t1[m] - t0[m]) < 1e-12 * scale[m], m
+    assert divb(st) < 1e-12 * np.abs(st[4:7]).max() / d + 10 * d0
+
+
+def test_3d_uniform_state_stays_uniform(oracle):
+    st = np.zeros((8, 16, 16, 16))
+    st[0] = 1.3
+    st[1], st[2], st[3] = 0.2, -0.1, 0.05
+    st[4], st[5], st[6] = 0.3, 0.4, -0.2
+    gam = 5.0 / 3.0
+    pr = 0.7
+    st[7] = pr / (gam - 1) + 0.5 * (st[1] ** 2 + st[2] ** 2 + st[3] ** 2) / st[0] + 0.5 * (0.09 + 0.16 + 0.04)
+    cfg = _cfg3(oracle, st, 0.1, gam=gam)
+    s = st.copy()
+    oracle.run(cfg, s, 3, 1)
+    assert np.allclose(s, st, rtol=1e-13, atol=1e-13)
+
+
+def test_2d_static_zero_fields_stay_exactly_zero(oracle):
+    """v = 0 and B = 0 stencils are kept exactly zero (mhd_eigen.c:575-587)."""
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=16, seed=1, vrms=0.0, brms=0.0)
+    st = p.state.copy()
+    st[1:7] = 0.0
+    st[0] *= 1 + 0.1 * np.sin(np.arange(st.shape[-1]))[None, :]
+    cfg = oracle.make_cfg(st.shape[1:], p.dx, p.dy, gam=p.gam)
+    oracle.run(cfg, st, 3, 1)
+    assert np.all(st[3] == 0.0) and np.all(st[4:7] == 0.0)
