@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the qasMHD hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload turb2d]
+
+Workload (default): 2D periodic decaying MHD, 1024^2 interior cells on
+[0, 2pi]^2, gam = 5/3, rho = gam^2, p = gam, velocity and Az from 16 seeded
+Fourier modes (|k| <= 4) with rms|v| = rms|B| = 1; HLLD, WCNS-4, CUCT CT,
+SSP-RK3, CFL 0.4 with the reference's dt schedule (dt from setdt every 2nd
+step).  One "step" is one full RK3 step (3 stages with per-stage ghost fill)
+plus the CFL reduction every 2nd step, on a state already resident in HBM;
+the metric is interior zone-cycles per second.
+
+Also reported, on one JSON line:
+  roofline      the dominant kernel's algorithmic work per launch (flops and
+                compulsory bytes from the library's work model) over its
+                average launch time, measured with HIP events on the
+                library's stream during the timed region
+  cpu_baseline  the reference's own 2D driver (oracle/_ref, built from the
+                reference sources) timed on this host over a bounded sample
+                of the same workload; the C oracle when that is not built
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-cycles/s at 1 GPU, FP64 vs mixed at stated L1 err; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from qasmhd_amd import problems
+    if name == "turb2d":
+        return problems.turbulence2d(n=1024, seed=2), "2d_decaying_mhd_1024x1024_hlld_wcns4_ct_rk3"
+    if name == "turb2d_512":
+        return problems.turbulence2d(n=512, seed=2), "2d_decaying_mhd_512x512_hlld_wcns4_ct_rk3"
+    if name == "khi":
+        return problems.kelvin_helmholtz(n=2048), "2d_lowmach_khi_2048x2048_mlau_wcns4_ct_rk3"
+    if name == "turb3d":
+        return problems.turbulence3d(n=512), "3d_mhd_turbulence_512x512x512_hlld_wcns4_ct_rk3"
+    if name == "brio_wu":
+        return problems.brio_wu(1024), "1d_brio_wu_1024_hlld_wcns4_rk3"
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(p, nsteps):
+    """The reference's own driver on this host (kind 'reference'), else the C
+    oracle (kind 'port'); same initial state, first `nsteps` steps."""
+    import numpy as np
+    from oracle import build as obuild
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    zones = int(np.prod(p.interior))
+    if p.ndim == 2:
+        nx, ny = p.interior[1], p.interior[0]
+        binary = os.path.join(obuild.REFOUT, obuild.ref_binary_name(2, nx, ny, p.rmn, p.odr, p.r_k, p.ctw, p.cfl))
+        if os.path.exists(binary):
+            _, _, el = obuild.run_ref(binary, p.state, p.dx, p.gam, nsteps, 1, dnx=list(p.dn[0]),
+                                      dny=list(p.dn[1]), dy=p.dy, threads=threads, timing=True)
+            return {"value": zones * nsteps / el, "unit": "zone-cycles/s", "cores": threads, "kind": "reference",
+                    "sample": "first %d steps of the same %dx%d run (reference MHD2D::ideal/bound/setdt, g++ -O2 "
+                              "-fopenmp, OMP_NUM_THREADS=%d), %.2f s" % (nsteps, nx, ny, threads, el)}
+    from oracle import pyoracle
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    cfg = pyoracle.make_cfg(p.state.shape[1:], p.dx, p.dy, p.dz, gam=p.gam, cfl=p.cfl, rmn=p.rmn, odr=p.odr,
+                            r_k=p.r_k, ctw=p.ctw, dn=p.dn)
+    st = p.state.copy()
+    t0 = time.perf_counter()
+    pyoracle.run(cfg, st, nsteps, 1)
+    el = time.perf_counter() - t0
+    return {"value": zones * nsteps / el, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": "first %d steps of the same run (C oracle, OpenMP %d threads), %.2f s" % (nsteps, threads, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="turb2d")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the domain is one device-resident block); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+    import numpy as np
+    from qasmhd_amd import capi
+
+    p, wname = workload(args.workload)
+    zones = int(np.prod(p.interior))
+    _log("bench: workload %s, %d zones, %d warmup + %d timed steps" % (wname, zones, args.warmup, args.steps))
+    lib = capi.load()
+    lib.qas_prof_enable.argtypes = [C.c_void_p, C.c_uint, C.c_int]
+    lib.qas_prof_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    lib.qas_kernel_model.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    s = capi.solver_for(p)
+    s.upload(p.state)
+    s.run(args.warmup, flg=1, history=False)
+    s.sync()
+    # timing hooks on the sweep kernels (the dominant ones)
+    sweeps = [1, 2, 3][:p.ndim]
+    mask = 0
+    for k in sweeps:
+        mask |= 1 << k
+    stages = p.r_k * args.steps
+    s._check(lib.qas_prof_enable(s.ctx, mask, stages * len(sweeps) + 16), "qas_prof_enable")
+    s.sync()
+    t0 = time.perf_counter()
+    hist, dt_now = s.run(args.steps, flg=1, history=False)
+    s.sync()
+    el = time.perf_counter() - t0
+    out = s.download()
+    finite = bool(np.isfinite(out).all())
+    value = zones * args.steps / el
+    _log("bench: %.3f s for %d steps -> %.4g zone-cycles/s (finite=%s)" % (el, args.steps, value, finite))
+
+    # roofline of the dominant kernel
+    best = None
+    for k in sweeps:
+        tot, n = C.c_double(), C.c_longlong()
+        s._check(lib.qas_prof_query(s.ctx, k, C.byref(tot), C.byref(n)), "qas_prof_query")
+        fl, by = C.c_double(), C.c_double()
+        s._check(lib.qas_kernel_model(s.ctx, k, C.byref(fl), C.byref(by)), "qas_kernel_model")
+        if n.value == 0:
+            continue
+        avg_s = tot.value / n.value * 1e-3
+        rec = dict(kid=k, total_ms=tot.value, launches=n.value, avg_s=avg_s, flops=fl.value, bytes=by.value)
+        if best is None or tot.value > best["total_ms"]:
+            best = rec
+    names = {1: "sweep_x", 2: "sweep_y", 3: "sweep_z"}
+    roof = None
+    if best:
+        t_flop = best["flops"] / (FP64_PEAK_TFLOPS * 1e12)
+        t_hbm = best["bytes"] / (HBM_PEAK_GBS * 1e9)
+        tflops = best["flops"] / best["avg_s"] / 1e12
+        gbs = best["bytes"] / best["avg_s"] / 1e9
+        if t_hbm > t_flop:
+            roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+        else:
+            roof = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / FP64_PEAK_TFLOPS}
+        roof.update({"traffic": None, "kernel": names[best["kid"]], "avg_launch_us": best["avg_s"] * 1e6,
+                     "launches": best["launches"], "alg_flops_per_launch": best["flops"],
+                     "alg_bytes_per_launch": best["bytes"], "fp64_tflops": tflops, "hbm_gbs": gbs,
+                     "fp64_frac": tflops / FP64_PEAK_TFLOPS, "hbm_frac": gbs / HBM_PEAK_GBS,
+                     "compute": "FP64 VALU (no MFMA on this stencil path); peak = MI355X FP64 78.6 TFLOP/s"})
+    s.close()
+
+    cpu = None
+    if not args.no_cpu and p.ndim == 2:
+        try:
+            _log("bench: cpu baseline, %d steps" % args.cpu_steps)
+            cpu = cpu_baseline(p, args.cpu_steps)
+        except Exception as e:  # the baseline is reported, not required
+            _log("bench: cpu baseline failed: %s" % e)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wname, "cells": list(p.interior), "solver": ["roe", "hlld", "lhlld", "mlau"][p.rmn],
+                   "odr": p.odr, "r_k": p.r_k, "ctw": p.ctw, "cfl": p.cfl, "precision": "fp64",
+                   "dt_final": dt_now, "finite": finite},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

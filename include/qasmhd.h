@@ -116,6 +116,36 @@ int qas_sync(qas_ctx *c);
 /* Device memory held by the context (bytes). */
 long long qas_device_bytes(const qas_ctx *c);
 
+
+/* ---- instrumentation (no counterpart in the reference) -----------------
+ * Kernel identifiers of one stage, for the timing hooks and the work model. */
+#define QAS_K_PRIM 0      /* cell-centred B + primitives                    */
+#define QAS_K_SWEEP_X 1   /* reconstruction + Riemann + divergence along x  */
+#define QAS_K_SWEEP_Y 2
+#define QAS_K_SWEEP_Z 3
+#define QAS_K_EMF 4       /* CUCT edge EMFs                                 */
+#define QAS_K_CT 5        /* staggered-B update                             */
+#define QAS_K_BOUND 6     /* ghost fill                                     */
+#define QAS_K_CFL 7       /* CFL reduction                                  */
+#define QAS_K_COUNT 8
+
+/* Record HIP events around every launch of the kernels whose bit is set in
+ * `mask` (1 << QAS_K_*), for up to `capacity` launches; resets earlier
+ * records.  mask 0 turns recording off. */
+int qas_prof_enable(qas_ctx *c, unsigned mask, int capacity);
+/* Total device time (ms) and launch count recorded for kernel `kid`.
+ * Synchronises. */
+int qas_prof_query(qas_ctx *c, int kid, double *total_ms, long long *launches);
+/* Algorithmic work of one launch of kernel `kid` for this context: flops
+ * (+ - * / sqrt counted 1 each, the cheaper characteristic projection) and
+ * compulsory HBM bytes. */
+int qas_kernel_model(const qas_ctx *c, int kid, double *flops, double *bytes);
+/* Per-point flop counts for (rmn, odr): [0] reconstruction (fast/Alfven),
+ * [1] reconstruction (Alfven/entropy), [2] Riemann solver, [3] B-flux face
+ * states, [4] one-sided linear interpolation, [5] 4th-order difference,
+ * [6] face-to-centre, [7] CUCT weight. */
+int qas_point_flops(int rmn, int odr, double out[8]);
+
 #ifdef __cplusplus
 }
 #endif

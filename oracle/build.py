@@ -15,7 +15,7 @@ cpu_baseline leg:
   skipped by its include guard and the values below are used instead.
   Nothing from the reference is copied; only the build outputs land in
   oracle/_ref/.  The 3D driver needs mpi.h, which this image lacks, so it is
-  not built (see the 3D notes in tests/test_oracle_3d.py).
+  not built; the 3D oracle is pinned by tests/test_oracle_props.py.
 """
 from __future__ import annotations
 
@@ -125,9 +125,10 @@ def _hex(x: float) -> str:
 
 
 def run_ref(binary: str, state, dx: float, gam: float, nsteps: int, flg: int, dt: float = 0.0,
-            dnx=(0,) * 8, dny=(0,) * 8, dy: float = 1.0, threads: int | None = None):
+            dnx=(0,) * 8, dny=(0,) * 8, dy: float = 1.0, threads: int | None = None, timing: bool = False):
     """Run a reference harness on `state` (numpy (8, ...) float64, ghosts
-    included).  Returns (final_state, dt_history)."""
+    included).  Returns (final_state, dt_history), plus the wall time of the
+    step loop in seconds when timing is set."""
     import numpy as np
     state = np.ascontiguousarray(state, dtype=np.float64)
     with tempfile.TemporaryDirectory() as td:
@@ -146,6 +147,12 @@ def run_ref(binary: str, state, dx: float, gam: float, nsteps: int, flg: int, dt
             raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout, r.stderr))
         out = np.fromfile(fout, dtype=np.float64).reshape(state.shape)
         hist = np.fromfile(fdt, dtype=np.float64) if os.path.exists(fdt) else np.zeros(0)
+    if timing:
+        el = float("nan")
+        for line in r.stdout.splitlines():
+            if line.startswith("elapsed_s"):
+                el = float(line.split()[1])
+        return out, hist, el
     return out, hist
 
 

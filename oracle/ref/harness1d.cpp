@@ -6,6 +6,7 @@
  */
 #include "mhd1d_class.hpp"
 #include "harness_common.hpp"
+#include <chrono>
 
 class RefRun1D : public MHD1D {
  public:
@@ -18,12 +19,15 @@ class RefRun1D : public MHD1D {
     const int flg = p.i("flg");
     std::vector<double> hist;
     if (flg) setdt(1); else dt = p.d("dt");
+    const auto t0 = std::chrono::steady_clock::now();
     for (n = 1; n <= nsteps; n++) {
       hist.push_back(dt);
       bound(val, nm, dnxs);
       ideal(dt);
       setdt(flg * (n % 2 == 0));
     }
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("elapsed_s %.9f\n", el);
     if (!write_raw(p.s("out").c_str(), val, nm, nx)) return 3;
     if (!p.s("dthist").empty()) write_vec(p.s("dthist").c_str(), hist);
     return 0;

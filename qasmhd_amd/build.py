@@ -26,8 +26,9 @@ ARCH = os.environ.get("QAS_OFFLOAD_ARCH", "gfx950")
 COMMON_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-Wno-unused-command-line-argument", "-Wno-unused-result"]
-# per-source extra flags (the mixed-precision unit may contract FP32 arithmetic)
-EXTRA = {}
+# per-source flags: every unit is built without FMA contraction except the
+# FP32 sweeps of the mixed path, which also use the fast FP32 divide/sqrt
+EXTRA = {"qas_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def lib_path() -> str:

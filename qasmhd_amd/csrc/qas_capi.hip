@@ -171,6 +171,7 @@ extern "C" void qas_destroy(qas_ctx* c)
   if (c->slab) hipFree(c->slab);
   if (c->d_hist) hipFree(c->d_hist);
   if (c->h_pinned) hipHostFree(c->h_pinned);
+  for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -374,3 +375,40 @@ extern "C" int qas_sync(qas_ctx* c)
 }
 
 extern "C" long long qas_device_bytes(const qas_ctx* c) { return c ? (long long)c->slab_bytes : 0; }
+
+extern "C" int qas_prof_enable(qas_ctx* c, unsigned mask, int capacity)
+{
+  if (!ready(c) || capacity < 0) return fail(c, QAS_E_ARG, "qas_prof_enable: bad arguments");
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  c->prof_mask = 0;
+  c->prof_n = 0;
+  if (mask == 0) return QAS_OK;
+  while ((int)c->prof_kid.size() < capacity) {
+    hipEvent_t a, b;
+    QAS_CHECK(c, hipEventCreate(&a));
+    QAS_CHECK(c, hipEventCreate(&b));
+    c->prof_ev.push_back(a);
+    c->prof_ev.push_back(b);
+    c->prof_kid.push_back(-1);
+  }
+  c->prof_mask = mask;
+  return QAS_OK;
+}
+
+extern "C" int qas_prof_query(qas_ctx* c, int kid, double* total_ms, long long* launches)
+{
+  if (!ready(c) || !total_ms || !launches) return fail(c, QAS_E_ARG, "qas_prof_query: bad arguments");
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  long long n = 0;
+  for (size_t r = 0; r < c->prof_n; r++) {
+    if (c->prof_kid[r] != kid) continue;
+    float ms = 0.0f;
+    QAS_CHECK(c, hipEventElapsedTime(&ms, c->prof_ev[2 * r], c->prof_ev[2 * r + 1]));
+    tot += ms;
+    n++;
+  }
+  *total_ms = tot;
+  *launches = n;
+  return QAS_OK;
+}

@@ -80,7 +80,7 @@ hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar)
   for (int m = 0; m < 8; m++) a.f[m] = f[m];
   a.nvar = nvar;
   const long long tot = nghost * nvar;
-  hipLaunchKernelGGL(k_bound, (unsigned)((tot + 255) / 256), 256, 0, c->stream, a, g, c->bc, nghost, ox, oy, oz);
+  QAS_LAUNCH(c, QAS_K_BOUND, k_bound, (unsigned)((tot + 255) / 256), 256, 0, c->stream, a, g, c->bc, nghost, ox, oy, oz);
   return hipGetLastError();
 }
 
@@ -173,7 +173,7 @@ hipError_t launch_cfl(qas_ctx* c, const double* const U[8])
   a.stz6 = c->s.st[2][6];
   const int ox = qas::NG, oy = g.ndim >= 2 ? qas::NG : 0, oz = g.ndim >= 3 ? qas::NG : 0;
   const long long n = (long long)(g.nx - 2 * ox) * (g.ny - 2 * oy) * (g.nz - 2 * oz);
-  hipLaunchKernelGGL(k_cfl, (unsigned)((n + 255) / 256), 256, 0, c->stream, a, g);
+  QAS_LAUNCH(c, QAS_K_CFL, k_cfl, (unsigned)((n + 255) / 256), 256, 0, c->stream, a, g);
   return hipGetLastError();
 }
 

@@ -22,21 +22,25 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+// host+device: the host instantiation with an op-counting scalar gives the
+// algorithmic flop model (qas_flops.hip)
+#define QAS_HD __host__ __device__ __forceinline__
+
 namespace qas {
 
 // ternary max/min of common/common_func.h:4-14 (not fmax/fmin)
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return (a >= b) ? a : b; }
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return (a >= b) ? b : a; }
+template <typename T> QAS_HD T tmax(T a, T b) { return (a >= b) ? a : b; }
+template <typename T> QAS_HD T tmin(T a, T b) { return (a >= b) ? b : a; }
 
-template <typename T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
-template <typename T> __device__ __forceinline__ T dabs(T x) { return fabs(x); }
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
-template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf(x); }
+template <typename T> QAS_HD T dsqrt(T x) { return sqrt(x); }
+template <typename T> QAS_HD T dabs(T x) { return fabs(x); }
+template <> QAS_HD float dsqrt<float>(float x) { return sqrtf(x); }
+template <> QAS_HD float dabs<float>(float x) { return fabsf(x); }
 
 // ------------------------------------------------------------------ 1-D interpolation
 // f[0..4] = stencil i-2..i+2; fl = left state at i+1/2, fr = right state at i-1/2.
 template <int ODR, typename T>
-__device__ __forceinline__ void lin5(const T* f, T& fl, T& fr)
+QAS_HD void lin5(const T* f, T& fl, T& fr)
 {
   if constexpr (ODR == 1) {
     fl = f[2];
@@ -54,13 +58,13 @@ __device__ __forceinline__ void lin5(const T* f, T& fl, T& fr)
 }
 
 template <typename T>
-__device__ __forceinline__ T minmod(T a, T b)
+QAS_HD T minmod(T a, T b)
 {
   return (+tmin(tmax(a, b), T(0.0)) + tmax(tmin(a, b), T(0.0)));
 }
 
 template <int ODR, typename T>
-__device__ __forceinline__ void interp5(const T* f, T& fl, T& fr)
+QAS_HD void interp5(const T* f, T& fl, T& fr)
 {
   if constexpr (ODR == 1) {
     fl = f[2];
@@ -132,13 +136,13 @@ __device__ __forceinline__ void interp5(const T* f, T& fl, T& fr)
 
 // face -> centre (f0 = value at i-1/2) and 4th-order difference
 template <int ODR, typename T>
-__device__ __forceinline__ T fcen(T fm1, T f0, T f1, T f2)
+QAS_HD T fcen(T fm1, T f0, T f1, T f2)
 {
   if constexpr (ODR <= 2) return (T(0.5) * (f0 + f1));
   else return (T(0.0625) * (T(9.0) * (f0 + f1) - (fm1 + f2)));
 }
 template <int ODR, typename T>
-__device__ __forceinline__ T dfc(T fm1, T f0, T f1, T f2)
+QAS_HD T dfc(T fm1, T f0, T f1, T f2)
 {
   if constexpr (ODR <= 2) return (f1 - f0);
   else return ((T(27.0) * (f1 - f0) - (f2 - fm1)) / T(24.0));
@@ -147,8 +151,11 @@ __device__ __forceinline__ T dfc(T fm1, T f0, T f1, T f2)
 // ------------------------------------------------------------------ characteristic reconstruction
 // s[k][m]: k = (ro, vn, vt, vu, bt, bu, pr), m = stencil i-2..i+2.
 // bn: cell-centred normal B.  vl: face i+1/2 (left state), vr: face i-1/2.
-template <int ODR, typename T>
-__device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7])
+// MODE 0: both projections formed, selected per lane (the kernels);
+// MODE 1 / 2: only the fast/Alfven resp. Alfven/entropy projection is
+// evaluated (used by the flop model to count one branch).
+template <int ODR, typename T, int MODE = 0>
+QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7])
 {
   const T ro0 = s[0][2], bt0 = s[4][2], bu0 = s[5][2], pr0 = s[6][2];
   // exact-zero flags of mhd_eigen.c:575-587
@@ -158,7 +165,7 @@ __device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)
     zf[n - 1] = (s[n][0] == T(0.0)) & (s[n][1] == T(0.0)) & (s[n][2] == T(0.0)) & (s[n][3] == T(0.0)) &
                 (s[n][4] == T(0.0));
   // branch of mhd_func.c:143 (strict >)
-  const bool fast = (s[0][2] * (s[1][2] * s[1][2] + s[2][2] * s[2][2] + s[3][2] * s[3][2]) > gam * s[6][2]);
+  const bool fast = MODE == 1 ? true : (MODE == 2 ? false : (s[0][2] * (s[1][2] * s[1][2] + s[2][2] * s[2][2] + s[3][2] * s[3][2]) > gam * s[6][2]));
   // shared eigen quantities (mhd_eigen_mi / mhd_eigen_ae)
   const T iro = T(1.0) / ro0;
   const T cs2 = gam * pr0 * iro;
@@ -166,13 +173,18 @@ __device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)
   const T ct2 = (bt0 * bt0 + bu0 * bu0) * iro;
   const T cb2 = cs2 + ca2 + ct2;
   const T cf2 = T(0.5) * (cb2 + dsqrt(dabs(cb2 * cb2 - T(4.0) * cs2 * ca2)));
-  const T cf = dsqrt(cf2);
-  const T rcf = ro0 * cf;
+  T rcf = T(0.0), icf2h = T(0.0), ircfh = T(0.0);
+  if (MODE != 2) {
+    const T cf = dsqrt(cf2);
+    rcf = ro0 * cf;
+  }
   const T sro = dsqrt(ro0);
   const T isroh = T(0.5) / sro;
   const T icf2 = T(1.0) / cf2;
-  const T icf2h = T(0.5) * icf2;
-  const T ircfh = T(0.5) / rcf;
+  if (MODE != 2) {
+    icf2h = T(0.5) * icf2;
+    ircfh = T(0.5) / rcf;
+  }
   const T byh = T(0.5) * bt0;
   const T bzh = T(0.5) * bu0;
 
@@ -180,8 +192,12 @@ __device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)
   // k = 0: fast -> -rcf vn + bt0 bt + bu0 bu + pr ; Alfven/entropy -> vn
 #pragma unroll
   for (int m = 0; m < 5; m++) {
-    const T wf = -rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-    w[m] = fast ? wf : s[1][m];
+    if (MODE == 2) w[m] = s[1][m];
+    else if (MODE == 1) w[m] = -rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+    else {
+      const T wf = -rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+      w[m] = fast ? wf : s[1][m];
+    }
   }
   interp5<ODR>(w, wl[0], wr[0]);
   // k = 1: cf2 ro - bt0 bt - bu0 bu - pr (both)
@@ -191,9 +207,13 @@ __device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)
   // k = 2: fast -> rcf vn + bt0 bt + bu0 bu + pr ; A/E -> bt0 bt + bu0 bu + pr
 #pragma unroll
   for (int m = 0; m < 5; m++) {
-    const T wf = rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-    const T wa = bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-    w[m] = fast ? wf : wa;
+    if (MODE == 1) w[m] = rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+    else if (MODE == 2) w[m] = bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+    else {
+      const T wf = rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+      const T wa = bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+      w[m] = fast ? wf : wa;
+    }
   }
   interp5<ODR>(w, wl[2], wr[2]);
   // k = 3..6: Alfven pairs (both)
@@ -215,18 +235,28 @@ __device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)
   for (int side = 0; side < 2; side++) {
     const T* q = side ? wr : wl;
     T* u = side ? vr : vl;
-    const T rof = icf2h * q[0] + icf2 * q[1] + icf2h * q[2];
-    const T roa = icf2 * q[1] + icf2 * q[2];
-    const T vnf = -ircfh * q[0] + ircfh * q[2];
-    const T prf = T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
-    const T pra = q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
-    u[0] = fast ? rof : roa;
-    u[1] = fast ? vnf : q[0];
+    if (MODE == 1) {
+      u[0] = icf2h * q[0] + icf2 * q[1] + icf2h * q[2];
+      u[1] = -ircfh * q[0] + ircfh * q[2];
+      u[6] = T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
+    } else if (MODE == 2) {
+      u[0] = icf2 * q[1] + icf2 * q[2];
+      u[1] = q[0];
+      u[6] = q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
+    } else {
+      const T rof = icf2h * q[0] + icf2 * q[1] + icf2h * q[2];
+      const T roa = icf2 * q[1] + icf2 * q[2];
+      const T vnf = -ircfh * q[0] + ircfh * q[2];
+      const T prf = T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
+      const T pra = q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
+      u[0] = fast ? rof : roa;
+      u[1] = fast ? vnf : q[0];
+      u[6] = fast ? prf : pra;
+    }
     u[2] = isroh * q[3] + isroh * q[4];
     u[3] = isroh * q[5] + isroh * q[6];
     u[4] = T(0.5) * q[3] - T(0.5) * q[4];
     u[5] = T(0.5) * q[5] - T(0.5) * q[6];
-    u[6] = fast ? prf : pra;
   }
 #pragma unroll
   for (int n = 0; n < 5; n++) {
@@ -253,7 +283,7 @@ __device__ __forceinline__ void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)
 // caller passes the 5-point products bt*vn, the vt stencil and the two
 // staggered b_n values at i-1/2 (bnm) and i+1/2 (bnp).
 template <int ODR, typename T>
-__device__ __forceinline__ void lr_fb(const T* p, const T* q, T bnm, T bnp, T& vl, T& vr)
+QAS_HD void lr_fb(const T* p, const T* q, T bnm, T bnp, T& vl, T& vr)
 {
   T fl0, fr0, fl1, fr1;
   lin5<ODR>(p, fl0, fr0);
@@ -267,7 +297,7 @@ __device__ __forceinline__ void lr_fb(const T* p, const T* q, T bnm, T bnp, T& v
 #define QAS_FL_EPS 1e-8
 
 template <typename T>
-__device__ __forceinline__ void flux_hlld_family(const bool lowdiss, const T* L, const T* R, T bnc, T gamma,
+QAS_HD void flux_hlld_family(const bool lowdiss, const T* L, const T* R, T bnc, T gamma,
                                                  T dv0, T dv1, T* F)
 {
   const T FL_EPS = T(QAS_FL_EPS);
@@ -387,7 +417,7 @@ __device__ __forceinline__ void flux_hlld_family(const bool lowdiss, const T* L,
 }
 
 template <typename T>
-__device__ __forceinline__ void flux_mlau(const T* L, const T* R, T bnc, T gamma, T dv0, T dv1, T* F)
+QAS_HD void flux_mlau(const T* L, const T* R, T bnc, T gamma, T dv0, T dv1, T* F)
 {
   const T FL_EPS = T(QAS_FL_EPS);
   const T rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
@@ -550,7 +580,7 @@ __device__ __forceinline__ void flux_mlau(const T* L, const T* R, T bnc, T gamma
 }
 
 template <typename T>
-__device__ __forceinline__ void flux_roe(const T* L, const T* R, T bnc, T gamma, T* F)
+QAS_HD void flux_roe(const T* L, const T* R, T bnc, T gamma, T* F)
 {
   const T FL_EPS = T(QAS_FL_EPS);
   const T rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
@@ -731,7 +761,7 @@ __device__ __forceinline__ void flux_roe(const T* L, const T* R, T bnc, T gamma,
 
 // riemann[RMN] (common/mhd_class.hpp:102-105)
 template <int RMN, typename T>
-__device__ __forceinline__ void riemann(const T* L, const T* R, T bn, T gam, T dv0, T dv1, T* F)
+QAS_HD void riemann(const T* L, const T* R, T bn, T gam, T dv0, T dv1, T* F)
 {
   if constexpr (RMN == 0) flux_roe(L, R, bn, gam, F);
   else if constexpr (RMN == 1) flux_hlld_family(false, L, R, bn, gam, dv0, dv1, F);
@@ -741,7 +771,7 @@ __device__ __forceinline__ void riemann(const T* L, const T* R, T bn, T gam, T d
 
 // mhd_cuct_weight from the corner's four cell means and two face means
 template <typename T>
-__device__ __forceinline__ T cuct_weight(T ro0, T ro1, T ro2, T ro3, T va0, T va1, T va2, T va3, T vb0, T vb1,
+QAS_HD T cuct_weight(T ro0, T ro1, T ro2, T ro3, T va0, T va1, T va2, T va3, T vb0, T vb1,
                                          T vb2, T vb3, T ba1, T ba3, T bb2, T bb3)
 {
   const T eps = T(1e-12);
@@ -757,7 +787,7 @@ __device__ __forceinline__ T cuct_weight(T ro0, T ro1, T ro2, T ro3, T va0, T va
 }
 
 // rk_updt (common/common_func.h:28-32)
-__device__ __forceinline__ double rk_upd(double f1, double f0, double df, double a, double b)
+QAS_HD double rk_upd(double f1, double f0, double df, double a, double b)
 {
   return a * f0 + b * (f1 + df);
 }

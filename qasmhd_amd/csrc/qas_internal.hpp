@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 
 #include "../../include/qasmhd.h"
 
@@ -106,7 +107,36 @@ struct qas_ctx {
   bool bounded = false;
   long long nstep = 0;                // running step counter (exec_'s n)
   std::string err;
+  // timing hooks (qas_prof_enable)
+  unsigned prof_mask = 0;
+  std::vector<hipEvent_t> prof_ev;    // start/stop pairs
+  std::vector<int> prof_kid;
+  size_t prof_n = 0;
 };
+
+namespace qas {
+inline void prof_begin(qas_ctx* c, int kid)
+{
+  if ((c->prof_mask >> kid) & 1u)
+    if (c->prof_n < c->prof_kid.size()) (void)hipEventRecord(c->prof_ev[2 * c->prof_n], c->stream);
+}
+inline void prof_end(qas_ctx* c, int kid)
+{
+  if ((c->prof_mask >> kid) & 1u)
+    if (c->prof_n < c->prof_kid.size()) {
+      (void)hipEventRecord(c->prof_ev[2 * c->prof_n + 1], c->stream);
+      c->prof_kid[c->prof_n++] = kid;
+    }
+}
+}  // namespace qas
+
+// launch a kernel between the timing hooks of kernel id KID
+#define QAS_LAUNCH(c, KID, ...)  \
+  do {                            \
+    qas::prof_begin((c), (KID));  \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+    qas::prof_end((c), (KID));    \
+  } while (0)
 
 // ---- launchers (defined in the per-dimension translation units) ----
 namespace qas {
@@ -117,4 +147,6 @@ hipError_t launch_set_dt(qas_ctx* c, double dt);
 hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
+// FP32 face sweeps of the mixed path (qas_mixed.hip)
+void (*pick_sweep_f32(int ndim, int dir, int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
 }  // namespace qas

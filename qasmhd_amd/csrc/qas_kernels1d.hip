@@ -98,11 +98,11 @@ static sweep1_fn pick1(int rmn, int odr)
 hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p)
 {
   const Geo& g = c->geo;
-  hipLaunchKernelGGL(k_prim1d, (unsigned)((g.nd + 255) / 256), 256, 0, c->stream, p, g, c->s.gam);
+  QAS_LAUNCH(c, QAS_K_PRIM, k_prim1d, (unsigned)((g.nd + 255) / 256), 256, 0, c->stream, p, g, c->s.gam);
   const int nseg = (g.nx - 5 + SEG - 1) / SEG;
   sweep1_fn f = pick1(c->s.rmn, c->s.odr);
   if (!f) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(f, (unsigned)((nseg * WAVE + 255) / 256), 256, 0, c->stream, p, g, c->s.gam, nseg);
+  QAS_LAUNCH(c, QAS_K_SWEEP_X, f, (unsigned)((nseg * WAVE + 255) / 256), 256, 0, c->stream, p, g, c->s.gam, nseg);
   return hipGetLastError();
 }
 

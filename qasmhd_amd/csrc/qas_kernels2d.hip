@@ -19,6 +19,7 @@
 // order (the library is built with -ffp-contract=off).
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
+#include "qas_sweep2d.hpp"
 
 namespace qas {
 
@@ -65,151 +66,7 @@ __global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, 
   p.cy[t] = cy;
 }
 
-// shock sensor dv of mhd2d_solve.cpp:105-125 at (i, j) after its bound():
-// DIR 0 -> dvx (flags -dnxs[1], dnys[1]); DIR 1 -> dvy (dnxs[2], -dnys[2])
-template <int DIR>
-__device__ __forceinline__ double dv_at(const double* v, int i, int j, const Geo& g, const BcFlags& bc)
-{
-  const int dnx = DIR == 0 ? -bc.dn[0][1] : bc.dn[0][2];
-  const int dny = DIR == 0 ? bc.dn[1][1] : -bc.dn[1][2];
-  const Src sx = resolve(i, g.nx, 0, dnx);
-  const Src sy = resolve(j, g.ny, 0, dny);
-  const long long q = (long long)g.nx * sy.i + sx.i;
-  const long long o = DIR == 0 ? 1 : g.nx;
-  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
-  return apply(sy, apply(sx, r));
-}
-
-__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, WAVE); }
-__device__ __forceinline__ double shfl_dn(double v, int d) { return __shfl_down(v, d, WAVE); }
-
-// ------------------------------------------------------------------ sweeps
-// DIR 0: x faces for every row j; DIR 1: y faces for every column i.
-template <int DIR, int RMN, int ODR>
-__global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
-{
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-  const int nline = DIR == 0 ? g.ny : g.nx;
-  if (wave >= nseg * nline) return;  // whole wavefront exits together
-  const int line = wave / nseg;
-  const int seg = wave - line * nseg;
-  const int n = DIR == 0 ? g.nx : g.ny;       // points along the sweep
-  const long long o = DIR == 0 ? 1 : g.sy;    // stride along the sweep
-  const long long base = DIR == 0 ? g.sy * line : (long long)line;
-  const int f0 = 3 + seg * SEG;
-  const int q = f0 - 2 + lane;                // this lane's cell
-  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
-  const long long sc = base + o * qc;
-
-  // primitive stencil, cyclic order (vn, vt, vu), (bt, bu)
-  const double* vn = DIR == 0 ? p.vx : p.vy;
-  const double* vt = DIR == 0 ? p.vy : p.vz;
-  const double* vu = DIR == 0 ? p.vz : p.vx;
-  const double* bt = DIR == 0 ? p.cy : p.in[6];
-  const double* bu = DIR == 0 ? p.in[6] : p.cx;
-  const double* cn = DIR == 0 ? p.cx : p.cy;
-  const double* bf = DIR == 0 ? p.in[4] : p.in[5];  // staggered normal B
-  double s[7][5];
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    const long long a = sc + (m - 2) * o;
-    s[0][m] = p.in[0][a];
-    s[1][m] = vn[a];
-    s[2][m] = vt[a];
-    s[3][m] = vu[a];
-    s[4][m] = bt[a];
-    s[5][m] = bu[a];
-    s[6][m] = p.pr[a];
-  }
-  double vl[7], vr[7];
-  lrstate<ODR>(s, cn[sc], gam, vl, vr);
-
-  // B-flux face states (mhd_lr_fb): b_t v_n - b_n v_t with the tangential
-  // component that pairs with this sweep (x: By, y: Bx)
-  double ql, qr;
-  {
-    const double* bb = DIR == 0 ? p.cy : p.cx;
-    const double* vv = DIR == 0 ? p.vy : p.vx;
-    double pp[5], qq[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-      const long long a = sc + (m - 2) * o;
-      pp[m] = bb[a] * vn[a];
-      qq[m] = vv[a];
-    }
-    lr_fb<ODR>(pp, qq, bf[sc], bf[sc + o], ql, qr);
-  }
-
-  // flux at face q (between q-1 and q): left state from lane-1
-  double L[7], R[7], F[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) {
-    L[k] = shfl_up1(vl[k]);
-    R[k] = vr[k];
-  }
-  const double qlf = shfl_up1(ql);
-  const long long sf = base + o * (q < 3 ? 3 : (q > n - 3 ? n - 3 : q));  // face index (clamped)
-  const double bnf = 0.5 * (bf[sf] + bf[sf]);
-  double dv0 = 0.0, dv1 = 0.0;
-  if constexpr (RMN >= 2) {
-    const int qq = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
-    dv0 = vn[sf] - vn[sf - o];
-    if (DIR == 0) {
-      dv1 = tmin(dv_at<1>(p.vy, qq - 1, line, g, bc), dv_at<1>(p.vy, qq, line, g, bc));
-    } else {
-      dv1 = tmin(dv_at<0>(p.vx, line, qq - 1, g, bc), dv_at<0>(p.vx, line, qq, g, bc));
-    }
-  }
-  riemann<RMN>(L, R, bnf, gam, dv0, dv1, F);
-  const double fc = 0.5 * (qlf + qr);
-  // split of the B flux: x sweep -> By flux (F[4]); y sweep -> Bx flux (F[5])
-  const double fbu = (DIR == 0 ? F[4] : F[5]) - fc;
-
-  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
-  if (own) {
-    const long long fi = base + o * q;
-    p.emf[DIR * 4 + 0][fi] = fbu;
-    p.emf[DIR * 4 + 1][fi] = fc;
-  }
-
-  // flux divergence of ro, mx, my, mz, bz, en at cell q: faces q-1 .. q+2
-  // x: (F0, F1, F2, F3, F5, F6); y: (F0, F3, F1, F2, F4, F6)
-  double fv[6];
-  if (DIR == 0) {
-    fv[0] = F[0]; fv[1] = F[1]; fv[2] = F[2]; fv[3] = F[3]; fv[4] = F[5]; fv[5] = F[6];
-  } else {
-    fv[0] = F[0]; fv[1] = F[3]; fv[2] = F[1]; fv[3] = F[2]; fv[4] = F[4]; fv[5] = F[6];
-  }
-  double du[6];
-#pragma unroll
-  for (int v = 0; v < 6; v++) {
-    const double fm1 = shfl_up1(fv[v]);
-    const double f1 = shfl_dn(fv[v], 1);
-    const double f2 = shfl_dn(fv[v], 2);
-    du[v] = dfc<ODR>(fm1, fv[v], f1, f2);
-  }
-  const int tl = line;
-  const int ntl = DIR == 0 ? g.ny : g.nx;
-  const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (tl >= NG) && (tl <= ntl - NG - 1);
-  if (!upd) return;
-  const long long c = base + o * q;
-  const double dtv = *p.dt;
-  if (DIR == 0) {
-    const double dtdx = dtv / g.dx;
-#pragma unroll
-    for (int v = 0; v < 6; v++) p.acc[v][c] = -dtdx * du[v];
-  } else {
-    const double dtdy = dtv / g.dy;
-    const int vid[6] = {0, 1, 2, 3, 6, 7};
-#pragma unroll
-    for (int v = 0; v < 6; v++) {
-      const int m = vid[v];
-      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (p.acc[v][c] - dtdy * du[v]), p.a, p.b);
-    }
-  }
-}
-
+// the face sweeps live in qas_sweep2d.hpp (shared with the mixed path)
 // ------------------------------------------------------------------ EMF
 template <int ODR>
 __global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
@@ -293,7 +150,7 @@ template <int DIR>
 static sweep_fn pick_sweep(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return k_sweep2d<DIR, R, O>;
+  if (rmn == R && odr == O) return k_sweep2d<DIR, R, O, double>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
@@ -312,29 +169,30 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   hipStream_t st = c->stream;
   // 1) cell-centred B and primitives
   switch (odr) {
-    case 1: case 2: hipLaunchKernelGGL(k_prim2d<2>, nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
-    default: hipLaunchKernelGGL(k_prim2d<4>, nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
+    case 1: case 2: QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<2>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
+    default: QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
   }
   // 2) x sweep, 3) y sweep
   const int nsx = (g.nx - 5 + SEG - 1) / SEG;
   const int nsy = (g.ny - 5 + SEG - 1) / SEG;
-  sweep_fn fx = pick_sweep<0>(c->s.rmn, odr);
-  sweep_fn fy = pick_sweep<1>(c->s.rmn, odr);
+  const bool mixed = c->s.precision == 1;
+  sweep_fn fx = mixed ? pick_sweep_f32(2, 0, c->s.rmn, odr) : pick_sweep<0>(c->s.rmn, odr);
+  sweep_fn fy = mixed ? pick_sweep_f32(2, 1, c->s.rmn, odr) : pick_sweep<1>(c->s.rmn, odr);
   if (!fx || !fy) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx);
-  hipLaunchKernelGGL(fy, nblk((long long)nsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsy);
+  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx);
+  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, nblk((long long)nsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsy);
   // 4) corner EMF, 5) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
   switch (odr) {
-    case 1: hipLaunchKernelGGL(k_emf2d<1>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            hipLaunchKernelGGL(k_ct2d<1>, nblk(nc, 256), 256, 0, st, p, g); break;
-    case 2: hipLaunchKernelGGL(k_emf2d<2>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            hipLaunchKernelGGL(k_ct2d<2>, nblk(nc, 256), 256, 0, st, p, g); break;
-    case 3: hipLaunchKernelGGL(k_emf2d<3>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            hipLaunchKernelGGL(k_ct2d<3>, nblk(nc, 256), 256, 0, st, p, g); break;
-    default: hipLaunchKernelGGL(k_emf2d<4>, nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-             hipLaunchKernelGGL(k_ct2d<4>, nblk(nc, 256), 256, 0, st, p, g); break;
+    case 1: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<1>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<1>), nblk(nc, 256), 256, 0, st, p, g); break;
+    case 2: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<2>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<2>), nblk(nc, 256), 256, 0, st, p, g); break;
+    case 3: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<3>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<3>), nblk(nc, 256), 256, 0, st, p, g); break;
+    default: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<4>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
+             QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<4>), nblk(nc, 256), 256, 0, st, p, g); break;
   }
   return hipGetLastError();
 }

@@ -16,6 +16,7 @@
 // followed by the bound() kernel (:713-714).
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
+#include "qas_sweep3d.hpp"
 
 namespace qas {
 
@@ -55,172 +56,7 @@ __global__ __launch_bounds__(256) void k_prim3d(StagePtrs p, Geo g, BcFlags bc, 
   p.cz[t] = c3[2];
 }
 
-// dv of axis A (0 x, 1 y, 2 z) at (i, j, k) after its bound()
-// (mhd3d_solve.cpp:138-164): flags (-dnxs[1], dnys[1], dnzs[1]) for dvx,
-// (dnxs[2], -dnys[2], dnzs[2]) for dvy, (dnxs[3], dnys[3], -dnzs[3]) for dvz.
-template <int A>
-__device__ __forceinline__ double dv3(const double* v, int i, int j, int k, const Geo& g, const BcFlags& bc)
-{
-  const int dnx = A == 0 ? -bc.dn[0][1] : bc.dn[0][1 + A];
-  const int dny = A == 1 ? -bc.dn[1][2] : bc.dn[1][1 + A];
-  const int dnz = A == 2 ? -bc.dn[2][3] : bc.dn[2][1 + A];
-  const Src sx = resolve(i, g.nx, 0, dnx);
-  const Src sy = resolve(j, g.ny, 0, dny);
-  const Src sz = resolve(k, g.nz, 0, dnz);
-  const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
-  const long long o = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
-  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
-  return apply(sz, apply(sy, apply(sx, r)));
-}
-
-template <int A>
-__device__ __forceinline__ double dv3_face(const StagePtrs& p, int i, int j, int k, const Geo& g,
-                                           const BcFlags& bc)
-{
-  const double* v = A == 0 ? p.vx : (A == 1 ? p.vy : p.vz);
-  return dv3<A>(v, i, j, k, g, bc);
-}
-
-// ------------------------------------------------------------------ sweeps
-template <int DIR, int RMN, int ODR>
-__global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
-{
-  constexpr int TA = (DIR + 1) % 3;  // first transverse axis (t)
-  constexpr int TB = (DIR + 2) % 3;  // second transverse axis (u)
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-  const int dims[3] = {g.nx, g.ny, g.nz};
-  const long long strd[3] = {1, g.sy, g.sz};
-  // lines: x -> (j, k); y -> (i, k); z -> (i, j); the faster line index is
-  // the lower axis for coalescing
-  constexpr int LA = DIR == 0 ? 1 : 0;  // faster transverse axis
-  constexpr int LB = DIR == 2 ? 1 : 2;  // slower transverse axis
-  const int nla = dims[LA], nlb = dims[LB];
-  const long long nline = (long long)nla * nlb;
-  if ((long long)wave >= nline * nseg) return;
-  const long long line = wave / nseg;
-  const int seg = (int)(wave - line * nseg);
-  const int la = (int)(line % nla), lb = (int)(line / nla);
-  const int n = dims[DIR];
-  const long long o = strd[DIR];
-  const long long base = strd[LA] * la + strd[LB] * lb;
-  const int f0 = 3 + seg * SEG;
-  const int q = f0 - 2 + lane;
-  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
-  const long long sc = base + o * qc;
-
-  const double* vax[3] = {p.vx, p.vy, p.vz};
-  const double* cax[3] = {p.cx, p.cy, p.cz};
-  const double* vn = vax[DIR];
-  const double* vt = vax[TA];
-  const double* vu = vax[TB];
-  const double* bt = cax[TA];
-  const double* bu = cax[TB];
-  const double* bf = p.in[4 + DIR];
-  double s[7][5];
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    const long long a = sc + (m - 2) * o;
-    s[0][m] = p.in[0][a];
-    s[1][m] = vn[a];
-    s[2][m] = vt[a];
-    s[3][m] = vu[a];
-    s[4][m] = bt[a];
-    s[5][m] = bu[a];
-    s[6][m] = p.pr[a];
-  }
-  double vl[7], vr[7];
-  lrstate<ODR>(s, cax[DIR][sc], gam, vl, vr);
-
-  // B-flux face states for both tangential components (mhd_lr_fb)
-  double qlt, qrt, qlu, qru;
-  {
-    double pp[5], qq[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-      const long long a = sc + (m - 2) * o;
-      pp[m] = bt[a] * vn[a];
-      qq[m] = vt[a];
-    }
-    lr_fb<ODR>(pp, qq, bf[sc], bf[sc + o], qlt, qrt);
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-      const long long a = sc + (m - 2) * o;
-      pp[m] = bu[a] * vn[a];
-      qq[m] = vu[a];
-    }
-    lr_fb<ODR>(pp, qq, bf[sc], bf[sc + o], qlu, qru);
-  }
-
-  double L[7], F[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
-  const double qlt_f = __shfl_up(qlt, 1, WAVE);
-  const double qlu_f = __shfl_up(qlu, 1, WAVE);
-  const int qf = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
-  const long long sf = base + o * qf;
-  const double bnf = 0.5 * (bf[sf] + bf[sf]);
-  double dv0 = 0.0, dv1 = 0.0;
-  if constexpr (RMN >= 2) {
-    dv0 = vn[sf] - vn[sf - o];
-    int c0[3], c1[3];
-    c0[LA] = la; c0[LB] = lb; c0[DIR] = qf - 1;
-    c1[LA] = la; c1[LB] = lb; c1[DIR] = qf;
-    const double a0 = dv3_face<TA>(p, c0[0], c0[1], c0[2], g, bc);
-    const double a1 = dv3_face<TA>(p, c1[0], c1[1], c1[2], g, bc);
-    const double b0 = dv3_face<TB>(p, c0[0], c0[1], c0[2], g, bc);
-    const double b1 = dv3_face<TB>(p, c1[0], c1[1], c1[2], g, bc);
-    dv1 = tmin(tmin(a0, a1), tmin(b0, b1));
-  }
-  riemann<RMN>(L, vr, bnf, gam, dv0, dv1, F);
-  const double fct = 0.5 * (qlt_f + qrt);
-  const double fcu = 0.5 * (qlu_f + qru);
-  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
-  if (own) {
-    const long long fi = base + o * q;
-    p.emf[DIR * 4 + 0][fi] = F[4] - fct;  // B_t flux, upwind part
-    p.emf[DIR * 4 + 1][fi] = fct;         // B_t flux, central part
-    p.emf[DIR * 4 + 2][fi] = F[5] - fcu;  // B_u flux, upwind part
-    p.emf[DIR * 4 + 3][fi] = fcu;         // B_u flux, central part
-  }
-  // divergence of ro, mx, my, mz, en: (ro, mn, mt, mu) permuted back
-  double fv[5];
-  fv[0] = F[0];
-  fv[1 + DIR] = F[1];
-  fv[1 + TA] = F[2];
-  fv[1 + TB] = F[3];
-  fv[4] = F[6];
-  double du[5];
-#pragma unroll
-  for (int v = 0; v < 5; v++) {
-    const double fm1 = __shfl_up(fv[v], 1, WAVE);
-    const double f1 = __shfl_down(fv[v], 1, WAVE);
-    const double f2 = __shfl_down(fv[v], 2, WAVE);
-    du[v] = dfc<ODR>(fm1, fv[v], f1, f2);
-  }
-  const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (la >= NG) && (la <= nla - NG - 1) && (lb >= NG) &&
-                   (lb <= nlb - NG - 1);
-  if (!upd) return;
-  const long long c = base + o * q;
-  const double dtv = *p.dt;
-  const int vid[5] = {0, 1, 2, 3, 7};
-  if (DIR == 0) {
-    const double dtdx = dtv / g.dx;
-#pragma unroll
-    for (int v = 0; v < 5; v++) p.acc[v][c] = -dtdx * du[v];
-  } else if (DIR == 1) {
-    const double dtdy = dtv / g.dy;
-#pragma unroll
-    for (int v = 0; v < 5; v++) p.acc[v][c] = p.acc[v][c] - dtdy * du[v];
-  } else {
-    const double dtdz = dtv / g.dz;
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-      const int m = vid[v];
-      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], p.acc[v][c] - dtdz * du[v], p.a, p.b);
-    }
-  }
-}
+// the face sweeps live in qas_sweep3d.hpp (shared with the mixed path)
 
 // ------------------------------------------------------------------ EMF
 // One contribution: interpolate the face parts (upwind fu, central fc) of a
@@ -329,7 +165,7 @@ template <int DIR>
 static sweep3_fn pick3(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return k_sweep3d<DIR, R, O>;
+  if (rmn == R && odr == O) return k_sweep3d<DIR, R, O, double>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
@@ -346,27 +182,29 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   const int odr = c->s.odr;
   const double gam = c->s.gam;
   hipStream_t st = c->stream;
-  if (odr <= 2) hipLaunchKernelGGL(k_prim3d<2>, nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-  else hipLaunchKernelGGL(k_prim3d<4>, nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  if (odr <= 2) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<2>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<4>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
   const int dims[3] = {g.nx, g.ny, g.nz};
   sweep3_fn f[3] = {pick3<0>(c->s.rmn, odr), pick3<1>(c->s.rmn, odr), pick3<2>(c->s.rmn, odr)};
+  if (c->s.precision == 1)
+    for (int d = 0; d < 3; d++) f[d] = pick_sweep_f32(3, d, c->s.rmn, odr);
   for (int d = 0; d < 3; d++) {
     if (!f[d]) return hipErrorInvalidValue;
     const int nseg = (dims[d] - 5 + SEG - 1) / SEG;
     const long long nline = g.nd / dims[d];
-    hipLaunchKernelGGL(f[d], nblk3(nline * nseg * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nseg);
+    QAS_LAUNCH(c, QAS_K_SWEEP_X + d, f[d], nblk3(nline * nseg * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nseg);
   }
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5) * (g.nz - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7) * (g.nz - 7);
   switch (odr) {
-    case 1: hipLaunchKernelGGL(k_emf3d<1>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            hipLaunchKernelGGL(k_ct3d<1>, nblk3(nc, 256), 256, 0, st, p, g); break;
-    case 2: hipLaunchKernelGGL(k_emf3d<2>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            hipLaunchKernelGGL(k_ct3d<2>, nblk3(nc, 256), 256, 0, st, p, g); break;
-    case 3: hipLaunchKernelGGL(k_emf3d<3>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            hipLaunchKernelGGL(k_ct3d<3>, nblk3(nc, 256), 256, 0, st, p, g); break;
-    default: hipLaunchKernelGGL(k_emf3d<4>, nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-             hipLaunchKernelGGL(k_ct3d<4>, nblk3(nc, 256), 256, 0, st, p, g); break;
+    case 1: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<1>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<1>), nblk3(nc, 256), 256, 0, st, p, g); break;
+    case 2: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<2>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<2>), nblk3(nc, 256), 256, 0, st, p, g); break;
+    case 3: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<3>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+            QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<3>), nblk3(nc, 256), 256, 0, st, p, g); break;
+    default: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<4>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
+             QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<4>), nblk3(nc, 256), 256, 0, st, p, g); break;
   }
   return hipGetLastError();
 }

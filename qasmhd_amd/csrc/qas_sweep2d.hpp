@@ -1,0 +1,158 @@
+// qas_sweep2d.hpp -- the 2D face sweep kernel, templated on the precision T
+// of the reconstruction and Riemann solver (double: the FP64 path, bit-
+// identical to the reference; float: the mixed path).  Included by
+// qas_kernels2d.hip (T = double, -ffp-contract=off) and qas_mixed.hip
+// (T = float).  Lane layout: one 64-lane wavefront per 64 consecutive cells
+// of a line; lane l reconstructs cell f0-2+l, neighbour states and fluxes
+// move between lanes by cross-lane shuffles, and the wavefront owns the 60
+// faces/cells [f0, f0+60).
+#pragma once
+#include "qas_device.hpp"
+#include "qas_internal.hpp"
+
+namespace qas {
+
+// shock sensor dv of mhd2d_solve.cpp:105-125 at (i, j) after its bound():
+// DIR 0 -> dvx (flags -dnxs[1], dnys[1]); DIR 1 -> dvy (dnxs[2], -dnys[2])
+template <int DIR>
+__device__ __forceinline__ double dv_at(const double* v, int i, int j, const Geo& g, const BcFlags& bc)
+{
+  const int dnx = DIR == 0 ? -bc.dn[0][1] : bc.dn[0][2];
+  const int dny = DIR == 0 ? bc.dn[1][1] : -bc.dn[1][2];
+  const Src sx = resolve(i, g.nx, 0, dnx);
+  const Src sy = resolve(j, g.ny, 0, dny);
+  const long long q = (long long)g.nx * sy.i + sx.i;
+  const long long o = DIR == 0 ? 1 : g.nx;
+  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
+  return apply(sy, apply(sx, r));
+}
+
+// ------------------------------------------------------------------ sweeps
+// DIR 0: x faces for every row j; DIR 1: y faces for every column i.
+template <int DIR, int RMN, int ODR, typename T>
+__global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+{
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int nline = DIR == 0 ? g.ny : g.nx;
+  if (wave >= nseg * nline) return;  // whole wavefront exits together
+  const int line = wave / nseg;
+  const int seg = wave - line * nseg;
+  const int n = DIR == 0 ? g.nx : g.ny;       // points along the sweep
+  const long long o = DIR == 0 ? 1 : g.sy;    // stride along the sweep
+  const long long base = DIR == 0 ? g.sy * line : (long long)line;
+  const int f0 = 3 + seg * SEG;
+  const int q = f0 - 2 + lane;                // this lane's cell
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const long long sc = base + o * qc;
+
+  // primitive stencil, cyclic order (vn, vt, vu), (bt, bu)
+  const double* vn = DIR == 0 ? p.vx : p.vy;
+  const double* vt = DIR == 0 ? p.vy : p.vz;
+  const double* vu = DIR == 0 ? p.vz : p.vx;
+  const double* bt = DIR == 0 ? p.cy : p.in[6];
+  const double* bu = DIR == 0 ? p.in[6] : p.cx;
+  const double* cn = DIR == 0 ? p.cx : p.cy;
+  const double* bf = DIR == 0 ? p.in[4] : p.in[5];  // staggered normal B
+  T s[7][5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long long a = sc + (m - 2) * o;
+    s[0][m] = (T)p.in[0][a];
+    s[1][m] = (T)vn[a];
+    s[2][m] = (T)vt[a];
+    s[3][m] = (T)vu[a];
+    s[4][m] = (T)bt[a];
+    s[5][m] = (T)bu[a];
+    s[6][m] = (T)p.pr[a];
+  }
+  T vl[7], vr[7];
+  lrstate<ODR>(s, (T)cn[sc], (T)gam, vl, vr);
+
+  // B-flux face states (mhd_lr_fb): b_t v_n - b_n v_t with the tangential
+  // component that pairs with this sweep (x: By, y: Bx)
+  T ql, qr;
+  {
+    const double* bb = DIR == 0 ? p.cy : p.cx;
+    const double* vv = DIR == 0 ? p.vy : p.vx;
+    T pp[5], qq[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      pp[m] = (T)bb[a] * (T)vn[a];
+      qq[m] = (T)vv[a];
+    }
+    lr_fb<ODR>(pp, qq, (T)bf[sc], (T)bf[sc + o], ql, qr);
+  }
+
+  // flux at face q (between q-1 and q): left state from lane-1
+  T L[7], R[7], F[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    L[k] = __shfl_up(vl[k], 1, WAVE);
+    R[k] = vr[k];
+  }
+  const T qlf = __shfl_up(ql, 1, WAVE);
+  const long long sf = base + o * (q < 3 ? 3 : (q > n - 3 ? n - 3 : q));  // face index (clamped)
+  const T bnf = (T)(0.5 * (bf[sf] + bf[sf]));
+  T dv0 = T(0.0), dv1 = T(0.0);
+  if constexpr (RMN >= 2) {
+    const int qq = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
+    dv0 = (T)(vn[sf] - vn[sf - o]);
+    if (DIR == 0) {
+      dv1 = (T)tmin(dv_at<1>(p.vy, qq - 1, line, g, bc), dv_at<1>(p.vy, qq, line, g, bc));
+    } else {
+      dv1 = (T)tmin(dv_at<0>(p.vx, line, qq - 1, g, bc), dv_at<0>(p.vx, line, qq, g, bc));
+    }
+  }
+  riemann<RMN>(L, R, bnf, (T)gam, dv0, dv1, F);
+  const T fc = T(0.5) * (qlf + qr);
+  // split of the B flux: x sweep -> By flux (F[4]); y sweep -> Bx flux (F[5])
+  const T fbu = (DIR == 0 ? F[4] : F[5]) - fc;
+
+  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
+  if (own) {
+    const long long fi = base + o * q;
+    p.emf[DIR * 4 + 0][fi] = (double)fbu;
+    p.emf[DIR * 4 + 1][fi] = (double)fc;
+  }
+
+  // flux divergence of ro, mx, my, mz, bz, en at cell q: faces q-1 .. q+2
+  // x: (F0, F1, F2, F3, F5, F6); y: (F0, F3, F1, F2, F4, F6)
+  // (fluxes are differenced in FP64 in both precisions)
+  double fv[6];
+  if (DIR == 0) {
+    fv[0] = F[0]; fv[1] = F[1]; fv[2] = F[2]; fv[3] = F[3]; fv[4] = F[5]; fv[5] = F[6];
+  } else {
+    fv[0] = F[0]; fv[1] = F[3]; fv[2] = F[1]; fv[3] = F[2]; fv[4] = F[4]; fv[5] = F[6];
+  }
+  double du[6];
+#pragma unroll
+  for (int v = 0; v < 6; v++) {
+    const double fm1 = __shfl_up(fv[v], 1, WAVE);
+    const double f1 = __shfl_down(fv[v], 1, WAVE);
+    const double f2 = __shfl_down(fv[v], 2, WAVE);
+    du[v] = dfc<ODR>(fm1, fv[v], f1, f2);
+  }
+  const int tl = line;
+  const int ntl = DIR == 0 ? g.ny : g.nx;
+  const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (tl >= NG) && (tl <= ntl - NG - 1);
+  if (!upd) return;
+  const long long c = base + o * q;
+  const double dtv = *p.dt;
+  if (DIR == 0) {
+    const double dtdx = dtv / g.dx;
+#pragma unroll
+    for (int v = 0; v < 6; v++) p.acc[v][c] = -dtdx * du[v];
+  } else {
+    const double dtdy = dtv / g.dy;
+    const int vid[6] = {0, 1, 2, 3, 6, 7};
+#pragma unroll
+    for (int v = 0; v < 6; v++) {
+      const int m = vid[v];
+      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (p.acc[v][c] - dtdy * du[v]), p.a, p.b);
+    }
+  }
+}
+
+}  // namespace qas

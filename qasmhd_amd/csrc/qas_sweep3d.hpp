@@ -1,0 +1,177 @@
+// qas_sweep3d.hpp -- the 3D face sweep kernel, templated on the precision T
+// of the reconstruction and Riemann solver (double: FP64 path; float: mixed
+// path).  Included by qas_kernels3d.hip and qas_mixed.hip.
+#pragma once
+#include "qas_device.hpp"
+#include "qas_internal.hpp"
+
+namespace qas {
+
+// dv of axis A (0 x, 1 y, 2 z) at (i, j, k) after its bound()
+// (mhd3d_solve.cpp:138-164): flags (-dnxs[1], dnys[1], dnzs[1]) for dvx,
+// (dnxs[2], -dnys[2], dnzs[2]) for dvy, (dnxs[3], dnys[3], -dnzs[3]) for dvz.
+template <int A>
+__device__ __forceinline__ double dv3(const double* v, int i, int j, int k, const Geo& g, const BcFlags& bc)
+{
+  const int dnx = A == 0 ? -bc.dn[0][1] : bc.dn[0][1 + A];
+  const int dny = A == 1 ? -bc.dn[1][2] : bc.dn[1][1 + A];
+  const int dnz = A == 2 ? -bc.dn[2][3] : bc.dn[2][1 + A];
+  const Src sx = resolve(i, g.nx, 0, dnx);
+  const Src sy = resolve(j, g.ny, 0, dny);
+  const Src sz = resolve(k, g.nz, 0, dnz);
+  const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
+  const long long o = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
+  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
+  return apply(sz, apply(sy, apply(sx, r)));
+}
+
+template <int A>
+__device__ __forceinline__ double dv3_face(const StagePtrs& p, int i, int j, int k, const Geo& g,
+                                           const BcFlags& bc)
+{
+  const double* v = A == 0 ? p.vx : (A == 1 ? p.vy : p.vz);
+  return dv3<A>(v, i, j, k, g, bc);
+}
+
+// ------------------------------------------------------------------ sweeps
+template <int DIR, int RMN, int ODR, typename T>
+__global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+{
+  constexpr int TA = (DIR + 1) % 3;  // first transverse axis (t)
+  constexpr int TB = (DIR + 2) % 3;  // second transverse axis (u)
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  const long long strd[3] = {1, g.sy, g.sz};
+  // lines: x -> (j, k); y -> (i, k); z -> (i, j); the faster line index is
+  // the lower axis for coalescing
+  constexpr int LA = DIR == 0 ? 1 : 0;  // faster transverse axis
+  constexpr int LB = DIR == 2 ? 1 : 2;  // slower transverse axis
+  const int nla = dims[LA], nlb = dims[LB];
+  const long long nline = (long long)nla * nlb;
+  if ((long long)wave >= nline * nseg) return;
+  const long long line = wave / nseg;
+  const int seg = (int)(wave - line * nseg);
+  const int la = (int)(line % nla), lb = (int)(line / nla);
+  const int n = dims[DIR];
+  const long long o = strd[DIR];
+  const long long base = strd[LA] * la + strd[LB] * lb;
+  const int f0 = 3 + seg * SEG;
+  const int q = f0 - 2 + lane;
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const long long sc = base + o * qc;
+
+  const double* vax[3] = {p.vx, p.vy, p.vz};
+  const double* cax[3] = {p.cx, p.cy, p.cz};
+  const double* vn = vax[DIR];
+  const double* vt = vax[TA];
+  const double* vu = vax[TB];
+  const double* bt = cax[TA];
+  const double* bu = cax[TB];
+  const double* bf = p.in[4 + DIR];
+  T s[7][5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long long a = sc + (m - 2) * o;
+    s[0][m] = (T)p.in[0][a];
+    s[1][m] = (T)vn[a];
+    s[2][m] = (T)vt[a];
+    s[3][m] = (T)vu[a];
+    s[4][m] = (T)bt[a];
+    s[5][m] = (T)bu[a];
+    s[6][m] = (T)p.pr[a];
+  }
+  T vl[7], vr[7];
+  lrstate<ODR>(s, (T)cax[DIR][sc], (T)gam, vl, vr);
+
+  // B-flux face states for both tangential components (mhd_lr_fb)
+  T qlt, qrt, qlu, qru;
+  {
+    T pp[5], qq[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      pp[m] = (T)bt[a] * (T)vn[a];
+      qq[m] = (T)vt[a];
+    }
+    lr_fb<ODR>(pp, qq, (T)bf[sc], (T)bf[sc + o], qlt, qrt);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      pp[m] = (T)bu[a] * (T)vn[a];
+      qq[m] = (T)vu[a];
+    }
+    lr_fb<ODR>(pp, qq, (T)bf[sc], (T)bf[sc + o], qlu, qru);
+  }
+
+  T L[7], F[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
+  const T qlt_f = __shfl_up(qlt, 1, WAVE);
+  const T qlu_f = __shfl_up(qlu, 1, WAVE);
+  const int qf = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
+  const long long sf = base + o * qf;
+  const T bnf = (T)(0.5 * (bf[sf] + bf[sf]));
+  T dv0 = T(0.0), dv1 = T(0.0);
+  if constexpr (RMN >= 2) {
+    dv0 = (T)(vn[sf] - vn[sf - o]);
+    int c0[3], c1[3];
+    c0[LA] = la; c0[LB] = lb; c0[DIR] = qf - 1;
+    c1[LA] = la; c1[LB] = lb; c1[DIR] = qf;
+    const double a0 = dv3_face<TA>(p, c0[0], c0[1], c0[2], g, bc);
+    const double a1 = dv3_face<TA>(p, c1[0], c1[1], c1[2], g, bc);
+    const double b0 = dv3_face<TB>(p, c0[0], c0[1], c0[2], g, bc);
+    const double b1 = dv3_face<TB>(p, c1[0], c1[1], c1[2], g, bc);
+    dv1 = (T)tmin(tmin(a0, a1), tmin(b0, b1));
+  }
+  riemann<RMN>(L, vr, bnf, (T)gam, dv0, dv1, F);
+  const T fct = T(0.5) * (qlt_f + qrt);
+  const T fcu = T(0.5) * (qlu_f + qru);
+  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
+  if (own) {
+    const long long fi = base + o * q;
+    p.emf[DIR * 4 + 0][fi] = (double)(F[4] - fct);  // B_t flux, upwind part
+    p.emf[DIR * 4 + 1][fi] = (double)fct;           // B_t flux, central part
+    p.emf[DIR * 4 + 2][fi] = (double)(F[5] - fcu);  // B_u flux, upwind part
+    p.emf[DIR * 4 + 3][fi] = (double)fcu;           // B_u flux, central part
+  }
+  // divergence of ro, mx, my, mz, en: (ro, mn, mt, mu) permuted back
+  double fv[5];
+  fv[0] = F[0];
+  fv[1 + DIR] = F[1];
+  fv[1 + TA] = F[2];
+  fv[1 + TB] = F[3];
+  fv[4] = F[6];
+  double du[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    const double fm1 = __shfl_up(fv[v], 1, WAVE);
+    const double f1 = __shfl_down(fv[v], 1, WAVE);
+    const double f2 = __shfl_down(fv[v], 2, WAVE);
+    du[v] = dfc<ODR>(fm1, fv[v], f1, f2);
+  }
+  const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (la >= NG) && (la <= nla - NG - 1) && (lb >= NG) &&
+                   (lb <= nlb - NG - 1);
+  if (!upd) return;
+  const long long c = base + o * q;
+  const double dtv = *p.dt;
+  const int vid[5] = {0, 1, 2, 3, 7};
+  if (DIR == 0) {
+    const double dtdx = dtv / g.dx;
+#pragma unroll
+    for (int v = 0; v < 5; v++) p.acc[v][c] = -dtdx * du[v];
+  } else if (DIR == 1) {
+    const double dtdy = dtv / g.dy;
+#pragma unroll
+    for (int v = 0; v < 5; v++) p.acc[v][c] = p.acc[v][c] - dtdy * du[v];
+  } else {
+    const double dtdz = dtv / g.dz;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const int m = vid[v];
+      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], p.acc[v][c] - dtdz * du[v], p.a, p.b);
+    }
+  }
+}
+
+}  // namespace qas

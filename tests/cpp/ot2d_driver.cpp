@@ -1,0 +1,85 @@
+// ot2d_driver.cpp -- a 2D problem written against qasmhd_driver.hpp the way
+// a reference problem directory is written against MHD2D: paras() sets the
+// domain and boundary flags (cf. 2D/SERIAL/OTvortex/mhd2d_paras.cpp), init_()
+// the Orszag-Tang state (cf. mhd2d_init_.cpp), main() calls setdt(1) and
+// exec_(1).  It dumps the initial and final FP64 states so a test can rerun
+// the oracle on the same initial data.
+//
+// usage: ot2d_driver <xmesh> <ymesh> <nsteps> <rmn> <init.bin> <final.bin>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "qasmhd_driver.hpp"
+
+class OTVortex : public qasmhd::MHD2DDevice {
+ public:
+  OTVortex(int xm, int ym, int nsteps) : MHD2DDevice(xm, ym, 1, 1e6, 0.4), steps_(nsteps) {}
+  void paras()
+  {
+    xmin = 0.0;
+    xmax = 2.0 * M_PI;
+    ymin = 0.0;
+    ymax = 2.0 * M_PI;
+    dx = (xmax - xmin) / (nx - 2 * xoff);
+    dy = (ymax - ymin) / (ny - 2 * yoff);
+    dr = dx < dy ? dx : dy;
+  }
+  void init_()
+  {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int ss = nx * j + i;
+        const double x = (i - xoff) * dx + xmin, y = (j - yoff) * dy + ymin;
+        const double r = gam * gam, vx = -std::sin(y), vy = std::sin(x), p = gam;
+        const double cx = -std::sin(y), cy = std::sin(2 * x);
+        ro[ss] = r;
+        mx[ss] = r * vx;
+        my[ss] = r * vy;
+        mz[ss] = 0.0;
+        bx[ss] = cx;
+        by[ss] = cy;
+        bz[ss] = 0.0;
+        en[ss] = p / (gam - 1) + 0.5 * (r * (vx * vx + vy * vy) + (cx * cx + cy * cy));
+      }
+  }
+  void run_steps(int flg)
+  {
+    nmax = steps_;
+    exec_(flg);
+    download();
+  }
+  void save(const char* path)
+  {
+    FILE* f = std::fopen(path, "wb");
+    for (int m = 0; m < nm; m++) std::fwrite(val[m], sizeof(double), nd, f);
+    std::fclose(f);
+  }
+  void set_scheme(int r) { rmn = r; }
+
+ private:
+  int steps_;
+};
+
+int main(int argc, char** argv)
+{
+  if (argc < 7) {
+    std::fprintf(stderr, "usage: %s xmesh ymesh nsteps rmn init.bin final.bin\n", argv[0]);
+    return 1;
+  }
+  try {
+    OTVortex ot(std::atoi(argv[1]), std::atoi(argv[2]), std::atoi(argv[3]));
+    ot.set_scheme(std::atoi(argv[4]));
+    ot.paras();
+    ot.init_();
+    ot.save(argv[5]);
+    ot.setdt(1);
+    ot.run_steps(1);
+    ot.save(argv[6]);
+    std::printf("dt %.17g\n", ot.getdt());
+  } catch (const qasmhd::Error& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 2;
+  }
+  return 0;
+}

@@ -164,3 +164,24 @@ def test_3d_vs_oracle(qas, oracle, rmn, odr, bc):
         out = s.download()
     assert np.array_equal(hist, h_ref) and dt_now == dt_ref
     assert np.array_equal(out, ref)
+
+
+def test_cpp_driver_matches_oracle(oracle, tmp_path):
+    """The C++ host driver (include/qasmhd_driver.hpp), used the way a
+    reference problem uses MHD2D, gives the oracle's result bit for bit."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLDEN), "cpp", "_build", "ot2d_driver")
+    if not os.path.exists(exe):
+        pytest.skip("C++ driver not built (run __graft_entry__.build())")
+    init, final = str(tmp_path / "init.bin"), str(tmp_path / "final.bin")
+    _say("cpp driver")
+    r = subprocess.run([exe, "32", "24", "6", "1", init, final], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    shape = (8, 32, 40)
+    st0 = np.fromfile(init).reshape(shape)
+    out = np.fromfile(final).reshape(shape)
+    d = 2 * np.pi / 32, 2 * np.pi / 24
+    cfg = oracle.make_cfg(shape[1:], d[0], d[1])
+    ref = st0.copy()
+    oracle.run(cfg, ref, 6, 1)
+    assert np.array_equal(out, ref)
