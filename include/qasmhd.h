@@ -109,6 +109,9 @@ int qas_cfl_dt(qas_ctx *c, double *dt_out);
 int qas_run(qas_ctx *c, int nsteps, int flg, double *dt_inout, double *dt_hist);
 /* Running step count used by qas_run's n % 2 schedule; reset to 0 here. */
 int qas_reset_counter(qas_ctx *c);
+/* qas_run replays pairs of steps as one captured HIP graph (R_K 2 and 3,
+ * timing hooks off); on = 0 launches every kernel from the host instead. */
+int qas_use_graphs(qas_ctx *c, int on);
 
 /* Block until all work queued on the context has finished. */
 int qas_sync(qas_ctx *c);

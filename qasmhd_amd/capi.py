@@ -35,7 +35,8 @@ class QasScheme(C.Structure):
 
 EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", "qas_upload", "qas_download",
            "qas_upload_packed", "qas_download_packed", "qas_upload_device", "qas_download_device", "qas_bound",
-           "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_sync", "qas_device_bytes"]
+           "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_use_graphs", "qas_sync", "qas_device_bytes",
+           "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops"]
 
 _lib = None
 

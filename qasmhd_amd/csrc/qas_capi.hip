@@ -156,6 +156,7 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
   char* tail = (char*)p;
   c->d_dt = (double*)tail;
   c->d_vmax = (unsigned long long*)(tail + 64);
+  c->d_hcount = (long long*)(tail + 128);
   QAS_CHECK(c, hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
   const double one = 1.0;
   QAS_CHECK(c, hipMemcpyAsync(c->d_vmax, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -170,6 +171,8 @@ extern "C" void qas_destroy(qas_ctx* c)
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->slab) hipFree(c->slab);
   if (c->d_hist) hipFree(c->d_hist);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
   if (c->h_pinned) hipHostFree(c->h_pinned);
   for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -324,39 +327,110 @@ extern "C" int qas_cfl_dt(qas_ctx* c, double* dt_out)
   return QAS_OK;
 }
 
+// One step of exec_'s loop body on the device: [record dt] bound (if the
+// state is not known to be bounded) ideal(dt) [setdt when flg and n even].
+static int loop_step(qas_ctx* c, int flg, bool hist, bool even)
+{
+  if (hist) QAS_CHECK(c, launch_record_counted(c));
+  if (!c->bounded) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+  int r = do_step(c);
+  if (r) return r;
+  if (flg && even) {
+    QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
+    QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
+  }
+  return QAS_OK;
+}
+
+// Capture two consecutive steps (odd, even) as one graph.  The kernels take
+// dt from device memory and the buffers rotate back to the same current
+// buffer after R_K = 2 or 3 stages, so one graph serves every pair.
+static int build_pair_graph(qas_ctx* c, int flg, bool hist)
+{
+  if (c->gexec) { QAS_CHECK(c, hipGraphExecDestroy(c->gexec)); c->gexec = nullptr; }
+  if (c->graph) { QAS_CHECK(c, hipGraphDestroy(c->graph)); c->graph = nullptr; }
+  const int cur0 = c->cur;
+  const bool b0 = c->bounded;
+  QAS_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int r = loop_step(c, flg, hist, false);
+  if (!r) r = loop_step(c, flg, hist, true);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->cur = cur0;  // capture does not execute; restore the host view
+  c->bounded = b0;
+  if (r) {
+    if (g) hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+  c->graph = g;
+  QAS_CHECK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->graph_flg = flg;
+  c->graph_hist = hist ? 1 : 0;
+  c->graph_cur = cur0;
+  return QAS_OK;
+}
+
 extern "C" int qas_run(qas_ctx* c, int nsteps, int flg, double* dt_inout, double* dt_hist)
 {
   if (!ready(c)) return fail(c, QAS_E_ARG, "qas_run: null context");
   if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_run: no state uploaded");
   if (nsteps < 0 || !dt_inout) return fail(c, QAS_E_ARG, "qas_run: bad arguments");
-  if (dt_hist && nsteps > c->hist_cap) {
+  const bool hist = dt_hist != nullptr;
+  if (hist && nsteps > c->hist_cap) {
     if (c->d_hist) QAS_CHECK(c, hipFree(c->d_hist));
     c->d_hist = nullptr;
     QAS_CHECK(c, hipMalloc(&c->d_hist, sizeof(double) * (size_t)nsteps));
     c->hist_cap = nsteps;
+    c->graph_hist = -1;  // the graph holds the old history pointer
   }
+  if (hist) QAS_CHECK(c, launch_zero_counter(c));
   if (*dt_inout > 0.0) {
     QAS_CHECK(c, launch_set_dt(c, *dt_inout));
   } else {
     QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
     QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
   }
-  for (int n = 0; n < nsteps; n++) {
+  int n = 0;
+  // bring the running counter to an even count before pairing steps
+  if (n < nsteps && (c->nstep % 2) == 1) {
     c->nstep++;
-    if (dt_hist) QAS_CHECK(c, launch_record_dt(c, c->d_hist + n));
-    if (!c->bounded) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
-    int r = do_step(c);
+    int r = loop_step(c, flg, hist, true);
     if (r) return r;
-    if (flg && (c->nstep % 2 == 0)) {
-      QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
-      QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
+    n++;
+  }
+  const bool graphs = c->use_graph && c->s.r_k >= 2 && c->prof_mask == 0;
+  if (graphs && nsteps - n >= 2) {
+    if (!c->bounded) {
+      QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+      c->bounded = true;
     }
+    if (!c->gexec || c->graph_flg != flg || c->graph_hist != (hist ? 1 : 0) || c->graph_cur != c->cur) {
+      int r = build_pair_graph(c, flg, hist);
+      if (r) return r;
+    }
+    for (; n + 2 <= nsteps; n += 2) {
+      QAS_CHECK(c, hipGraphLaunch(c->gexec, c->stream));
+      c->nstep += 2;
+    }
+  }
+  for (; n < nsteps; n++) {
+    c->nstep++;
+    int r = loop_step(c, flg, hist, c->nstep % 2 == 0);
+    if (r) return r;
   }
   QAS_CHECK(c, hipMemcpyAsync(c->h_pinned, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   QAS_CHECK(c, hipStreamSynchronize(c->stream));
   *dt_inout = c->h_pinned[0];
-  if (dt_hist && nsteps > 0)
+  if (hist && nsteps > 0)
     QAS_CHECK(c, hipMemcpy(dt_hist, c->d_hist, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost));
+  return QAS_OK;
+}
+
+extern "C" int qas_use_graphs(qas_ctx* c, int on)
+{
+  if (!c) return QAS_E_ARG;
+  c->use_graph = on != 0;
   return QAS_OK;
 }
 

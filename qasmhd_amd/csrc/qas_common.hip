@@ -160,6 +160,13 @@ __global__ void k_cfl_final(unsigned long long* vmax, double* dt, double* hist_s
 
 __global__ void k_set_dt(double* dt, double v) { *dt = v; }
 __global__ void k_copy1(const double* src, double* dst) { *dst = *src; }
+// dt history for graph replays: hist[*cnt] = *dt; ++*cnt (one thread)
+__global__ void k_record(const double* dt, double* hist, long long* cnt, long long cap)
+{
+  const long long n = *cnt;
+  if (n < cap) hist[n] = *dt;
+  *cnt = n + 1;
+}
 
 hipError_t launch_cfl(qas_ctx* c, const double* const U[8])
 {
@@ -200,6 +207,20 @@ hipError_t launch_set_dt(qas_ctx* c, double dt)
 hipError_t launch_record_dt(qas_ctx* c, double* slot)
 {
   hipLaunchKernelGGL(k_copy1, 1, 1, 0, c->stream, c->d_dt, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_record_counted(qas_ctx* c)
+{
+  hipLaunchKernelGGL(k_record, 1, 1, 0, c->stream, c->d_dt, c->d_hist, c->d_hcount, (long long)c->hist_cap);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_ll(long long* p) { *p = 0; }
+
+hipError_t launch_zero_counter(qas_ctx* c)
+{
+  hipLaunchKernelGGL(k_zero_ll, 1, 1, 0, c->stream, c->d_hcount);
   return hipGetLastError();
 }
 

@@ -102,6 +102,12 @@ struct qas_ctx {
   double* d_hist = nullptr;           // dt history of qas_run
   int hist_cap = 0;
   unsigned long long* d_vmax = nullptr;
+  long long* d_hcount = nullptr;      // dt-history slot counter (graph replays)
+  // hipGraph of two consecutive steps of qas_run (see qas_run)
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_flg = -1, graph_hist = -1, graph_cur = -1;
+  bool use_graph = true;
   double* h_pinned = nullptr;         // small pinned staging
   bool uploaded = false;
   bool bounded = false;
@@ -144,6 +150,8 @@ hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar);
 hipError_t launch_cfl(qas_ctx* c, const double* const U[8]);
 hipError_t launch_cfl_finalize(qas_ctx* c, double* hist_slot);
 hipError_t launch_set_dt(qas_ctx* c, double dt);
+hipError_t launch_record_counted(qas_ctx* c);
+hipError_t launch_zero_counter(qas_ctx* c);
 hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);

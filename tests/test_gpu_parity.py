@@ -185,3 +185,31 @@ def test_cpp_driver_matches_oracle(oracle, tmp_path):
     ref = st0.copy()
     oracle.run(cfg, ref, 6, 1)
     assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("nsteps", [7, 10])
+def test_graph_replay_matches_direct_launches(qas, oracle, nsteps):
+    """qas_run with captured 2-step graphs == host-launched steps == oracle,
+    including the dt history, over an odd and an even step count and a
+    second call that continues the running counter."""
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=48, seed=4)
+    _say("graphs", nsteps)
+    outs = []
+    for graphs in (1, 0):
+        with qas.solver_for(p) as s:
+            s.lib.qas_use_graphs(s.ctx, graphs)
+            s.upload(p.state)
+            h1, _ = s.run(nsteps, flg=1)
+            h2, _ = s.run(5, flg=1)
+            outs.append((s.download(), np.concatenate([h1, h2])))
+    cfg = oracle.make_cfg(p.state.shape[1:], p.dx, p.dy, gam=p.gam)
+    ref = p.state.copy()
+    href, dt = oracle.run(cfg, ref, nsteps, 1)
+    href2, _ = oracle.run(cfg, ref, 5, 1, dt=dt) if nsteps % 2 == 0 else (None, None)
+    for out, h in outs:
+        assert np.array_equal(h[:nsteps], href)
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    if nsteps % 2 == 0:
+        assert np.array_equal(outs[0][0], ref)
