@@ -149,6 +149,17 @@ int qas_kernel_model(const qas_ctx *c, int kid, double *flops, double *bytes);
  * [6] face-to-centre, [7] CUCT weight. */
 int qas_point_flops(int rmn, int odr, double out[8]);
 
+/* Host-side dot-product check of a hand-written per-point adjoint against
+ * central finite differences of its forward function, on `ntrials` random
+ * inputs; writes one relative error per trial.  No device call. */
+#define QAS_ADJ_WCNS4 0   /* WCNS-4 face values                             */
+#define QAS_ADJ_LIN4 1    /* linear 4th-order face values                   */
+#define QAS_ADJ_LRSTATE 2 /* characteristic reconstruction (ODR 4)          */
+#define QAS_ADJ_LRFB 3    /* B-flux face states                             */
+#define QAS_ADJ_CUCT 4    /* CUCT upwind weight                             */
+#define QAS_ADJ_HLLD 5    /* HLLD flux                                      */
+int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long seed, double *relerr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,7 @@ class QasScheme(C.Structure):
 EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", "qas_upload", "qas_download",
            "qas_upload_packed", "qas_download_packed", "qas_upload_device", "qas_download_device", "qas_bound",
            "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_use_graphs", "qas_sync", "qas_device_bytes",
-           "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops"]
+           "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck"]
 
 _lib = None
 
@@ -77,8 +77,24 @@ def load(path: str | None = None):
     l.qas_sync.argtypes = [vp]
     l.qas_device_bytes.argtypes = [vp]
     l.qas_device_bytes.restype = C.c_longlong
+    l.qas_adjoint_selfcheck.argtypes = [C.c_int, C.c_int, C.c_ulonglong, dp]
     _lib = l
     return l
+
+
+ADJ_POINT = {"wcns4": 0, "lin4": 1, "lrstate": 2, "lrfb": 3, "cuct": 4, "hlld": 5}
+
+
+def adjoint_selfcheck(name: str, ntrials: int = 1000, seed: int = 1):
+    """Relative errors of one per-point adjoint against Richardson-extrapolated
+    central differences of its forward function (host only, no device)."""
+    l = C.CDLL(LIB_PATH)
+    l.qas_adjoint_selfcheck.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_double)]
+    err = np.zeros(ntrials)
+    r = l.qas_adjoint_selfcheck(ADJ_POINT[name], ntrials, seed, err.ctypes.data_as(C.POINTER(C.c_double)))
+    if r != QAS_OK:
+        raise QasError("qas_adjoint_selfcheck(%s): %d" % (name, r))
+    return err
 
 
 def exported_symbols(path: str | None = None):

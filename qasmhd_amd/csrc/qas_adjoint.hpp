@@ -1,0 +1,698 @@
+// qas_adjoint.hpp -- reverse-mode (adjoint) versions of the per-point
+// functions of qas_device.hpp, written by hand.
+//
+// Each *_adj function recomputes the forward intermediates of its primal
+// (same formulas as qas_device.hpp) and then propagates output adjoints to
+// input adjoints, accumulating (+=) into the caller's input-adjoint arrays.
+// Non-smooth operations are differentiated at the branch the forward pass
+// takes: the WCNS nonlinear weights are differentiated exactly; max/min and
+// the HLLD region flags route the derivative to the selected operand / fix
+// the region; the positivity fallback and the exact-zero flags of the
+// characteristic reconstruction keep the branch the forward takes; |x| has
+// derivative sign(x).
+#pragma once
+#include "qas_device.hpp"
+
+namespace qas {
+
+template <typename T> QAS_HD T sgnd(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
+
+// ---------------------------------------------------------------- WCNS-4 / lin4
+// gf[0..4] += d(fl, fr)/df ^T (gl, gr)
+template <typename T>
+QAS_HD void wcns4_adj(const T* f, T gl, T gr, T* gf)
+{
+  const T eps = T(1e-40);
+  const T c13 = T(13. / 12.);
+  const T a0 = f[0] - T(4.0) * f[1] + T(3.0) * f[2];
+  const T b0 = f[0] - T(2.0) * f[1] + f[2];
+  const T s0 = c13 * b0 * b0 + T(0.25) * a0 * a0;
+  const T a1 = f[1] - f[3];
+  const T b1 = f[1] - T(2.0) * f[2] + f[3];
+  const T s1 = c13 * b1 * b1 + T(0.25) * a1 * a1;
+  const T a2 = f[4] - T(4.0) * f[3] + T(3.0) * f[2];
+  const T b2 = f[4] - T(2.0) * f[3] + f[2];
+  const T s2 = c13 * b2 * b2 + T(0.25) * a2 * a2;
+  const T S0 = T(1.0) / (s0 * s0 + eps), S1 = T(1.0) / (s1 * s1 + eps), S2 = T(1.0) / (s2 * s2 + eps);
+  const T qa = T(3.0) * f[0] - T(10.0) * f[1] + T(15.0) * f[2];
+  const T qb = -f[1] + T(6.0) * f[2] + T(3.0) * f[3];
+  const T qc = T(3.0) * f[2] + T(6.0) * f[3] - f[4];
+  const T ra = T(3.0) * f[4] - T(10.0) * f[3] + T(15.0) * f[2];
+  const T rb = -f[3] + T(6.0) * f[2] + T(3.0) * f[1];
+  const T rc = T(3.0) * f[2] + T(6.0) * f[1] - f[0];
+  // left weights
+  const T u0 = T(0.075) * S0, u1 = T(0.650) * S1, u2 = T(0.275) * S2;
+  const T D = T(1.0) / (u0 + u1 + u2);
+  const T w0 = u0 * D, w1 = u1 * D, w2 = T(1.0) - w0 - w1;
+  // right weights
+  const T v0 = T(0.275) * S0, v1 = T(0.650) * S1, v2 = T(0.075) * S2;
+  const T E = T(1.0) / (v0 + v1 + v2);
+  const T z0 = v0 * E, z1 = v1 * E, z2 = T(1.0) - z0 - z1;
+  // reverse: fl = 0.125 (w0 qa + w1 qb + w2 qc)
+  T gw0 = T(0.125) * qa * gl, gw1 = T(0.125) * qb * gl, gw2 = T(0.125) * qc * gl;
+  const T gqa = T(0.125) * w0 * gl, gqb = T(0.125) * w1 * gl, gqc = T(0.125) * w2 * gl;
+  gw0 -= gw2;
+  gw1 -= gw2;
+  T gu0 = D * gw0, gu1 = D * gw1;
+  const T gD = u0 * gw0 + u1 * gw1;
+  const T gsu = -D * D * gD;
+  gu0 += gsu;
+  gu1 += gsu;
+  const T gu2 = gsu;
+  // fr = 0.125 (z2 ra + z1 rb + z0 rc)
+  T gz2 = T(0.125) * ra * gr, gz1 = T(0.125) * rb * gr, gz0 = T(0.125) * rc * gr;
+  const T gra = T(0.125) * z2 * gr, grb = T(0.125) * z1 * gr, grc = T(0.125) * z0 * gr;
+  gz0 -= gz2;
+  gz1 -= gz2;
+  T gv0 = E * gz0, gv1 = E * gz1;
+  const T gE = v0 * gz0 + v1 * gz1;
+  const T gsv = -E * E * gE;
+  gv0 += gsv;
+  gv1 += gsv;
+  const T gv2 = gsv;
+  const T gS0 = T(0.075) * gu0 + T(0.275) * gv0;
+  const T gS1 = T(0.650) * gu1 + T(0.650) * gv1;
+  const T gS2 = T(0.275) * gu2 + T(0.075) * gv2;
+  const T gs0 = -S0 * S0 * T(2.0) * s0 * gS0;
+  const T gs1 = -S1 * S1 * T(2.0) * s1 * gS1;
+  const T gs2 = -S2 * S2 * T(2.0) * s2 * gS2;
+  const T ga0 = T(0.5) * a0 * gs0, gb0 = T(2.0) * c13 * b0 * gs0;
+  const T ga1 = T(0.5) * a1 * gs1, gb1 = T(2.0) * c13 * b1 * gs1;
+  const T ga2 = T(0.5) * a2 * gs2, gb2 = T(2.0) * c13 * b2 * gs2;
+  gf[0] += ga0 + gb0 + T(3.0) * gqa - grc;
+  gf[1] += T(-4.0) * ga0 - T(2.0) * gb0 + ga1 + gb1 - T(10.0) * gqa - gqb + T(3.0) * grb + T(6.0) * grc;
+  gf[2] += T(3.0) * ga0 + gb0 - T(2.0) * gb1 + T(3.0) * ga2 + gb2 + T(15.0) * gqa + T(6.0) * gqb + T(3.0) * gqc +
+           T(15.0) * gra + T(6.0) * grb + T(3.0) * grc;
+  gf[3] += -ga1 + gb1 - T(4.0) * ga2 - T(2.0) * gb2 + T(3.0) * gqb + T(6.0) * gqc - T(10.0) * gra - grb;
+  gf[4] += ga2 + gb2 - gqc + T(3.0) * gra;
+}
+
+// linear 4th-order face interpolation (cal_flr_4th), adjoint
+template <typename T>
+QAS_HD void lin4_adj(T gl, T gr, T* gf)
+{
+  const T c = T(0.003125);
+  gf[0] += c * (T(9.0) * gl - T(11.0) * gr);
+  gf[1] += c * (T(-56.0) * gl + T(144.0) * gr);
+  gf[2] += c * (T(234.0) * gl + T(234.0) * gr);
+  gf[3] += c * (T(144.0) * gl - T(56.0) * gr);
+  gf[4] += c * (T(-11.0) * gl + T(9.0) * gr);
+}
+
+// ---------------------------------------------------------------- lrstate
+// gs[7][5] += adjoint of the stencil, *gbn += adjoint of the cell-centred bn
+template <typename T>
+QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl_in)[7], const T (&gvr_in)[7],
+                        T (&gs)[7][5], T& gbn)
+{
+  T gvl[7], gvr[7];
+  for (int k = 0; k < 7; k++) {
+    gvl[k] = gvl_in[k];
+    gvr[k] = gvr_in[k];
+  }
+  // forward recompute (the projection branch of mhd_func.c:143)
+  const T ro0 = s[0][2], bt0 = s[4][2], bu0 = s[5][2], pr0 = s[6][2];
+  const bool fast = (s[0][2] * (s[1][2] * s[1][2] + s[2][2] * s[2][2] + s[3][2] * s[3][2]) > gam * s[6][2]);
+  const T eps = T(1e-12);
+  // exact-zero flags
+  bool zf[5];
+  for (int n = 1; n <= 5; n++)
+    zf[n - 1] = (s[n][0] == T(0.0)) && (s[n][1] == T(0.0)) && (s[n][2] == T(0.0)) && (s[n][3] == T(0.0)) &&
+                (s[n][4] == T(0.0));
+  // eigen quantities
+  const T iro = T(1.0) / ro0;
+  const T cs2 = gam * pr0 * iro;
+  const T ca2 = bn * bn * iro;
+  const T ct2 = (bt0 * bt0 + bu0 * bu0) * iro;
+  const T cb2 = cs2 + ca2 + ct2;
+  const T disc = cb2 * cb2 - T(4.0) * cs2 * ca2;
+  const T sq = dsqrt(dabs(disc));
+  const T cf2 = T(0.5) * (cb2 + sq);
+  const T cf = dsqrt(cf2);
+  const T rcf = ro0 * cf;
+  const T sro = dsqrt(ro0);
+  const T isroh = T(0.5) / sro;
+  const T icf2 = T(1.0) / cf2;
+  const T icf2h = T(0.5) * icf2;
+  const T ircfh = T(0.5) / rcf;
+  const T byh = T(0.5) * bt0;
+  const T bzh = T(0.5) * bu0;
+  // characteristic stencils and face values (pre-fallback)
+  T w[7][5], wl[7], wr[7];
+  for (int m = 0; m < 5; m++) {
+    w[0][m] = fast ? (-rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m]) : s[1][m];
+    w[1][m] = cf2 * s[0][m] - bt0 * s[4][m] - bu0 * s[5][m] - s[6][m];
+    w[2][m] = fast ? (rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m])
+                   : (bt0 * s[4][m] + bu0 * s[5][m] + s[6][m]);
+    w[3][m] = sro * s[2][m] + s[4][m];
+    w[4][m] = sro * s[2][m] - s[4][m];
+    w[5][m] = sro * s[3][m] + s[5][m];
+    w[6][m] = sro * s[3][m] - s[5][m];
+  }
+  for (int k = 0; k < 7; k++) interp5<4>(w[k], wl[k], wr[k]);
+  T ul[7], ur[7];
+  for (int side = 0; side < 2; side++) {
+    const T* q = side ? wr : wl;
+    T* u = side ? ur : ul;
+    u[0] = fast ? (icf2h * q[0] + icf2 * q[1] + icf2h * q[2]) : (icf2 * q[1] + icf2 * q[2]);
+    u[1] = fast ? (-ircfh * q[0] + ircfh * q[2]) : q[0];
+    u[2] = isroh * q[3] + isroh * q[4];
+    u[3] = isroh * q[5] + isroh * q[6];
+    u[4] = T(0.5) * q[3] - T(0.5) * q[4];
+    u[5] = T(0.5) * q[5] - T(0.5) * q[6];
+    u[6] = fast ? (T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6])
+                : (q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6]);
+  }
+  for (int n = 0; n < 5; n++)
+    if (zf[n]) ul[n + 1] = ur[n + 1] = T(0.0);
+  // reverse: positivity fallback
+  if (ul[0] <= eps || ul[6] <= eps) {
+    gs[0][2] += gvl[0];
+    gs[1][2] += gvl[1];
+    gs[6][2] += gvl[6];
+    gvl[0] = gvl[1] = gvl[6] = T(0.0);
+  }
+  if (ur[0] <= eps || ur[6] <= eps) {
+    gs[0][2] += gvr[0];
+    gs[1][2] += gvr[1];
+    gs[6][2] += gvr[6];
+    gvr[0] = gvr[1] = gvr[6] = T(0.0);
+  }
+  for (int n = 0; n < 5; n++)
+    if (zf[n]) gvl[n + 1] = gvr[n + 1] = T(0.0);
+  // reverse: back projection (both sides)
+  T gwl[7] = {}, gwr[7] = {};
+  T gicf2 = T(0), gicf2h = T(0), gircfh = T(0), gisroh = T(0), gbyh = T(0), gbzh = T(0);
+  for (int side = 0; side < 2; side++) {
+    const T* q = side ? wr : wl;
+    const T* g = side ? gvr : gvl;
+    T* gq = side ? gwr : gwl;
+    if (fast) {
+      gq[0] += icf2h * g[0] - ircfh * g[1] + T(0.5) * g[6];
+      gq[1] += icf2 * g[0];
+      gq[2] += icf2h * g[0] + ircfh * g[1] + T(0.5) * g[6];
+      gicf2h += (q[0] + q[2]) * g[0];
+      gicf2 += q[1] * g[0];
+      gircfh += (q[2] - q[0]) * g[1];
+    } else {
+      gq[0] += g[1];
+      gq[1] += icf2 * g[0];
+      gq[2] += icf2 * g[0] + g[6];
+      gicf2 += (q[1] + q[2]) * g[0];
+    }
+    gq[3] += isroh * g[2] + T(0.5) * g[4] - byh * g[6];
+    gq[4] += isroh * g[2] - T(0.5) * g[4] + byh * g[6];
+    gq[5] += isroh * g[3] + T(0.5) * g[5] - bzh * g[6];
+    gq[6] += isroh * g[3] - T(0.5) * g[5] + bzh * g[6];
+    gisroh += (q[3] + q[4]) * g[2] + (q[5] + q[6]) * g[3];
+    gbyh += (q[4] - q[3]) * g[6];
+    gbzh += (q[6] - q[5]) * g[6];
+  }
+  // reverse: WCNS per characteristic field
+  T gw[7][5] = {};
+  for (int k = 0; k < 7; k++) wcns4_adj(w[k], gwl[k], gwr[k], gw[k]);
+  // reverse: projection w = L s
+  T grcf = T(0), gcf2 = T(0), gsro = T(0), gbt0 = T(0), gbu0 = T(0);
+  for (int m = 0; m < 5; m++) {
+    const T g0 = gw[0][m], g1 = gw[1][m], g2 = gw[2][m], g3 = gw[3][m], g4 = gw[4][m], g5 = gw[5][m],
+            g6 = gw[6][m];
+    if (fast) {
+      gs[1][m] += -rcf * g0 + rcf * g2;
+      gs[4][m] += bt0 * (g0 + g2);
+      gs[5][m] += bu0 * (g0 + g2);
+      gs[6][m] += g0 + g2;
+      grcf += s[1][m] * (g2 - g0);
+      gbt0 += s[4][m] * (g0 + g2);
+      gbu0 += s[5][m] * (g0 + g2);
+    } else {
+      gs[1][m] += g0;
+      gs[4][m] += bt0 * g2;
+      gs[5][m] += bu0 * g2;
+      gs[6][m] += g2;
+      gbt0 += s[4][m] * g2;
+      gbu0 += s[5][m] * g2;
+    }
+    gs[0][m] += cf2 * g1;
+    gs[4][m] += -bt0 * g1 + g3 - g4;
+    gs[5][m] += -bu0 * g1 + g5 - g6;
+    gs[6][m] += -g1;
+    gcf2 += s[0][m] * g1;
+    gbt0 += -s[4][m] * g1;
+    gbu0 += -s[5][m] * g1;
+    gs[2][m] += sro * (g3 + g4);
+    gs[3][m] += sro * (g5 + g6);
+    gsro += s[2][m] * (g3 + g4) + s[3][m] * (g5 + g6);
+  }
+  // reverse: eigen quantities
+  gbt0 += T(0.5) * gbyh;
+  gbu0 += T(0.5) * gbzh;
+  // ircfh = 0.5 / rcf ; icf2h = 0.5 icf2 ; icf2 = 1/cf2 ; isroh = 0.5/sro
+  grcf += -ircfh / rcf * gircfh;
+  gicf2 += T(0.5) * gicf2h;
+  gcf2 += -icf2 * icf2 * gicf2;
+  gsro += -isroh / sro * gisroh;
+  // rcf = ro0 cf ; cf = sqrt(cf2) ; sro = sqrt(ro0)
+  T gro0 = cf * grcf;
+  const T gcf = ro0 * grcf;
+  gcf2 += (cf > T(0) ? T(0.5) / cf * gcf : T(0));
+  gro0 += (sro > T(0) ? T(0.5) / sro * gsro : T(0));
+  // cf2 = 0.5 (cb2 + sq) ; sq = sqrt(|disc|)
+  T gcb2 = T(0.5) * gcf2;
+  const T gsq = T(0.5) * gcf2;
+  const T gdisc = (sq > T(0) ? T(0.5) / sq * gsq : T(0)) * sgnd(disc);
+  // disc = cb2^2 - 4 cs2 ca2
+  gcb2 += T(2.0) * cb2 * gdisc;
+  T gcs2 = gcb2 - T(4.0) * ca2 * gdisc;
+  T gca2 = gcb2 - T(4.0) * cs2 * gdisc;
+  const T gct2 = gcb2;
+  // cs2 = gam pr0 iro ; ca2 = bn^2 iro ; ct2 = (bt0^2 + bu0^2) iro
+  T giro = gam * pr0 * gcs2 + bn * bn * gca2 + (bt0 * bt0 + bu0 * bu0) * gct2;
+  T gpr0 = gam * iro * gcs2;
+  gbn += T(2.0) * bn * iro * gca2;
+  gbt0 += T(2.0) * bt0 * iro * gct2;
+  gbu0 += T(2.0) * bu0 * iro * gct2;
+  gro0 += -iro * iro * giro;
+  gs[0][2] += gro0;
+  gs[4][2] += gbt0;
+  gs[5][2] += gbu0;
+  gs[6][2] += gpr0;
+}
+
+// ---------------------------------------------------------------- lr_fb
+// vl = lin_l(p) - bnp lin_l(q) ; vr = lin_r(p) - bnm lin_r(q)  (ODR 4)
+template <typename T>
+QAS_HD void lr_fb_adj(const T* p, const T* q, T bnm, T bnp, T gvl, T gvr, T* gp, T* gq, T& gbnm, T& gbnp)
+{
+  T fl1, fr1;
+  lin5<4>(q, fl1, fr1);
+  lin4_adj(gvl, gvr, gp);
+  lin4_adj(-bnp * gvl, -bnm * gvr, gq);
+  gbnp += -fl1 * gvl;
+  gbnm += -fr1 * gvr;
+}
+
+// ---------------------------------------------------------------- CUCT weight
+// g[0..3] ro, g[4..7] va, g[8..11] vb, g[12] ba1, g[13] ba3, g[14] bb2, g[15] bb3
+template <typename T>
+QAS_HD void cuct_weight_adj(const T* x, T gw, T* g)
+{
+  const T eps = T(1e-12);
+  const T rs = T(0.25) * (x[0] + x[1] + x[2] + x[3]);
+  const T srs = dsqrt(rs);
+  const T roc = T(1.0) / srs;
+  const T vac0 = T(0.25) * (x[4] + x[5] + x[6] + x[7]);
+  const T vbc0 = T(0.25) * (x[8] + x[9] + x[10] + x[11]);
+  const T bac = T(0.5) * (x[12] + x[13]);
+  const T bbc = T(0.5) * (x[14] + x[15]);
+  const T va = dabs(vac0) + dabs(bac * roc);
+  const T vb = dabs(vbc0) + dabs(bbc * roc);
+  const T num = va + T(0.5) * eps, den = va + vb + eps;
+  // w = num / den
+  const T gnum = gw / den;
+  const T gden = -num / (den * den) * gw;
+  const T gva = gnum + gden;
+  const T gvb = gden;
+  const T gvac0 = sgnd(vac0) * gva, gvbc0 = sgnd(vbc0) * gvb;
+  const T ga = sgnd(bac * roc) * gva, gb = sgnd(bbc * roc) * gvb;
+  const T gbac = roc * ga, gbbc = roc * gb;
+  const T groc = bac * ga + bbc * gb;
+  const T grs = -roc / srs * T(0.5) / srs * groc;  // d(1/sqrt(rs)) = -0.5 rs^-3/2
+  for (int k = 0; k < 4; k++) {
+    g[k] += T(0.25) * grs;
+    g[4 + k] += T(0.25) * gvac0;
+    g[8 + k] += T(0.25) * gvbc0;
+  }
+  g[12] += T(0.5) * gbac;
+  g[13] += T(0.5) * gbac;
+  g[14] += T(0.5) * gbbc;
+  g[15] += T(0.5) * gbbc;
+}
+
+// ---------------------------------------------------------------- HLLD
+// gL, gR (7 each) and gbn += adjoint of calc_flux_hlld; gm1i = 1/(gam-1)
+template <typename T>
+QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T* gR, T& gbn)
+{
+  const T FL_EPS = T(QAS_FL_EPS);
+  const T rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
+  const T ror = R[0], vnr = R[1], vtr = R[2], vur = R[3], btr = R[4], bur = R[5], prr = R[6];
+  // ---- forward
+  const T gm1i = T(1.0) / (gam - T(1.0));
+  const T bnc2 = bn * bn;
+  const T sgn = (bn > T(0)) ? T(1.0) : T(-1.0);
+  const T roli = T(1.0) / rol, rori = T(1.0) / ror;
+  const T pml = T(0.5) * (btl * btl + bul * bul), pmr = T(0.5) * (btr * btr + bur * bur);
+  const T ptl = prl + pml, ptr = prr + pmr;
+  const T vl2 = vnl * vnl + vtl * vtl + vul * vul, vr2 = vnr * vnr + vtr * vtr + vur * vur;
+  const T enl = gm1i * prl + pml + T(0.5) * rol * vl2, enr = gm1i * prr + pmr + T(0.5) * ror * vr2;
+  const T vbl = vtl * btl + vul * bul, vbr = vtr * btr + vur * bur;
+  const T cl2 = gam * prl * roli, cr2 = gam * prr * rori;
+  const T cal2 = bnc2 * roli, car2 = bnc2 * rori;
+  const T cbl2 = cl2 + cal2 + T(2.0) * pml * roli, cbr2 = cr2 + car2 + T(2.0) * pmr * rori;
+  const T dl = cbl2 * cbl2 - T(4.0) * cl2 * cal2, dr = cbr2 * cbr2 - T(4.0) * cr2 * car2;
+  const T sql = dsqrt(dabs(dl)), sqr = dsqrt(dabs(dr));
+  const T cfl2 = T(0.5) * (cbl2 + sql), cfr2 = T(0.5) * (cbr2 + sqr);
+  const T cfl = dsqrt(cfl2), cfr = dsqrt(cfr2);
+  const bool selL = cfl >= cfr;
+  const T cmax = selL ? cfl : cfr;
+  const bool nr_min = vnl >= vnr;  // min(vnl,vnr) = vnr, max = vnl
+  const T mnv = nr_min ? vnr : vnl, mxv = nr_min ? vnl : vnr;
+  const T xs = mnv - cmax, ys = mxv + cmax;
+  const bool slx = (T(0.0) >= xs), sry = !(T(0.0) >= ys);
+  const T sl = slx ? xs : T(0.0), sr = sry ? ys : T(0.0);
+  const T slvl = sl - vnl, srvr = sr - vnr;
+  const T rslvl = rol * slvl, rsrvr = ror * srvr;
+  const T drsvi = T(1.0) / (rsrvr - rslvl);
+  const T Nvnc = rsrvr * vnr - rslvl * vnl - ptr + ptl;
+  const T vnc = Nvnc * drsvi;
+  const T Nptc = rsrvr * ptl - rslvl * ptr + rsrvr * rslvl * (vnr - vnl);
+  const T ptc = Nptc * drsvi;
+  const T slvc = sl - vnc, srvc = sr - vnc;
+  const T ro2l = rslvl / slvc, ro2r = rsrvr / srvc;
+  const T rhdl = rslvl * slvc - bnc2, rhdr = rsrvr * srvc - bnc2;
+  const bool gl_ = dabs(rhdl) > FL_EPS, gr_ = dabs(rhdr) > FL_EPS;
+  T rhdli = T(0), rhnvl = T(0), rhnbl = T(0), Al = T(0), Bl = T(0);
+  T rhdri = T(0), rhnvr = T(0), rhnbr = T(0), Ar = T(0), Br = T(0);
+  T vt2l = vtl, vu2l = vul, bt2l = btl, bu2l = bul, vt2r = vtr, vu2r = vur, bt2r = btr, bu2r = bur;
+  if (gl_) {
+    rhdli = T(1.0) / rhdl;
+    rhnvl = (vnl - vnc) * bn;
+    rhnbl = rslvl * slvl - bnc2;
+    Al = rhnvl * rhdli;
+    Bl = rhnbl * rhdli;
+    vt2l = vtl + Al * btl;
+    vu2l = vul + Al * bul;
+    bt2l = Bl * btl;
+    bu2l = Bl * bul;
+  }
+  if (gr_) {
+    rhdri = T(1.0) / rhdr;
+    rhnvr = (vnr - vnc) * bn;
+    rhnbr = rsrvr * srvr - bnc2;
+    Ar = rhnvr * rhdri;
+    Br = rhnbr * rhdri;
+    vt2r = vtr + Ar * btr;
+    vu2r = vur + Ar * bur;
+    bt2r = Br * btr;
+    bu2r = Br * bur;
+  }
+  const T vb2l = vt2l * bt2l + vu2l * bu2l, vb2r = vt2r * bt2r + vu2r * bu2r;
+  const T Nen2l = slvl * enl - ptl * vnl + ptc * vnc + bn * (vbl - vb2l);
+  const T Nen2r = srvr * enr - ptr * vnr + ptc * vnc + bn * (vbr - vb2r);
+  const T en2l = Nen2l / slvc, en2r = Nen2r / srvc;
+  const T rro2l = dsqrt(ro2l), rro2r = dsqrt(ro2r);
+  const T rro2i = T(1.0) / (rro2r + rro2l);
+  const T Nvt = rro2r * vt2r + rro2l * vt2l + (bt2r - bt2l) * sgn;
+  const T Nvu = rro2r * vu2r + rro2l * vu2l + (bu2r - bu2l) * sgn;
+  const T Nbt = rro2l * bt2r + rro2r * bt2l + rro2r * rro2l * (vt2r - vt2l) * sgn;
+  const T Nbu = rro2l * bu2r + rro2r * bu2l + rro2r * rro2l * (vu2r - vu2l) * sgn;
+  const T vt3m = Nvt * rro2i, vu3m = Nvu * rro2i, bt3m = Nbt * rro2i, bu3m = Nbu * rro2i;
+  const T vb3m = vt3m * bt3m + vu3m * bu3m;
+  const T en3l = en2l - rro2l * (vb2l - vb3m) * sgn;
+  const T en3r = en2r + rro2r * (vb2r - vb3m) * sgn;
+  (void)en3l; (void)en3r;
+  const T hl = (vnc > T(0)) ? T(1.0) : T(0.0);
+  const T hr = T(1.0) - hl;
+  const T h2l = (vnc - dabs(bn) / rro2l > T(0)) ? T(1.0) : T(0.0);
+  const T h3l = (T(1.0) - h2l) * hl;
+  const T h2r = (vnc + dabs(bn) / rro2r > T(0)) ? T(0.0) : T(1.0);
+  const T h3r = (T(1.0) - h2r) * hr;
+  const T rou = ro2l * hl + ro2r * hr;
+  const T vtu = vt2l * h2l + vt3m * (h3l + h3r) + vt2r * h2r;
+  const T vuu = vu2l * h2l + vu3m * (h3l + h3r) + vu2r * h2r;
+  const T btu = bt2l * h2l + bt3m * (h3l + h3r) + bt2r * h2r;
+  const T buu = bu2l * h2l + bu3m * (h3l + h3r) + bu2r * h2r;
+  const T enu = en2l * h2l + en3l * h3l + en3r * h3r + en2r * h2r;
+  // ---- reverse
+  const T g0 = gF[0], g1 = gF[1], g2 = gF[2], g3 = gF[3], g4 = gF[4], g5 = gF[5], g6 = gF[6];
+  T grou = vnc * g0 + vnc * vnc * g1 + vtu * vnc * g2 + vuu * vnc * g3;
+  T gvnc = rou * g0 + T(2.0) * rou * vnc * g1 + rou * vtu * g2 + rou * vuu * g3 + btu * g4 + buu * g5 + (enu + ptc) * g6;
+  T gptc = g1 + vnc * g6;
+  T gbnc2 = T(-0.5) * g1;
+  T gbn_ = -btu * g2 - buu * g3 - vtu * g4 - vuu * g5 - (vtu * btu + vuu * buu) * g6;
+  const T gvtu = rou * vnc * g2 - bn * g4 - bn * btu * g6;
+  const T gbtu = -bn * g2 + vnc * g4 - bn * vtu * g6;
+  const T gvuu = rou * vnc * g3 - bn * g5 - bn * buu * g6;
+  const T gbuu = -bn * g3 + vnc * g5 - bn * vuu * g6;
+  const T genu = vnc * g6;
+  T gro2l = hl * grou, gro2r = hr * grou;
+  const T h33 = h3l + h3r;
+  T gvt2l = h2l * gvtu, gvt2r = h2r * gvtu, gvt3m = h33 * gvtu;
+  T gvu2l = h2l * gvuu, gvu2r = h2r * gvuu, gvu3m = h33 * gvuu;
+  T gbt2l = h2l * gbtu, gbt2r = h2r * gbtu, gbt3m = h33 * gbtu;
+  T gbu2l = h2l * gbuu, gbu2r = h2r * gbuu, gbu3m = h33 * gbuu;
+  T gen2l = h2l * genu, gen2r = h2r * genu;
+  const T gen3l = h3l * genu, gen3r = h3r * genu;
+  T grro2l = T(0), grro2r = T(0), gvb2l = T(0), gvb2r = T(0), gvb3m = T(0);
+  gen2l += gen3l;
+  grro2l += -(vb2l - vb3m) * sgn * gen3l;
+  gvb2l += -rro2l * sgn * gen3l;
+  gvb3m += rro2l * sgn * gen3l;
+  gen2r += gen3r;
+  grro2r += (vb2r - vb3m) * sgn * gen3r;
+  gvb2r += rro2r * sgn * gen3r;
+  gvb3m += -rro2r * sgn * gen3r;
+  gvt3m += bt3m * gvb3m;
+  gbt3m += vt3m * gvb3m;
+  gvu3m += bu3m * gvb3m;
+  gbu3m += vu3m * gvb3m;
+  T grro2i = Nvt * gvt3m + Nvu * gvu3m + Nbt * gbt3m + Nbu * gbu3m;
+  {  // Nbt, Nbu
+    const T gN = rro2i * gbt3m;
+    grro2l += bt2r * gN + rro2r * (vt2r - vt2l) * sgn * gN;
+    grro2r += bt2l * gN + rro2l * (vt2r - vt2l) * sgn * gN;
+    gbt2r += rro2l * gN;
+    gbt2l += rro2r * gN;
+    gvt2r += rro2r * rro2l * sgn * gN;
+    gvt2l -= rro2r * rro2l * sgn * gN;
+    const T gM = rro2i * gbu3m;
+    grro2l += bu2r * gM + rro2r * (vu2r - vu2l) * sgn * gM;
+    grro2r += bu2l * gM + rro2l * (vu2r - vu2l) * sgn * gM;
+    gbu2r += rro2l * gM;
+    gbu2l += rro2r * gM;
+    gvu2r += rro2r * rro2l * sgn * gM;
+    gvu2l -= rro2r * rro2l * sgn * gM;
+  }
+  {  // Nvt, Nvu
+    const T gN = rro2i * gvt3m;
+    grro2r += vt2r * gN;
+    gvt2r += rro2r * gN;
+    grro2l += vt2l * gN;
+    gvt2l += rro2l * gN;
+    gbt2r += sgn * gN;
+    gbt2l -= sgn * gN;
+    const T gM = rro2i * gvu3m;
+    grro2r += vu2r * gM;
+    gvu2r += rro2r * gM;
+    grro2l += vu2l * gM;
+    gvu2l += rro2l * gM;
+    gbu2r += sgn * gM;
+    gbu2l -= sgn * gM;
+  }
+  {
+    const T gs_ = -rro2i * rro2i * grro2i;
+    grro2r += gs_;
+    grro2l += gs_;
+  }
+  gro2l += (rro2l > T(0) ? T(0.5) / rro2l : T(0)) * grro2l;
+  gro2r += (rro2r > T(0) ? T(0.5) / rro2r : T(0)) * grro2r;
+  // en2l = Nen2l / slvc
+  T gslvc = -en2l / slvc * gen2l, gsrvc = -en2r / srvc * gen2r;
+  T gslvl = T(0), gsrvr = T(0), genl = T(0), genr = T(0), gptl = T(0), gptr = T(0), gvnl = T(0), gvnr = T(0);
+  T gvbl = T(0), gvbr = T(0);
+  {
+    const T gN = gen2l / slvc;
+    gslvl += enl * gN;
+    genl += slvl * gN;
+    gptl += -vnl * gN;
+    gvnl += -ptl * gN;
+    gptc += vnc * gN;
+    gvnc += ptc * gN;
+    gbn_ += (vbl - vb2l) * gN;
+    gvbl += bn * gN;
+    gvb2l += -bn * gN;
+    const T gM = gen2r / srvc;
+    gsrvr += enr * gM;
+    genr += srvr * gM;
+    gptr += -vnr * gM;
+    gvnr += -ptr * gM;
+    gptc += vnc * gM;
+    gvnc += ptc * gM;
+    gbn_ += (vbr - vb2r) * gM;
+    gvbr += bn * gM;
+    gvb2r += -bn * gM;
+  }
+  gvt2l += bt2l * gvb2l;
+  gbt2l += vt2l * gvb2l;
+  gvu2l += bu2l * gvb2l;
+  gbu2l += vu2l * gvb2l;
+  gvt2r += bt2r * gvb2r;
+  gbt2r += vt2r * gvb2r;
+  gvu2r += bu2r * gvb2r;
+  gbu2r += vu2r * gvb2r;
+  T grslvl = T(0), grsrvr = T(0), gvtl = T(0), gvul = T(0), gbtl = T(0), gbul = T(0);
+  T gvtr = T(0), gvur = T(0), gbtr = T(0), gbur = T(0);
+  if (gl_) {
+    gvtl += gvt2l;
+    gvul += gvu2l;
+    const T gA = btl * gvt2l + bul * gvu2l;
+    const T gB = btl * gbt2l + bul * gbu2l;
+    gbtl += Al * gvt2l + Bl * gbt2l;
+    gbul += Al * gvu2l + Bl * gbu2l;
+    const T grhnvl = rhdli * gA, grhnbl = rhdli * gB;
+    const T grhdli = rhnvl * gA + rhnbl * gB;
+    const T grhdl = -rhdli * rhdli * grhdli;
+    gvnl += bn * grhnvl;
+    gvnc -= bn * grhnvl;
+    gbn_ += (vnl - vnc) * grhnvl;
+    grslvl += slvl * grhnbl + slvc * grhdl;
+    gslvl += rslvl * grhnbl;
+    gbnc2 -= grhnbl + grhdl;
+    gslvc += rslvl * grhdl;
+  } else {
+    gvtl += gvt2l;
+    gvul += gvu2l;
+    gbtl += gbt2l;
+    gbul += gbu2l;
+  }
+  if (gr_) {
+    gvtr += gvt2r;
+    gvur += gvu2r;
+    const T gA = btr * gvt2r + bur * gvu2r;
+    const T gB = btr * gbt2r + bur * gbu2r;
+    gbtr += Ar * gvt2r + Br * gbt2r;
+    gbur += Ar * gvu2r + Br * gbu2r;
+    const T grhnvr = rhdri * gA, grhnbr = rhdri * gB;
+    const T grhdri = rhnvr * gA + rhnbr * gB;
+    const T grhdr = -rhdri * rhdri * grhdri;
+    gvnr += bn * grhnvr;
+    gvnc -= bn * grhnvr;
+    gbn_ += (vnr - vnc) * grhnvr;
+    grsrvr += srvr * grhnbr + srvc * grhdr;
+    gsrvr += rsrvr * grhnbr;
+    gbnc2 -= grhnbr + grhdr;
+    gsrvc += rsrvr * grhdr;
+  } else {
+    gvtr += gvt2r;
+    gvur += gvu2r;
+    gbtr += gbt2r;
+    gbur += gbu2r;
+  }
+  // ro2l = rslvl / slvc
+  grslvl += gro2l / slvc;
+  gslvc += -ro2l / slvc * gro2l;
+  grsrvr += gro2r / srvc;
+  gsrvc += -ro2r / srvc * gro2r;
+  T gsl = gslvc, gsr = gsrvc;
+  gvnc -= gslvc + gsrvc;
+  // vnc, ptc
+  T gdrsvi = Nvnc * gvnc + Nptc * gptc;
+  {
+    const T gN = drsvi * gvnc;
+    grsrvr += vnr * gN;
+    gvnr += rsrvr * gN;
+    grslvl -= vnl * gN;
+    gvnl -= rslvl * gN;
+    gptr -= gN;
+    gptl += gN;
+    const T gM = drsvi * gptc;
+    grsrvr += ptl * gM + rslvl * (vnr - vnl) * gM;
+    gptl += rsrvr * gM;
+    grslvl += -ptr * gM + rsrvr * (vnr - vnl) * gM;
+    gptr -= rslvl * gM;
+    gvnr += rsrvr * rslvl * gM;
+    gvnl -= rsrvr * rslvl * gM;
+  }
+  {
+    const T gd = -drsvi * drsvi * gdrsvi;
+    grsrvr += gd;
+    grslvl -= gd;
+  }
+  T grol = slvl * grslvl, gror = srvr * grsrvr;
+  gslvl += rol * grslvl;
+  gsrvr += ror * grsrvr;
+  gsl += gslvl;
+  gvnl -= gslvl;
+  gsr += gsrvr;
+  gvnr -= gsrvr;
+  // sl, sr, cmax
+  T gcmax = T(0), gmnv = T(0), gmxv = T(0);
+  if (slx) {
+    gmnv += gsl;
+    gcmax -= gsl;
+  }
+  if (sry) {
+    gmxv += gsr;
+    gcmax += gsr;
+  }
+  if (nr_min) {
+    gvnr += gmnv;
+    gvnl += gmxv;
+  } else {
+    gvnl += gmnv;
+    gvnr += gmxv;
+  }
+  T gcfl2 = T(0), gcfr2 = T(0);
+  if (selL) gcfl2 += (cfl > T(0) ? T(0.5) / cfl : T(0)) * gcmax;
+  else gcfr2 += (cfr > T(0) ? T(0.5) / cfr : T(0)) * gcmax;
+  T groli = T(0), grori = T(0), gpml = T(0), gpmr = T(0), gprl = T(0), gprr = T(0);
+  {  // left fast speed
+    T gcb = T(0.5) * gcfl2;
+    const T gsq = T(0.5) * gcfl2;
+    const T gd = (sql > T(0) ? T(0.5) / sql : T(0)) * sgnd(dl) * gsq;
+    gcb += T(2.0) * cbl2 * gd;
+    const T gcl2 = gcb - T(4.0) * cal2 * gd;
+    const T gcal2 = gcb - T(4.0) * cl2 * gd;
+    gpml += T(2.0) * roli * gcb;
+    groli += T(2.0) * pml * gcb + bnc2 * gcal2 + gam * prl * gcl2;
+    gbnc2 += roli * gcal2;
+    gprl += gam * roli * gcl2;
+  }
+  {  // right fast speed
+    T gcb = T(0.5) * gcfr2;
+    const T gsq = T(0.5) * gcfr2;
+    const T gd = (sqr > T(0) ? T(0.5) / sqr : T(0)) * sgnd(dr) * gsq;
+    gcb += T(2.0) * cbr2 * gd;
+    const T gcr2 = gcb - T(4.0) * car2 * gd;
+    const T gcar2 = gcb - T(4.0) * cr2 * gd;
+    gpmr += T(2.0) * rori * gcb;
+    grori += T(2.0) * pmr * gcb + bnc2 * gcar2 + gam * prr * gcr2;
+    gbnc2 += rori * gcar2;
+    gprr += gam * rori * gcr2;
+  }
+  // vbl, enl, ptl, pml
+  gvtl += btl * gvbl;
+  gbtl += vtl * gvbl;
+  gvul += bul * gvbl;
+  gbul += vul * gvbl;
+  gvtr += btr * gvbr;
+  gbtr += vtr * gvbr;
+  gvur += bur * gvbr;
+  gbur += vur * gvbr;
+  gprl += gm1i * genl + gptl;
+  gpml += genl + gptl;
+  grol += T(0.5) * vl2 * genl;
+  const T gvl2 = T(0.5) * rol * genl;
+  gprr += gm1i * genr + gptr;
+  gpmr += genr + gptr;
+  gror += T(0.5) * vr2 * genr;
+  const T gvr2 = T(0.5) * ror * genr;
+  gvnl += T(2.0) * vnl * gvl2;
+  gvtl += T(2.0) * vtl * gvl2;
+  gvul += T(2.0) * vul * gvl2;
+  gvnr += T(2.0) * vnr * gvr2;
+  gvtr += T(2.0) * vtr * gvr2;
+  gvur += T(2.0) * vur * gvr2;
+  gbtl += btl * gpml;
+  gbul += bul * gpml;
+  gbtr += btr * gpmr;
+  gbur += bur * gpmr;
+  grol += -roli * roli * groli;
+  gror += -rori * rori * grori;
+  gbn_ += T(2.0) * bn * gbnc2;
+  gL[0] += grol; gL[1] += gvnl; gL[2] += gvtl; gL[3] += gvul; gL[4] += gbtl; gL[5] += gbul; gL[6] += gprl;
+  gR[0] += gror; gR[1] += gvnr; gR[2] += gvtr; gR[3] += gvur; gR[4] += gbtr; gR[5] += gbur; gR[6] += gprr;
+  gbn += gbn_;
+}
+
+}  // namespace qas
