@@ -160,6 +160,30 @@ int qas_point_flops(int rmn, int odr, double out[8]);
 #define QAS_ADJ_HLLD 5    /* HLLD flux                                      */
 int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long seed, double *relerr);
 
+/* ---- batched per-point evaluation on the device (tests) -----------------
+ * One GPU thread per item through the same device functions the stage
+ * kernels use; host arrays in and out; each call allocates its own device
+ * buffers.  They replace, for testing, direct calls of the reference's
+ * per-point functions:
+ *   qas_test_interp   interpol[ODR-1] (kind 0..3) / l_interp[ODR-1] (kind
+ *                     10..13), common/common_interp.c:7-150; f5 = n x 5
+ *                     stencil, fl at i+1/2, fr at i-1/2
+ *   qas_test_f4       fcen[ODR-1] (op 0) / df1[ODR-1] (op 1),
+ *                     common/common_interp.h:10-24; f4 = n x (i-1, i, i+1, i+2)
+ *   qas_test_lrstate  mhd_lrstate, common/mhd_func.c:108-161; st = n x 7 x 5
+ *                     (ro, vn, vt, vu, bt, bu, pr), bn = cell-centred normal B
+ *   qas_test_flux     riemann[RMN], common/mhd_flux.c:7-698; n x 7 states,
+ *                     dv = n x 2 shock sensor
+ *   qas_test_cuct     mhd_cuct_weight, common/mhd_func.c:56-76; cells = n x
+ *                     (ro[4], vx[4], vy[4]) of cells (i-1,j-1),(i,j-1),(i-1,j),
+ *                     (i,j); faces = n x (bx(i,j-1), bx(i,j), by(i-1,j), by(i,j)) */
+int qas_test_interp(int kind, int n, const double *f5, double *fl, double *fr);
+int qas_test_f4(int op, int odr, int n, const double *f4, double *out);
+int qas_test_lrstate(int odr, int n, const double *st, const double *bn, double gam, double *vl, double *vr);
+int qas_test_flux(int rmn, int n, const double *ql, const double *qr, const double *bn, double gam,
+                  const double *dv, double *f);
+int qas_test_cuct(int n, const double *cells, const double *faces, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,8 @@ class QasScheme(C.Structure):
 EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", "qas_upload", "qas_download",
            "qas_upload_packed", "qas_download_packed", "qas_upload_device", "qas_download_device", "qas_bound",
            "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_use_graphs", "qas_sync", "qas_device_bytes",
-           "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck"]
+           "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck",
+           "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct"]
 
 _lib = None
 
@@ -78,11 +79,68 @@ def load(path: str | None = None):
     l.qas_device_bytes.argtypes = [vp]
     l.qas_device_bytes.restype = C.c_longlong
     l.qas_adjoint_selfcheck.argtypes = [C.c_int, C.c_int, C.c_ulonglong, dp]
+    l.qas_test_interp.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+    l.qas_test_f4.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    l.qas_test_lrstate.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
+    l.qas_test_flux.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, dp, dp]
+    l.qas_test_cuct.argtypes = [C.c_int, dp, dp, dp]
     _lib = l
     return l
 
 
 ADJ_POINT = {"wcns4": 0, "lin4": 1, "lrstate": 2, "lrfb": 3, "cuct": 4, "hlld": 5}
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _chk(r, what):
+    if r != QAS_OK:
+        raise QasError("%s failed (%d)" % (what, r))
+
+
+class points:
+    """Batched device evaluation of the per-point functions (qas_test_*)."""
+
+    @staticmethod
+    def interp(kind, f5):
+        f5 = np.ascontiguousarray(f5, dtype=np.float64)
+        n = f5.shape[0]
+        fl, fr = np.zeros(n), np.zeros(n)
+        _chk(load().qas_test_interp(kind, n, _p(f5), _p(fl), _p(fr)), "qas_test_interp")
+        return fl, fr
+
+    @staticmethod
+    def f4(op, odr, f4):
+        f4 = np.ascontiguousarray(f4, dtype=np.float64)
+        out = np.zeros(f4.shape[0])
+        _chk(load().qas_test_f4(op, odr, f4.shape[0], _p(f4), _p(out)), "qas_test_f4")
+        return out
+
+    @staticmethod
+    def lrstate(odr, st, bn, gam):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        bn = np.ascontiguousarray(bn, dtype=np.float64)
+        n = st.shape[0]
+        vl, vr = np.zeros((n, 7)), np.zeros((n, 7))
+        _chk(load().qas_test_lrstate(odr, n, _p(st), _p(bn), gam, _p(vl), _p(vr)), "qas_test_lrstate")
+        return vl, vr
+
+    @staticmethod
+    def flux(rmn, ql, qr, bn, gam, dv):
+        ql, qr, bn, dv = (np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, bn, dv))
+        f = np.zeros((ql.shape[0], 7))
+        _chk(load().qas_test_flux(rmn, ql.shape[0], _p(ql), _p(qr), _p(bn), gam, _p(dv), _p(f)), "qas_test_flux")
+        return f
+
+    @staticmethod
+    def cuct(cells, faces):
+        cells = np.ascontiguousarray(cells, dtype=np.float64)
+        faces = np.ascontiguousarray(faces, dtype=np.float64)
+        out = np.zeros(cells.shape[0])
+        _chk(load().qas_test_cuct(cells.shape[0], _p(cells), _p(faces), _p(out)), "qas_test_cuct")
+        return out
 
 
 def adjoint_selfcheck(name: str, ntrials: int = 1000, seed: int = 1):
