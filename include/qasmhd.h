@@ -120,6 +120,28 @@ int qas_sync(qas_ctx *c);
 long long qas_device_bytes(const qas_ctx *c);
 
 
+/* ---- discrete adjoint (no counterpart in the reference) ----------------
+ * Reverse-mode derivative of `nsteps` steps of ideal(dt) (with the bound()
+ * before the first step) at frozen step sizes.  Scope: 2D, FP64, HLLD
+ * (rmn 1), odr 4, r_k 3, ctw 0 or 1, periodic on both axes; other schemes
+ * are rejected with QAS_E_ARG.
+ * qas_adj_forward   advances the current state by nsteps steps with the
+ *                   given dt values (as qas_step would), keeping each step's
+ *                   input state on the device (8 x nd doubles per step).
+ * qas_adj_backward  given the adjoint of the final state (packed 8 x nd,
+ *                   ghost entries included: they are added onto the cells
+ *                   bound() copies them from), returns the adjoint of the
+ *                   state passed to the first step (packed 8 x nd; ghost
+ *                   entries zero, as the forward overwrites them).  Host
+ *                   arrays; the _device variant takes device pointers.
+ *                   Non-smooth operations (WCNS weights, wave-speed and
+ *                   HLLD region selections, positivity fallback) are
+ *                   differentiated at the branch the forward takes. */
+int qas_adj_forward(qas_ctx *c, int nsteps, const double *dts);
+int qas_adj_backward(qas_ctx *c, const double *g_final, double *g_init);
+int qas_adj_backward_device(qas_ctx *c, const double *g_final, double *g_init);
+
+
 /* ---- instrumentation (no counterpart in the reference) -----------------
  * Kernel identifiers of one stage, for the timing hooks and the work model. */
 #define QAS_K_PRIM 0      /* cell-centred B + primitives                    */

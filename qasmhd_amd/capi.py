@@ -37,6 +37,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_upload_packed", "qas_download_packed", "qas_upload_device", "qas_download_device", "qas_bound",
            "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_use_graphs", "qas_sync", "qas_device_bytes",
            "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck",
+           "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct"]
 
 _lib = None
@@ -79,6 +80,9 @@ def load(path: str | None = None):
     l.qas_device_bytes.argtypes = [vp]
     l.qas_device_bytes.restype = C.c_longlong
     l.qas_adjoint_selfcheck.argtypes = [C.c_int, C.c_int, C.c_ulonglong, dp]
+    l.qas_adj_forward.argtypes = [vp, C.c_int, dp]
+    l.qas_adj_backward.argtypes = [vp, dp, dp]
+    l.qas_adj_backward_device.argtypes = [vp, vp, vp]
     l.qas_test_interp.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     l.qas_test_f4.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
     l.qas_test_lrstate.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
@@ -239,6 +243,15 @@ class Solver:
                     "qas_download")
         return out
 
+    def upload_device(self, ptr):
+        """Copy a packed (8, ...) float64 device array (raw pointer) in."""
+        self._check(self.lib.qas_upload_device(self.ctx, C.c_void_p(ptr)), "qas_upload_device")
+
+    def download_device(self, ptr):
+        """Copy the state into a packed (8, ...) float64 device array (raw
+        pointer); ordered on the context's stream, call sync() before use."""
+        self._check(self.lib.qas_download_device(self.ctx, C.c_void_p(ptr)), "qas_download_device")
+
     def bound(self):
         self._check(self.lib.qas_bound(self.ctx), "qas_bound")
 
@@ -263,6 +276,21 @@ class Solver:
 
     def sync(self):
         self._check(self.lib.qas_sync(self.ctx), "qas_sync")
+
+    def adj_forward(self, dts):
+        """Advance len(dts) steps at the given dt values, keeping checkpoints
+        for adj_backward (qas_adj_forward)."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(self.lib.qas_adj_forward(self.ctx, int(d.size), _p(d)), "qas_adj_forward")
+
+    def adj_backward(self, g_final):
+        """Adjoint of the recorded steps: d(loss)/d(initial state) from
+        d(loss)/d(final state), both (8, ...) arrays with ghosts."""
+        g = np.ascontiguousarray(g_final, dtype=np.float64)
+        assert g.shape == (8,) + self.shape
+        out = np.empty_like(g)
+        self._check(self.lib.qas_adj_backward(self.ctx, _p(g), _p(out)), "qas_adj_backward")
+        return out
 
     @property
     def device_bytes(self):

@@ -1,0 +1,772 @@
+// qas_adjoint2d.hip -- hand-written discrete adjoint of the 2D step.
+//
+// Scope: FP64, HLLD (RMN 1), ODR 4 (WCNS-4 / 4th-order linear / fcen /
+// difference), SSP-RK3, CUCT on or off, periodic on both axes, dt frozen
+// (the step sizes are inputs, as a recorded dt history is).  The adjoint is
+// that of exactly the forward kernels of qas_kernels2d.hip / qas_sweep2d.hpp,
+// which reproduce MHD2D::ideal (2D/SERIAL/OTvortex/mhd2d_solve.cpp:3-400)
+// bit for bit; non-smooth operations are differentiated at the branch the
+// forward takes (qas_adjoint.hpp).
+//
+// Checkpointing: qas_adj_forward keeps every step's input state on the
+// device.  qas_adj_backward walks the steps in reverse; for each step it
+// recomputes the three forward stages from the checkpoint, keeping each
+// stage's primitives and face EMF parts, and then runs the stage adjoints
+// 3, 2, 1.  A stage adjoint is seven launches, all in gather form (every
+// output element is written by one thread; no atomics):
+//   corner    adjoint of the CT update -> g_ez at corners; forward CUCT
+//             weight and g_ct
+//   faceparts adjoint of the corner EMF interpolation -> g of the four face
+//             parts (x/y sweep, upwind/central)
+//   sweep<X>, sweep<Y>
+//             per line segment (one wavefront, 58 owned cells): forward
+//             reconstruction + lr_fb, face flux adjoint (HLLD), adjoint of
+//             the reconstruction, gathered onto cells by cross-lane
+//             shuffles -> g of ro, vx, vy, vz, cx, cy, bz, pr and face B
+//   prim      adjoint of the primitive recovery + CUCT corner terms of
+//             ro, vx, vy + the RK pass-through b * g_out -> g_in of
+//             ro, mx, my, mz, bz, en; a * g_out into the step-base adjoint
+//   face      adjoint of fcen (with the periodic ghost copies of cx, cy
+//             folded onto their sources) + CUCT and sweep terms -> g_in of
+//             bx, by
+//   fold      adjoint of bound(): ghost entries added onto the cells they
+//             copy, ghosts zeroed
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qas_adjoint.hpp"
+#include "qas_device.hpp"
+#include "qas_internal.hpp"
+
+namespace qas {
+
+constexpr int SEGA = 58;  // owned cells per wavefront segment of the adjoint sweeps (lanes 3..60)
+
+struct AdjPtrs {
+  const double* in[8];   // stage input state (ghosts filled)
+  const double *vx, *vy, *vz, *pr, *cx, *cy;  // forward primitives of `in`
+  const double* emf[8];  // forward face parts ([0] x upwind, [1] x central, [4] y upwind, [5] y central)
+  const double* gO[8];   // adjoint of the stage output (ghost entries zero)
+  double* gI[8];         // adjoint of the stage input (before the ghost fold)
+  double* gU0[8];        // adjoint of the step base (accumulated over the stages)
+  double* gp[8];         // adjoint of ro, vx, vy, vz, cx, cy, bz, pr as the sweeps read them
+  double* gbf[2];        // adjoint of face bx (x sweep), by (y sweep)
+  double* gemf[4];       // adjoint of the face parts: x upwind, x central, y upwind, y central
+  double *gez, *gct, *ctv;  // corner adjoints of ez and of the CUCT weight; forward weight
+  const double* dt;
+  double a, b;
+};
+
+// ---------------------------------------------------------------- helpers
+// periodic images of a kept index q in [4, n-5]: q itself and the ghost that
+// bound() fills from it (common_func.c:67-74)
+struct Img {
+  int i[2];
+  int n;
+};
+__device__ __forceinline__ Img images(int q, int n)
+{
+  Img r;
+  r.i[0] = q;
+  r.n = 1;
+  const int p = n - 2 * NG;
+  if (q + p < n) r.i[r.n++] = q + p;
+  else if (q - p >= 0) r.i[r.n++] = q - p;
+  return r;
+}
+
+__device__ __forceinline__ double fold_at(const double* f, int i, int j, const Geo& g)
+{
+  const Img ix = images(i, g.nx), iy = images(j, g.ny);
+  double v = 0.0;
+  for (int b = 0; b < iy.n; b++)
+    for (int a = 0; a < ix.n; a++) v += f[(long long)g.nx * iy.i[b] + ix.i[a]];
+  return v;
+}
+
+__device__ __forceinline__ bool kept(int i, int n) { return i >= NG && i <= n - NG - 1; }
+
+// inputs of the CUCT weight at corner ss (k_emf2d): ro, vx, vy of the cells
+// (i-1,j-1), (i,j-1), (i-1,j), (i,j); bx(i,j-1), bx(i,j), by(i-1,j), by(i,j)
+__device__ __forceinline__ void cuct_inputs(const AdjPtrs& p, long long ss, long long nx, double* x)
+{
+  const long long c[4] = {ss - 1 - nx, ss - nx, ss - 1, ss};
+  for (int k = 0; k < 4; k++) {
+    x[k] = p.in[0][c[k]];
+    x[4 + k] = p.vx[c[k]];
+    x[8 + k] = p.vy[c[k]];
+  }
+  x[12] = p.in[4][ss - nx];
+  x[13] = p.in[4][ss];
+  x[14] = p.in[5][ss - 1];
+  x[15] = p.in[5][ss];
+}
+
+__device__ __forceinline__ bool corner_ok(int i, int j, const Geo& g)
+{
+  return i >= 3 && i <= g.nx - 3 && j >= 3 && j <= g.ny - 3;
+}
+
+// ---------------------------------------------------------------- zero
+__global__ __launch_bounds__(256) void k_adj_zero(AdjPtrs p, long long nd)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nd) return;
+  for (int m = 0; m < 8; m++) p.gp[m][t] = 0.0;
+  p.gbf[0][t] = 0.0;
+  p.gbf[1][t] = 0.0;
+}
+
+// ---------------------------------------------------------------- corner
+// k_ct2d: bx(i,j) uses dfc(ez(i,j-1..j+2)) scaled by -dt/dy; by(i,j) uses
+// dfc(ez(i-1..i+2,j)) scaled by +dt/dx; out = a u0 + b (in + ...).
+template <int CTW>
+__global__ __launch_bounds__(256) void k_adj_corner(AdjPtrs p, Geo g)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  const long long nx = g.nx;
+  double gez = 0.0, gct = 0.0, ctv = 0.5;
+  if (corner_ok(i, j, g)) {
+    const double dtv = *p.dt;
+    const double fy = -p.b * (dtv / g.dy), fx = p.b * (dtv / g.dx);
+    const double w[4] = {1.0 / 24.0, -27.0 / 24.0, 27.0 / 24.0, -1.0 / 24.0};  // offsets -1..2
+    for (int kk = 0; kk < 4; kk++) {
+      const int k = kk - 1;
+      const int jf = j - k, iff = i - k;
+      if (jf >= 0 && jf < g.ny) gez += w[kk] * fy * p.gO[4][nx * jf + i];
+      if (iff >= 0 && iff < g.nx) gez += w[kk] * fx * p.gO[5][nx * j + iff];
+    }
+    if (CTW) {
+      double x[16];
+      cuct_inputs(p, t, nx, x);
+      ctv = cuct_weight(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], x[9], x[10], x[11], x[12], x[13],
+                        x[14], x[15]);
+      double a[5], bq[5], l0, r0, l1, r1, dum;
+      for (int m = 0; m < 5; m++) {
+        a[m] = p.emf[1][t + (m - 3) * nx];
+        bq[m] = p.emf[1][t + (m - 2) * nx];
+      }
+      lin5<4>(a, l0, dum);
+      lin5<4>(bq, dum, r0);
+      for (int m = 0; m < 5; m++) {
+        a[m] = p.emf[5][t + (m - 3)];
+        bq[m] = p.emf[5][t + (m - 2)];
+      }
+      lin5<4>(a, l1, dum);
+      lin5<4>(bq, dum, r1);
+      gct = gez * (0.5 * (l0 + r0) + 0.5 * (l1 + r1));
+    }
+  }
+  p.gez[t] = gez;
+  p.gct[t] = gct;
+  p.ctv[t] = ctv;
+}
+
+// ---------------------------------------------------------------- face parts
+// ez(c) = -1/2 [(uL+uR) + (1-w)(cL+cR)]_x + 1/2 [(uL+uR) + w (cL+cR)]_y with
+// the one-sided linear interpolations of k_emf2d
+__global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  const long long nx = g.nx;
+  const double cl[5] = {0.003125 * 9.0, 0.003125 * -56.0, 0.003125 * 234.0, 0.003125 * 144.0, 0.003125 * -11.0};
+  const double cr[5] = {0.003125 * -11.0, 0.003125 * 144.0, 0.003125 * 234.0, 0.003125 * -56.0, 0.003125 * 9.0};
+  double xu = 0.0, xc = 0.0, yu = 0.0, yc = 0.0;
+  for (int s = -3; s <= 2; s++) {
+    const double coef = (s <= 1 ? cl[s + 3] : 0.0) + (s >= -2 ? cr[s + 2] : 0.0);
+    const int jc = j - s, ic = i - s;
+    if (corner_ok(i, jc, g)) {
+      const long long c = nx * jc + i;
+      const double ge = p.gez[c];
+      xu += -0.5 * coef * ge;
+      xc += -0.5 * (1.0 - p.ctv[c]) * coef * ge;
+    }
+    if (corner_ok(ic, j, g)) {
+      const long long c = nx * j + ic;
+      const double ge = p.gez[c];
+      yu += 0.5 * coef * ge;
+      yc += 0.5 * p.ctv[c] * coef * ge;
+    }
+  }
+  p.gemf[0][t] = xu;
+  p.gemf[1][t] = xc;
+  p.gemf[2][t] = yu;
+  p.gemf[3][t] = yc;
+}
+
+// ---------------------------------------------------------------- sweeps
+// Lane l of a segment handles cell q = seg*SEGA - 3 + l and face q; lanes
+// 3..60 own their cell and face.  Forward as in k_sweep2d (RMN 1, ODR 4).
+template <int DIR>
+__global__ __launch_bounds__(256) void k_adj_sweep2d(AdjPtrs p, Geo g, double gam, int nseg)
+{
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int nline = DIR == 0 ? g.ny : g.nx;
+  if (wave >= nseg * nline) return;  // whole wavefront exits together
+  const int line = wave / nseg;
+  const int seg = wave - line * nseg;
+  const int n = DIR == 0 ? g.nx : g.ny;
+  const long long o = DIR == 0 ? 1 : g.sy;
+  const long long base = DIR == 0 ? g.sy * line : (long long)line;
+  const int q = seg * SEGA - 3 + lane;
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const long long sc = base + o * qc;
+
+  const double* vn = DIR == 0 ? p.vx : p.vy;
+  const double* vt = DIR == 0 ? p.vy : p.vz;
+  const double* vu = DIR == 0 ? p.vz : p.vx;
+  const double* bt = DIR == 0 ? p.cy : p.in[6];
+  const double* bu = DIR == 0 ? p.in[6] : p.cx;
+  const double* cn = DIR == 0 ? p.cx : p.cy;
+  const double* bf = DIR == 0 ? p.in[4] : p.in[5];
+  double s[7][5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const long long a = sc + (m - 2) * o;
+    s[0][m] = p.in[0][a];
+    s[1][m] = vn[a];
+    s[2][m] = vt[a];
+    s[3][m] = vu[a];
+    s[4][m] = bt[a];
+    s[5][m] = bu[a];
+    s[6][m] = p.pr[a];
+  }
+  const double bnc = cn[sc];
+  double vl[7], vr[7];
+  lrstate<4>(s, bnc, gam, vl, vr);
+  // lr_fb: p = b_pair * v_n, q = v_pair (x: cy, vy; y: cx, vx)
+  double pp[5], qq[5], ql, qr;
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    pp[m] = (DIR == 0 ? s[4][m] : s[5][m]) * s[1][m];
+    qq[m] = DIR == 0 ? s[2][m] : s[3][m];
+  }
+  const double bnm = bf[sc], bnp = bf[sc + o];
+  lr_fb<4>(pp, qq, bnm, bnp, ql, qr);
+
+  double L[7], R[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    L[k] = __shfl_up(vl[k], 1, WAVE);
+    R[k] = vr[k];
+  }
+  const bool face_ok = lane >= 1 && q >= 3 && q <= n - 3;
+  double gL[7] = {}, gR[7] = {};
+  double gbnf = 0.0, gqlf = 0.0, gqr = 0.0;
+  if (face_ok) {
+    const long long fq = base + o * q;
+    const int tl = line;
+    const int ntl = DIR == 0 ? g.ny : g.nx;
+    double gF[7] = {};
+    if (tl >= NG && tl <= ntl - NG - 1) {
+      const double sfac = -p.b * (*p.dt / (DIR == 0 ? g.dx : g.dy));
+      // flux component fed into each updated variable (k_sweep2d fv[])
+      const int fk[6] = {0, DIR == 0 ? 1 : 3, DIR == 0 ? 2 : 1, DIR == 0 ? 3 : 2, DIR == 0 ? 5 : 4, 6};
+      const int vm[6] = {0, 1, 2, 3, 6, 7};
+      const double w[4] = {1.0 / 24.0, -27.0 / 24.0, 27.0 / 24.0, -1.0 / 24.0};  // F(c+k), k = -1..2
+#pragma unroll
+      for (int v = 0; v < 6; v++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+          const int c = q - (kk - 1);
+          if (c >= NG && c <= n - NG - 1) acc += w[kk] * p.gO[vm[v]][base + o * c];
+        }
+        gF[fk[v]] = sfac * acc;
+      }
+    }
+    // face parts of the B flux: fbu = F_B - fc, fc = (ql(q-1) + qr(q)) / 2
+    const double gu = p.gemf[DIR * 2 + 0][fq], gc = p.gemf[DIR * 2 + 1][fq];
+    gF[DIR == 0 ? 4 : 5] += gu;
+    const double gfc = gc - gu;
+    gqlf = 0.5 * gfc;
+    gqr = 0.5 * gfc;
+    const double bnf = 0.5 * (bf[fq] + bf[fq]);
+    double gbnh = 0.0;
+    hlld_adj(L, R, bnf, gam, gF, gL, gR, gbnh);
+    gbnf = gbnh;  // d(0.5 (b + b))/db = 1
+  }
+  // this cell's left state feeds face q+1 (lane+1)
+  double gvl[7], gvr[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    gvl[k] = __shfl_down(gL[k], 1, WAVE);
+    gvr[k] = gR[k];
+  }
+  const double gql = __shfl_down(gqlf, 1, WAVE);
+  double gs[7][5] = {};
+  double gcn = 0.0, gbm = 0.0, gbp = 0.0;
+  const bool cell_ok = q >= 2 && q <= n - 3;
+  if (cell_ok) {
+    lrstate_adj(s, bnc, gam, gvl, gvr, gs, gcn);
+    double gpp[5] = {}, gqq[5] = {};
+    lr_fb_adj(pp, qq, bnm, bnp, gql, gqr, gpp, gqq, gbm, gbp);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      if (DIR == 0) {
+        gs[1][m] += gpp[m] * s[4][m];
+        gs[4][m] += gpp[m] * s[1][m];
+        gs[2][m] += gqq[m];
+      } else {
+        gs[1][m] += gpp[m] * s[5][m];
+        gs[5][m] += gpp[m] * s[1][m];
+        gs[3][m] += gqq[m];
+      }
+    }
+  }
+  // gather the stencil adjoints onto cells: cell of lane l gets entry m of
+  // lane l - (m - 2)
+  double gc7[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    gc7[k] = __shfl_down(gs[k][0], 2, WAVE) + __shfl_down(gs[k][1], 1, WAVE) + gs[k][2] +
+             __shfl_up(gs[k][3], 1, WAVE) + __shfl_up(gs[k][4], 2, WAVE);
+  }
+  const double gface = gbnf + gbm + __shfl_up(gbp, 1, WAVE);
+  const bool own = lane >= 3 && lane < 3 + SEGA && q >= 0 && q < n;
+  if (!own) return;
+  const long long c = base + o * q;
+  // stencil rows -> adjoint arrays (ro vx vy vz cx cy bz pr)
+  const int gi[7] = {0, DIR == 0 ? 1 : 2, DIR == 0 ? 2 : 3, DIR == 0 ? 3 : 1, DIR == 0 ? 5 : 6, DIR == 0 ? 6 : 4, 7};
+#pragma unroll
+  for (int k = 0; k < 7; k++) p.gp[gi[k]][c] += gc7[k];
+  p.gp[DIR == 0 ? 4 : 5][c] += gcn;
+  p.gbf[DIR][c] += gface;
+}
+
+// ---------------------------------------------------------------- prim
+template <int CTW>
+__global__ __launch_bounds__(256) void k_adj_prim(AdjPtrs p, Geo g, double gam)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  const long long nx = g.nx;
+  double cro = 0.0, cvx = 0.0, cvy = 0.0;
+  if (CTW) {
+    // this cell is entry `role` of the corners (i+1,j+1), (i,j+1), (i+1,j), (i,j)
+#pragma unroll
+    for (int role = 0; role < 4; role++) {
+      const int ic = i + ((role == 0 || role == 2) ? 1 : 0);
+      const int jc = j + (role <= 1 ? 1 : 0);
+      if (!corner_ok(ic, jc, g)) continue;
+      const long long ss = nx * jc + ic;
+      const double gct = p.gct[ss];
+      if (gct == 0.0) continue;
+      double x[16], gx[16] = {};
+      cuct_inputs(p, ss, nx, x);
+      cuct_weight_adj(x, gct, gx);
+      cro += gx[role];
+      cvx += gx[4 + role];
+      cvy += gx[8 + role];
+    }
+  }
+  double gro = p.gp[0][t] + cro;
+  double gvx = p.gp[1][t] + cvx, gvy = p.gp[2][t] + cvy, gvz = p.gp[3][t];
+  double gcx = p.gp[4][t], gcy = p.gp[5][t], gbz = p.gp[6][t];
+  const double gpr = p.gp[7][t];
+  // forward of k_prim2d
+  const double ro = p.in[0][t];
+  const double iro = 1.0 / ro;
+  const double mx = p.in[1][t], my = p.in[2][t], mz = p.in[3][t];
+  const double vx = mx * iro, vy = my * iro, vz = mz * iro;
+  const double cx = p.cx[t], cy = p.cy[t], cz = p.in[6][t];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double gm1 = gam - 1.0;
+  // pr = (gam-1) (en - (ro v2 + b2) / 2)
+  const double gen = gm1 * gpr;
+  gro += -gm1 * 0.5 * v2 * gpr;
+  gvx += -gm1 * ro * vx * gpr;
+  gvy += -gm1 * ro * vy * gpr;
+  gvz += -gm1 * ro * vz * gpr;
+  gcx += -gm1 * cx * gpr;
+  gcy += -gm1 * cy * gpr;
+  gbz += -gm1 * cz * gpr;
+  // v = m / ro
+  const double giro = mx * gvx + my * gvy + mz * gvz;
+  gro += -iro * iro * giro;
+  const double b = p.b;
+  p.gI[0][t] = b * p.gO[0][t] + gro;
+  p.gI[1][t] = b * p.gO[1][t] + iro * gvx;
+  p.gI[2][t] = b * p.gO[2][t] + iro * gvy;
+  p.gI[3][t] = b * p.gO[3][t] + iro * gvz;
+  p.gI[6][t] = b * p.gO[6][t] + gbz;
+  p.gI[7][t] = b * p.gO[7][t] + gen;
+  p.gp[4][t] = gcx;  // totals of the cell-centred B adjoint, for k_adj_face
+  p.gp[5][t] = gcy;
+  if (p.a != 0.0)
+    for (int m = 0; m < 8; m++) p.gU0[m][t] += p.a * p.gO[m][t];
+}
+
+// ---------------------------------------------------------------- face
+template <int CTW>
+__global__ __launch_bounds__(256) void k_adj_face(AdjPtrs p, Geo g)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  const long long nx = g.nx;
+  double gbx = p.b * p.gO[4][t] + p.gbf[0][t];
+  double gby = p.b * p.gO[5][t] + p.gbf[1][t];
+  if (CTW) {
+    // bx(i,j) is entry 13 of corner (i,j) and entry 12 of corner (i,j+1);
+    // by(i,j) is entry 15 of corner (i,j) and entry 14 of corner (i+1,j)
+    const int ci[3] = {i, i, i + 1}, cj[3] = {j, j + 1, j};
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      if (!corner_ok(ci[r], cj[r], g)) continue;
+      const long long ss = nx * cj[r] + ci[r];
+      const double gct = p.gct[ss];
+      if (gct == 0.0) continue;
+      double x[16], gx[16] = {};
+      cuct_inputs(p, ss, nx, x);
+      cuct_weight_adj(x, gct, gx);
+      if (r == 0) {
+        gbx += gx[13];
+        gby += gx[15];
+      } else if (r == 1) {
+        gbx += gx[12];
+      } else {
+        gby += gx[14];
+      }
+    }
+  }
+  // cx(q) = fcen(bx(q-1..q+2)) at kept q; ghost copies of cx fold onto q
+  const double fw[4] = {-0.0625, 0.5625, 0.5625, -0.0625};  // offsets -1..2
+  if (kept(j, g.ny)) {
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int iq = i - (kk - 1);
+      if (kept(iq, g.nx)) gbx += fw[kk] * fold_at(p.gp[4], iq, j, g);
+    }
+  }
+  if (kept(i, g.nx)) {
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int jq = j - (kk - 1);
+      if (kept(jq, g.ny)) gby += fw[kk] * fold_at(p.gp[5], i, jq, g);
+    }
+  }
+  p.gI[4][t] = gbx;
+  p.gI[5][t] = gby;
+}
+
+// ---------------------------------------------------------------- fold
+struct FoldArgs {
+  const double* src[8];
+  double* dst[8];
+  const double* extra[8];  // added at kept cells when non-null
+};
+
+__global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  const bool k = kept(i, g.nx) && kept(j, g.ny);
+  for (int m = 0; m < 8; m++) {
+    double v = 0.0;
+    if (k) {
+      v = fold_at(f.src[m], i, j, g);
+      if (f.extra[m]) v += f.extra[m][t];
+    }
+    f.dst[m][t] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_clear8(FoldArgs f, long long nd)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nd) return;
+  for (int m = 0; m < 8; m++) f.dst[m][t] = 0.0;
+}
+
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace qas
+
+using namespace qas;
+
+// ---------------------------------------------------------------- workspace
+struct qas_adj_work {
+  void* slab = nullptr;
+  size_t stride = 0;
+  double* S[3][8] = {};       // recomputed stage inputs S1, S2 and a discard output
+  double* prim[3][6] = {};    // per stage: vx vy vz pr cx cy
+  double* emf[3][8] = {};     // per stage: face parts (slots 0, 1, 4, 5 used)
+  double* ez[3] = {};
+  double* gO[8] = {};
+  double* gI[8] = {};
+  double* gU0[8] = {};
+  double* gp[8] = {};
+  double* gbf[2] = {};
+  double* gemf[4] = {};
+  double *gez = nullptr, *gct = nullptr, *ctv = nullptr;
+  double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
+  int ck_cap = 0;
+  int nrec = 0;               // steps recorded
+  std::vector<double> dts;
+};
+
+namespace qas {
+void adj_release(qas_ctx* c)
+{
+  if (!c || !c->adj) return;
+  if (c->adj->slab) (void)hipFree(c->adj->slab);
+  if (c->adj->ck) (void)hipFree(c->adj->ck);
+  delete c->adj;
+  c->adj = nullptr;
+}
+}  // namespace qas
+
+static int afail(qas_ctx* c, int code, const std::string& m)
+{
+  if (c) c->err = m;
+  return code;
+}
+
+#define QAS_ACHECK(c, expr)                                                                               \
+  do {                                                                                                    \
+    hipError_t _e = (expr);                                                                               \
+    if (_e != hipSuccess) return afail((c), QAS_E_HIP, std::string(#expr) + ": " + hipGetErrorName(_e)); \
+  } while (0)
+
+static std::string adj_supported(const qas_ctx* c)
+{
+  if (c->g.ndim != 2) return "the adjoint covers the 2D path only";
+  if (c->s.precision != 0) return "the adjoint covers the FP64 path only";
+  if (c->s.rmn != 1) return "the adjoint covers the HLLD solver (rmn 1) only";
+  if (c->s.odr != 4) return "the adjoint covers odr 4 only";
+  if (c->s.r_k != 3) return "the adjoint covers r_k 3 only";
+  for (int a = 0; a < 2; a++)
+    for (int m = 0; m < 8; m++)
+      if (c->s.dn[a][m] != 0) return "the adjoint covers periodic boundaries only";
+  return "";
+}
+
+static int adj_workspace(qas_ctx* c)
+{
+  if (c->adj) return QAS_OK;
+  qas_adj_work* w = new qas_adj_work();
+  c->adj = w;
+  const size_t stride = (size_t)((c->geo.nd + 63) / 64 * 64);
+  w->stride = stride;
+  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 3;
+  hipError_t e = hipMalloc(&w->slab, narr * stride * sizeof(double));
+  if (e != hipSuccess) {
+    adj_release(c);
+    return afail(c, QAS_E_NOMEM, std::string("adjoint workspace: ") + hipGetErrorName(e));
+  }
+  double* p = (double*)w->slab;
+  auto take = [&]() { double* r = p; p += stride; return r; };
+  for (int s = 0; s < 3; s++)
+    for (int m = 0; m < 8; m++) w->S[s][m] = take();
+  for (int s = 0; s < 3; s++)
+    for (int k = 0; k < 6; k++) w->prim[s][k] = take();
+  for (int s = 0; s < 3; s++)
+    for (int k = 0; k < 8; k++) w->emf[s][k] = take();
+  for (int s = 0; s < 3; s++) w->ez[s] = take();
+  for (int m = 0; m < 8; m++) w->gO[m] = take();
+  for (int m = 0; m < 8; m++) w->gI[m] = take();
+  for (int m = 0; m < 8; m++) w->gU0[m] = take();
+  for (int m = 0; m < 8; m++) w->gp[m] = take();
+  for (int k = 0; k < 2; k++) w->gbf[k] = take();
+  for (int k = 0; k < 4; k++) w->gemf[k] = take();
+  w->gez = take();
+  w->gct = take();
+  w->ctv = take();
+  QAS_ACHECK(c, hipMemsetAsync(w->slab, 0, narr * stride * sizeof(double), c->stream));
+  return QAS_OK;
+}
+
+// forward stage with its primitives and face parts kept in stage slot s
+static hipError_t fwd_stage(qas_ctx* c, int s, double* const in[8], double* const u0[8], double* const out[8],
+                            int rk)
+{
+  qas_adj_work* w = c->adj;
+  StagePtrs p;
+  std::memset(&p, 0, sizeof(p));
+  for (int m = 0; m < 8; m++) {
+    p.in[m] = in[m];
+    p.out[m] = out[m];
+    p.u0[m] = u0[m];
+  }
+  p.vx = w->prim[s][0]; p.vy = w->prim[s][1]; p.vz = w->prim[s][2]; p.pr = w->prim[s][3];
+  p.cx = w->prim[s][4]; p.cy = w->prim[s][5];
+  for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
+  for (int k = 0; k < 8; k++) p.emf[k] = w->emf[s][k];
+  p.ez = w->ez[s];
+  p.dt = c->d_dt;
+  const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
+  p.a = ra[rk];
+  p.b = rb[rk];
+  hipError_t e = launch_stage2d(c, p);
+  if (e != hipSuccess) return e;
+  return launch_bound(c, out, 8);
+}
+
+// adjoint of stage rk (0..2) whose input is `in` and whose forward
+// intermediates sit in slot rk; reads w->gO, leaves the folded input adjoint
+// in w->gO (plus the step-base adjoint when `last`)
+static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
+{
+  qas_adj_work* w = c->adj;
+  const Geo& g = c->geo;
+  hipStream_t st = c->stream;
+  AdjPtrs p;
+  std::memset(&p, 0, sizeof(p));
+  for (int m = 0; m < 8; m++) {
+    p.in[m] = in[m];
+    p.gO[m] = w->gO[m];
+    p.gI[m] = w->gI[m];
+    p.gU0[m] = w->gU0[m];
+    p.gp[m] = w->gp[m];
+    p.emf[m] = w->emf[rk][m];
+  }
+  p.vx = w->prim[rk][0]; p.vy = w->prim[rk][1]; p.vz = w->prim[rk][2]; p.pr = w->prim[rk][3];
+  p.cx = w->prim[rk][4]; p.cy = w->prim[rk][5];
+  for (int k = 0; k < 2; k++) p.gbf[k] = w->gbf[k];
+  for (int k = 0; k < 4; k++) p.gemf[k] = w->gemf[k];
+  p.gez = w->gez;
+  p.gct = w->gct;
+  p.ctv = w->ctv;
+  p.dt = c->d_dt;
+  const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
+  p.a = ra[rk];
+  p.b = rb[rk];
+  const unsigned nb = nblk(g.nd, 256);
+  const bool ctw = c->s.ctw != 0;
+  hipLaunchKernelGGL(k_adj_zero, nb, 256, 0, st, p, g.nd);
+  if (ctw) hipLaunchKernelGGL((k_adj_corner<1>), nb, 256, 0, st, p, g);
+  else hipLaunchKernelGGL((k_adj_corner<0>), nb, 256, 0, st, p, g);
+  hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
+  const int nsx = (g.nx + SEGA - 1) / SEGA, nsy = (g.ny + SEGA - 1) / SEGA;
+  hipLaunchKernelGGL((k_adj_sweep2d<0>), nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam, nsx);
+  hipLaunchKernelGGL((k_adj_sweep2d<1>), nblk((long long)nsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->s.gam, nsy);
+  if (ctw) {
+    hipLaunchKernelGGL((k_adj_prim<1>), nb, 256, 0, st, p, g, c->s.gam);
+    hipLaunchKernelGGL((k_adj_face<1>), nb, 256, 0, st, p, g);
+  } else {
+    hipLaunchKernelGGL((k_adj_prim<0>), nb, 256, 0, st, p, g, c->s.gam);
+    hipLaunchKernelGGL((k_adj_face<0>), nb, 256, 0, st, p, g);
+  }
+  FoldArgs f;
+  for (int m = 0; m < 8; m++) {
+    f.src[m] = w->gI[m];
+    f.dst[m] = w->gO[m];
+    f.extra[m] = last ? w->gU0[m] : nullptr;
+  }
+  hipLaunchKernelGGL(k_adj_fold, nb, 256, 0, st, f, g);
+  return hipGetLastError();
+}
+
+extern "C" int qas_adj_forward(qas_ctx* c, int nsteps, const double* dts)
+{
+  if (!c || !c->slab) return afail(c, QAS_E_ARG, "qas_adj_forward: null context");
+  if (!c->uploaded) return afail(c, QAS_E_STATE, "qas_adj_forward: no state uploaded");
+  if (nsteps <= 0 || !dts) return afail(c, QAS_E_ARG, "qas_adj_forward: bad arguments");
+  for (int n = 0; n < nsteps; n++)
+    if (!(dts[n] > 0.0)) return afail(c, QAS_E_ARG, "qas_adj_forward: dt must be positive");
+  const std::string why = adj_supported(c);
+  if (!why.empty()) return afail(c, QAS_E_ARG, "qas_adj_forward: " + why);
+  int r = adj_workspace(c);
+  if (r) return r;
+  qas_adj_work* w = c->adj;
+  const size_t per = 8 * w->stride;
+  if (nsteps > w->ck_cap) {
+    if (w->ck) QAS_ACHECK(c, hipFree(w->ck));
+    w->ck = nullptr;
+    w->ck_cap = 0;
+    hipError_t e = hipMalloc(&w->ck, per * (size_t)nsteps * sizeof(double));
+    if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint checkpoints: ") + hipGetErrorName(e));
+    w->ck_cap = nsteps;
+  }
+  w->dts.assign(dts, dts + nsteps);
+  w->nrec = 0;
+  if (!c->bounded) {
+    QAS_ACHECK(c, launch_bound(c, c->U[c->cur], 8));
+    c->bounded = true;
+  }
+  const size_t bytes = (size_t)c->geo.nd * sizeof(double);
+  for (int n = 0; n < nsteps; n++) {
+    for (int m = 0; m < 8; m++)
+      QAS_ACHECK(c, hipMemcpyAsync(w->ck + per * n + w->stride * m, c->U[c->cur][m], bytes,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    r = qas_step(c, dts[n]);
+    if (r) return r;
+  }
+  w->nrec = nsteps;
+  QAS_ACHECK(c, hipStreamSynchronize(c->stream));
+  return QAS_OK;
+}
+
+extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double* g_init)
+{
+  if (!c || !c->slab || !g_final || !g_init) return afail(c, QAS_E_ARG, "qas_adj_backward: bad arguments");
+  if (!c->adj || c->adj->nrec == 0) return afail(c, QAS_E_STATE, "qas_adj_backward: no steps recorded");
+  qas_adj_work* w = c->adj;
+  const Geo& g = c->geo;
+  const size_t per = 8 * w->stride;
+  const unsigned nb = nblk(g.nd, 256);
+  // the final state came out of bound(): fold the given adjoint onto kept cells
+  {
+    FoldArgs f;
+    for (int m = 0; m < 8; m++) {
+      f.src[m] = g_final + (size_t)m * g.nd;
+      f.dst[m] = w->gO[m];
+      f.extra[m] = nullptr;
+    }
+    hipLaunchKernelGGL(k_adj_fold, nb, 256, 0, c->stream, f, g);
+    QAS_ACHECK(c, hipGetLastError());
+  }
+  for (int n = w->nrec - 1; n >= 0; n--) {
+    QAS_ACHECK(c, launch_set_dt(c, w->dts[n]));
+    double* U[8];
+    for (int m = 0; m < 8; m++) U[m] = w->ck + per * n + w->stride * m;
+    // forward recompute: stage inputs S1, S2 and every stage's intermediates
+    QAS_ACHECK(c, fwd_stage(c, 0, U, U, w->S[0], 0));
+    QAS_ACHECK(c, fwd_stage(c, 1, w->S[0], U, w->S[1], 1));
+    QAS_ACHECK(c, fwd_stage(c, 2, w->S[1], U, w->S[2], 2));
+    {
+      FoldArgs f;
+      for (int m = 0; m < 8; m++) f.dst[m] = w->gU0[m];
+      hipLaunchKernelGGL(k_adj_clear8, nb, 256, 0, c->stream, f, g.nd);
+    }
+    QAS_ACHECK(c, adj_stage(c, 2, w->S[1], false));
+    QAS_ACHECK(c, adj_stage(c, 1, w->S[0], false));
+    QAS_ACHECK(c, adj_stage(c, 0, U, true));
+  }
+  for (int m = 0; m < 8; m++)
+    QAS_ACHECK(c, hipMemcpyAsync(g_init + (size_t)m * g.nd, w->gO[m], (size_t)g.nd * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+  QAS_ACHECK(c, hipStreamSynchronize(c->stream));
+  return QAS_OK;
+}
+
+extern "C" int qas_adj_backward(qas_ctx* c, const double* g_final, double* g_init)
+{
+  if (!c || !c->slab || !g_final || !g_init) return afail(c, QAS_E_ARG, "qas_adj_backward: bad arguments");
+  const size_t bytes = 8 * (size_t)c->geo.nd * sizeof(double);
+  double *dg = nullptr, *di = nullptr;
+  QAS_ACHECK(c, hipMalloc(&dg, bytes));
+  hipError_t e = hipMalloc(&di, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(dg);
+    return afail(c, QAS_E_NOMEM, "qas_adj_backward: staging");
+  }
+  int r = QAS_OK;
+  e = hipMemcpy(dg, g_final, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) r = afail(c, QAS_E_HIP, "qas_adj_backward: upload");
+  if (!r) r = qas_adj_backward_device(c, dg, di);
+  if (!r && hipMemcpy(g_init, di, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    r = afail(c, QAS_E_HIP, "qas_adj_backward: download");
+  (void)hipFree(dg);
+  (void)hipFree(di);
+  return r;
+}

@@ -169,6 +169,7 @@ extern "C" void qas_destroy(qas_ctx* c)
 {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  adj_release(c);
   if (c->slab) hipFree(c->slab);
   if (c->d_hist) hipFree(c->d_hist);
   if (c->gexec) hipGraphExecDestroy(c->gexec);

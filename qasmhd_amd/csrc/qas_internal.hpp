@@ -118,6 +118,8 @@ struct qas_ctx {
   std::vector<hipEvent_t> prof_ev;    // start/stop pairs
   std::vector<int> prof_kid;
   size_t prof_n = 0;
+  // discrete-adjoint workspace (qas_adjoint2d.hip), created on first use
+  struct qas_adj_work* adj = nullptr;
 };
 
 namespace qas {
@@ -155,6 +157,7 @@ hipError_t launch_zero_counter(qas_ctx* c);
 hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
+void adj_release(qas_ctx* c);  // frees the adjoint workspace (qas_adjoint2d.hip)
 // FP32 face sweeps of the mixed path (qas_mixed.hip)
 void (*pick_sweep_f32(int ndim, int dir, int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
 }  // namespace qas
