@@ -18,11 +18,13 @@
 //             weight and g_ct
 //   faceparts adjoint of the corner EMF interpolation -> g of the four face
 //             parts (x/y sweep, upwind/central)
-//   sweep<X>, sweep<Y>
-//             per line segment (one wavefront, 58 owned cells): forward
-//             reconstruction + lr_fb, face flux adjoint (HLLD), adjoint of
-//             the reconstruction, gathered onto cells by cross-lane
-//             shuffles -> g of ro, vx, vy, vz, cx, cy, bz, pr and face B
+//   sweep     per direction two passes over line segments (one wavefront
+//             each): the face pass reconstructs the face states and runs
+//             the flux adjoint (HLLD) per face, writing the adjoints of the
+//             two face states; the cell pass runs the adjoint of the
+//             reconstruction per cell and gathers the stencil adjoints onto
+//             cells by cross-lane shuffles (58 owned cells per wavefront)
+//             -> g of ro, vx, vy, vz, cx, cy, bz, pr and face B
 //   prim      adjoint of the primitive recovery + CUCT corner terms of
 //             ro, vx, vy + the RK pass-through b * g_out -> g_in of
 //             ro, mx, my, mz, bz, en; a * g_out into the step-base adjoint
@@ -54,6 +56,7 @@ struct AdjPtrs {
   double* gbf[2];        // adjoint of face bx (x sweep), by (y sweep)
   double* gemf[4];       // adjoint of the face parts: x upwind, x central, y upwind, y central
   double *gez, *gct, *ctv;  // corner adjoints of ez and of the CUCT weight; forward weight
+  double* gfc[17];       // per-face adjoints of the face states (sweep face pass -> cell pass)
   const double* dt;
   double a, b;
 };
@@ -200,10 +203,55 @@ __global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
 }
 
 // ---------------------------------------------------------------- sweeps
-// Lane l of a segment handles cell q = seg*SEGA - 3 + l and face q; lanes
-// 3..60 own their cell and face.  Forward as in k_sweep2d (RMN 1, ODR 4).
+// Forward of k_sweep2d (RMN 1, ODR 4) up to the face states, for the cell
+// of this lane; shared by the two adjoint sweep kernels.
 template <int DIR>
-__global__ __launch_bounds__(256) void k_adj_sweep2d(AdjPtrs p, Geo g, double gam, int nseg)
+struct SweepFwd {
+  const double *vn, *vt, *vu, *bt, *bu, *cn, *bf;
+  double s[7][5];
+  double pp[5], qq[5];
+  double bnc, bnm, bnp;
+  __device__ __forceinline__ void load(const AdjPtrs& p, long long sc, long long o)
+  {
+    vn = DIR == 0 ? p.vx : p.vy;
+    vt = DIR == 0 ? p.vy : p.vz;
+    vu = DIR == 0 ? p.vz : p.vx;
+    bt = DIR == 0 ? p.cy : p.in[6];
+    bu = DIR == 0 ? p.in[6] : p.cx;
+    cn = DIR == 0 ? p.cx : p.cy;
+    bf = DIR == 0 ? p.in[4] : p.in[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = sc + (m - 2) * o;
+      s[0][m] = p.in[0][a];
+      s[1][m] = vn[a];
+      s[2][m] = vt[a];
+      s[3][m] = vu[a];
+      s[4][m] = bt[a];
+      s[5][m] = bu[a];
+      s[6][m] = p.pr[a];
+    }
+    bnc = cn[sc];
+    // lr_fb inputs: p = b_pair * v_n, q = v_pair (x: cy, vy; y: cx, vx)
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      pp[m] = (DIR == 0 ? s[4][m] : s[5][m]) * s[1][m];
+      qq[m] = DIR == 0 ? s[2][m] : s[3][m];
+    }
+    bnm = bf[sc];
+    bnp = bf[sc + o];
+  }
+};
+
+// number of adjoint-face arrays: gL[7] (left state of the face, i.e. the
+// right face state of cell q-1), gR[7], g ql(q-1), g qr(q), g bn(face)
+constexpr int NGF = 17;
+
+// Face pass: lane l reconstructs cell q = seg*63 - 1 + l and, for l >= 1,
+// owns face q (between cells q-1 and q): flux adjoint (HLLD) from the
+// divergence and EMF-part adjoints, written to p.gfc[.][face].
+template <int DIR>
+__global__ __launch_bounds__(256) void k_adj_sweep_face(AdjPtrs p, Geo g, double gam, int nseg)
 {
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
@@ -214,57 +262,25 @@ __global__ __launch_bounds__(256) void k_adj_sweep2d(AdjPtrs p, Geo g, double ga
   const int n = DIR == 0 ? g.nx : g.ny;
   const long long o = DIR == 0 ? 1 : g.sy;
   const long long base = DIR == 0 ? g.sy * line : (long long)line;
-  const int q = seg * SEGA - 3 + lane;
+  const int q = seg * (WAVE - 1) - 1 + lane;
   const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
-  const long long sc = base + o * qc;
-
-  const double* vn = DIR == 0 ? p.vx : p.vy;
-  const double* vt = DIR == 0 ? p.vy : p.vz;
-  const double* vu = DIR == 0 ? p.vz : p.vx;
-  const double* bt = DIR == 0 ? p.cy : p.in[6];
-  const double* bu = DIR == 0 ? p.in[6] : p.cx;
-  const double* cn = DIR == 0 ? p.cx : p.cy;
-  const double* bf = DIR == 0 ? p.in[4] : p.in[5];
-  double s[7][5];
+  SweepFwd<DIR> f;
+  f.load(p, base + o * qc, o);
+  double vl[7], vr[7], ql, qr;
+  lrstate<4>(f.s, f.bnc, gam, vl, vr);
+  lr_fb<4>(f.pp, f.qq, f.bnm, f.bnp, ql, qr);
+  double L[7];
 #pragma unroll
-  for (int m = 0; m < 5; m++) {
-    const long long a = sc + (m - 2) * o;
-    s[0][m] = p.in[0][a];
-    s[1][m] = vn[a];
-    s[2][m] = vt[a];
-    s[3][m] = vu[a];
-    s[4][m] = bt[a];
-    s[5][m] = bu[a];
-    s[6][m] = p.pr[a];
-  }
-  const double bnc = cn[sc];
-  double vl[7], vr[7];
-  lrstate<4>(s, bnc, gam, vl, vr);
-  // lr_fb: p = b_pair * v_n, q = v_pair (x: cy, vy; y: cx, vx)
-  double pp[5], qq[5], ql, qr;
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    pp[m] = (DIR == 0 ? s[4][m] : s[5][m]) * s[1][m];
-    qq[m] = DIR == 0 ? s[2][m] : s[3][m];
-  }
-  const double bnm = bf[sc], bnp = bf[sc + o];
-  lr_fb<4>(pp, qq, bnm, bnp, ql, qr);
-
-  double L[7], R[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) {
-    L[k] = __shfl_up(vl[k], 1, WAVE);
-    R[k] = vr[k];
-  }
-  const bool face_ok = lane >= 1 && q >= 3 && q <= n - 3;
+  for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
+  (void)ql;
+  if (lane < 1 || q < 0 || q >= n) return;
+  const long long fq = base + o * q;
   double gL[7] = {}, gR[7] = {};
-  double gbnf = 0.0, gqlf = 0.0, gqr = 0.0;
-  if (face_ok) {
-    const long long fq = base + o * q;
-    const int tl = line;
+  double gbn = 0.0, gqlf = 0.0, gqr = 0.0;
+  if (q >= 3 && q <= n - 3) {
     const int ntl = DIR == 0 ? g.ny : g.nx;
     double gF[7] = {};
-    if (tl >= NG && tl <= ntl - NG - 1) {
+    if (line >= NG && line <= ntl - NG - 1) {
       const double sfac = -p.b * (*p.dt / (DIR == 0 ? g.dx : g.dy));
       // flux component fed into each updated variable (k_sweep2d fv[])
       const int fk[6] = {0, DIR == 0 ? 1 : 3, DIR == 0 ? 2 : 1, DIR == 0 ? 3 : 2, DIR == 0 ? 5 : 4, 6};
@@ -287,35 +303,63 @@ __global__ __launch_bounds__(256) void k_adj_sweep2d(AdjPtrs p, Geo g, double ga
     const double gfc = gc - gu;
     gqlf = 0.5 * gfc;
     gqr = 0.5 * gfc;
-    const double bnf = 0.5 * (bf[fq] + bf[fq]);
-    double gbnh = 0.0;
-    hlld_adj(L, R, bnf, gam, gF, gL, gR, gbnh);
-    gbnf = gbnh;  // d(0.5 (b + b))/db = 1
+    const double bnf = 0.5 * (f.bf[fq] + f.bf[fq]);
+    hlld_adj(L, vr, bnf, gam, gF, gL, gR, gbn);  // d(0.5 (b + b))/db = 1
   }
-  // this cell's left state feeds face q+1 (lane+1)
-  double gvl[7], gvr[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
-    gvl[k] = __shfl_down(gL[k], 1, WAVE);
-    gvr[k] = gR[k];
+    p.gfc[k][fq] = gL[k];
+    p.gfc[7 + k][fq] = gR[k];
   }
-  const double gql = __shfl_down(gqlf, 1, WAVE);
+  p.gfc[14][fq] = gqlf;
+  p.gfc[15][fq] = gqr;
+  p.gfc[16][fq] = gbn;
+}
+
+// Cell pass: lane l handles cell q = seg*SEGA - 3 + l; lanes 3..60 own
+// their cell and face.  Reconstruction adjoint from the face adjoints of
+// faces q (right state) and q+1 (left state), gathered onto cells.
+template <int DIR>
+__global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
+{
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int nline = DIR == 0 ? g.ny : g.nx;
+  if (wave >= nseg * nline) return;  // whole wavefront exits together
+  const int line = wave / nseg;
+  const int seg = wave - line * nseg;
+  const int n = DIR == 0 ? g.nx : g.ny;
+  const long long o = DIR == 0 ? 1 : g.sy;
+  const long long base = DIR == 0 ? g.sy * line : (long long)line;
+  const int q = seg * SEGA - 3 + lane;
   double gs[7][5] = {};
   double gcn = 0.0, gbm = 0.0, gbp = 0.0;
   const bool cell_ok = q >= 2 && q <= n - 3;
   if (cell_ok) {
-    lrstate_adj(s, bnc, gam, gvl, gvr, gs, gcn);
+    const long long sc = base + o * q;
+    SweepFwd<DIR> f;
+    f.load(p, sc, o);
+    double gvl[7], gvr[7];
+    const bool fr_ok = q >= 3, fl_ok = q + 1 <= n - 3;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      gvl[k] = fl_ok ? p.gfc[k][sc + o] : 0.0;
+      gvr[k] = fr_ok ? p.gfc[7 + k][sc] : 0.0;
+    }
+    const double gql = fl_ok ? p.gfc[14][sc + o] : 0.0;
+    const double gqr = fr_ok ? p.gfc[15][sc] : 0.0;
+    lrstate_adj(f.s, f.bnc, gam, gvl, gvr, gs, gcn);
     double gpp[5] = {}, gqq[5] = {};
-    lr_fb_adj(pp, qq, bnm, bnp, gql, gqr, gpp, gqq, gbm, gbp);
+    lr_fb_adj(f.pp, f.qq, f.bnm, f.bnp, gql, gqr, gpp, gqq, gbm, gbp);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       if (DIR == 0) {
-        gs[1][m] += gpp[m] * s[4][m];
-        gs[4][m] += gpp[m] * s[1][m];
+        gs[1][m] += gpp[m] * f.s[4][m];
+        gs[4][m] += gpp[m] * f.s[1][m];
         gs[2][m] += gqq[m];
       } else {
-        gs[1][m] += gpp[m] * s[5][m];
-        gs[5][m] += gpp[m] * s[1][m];
+        gs[1][m] += gpp[m] * f.s[5][m];
+        gs[5][m] += gpp[m] * f.s[1][m];
         gs[3][m] += gqq[m];
       }
     }
@@ -328,10 +372,11 @@ __global__ __launch_bounds__(256) void k_adj_sweep2d(AdjPtrs p, Geo g, double ga
     gc7[k] = __shfl_down(gs[k][0], 2, WAVE) + __shfl_down(gs[k][1], 1, WAVE) + gs[k][2] +
              __shfl_up(gs[k][3], 1, WAVE) + __shfl_up(gs[k][4], 2, WAVE);
   }
-  const double gface = gbnf + gbm + __shfl_up(gbp, 1, WAVE);
+  const double gbpl = __shfl_up(gbp, 1, WAVE);
   const bool own = lane >= 3 && lane < 3 + SEGA && q >= 0 && q < n;
   if (!own) return;
   const long long c = base + o * q;
+  const double gface = ((q >= 3 && q <= n - 3) ? p.gfc[16][c] : 0.0) + gbm + gbpl;
   // stencil rows -> adjoint arrays (ro vx vy vz cx cy bz pr)
   const int gi[7] = {0, DIR == 0 ? 1 : 2, DIR == 0 ? 2 : 3, DIR == 0 ? 3 : 1, DIR == 0 ? 5 : 6, DIR == 0 ? 6 : 4, 7};
 #pragma unroll
@@ -508,6 +553,7 @@ struct qas_adj_work {
   double* gbf[2] = {};
   double* gemf[4] = {};
   double *gez = nullptr, *gct = nullptr, *ctv = nullptr;
+  double* gfc[17] = {};
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
   int ck_cap = 0;
   int nrec = 0;               // steps recorded
@@ -557,7 +603,7 @@ static int adj_workspace(qas_ctx* c)
   c->adj = w;
   const size_t stride = (size_t)((c->geo.nd + 63) / 64 * 64);
   w->stride = stride;
-  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 3;
+  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 3 + 17;
   hipError_t e = hipMalloc(&w->slab, narr * stride * sizeof(double));
   if (e != hipSuccess) {
     adj_release(c);
@@ -581,6 +627,7 @@ static int adj_workspace(qas_ctx* c)
   w->gez = take();
   w->gct = take();
   w->ctv = take();
+  for (int k = 0; k < 17; k++) w->gfc[k] = take();
   QAS_ACHECK(c, hipMemsetAsync(w->slab, 0, narr * stride * sizeof(double), c->stream));
   return QAS_OK;
 }
@@ -636,6 +683,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   p.gez = w->gez;
   p.gct = w->gct;
   p.ctv = w->ctv;
+  for (int k = 0; k < NGF; k++) p.gfc[k] = w->gfc[k];
   p.dt = c->d_dt;
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
@@ -646,9 +694,12 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   if (ctw) hipLaunchKernelGGL((k_adj_corner<1>), nb, 256, 0, st, p, g);
   else hipLaunchKernelGGL((k_adj_corner<0>), nb, 256, 0, st, p, g);
   hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
-  const int nsx = (g.nx + SEGA - 1) / SEGA, nsy = (g.ny + SEGA - 1) / SEGA;
-  hipLaunchKernelGGL((k_adj_sweep2d<0>), nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam, nsx);
-  hipLaunchKernelGGL((k_adj_sweep2d<1>), nblk((long long)nsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->s.gam, nsy);
+  const int fsx = (g.nx + WAVE - 2) / (WAVE - 1), fsy = (g.ny + WAVE - 2) / (WAVE - 1);
+  const int csx = (g.nx + SEGA - 1) / SEGA, csy = (g.ny + SEGA - 1) / SEGA;
+  hipLaunchKernelGGL((k_adj_sweep_face<0>), nblk((long long)fsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam, fsx);
+  hipLaunchKernelGGL((k_adj_sweep_cell<0>), nblk((long long)csx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam, csx);
+  hipLaunchKernelGGL((k_adj_sweep_face<1>), nblk((long long)fsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->s.gam, fsy);
+  hipLaunchKernelGGL((k_adj_sweep_cell<1>), nblk((long long)csy * g.nx * WAVE, 256), 256, 0, st, p, g, c->s.gam, csy);
   if (ctw) {
     hipLaunchKernelGGL((k_adj_prim<1>), nb, 256, 0, st, p, g, c->s.gam);
     hipLaunchKernelGGL((k_adj_face<1>), nb, 256, 0, st, p, g);
