@@ -133,7 +133,7 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
   const int nemf = g->ndim == 3 ? 12 : (g->ndim == 2 ? 8 : 0);
   const int nedge = g->ndim == 3 ? 3 : (g->ndim == 2 ? 1 : 0);
   const size_t narr = 24 + nprim + nacc + nemf + nedge;
-  c->slab_bytes = narr * stride * sizeof(double) + 4096;
+  c->slab_bytes = narr * stride * sizeof(double) + 4096 + sizeof(double) * CFL_MAX_BLOCKS;
   e = hipMalloc(&c->slab, c->slab_bytes);
   if (e != hipSuccess) return fail(c, QAS_E_NOMEM, std::string("hipMalloc of the domain failed: ") + hipGetErrorName(e));
   double* p = (double*)c->slab;
@@ -152,14 +152,12 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
   for (int k = 0; k < nemf; k++) { c->emf[k] = p; p += stride; }
   if (g->ndim == 2) { c->ez = p; p += stride; }
   if (g->ndim == 3) { c->ex = p; p += stride; c->ey = p; p += stride; c->ez = p; p += stride; }
-  // small device words: dt, vmax accumulator
+  // small device words: dt, history counter; CFL partial maxima
   char* tail = (char*)p;
   c->d_dt = (double*)tail;
-  c->d_vmax = (unsigned long long*)(tail + 64);
+  c->d_cflpart = (double*)(tail + 4096);  // per-block maxima of the CFL reduction
   c->d_hcount = (long long*)(tail + 128);
   QAS_CHECK(c, hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
-  const double one = 1.0;
-  QAS_CHECK(c, hipMemcpyAsync(c->d_vmax, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
   QAS_CHECK(c, hipHostMalloc((void**)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
   QAS_CHECK(c, hipStreamSynchronize(c->stream));
   return QAS_OK;

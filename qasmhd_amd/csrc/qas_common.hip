@@ -6,12 +6,11 @@
 //               reference would copy into it, times the pass factors.
 //   k_cfl       setdt(1)'s max of |v| + vfast over the interior with the
 //               2nd-order cell-centred B (mhd2d_class.cpp:11-33,
-//               mhd1d_class.cpp:11-30, mhd3d_class.cpp:15-46): wavefront
-//               max by cross-lane shuffles, one 64-bit atomic max per
-//               wavefront on the bit pattern (all candidates are >= 0, so
-//               their IEEE bit patterns order like the values).
-//   k_cfl_final dt = cfl*dr/vmax into device memory; re-arms the
-//               accumulator at 1.0 (the reference's floor on vmax).
+//               mhd1d_class.cpp:11-30, mhd3d_class.cpp:15-46): grid-stride
+//               loop, wavefront max by cross-lane shuffles, block max in
+//               LDS, one partial maximum per block (no atomics).
+//   k_cfl_final dt = cfl*dr/vmax into device memory from the partials,
+//               with the reference's floor vmax >= 1.0.
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
 
@@ -87,7 +86,7 @@ hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar)
 // ------------------------------------------------------------------ CFL
 struct CflArgs {
   const double* u[8];
-  unsigned long long* vmax;
+  double* part;  // one maximum per block
   double gam;
   int stx4, sty5, stz6;
 };
@@ -102,6 +101,21 @@ __device__ __forceinline__ double wave_max(double v)
   return v;
 }
 
+// max over the block's 4 wavefronts, valid in thread 0
+__device__ __forceinline__ double block_max(double v)
+{
+  __shared__ double wm[4];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) v = (wm[w] > v) ? wm[w] : v;
+  }
+  return v;
+}
+
+// grid-stride over the interior; each block writes its maximum of
+// |v| + vfast (floor 1.0, the reference's initial vmax) to part[block]
 __global__ __launch_bounds__(256) void k_cfl(CflArgs a, Geo g)
 {
   const int ox = qas::NG;
@@ -109,9 +123,9 @@ __global__ __launch_bounds__(256) void k_cfl(CflArgs a, Geo g)
   const int oz = g.ndim >= 3 ? qas::NG : 0;
   const int ni = g.nx - 2 * ox, nj = g.ny - 2 * oy, nk = g.nz - 2 * oz;
   const long long ntot = (long long)ni * nj * nk;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double vm = 1.0;
-  if (t < ntot) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntot;
+       t += (long long)gridDim.x * blockDim.x) {
     const int i = ox + (int)(t % ni);
     const int j = oy + (int)((t / ni) % nj);
     const int k = oz + (int)(t / ((long long)ni * nj));
@@ -145,17 +159,23 @@ __global__ __launch_bounds__(256) void k_cfl(CflArgs a, Geo g)
     else v = sqrt(vx * vx + vy * vy + vz * vz) + vf;
     if (v > vm) vm = v;
   }
-  vm = wave_max(vm);
-  if ((threadIdx.x & 63) == 0) atomicMax(a.vmax, (unsigned long long)__double_as_longlong(vm));
+  vm = block_max(vm);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = vm;
 }
 
-__global__ void k_cfl_final(unsigned long long* vmax, double* dt, double* hist_slot, double cfl_dr)
+// dt = cfl * dr / max(part[0..nb)); one block of 256 threads.  max is
+// exact and order-independent, so dt equals the reference's serial loop.
+__global__ __launch_bounds__(256) void k_cfl_final(const double* part, int nb, double* dt, double* hist_slot,
+                                                   double cfl_dr)
 {
-  const double vm = __longlong_as_double((long long)*vmax);
-  const double d = cfl_dr / vm;
-  *dt = d;
-  if (hist_slot) *hist_slot = d;
-  *vmax = (unsigned long long)__double_as_longlong(1.0);
+  double vm = 1.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) vm = (part[b] > vm) ? part[b] : vm;
+  vm = block_max(vm);
+  if (threadIdx.x == 0) {
+    const double d = cfl_dr / vm;
+    *dt = d;
+    if (hist_slot) *hist_slot = d;
+  }
 }
 
 __global__ void k_set_dt(double* dt, double v) { *dt = v; }
@@ -173,14 +193,17 @@ hipError_t launch_cfl(qas_ctx* c, const double* const U[8])
   const Geo& g = c->geo;
   CflArgs a;
   for (int m = 0; m < 8; m++) a.u[m] = U[m];
-  a.vmax = c->d_vmax;
+  a.part = c->d_cflpart;
   a.gam = c->s.gam;
   a.stx4 = c->s.st[0][4];
   a.sty5 = c->s.st[1][5];
   a.stz6 = c->s.st[2][6];
   const int ox = qas::NG, oy = g.ndim >= 2 ? qas::NG : 0, oz = g.ndim >= 3 ? qas::NG : 0;
   const long long n = (long long)(g.nx - 2 * ox) * (g.ny - 2 * oy) * (g.nz - 2 * oz);
-  QAS_LAUNCH(c, QAS_K_CFL, k_cfl, (unsigned)((n + 255) / 256), 256, 0, c->stream, a, g);
+  long long nb = (n + 255) / 256;
+  if (nb > CFL_MAX_BLOCKS) nb = CFL_MAX_BLOCKS;
+  c->cfl_blocks = (int)nb;
+  QAS_LAUNCH(c, QAS_K_CFL, k_cfl, (unsigned)nb, 256, 0, c->stream, a, g);
   return hipGetLastError();
 }
 
@@ -194,7 +217,7 @@ hipError_t launch_cfl_finalize(qas_ctx* c, double* hist_slot)
     dr = (t >= g.dz) ? g.dz : t;
   }
   const double cfl_dr = c->s.cfl * dr;
-  hipLaunchKernelGGL(k_cfl_final, 1, 1, 0, c->stream, c->d_vmax, c->d_dt, hist_slot, cfl_dr);
+  hipLaunchKernelGGL(k_cfl_final, 1, 256, 0, c->stream, c->d_cflpart, c->cfl_blocks, c->d_dt, hist_slot, cfl_dr);
   return hipGetLastError();
 }
 

@@ -57,6 +57,13 @@ QAS_HD void lin5(const T* f, T& fl, T& fr)
   }
 }
 
+// eps of the WCNS smoothness weights 1/(is^2 + eps): the reference's 1e-40
+// in FP64.  1e-40 is not a normal FP32 number and 1/1e-40 overflows FP32,
+// so the FP32 (mixed) instantiation uses 1e-30, still far below is^2 of any
+// non-uniform stencil.
+template <typename T> QAS_HD T wcns_eps() { return T(1e-40); }
+template <> QAS_HD float wcns_eps<float>() { return 1e-30f; }
+
 template <typename T>
 QAS_HD T minmod(T a, T b)
 {
@@ -74,7 +81,7 @@ QAS_HD void interp5(const T* f, T& fl, T& fr)
     fl = f[2] + d;
     fr = f[2] - d;
   } else if constexpr (ODR == 3) {  // WCNS3
-    const T eps = T(1e-40);
+    const T eps = wcns_eps<T>();
     T a = f[1] - f[2];
     T s0 = a * a;
     a = f[3] - f[2];
@@ -94,7 +101,7 @@ QAS_HD void interp5(const T* f, T& fl, T& fr)
     w1 = T(1.0) - w0;
     fr = T(0.5) * (w1 * (-f[3] + T(3.0) * f[2]) + w0 * (f[2] + f[1]));
   } else {  // WCNS4
-    const T eps = T(1e-40);
+    const T eps = wcns_eps<T>();
     const T c13 = T(13. / 12.);
     T a = f[0] - T(4.0) * f[1] + T(3.0) * f[2];
     T b = f[0] - T(2.0) * f[1] + f[2];

@@ -12,6 +12,7 @@ namespace qas {
 constexpr int NG = 4;       // ghost layers (mhd2d_class.hpp:10)
 constexpr int SEG = 60;     // faces owned by one wavefront segment (64 lanes - 4 halo)
 constexpr int WAVE = 64;
+constexpr int CFL_MAX_BLOCKS = 1024;  // blocks of the CFL reduction (grid-stride)
 
 // Boundary flags as kernel arguments (dnxs/dnys/dnzs, stxs/stys/stzs).
 struct BcFlags {
@@ -101,7 +102,8 @@ struct qas_ctx {
   double* d_dt = nullptr;             // current dt
   double* d_hist = nullptr;           // dt history of qas_run
   int hist_cap = 0;
-  unsigned long long* d_vmax = nullptr;
+  double* d_cflpart = nullptr;        // CFL_MAX_BLOCKS per-block maxima
+  int cfl_blocks = 1;                 // blocks of the last CFL launch
   long long* d_hcount = nullptr;      // dt-history slot counter (graph replays)
   // hipGraph of two consecutive steps of qas_run (see qas_run)
   hipGraphExec_t gexec = nullptr;

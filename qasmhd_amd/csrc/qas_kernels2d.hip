@@ -150,7 +150,7 @@ template <int DIR>
 static sweep_fn pick_sweep(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return k_sweep2d<DIR, R, O, double>;
+  if (rmn == R && odr == O) return DIR == 0 ? k_sweep2d<0, R, O, double> : k_sweep2d_y<R, O, double>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
@@ -180,7 +180,8 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   sweep_fn fy = mixed ? pick_sweep_f32(2, 1, c->s.rmn, odr) : pick_sweep<1>(c->s.rmn, odr);
   if (!fx || !fy) return hipErrorInvalidValue;
   QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx);
-  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, nblk((long long)nsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsy);
+  // y sweep: one 256-thread workgroup per (4-column group, segment)
+  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, p, g, c->bc, gam, nsy);
   // 4) corner EMF, 5) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);

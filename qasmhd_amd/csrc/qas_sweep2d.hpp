@@ -27,8 +27,157 @@ __device__ __forceinline__ double dv_at(const double* v, int i, int j, const Geo
   return apply(sy, apply(sx, r));
 }
 
+// XCD-aware tile order: the hardware hands consecutive workgroups to the 8
+// XCDs in turn; remap block b of nb so that each XCD runs one contiguous
+// range of logical tiles (neighbouring column groups then share that XCD's
+// L2 lines).
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+  const int x = b % 8, slot = b / 8;
+  const int q = nb / 8, r = nb % 8;
+  return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+// y sweep, LDS-staged.  One workgroup = 4 wavefronts = 4 adjacent columns
+// of one 60-face segment.  The stencil inputs of the 68 rows f0-4..f0+63 of
+// the 4 columns are loaded cooperatively (4 consecutive threads read 4
+// consecutive doubles of a row) into LDS, column-major; each wavefront then
+// runs the same per-lane computation as the x sweep on its column from LDS;
+// the outputs (y divergence of 6 variables, 2 face parts) go back through
+// LDS and are written, and combined into the RK stage sum, by coalesced
+// threads.  Same arithmetic as a direct per-lane sweep: bit-identical.
+constexpr int YC = 4;          // columns per workgroup
+constexpr int YR = WAVE + 4;   // stencil rows per segment
+constexpr int YRP = YR + 1;    // padded LDS column length
+
+template <int RMN, int ODR, typename T>
+__device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam, int nseg)
+{
+  __shared__ double sh[9][YC][YRP];  // ro vy vz vx bz cx pr cy by
+  __shared__ double so[8][YC][WAVE];  // du[6], upwind and central B-flux parts
+  const int ncg = (g.nx + YC - 1) / YC;
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  if (L >= ncg * nseg) return;  // whole workgroup
+  const int cg = L % ncg, seg = L / ncg;
+  const int i0 = cg * YC;
+  const int f0 = 3 + seg * SEG;
+  const int n = g.ny;
+  const long long sy = g.sy;
+  {
+    const double* src[9] = {p.in[0], p.vy, p.vz, p.vx, p.in[6], p.cx, p.pr, p.cy, p.in[5]};
+    for (int e = threadIdx.x; e < YC * YR; e += blockDim.x) {
+      const int c = e % YC, r = e / YC;
+      int row = f0 - 4 + r;
+      row = row < 0 ? 0 : (row > n - 1 ? n - 1 : row);
+      int col = i0 + c;
+      col = col > g.nx - 1 ? g.nx - 1 : col;
+      const long long a = sy * row + col;
+#pragma unroll
+      for (int k = 0; k < 9; k++) sh[k][c][r] = src[k][a];
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int line = i0 + w;  // this wavefront's column (>= nx: computes, never stores)
+  const int q = f0 - 2 + lane;
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const int rb = qc - (f0 - 4);
+  T s[7][5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const int r = rb + m - 2;
+    s[0][m] = (T)sh[0][w][r];
+    s[1][m] = (T)sh[1][w][r];
+    s[2][m] = (T)sh[2][w][r];
+    s[3][m] = (T)sh[3][w][r];
+    s[4][m] = (T)sh[4][w][r];
+    s[5][m] = (T)sh[5][w][r];
+    s[6][m] = (T)sh[6][w][r];
+  }
+  // B-flux face states first (their inputs are already in s[]), then the
+  // characteristic reconstruction
+  T ql, qr;
+  {
+    T pp[5], qq[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      pp[m] = s[5][m] * s[1][m];  // cx * vy
+      qq[m] = s[3][m];            // vx
+    }
+    lr_fb<ODR>(pp, qq, (T)sh[8][w][rb], (T)sh[8][w][rb + 1], ql, qr);
+  }
+  T vl[7], vr[7];
+  lrstate<ODR>(s, (T)sh[7][w][rb], (T)gam, vl, vr);
+  T Lf[7], Rf[7], F[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    Lf[k] = __shfl_up(vl[k], 1, WAVE);
+    Rf[k] = vr[k];
+  }
+  const T qlf = __shfl_up(ql, 1, WAVE);
+  const int sf = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
+  const int rf = sf - (f0 - 4);
+  const T bnf = (T)(0.5 * (sh[8][w][rf] + sh[8][w][rf]));
+  T dv0 = T(0.0), dv1 = T(0.0);
+  if constexpr (RMN >= 2) {
+    const int lc = line > g.nx - 1 ? g.nx - 1 : line;
+    dv0 = (T)(sh[1][w][rf] - sh[1][w][rf - 1]);
+    dv1 = (T)tmin(dv_at<0>(p.vx, lc, sf - 1, g, bc), dv_at<0>(p.vx, lc, sf, g, bc));
+  }
+  riemann<RMN>(Lf, Rf, bnf, (T)gam, dv0, dv1, F);
+  const T fc = T(0.5) * (qlf + qr);
+  const T fbu = F[5] - fc;
+  const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
+  // y: (F0, F3, F1, F2, F4, F6) feed (ro, mx, my, mz, bz, en)
+  double fv[6];
+  fv[0] = F[0]; fv[1] = F[3]; fv[2] = F[1]; fv[3] = F[2]; fv[4] = F[4]; fv[5] = F[6];
+#pragma unroll
+  for (int v = 0; v < 6; v++) {
+    const double fm1 = __shfl_up(fv[v], 1, WAVE);
+    const double f1 = __shfl_down(fv[v], 1, WAVE);
+    const double f2 = __shfl_down(fv[v], 2, WAVE);
+    so[v][w][lane] = dfc<ODR>(fm1, fv[v], f1, f2);
+  }
+  so[6][w][lane] = (double)fbu;
+  so[7][w][lane] = (double)fc;
+  (void)own;
+  __syncthreads();
+  // coalesced write-back: thread -> (column c, lane r) of the 4 x 64 tile
+  {
+    const int c = threadIdx.x % YC, r = threadIdx.x / YC;
+    const int qq = f0 - 2 + r;
+    const int i = i0 + c;
+    const bool ownr = (r >= 2) && (r < 2 + SEG) && (qq <= n - 3) && (i < g.nx);
+    if (ownr) {
+      const long long a = sy * qq + i;
+      p.emf[4][a] = so[6][c][r];
+      p.emf[5][a] = so[7][c][r];
+      const bool upd = (qq >= NG) && (qq <= n - NG - 1) && (i >= NG) && (i <= g.nx - NG - 1);
+      if (upd) {
+        const double dtdy = *p.dt / g.dy;
+        const int vid[6] = {0, 1, 2, 3, 6, 7};
+#pragma unroll
+        for (int v = 0; v < 6; v++) {
+          const int m = vid[v];
+          p.out[m][a] = rk_upd(p.in[m][a], p.u0[m][a], (p.acc[v][a] - dtdy * so[v][c][r]), p.a, p.b);
+        }
+      }
+    }
+  }
+}
+
+// y faces for every column (grid: 4-column groups x segments)
+template <int RMN, int ODR, typename T>
+__global__ __launch_bounds__(256, 2) void k_sweep2d_y(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+{
+  sweep2d_y<RMN, ODR, T>(p, g, bc, gam, nseg);
+}
+
 // ------------------------------------------------------------------ sweeps
-// DIR 0: x faces for every row j; DIR 1: y faces for every column i.
+// DIR 0: x faces for every row j (one wavefront per 60-face segment of a
+// row).  DIR 1, the same per-lane layout along columns with strided loads,
+// is kept for reference and testing; the stage uses k_sweep2d_y.
 template <int DIR, int RMN, int ODR, typename T>
 __global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
 {

@@ -14,11 +14,19 @@ interface; this is a thin binding of qas_adj_forward / qas_adj_backward_device.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
-import torch
 
 from . import capi
+
+# torch ships its own libamdhip64 with the same soname as ROCm's; whichever
+# is loaded first serves the process.  torch must come first, or it finds no
+# GPU once the library has initialised the other copy.
+if "torch" not in sys.modules and capi._lib is not None:
+    raise ImportError("qasmhd_amd.torch_adjoint: import torch before loading libqasmhd_amd "
+                      "(both bind libamdhip64.so.7; torch's copy must be loaded first)")
+import torch  # noqa: E402
 
 
 class IdealSteps(torch.autograd.Function):

@@ -73,7 +73,10 @@ def test_1d_brio_wu_golden(qas, rmn):
                                                (4, 20, 36, 0, 2), (5, 64, 64, 1, 4)])
 def test_2d_vs_oracle(qas, oracle, seed, n, ny, rmn, odr):
     from qasmhd_amd import problems
-    p = problems.turbulence2d(n=n, ny=ny, seed=seed, vrms=1.2)
+    # Roe at 2nd order with open/mirror walls breaks down (NaN in the oracle
+    # too) at rms Mach 1.2 on this grid; that case runs at 0.8
+    p = problems.turbulence2d(n=n, ny=ny, seed=seed, vrms=0.8 if (rmn, odr) == (0, 2) else 1.2)
+    assert np.isfinite(p.state).all()
     dn = np.zeros((3, 8), dtype=int)
     if seed % 2 == 0:
         dn[0] = [2] * 8
@@ -86,6 +89,7 @@ def test_2d_vs_oracle(qas, oracle, seed, n, ny, rmn, odr):
         s.upload(p.state)
         hist, dt_now = s.run(10, flg=1)
         out = s.download()
+    assert np.isfinite(ref).all()
     assert np.array_equal(hist, h_ref) and dt_now == dt_ref
     assert np.array_equal(out, ref)
 
