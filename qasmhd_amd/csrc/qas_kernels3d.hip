@@ -165,7 +165,7 @@ template <int DIR>
 static sweep3_fn pick3(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return k_sweep3d<DIR, R, O, double>;
+  if (rmn == R && odr == O) return DIR == 0 ? k_sweep3d<0, R, O, double> : k_sweep3d_t<DIR == 0 ? 1 : DIR, R, O, double>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
@@ -191,8 +191,14 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   for (int d = 0; d < 3; d++) {
     if (!f[d]) return hipErrorInvalidValue;
     const int nseg = (dims[d] - 5 + SEG - 1) / SEG;
-    const long long nline = g.nd / dims[d];
-    QAS_LAUNCH(c, QAS_K_SWEEP_X + d, f[d], nblk3(nline * nseg * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nseg);
+    if (d == 0) {
+      const long long nline = g.nd / dims[d];
+      QAS_LAUNCH(c, QAS_K_SWEEP_X, f[d], nblk3(nline * nseg * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nseg);
+    } else {
+      // one 256-thread workgroup per (4-column group, segment, other transverse index)
+      const long long nwg = (long long)((g.nx + YC - 1) / YC) * nseg * (d == 1 ? g.nz : g.ny);
+      QAS_LAUNCH(c, QAS_K_SWEEP_X + d, f[d], (unsigned)nwg, 256, 0, st, p, g, c->bc, gam, nseg);
+    }
   }
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5) * (g.nz - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7) * (g.nz - 7);

@@ -32,7 +32,7 @@ template <int DIR>
 static sweep_fn_t pick3(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return k_sweep3d<DIR, R, O, float>;
+  if (rmn == R && odr == O) return DIR == 0 ? k_sweep3d<0, R, O, float> : k_sweep3d_t<DIR == 0 ? 1 : DIR, R, O, float>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
