@@ -12,6 +12,12 @@ plus the CFL reduction every 2nd step, on a state already resident in HBM;
 the metric is interior zone-cycles per second.
 
 Also reported, on one JSON line:
+  mixed         FP64 vs mixed precision (FP32 reconstruction / flux, FP64
+                state, RK and pressure recovery) on the low-Mach
+                Kelvin-Helmholtz workload at 2048^2: both rates and the L1
+                difference of density and pressure
+  adjoint       time of the hand-written discrete adjoint over 20 steps of a
+                256^2 periodic run against the plain forward of those steps
   roofline      the dominant kernel's algorithmic work per launch (flops and
                 compulsory bytes from the library's work model) over its
                 average launch time, measured with HIP events on the
@@ -83,6 +89,89 @@ def cpu_baseline(p, nsteps):
             "sample": "first %d steps of the same run (C oracle, OpenMP %d threads), %.2f s" % (nsteps, threads, el)}
 
 
+def _interior(a, ndim):
+    g = 4
+    return a[(Ellipsis,) + (slice(g, -g),) * ndim]
+
+
+def mixed_line(n, steps):
+    """FP64 vs mixed precision on the low-Mach Kelvin-Helmholtz workload
+    (MLAU, WCNS-4, CUCT, RK3): zone-cycles/s of both at the same fixed dt
+    and the L1 difference of density and pressure after `steps` steps."""
+    import numpy as np
+    from qasmhd_amd import capi, problems
+    p = problems.kelvin_helmholtz(n=n)
+    zones = int(np.prod(p.interior))
+    with capi.solver_for(p) as s:
+        s.upload(p.state)
+        dt = s.cfl_dt()
+    out, rate = {}, {}
+    for prec in (0, 1):
+        with capi.solver_for(p, precision=prec) as s:
+            s.upload(p.state)
+            s.run(4, flg=0, dt=dt, history=False)  # builds the step graph
+            s.upload(p.state)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(steps, flg=0, dt=dt, history=False)
+            s.sync()
+            el = time.perf_counter() - t0
+            out[prec] = s.download()
+            rate[prec] = zones * steps / el
+    gam = p.gam
+
+    def pres(st):
+        ro = st[0]
+        v2 = (st[1] ** 2 + st[2] ** 2 + st[3] ** 2) / ro
+        bx = 0.5 * (st[4] + np.roll(st[4], -1, axis=-1))
+        by = 0.5 * (st[5] + np.roll(st[5], -1, axis=-2))
+        return (gam - 1) * (st[7] - 0.5 * (v2 + bx ** 2 + by ** 2 + st[6] ** 2))
+    a, b = out[0], out[1]
+    l1r = float(np.mean(np.abs(_interior(b[0] - a[0], 2))))
+    l1p = float(np.mean(np.abs(_interior(pres(b) - pres(a), 2))))
+    return {"workload": "2d_lowmach_khi_%dx%d_mlau_wcns4_ct_rk3" % (n, n), "steps": steps, "dt": dt,
+            "fp64_value": rate[0], "mixed_value": rate[1], "speedup": rate[1] / rate[0], "l1_rho": l1r,
+            "l1_p": l1p, "finite": bool(np.isfinite(b).all())}
+
+
+def adjoint_line(n, steps):
+    """Discrete adjoint (qas_adj_forward / qas_adj_backward) on the 2D
+    periodic workload at rms Mach 0.3 (HLLD, WCNS-4, CUCT, RK3): time of the
+    backward pass over `steps` steps against the plain forward of the same
+    steps at the same dt values."""
+    import numpy as np
+    from qasmhd_amd import capi, problems
+    p = problems.turbulence2d(n=n, seed=5, vrms=0.3, brms=0.3)
+    rng = np.random.default_rng(5)
+    W = rng.standard_normal(p.state.shape)
+    with capi.solver_for(p) as s:
+        s.upload(p.state)
+        dts, _ = s.run(steps, flg=1)
+        # plain forward at the recorded step sizes
+        s.upload(p.state)
+        s.bound()
+        s.sync()
+        t0 = time.perf_counter()
+        for dt in dts:
+            s.step(dt)
+        s.sync()
+        t_fwd = time.perf_counter() - t0
+        s.upload(p.state)
+        s.sync()
+        t0 = time.perf_counter()
+        s.adj_forward(dts)
+        t_rec = time.perf_counter() - t0
+        s.adj_backward(W)  # first call allocates and warms up
+        t0 = time.perf_counter()
+        g = s.adj_backward(W)
+        t_bwd = time.perf_counter() - t0
+    return {"workload": "2d_decaying_mhd_%dx%d_mach0.3_hlld_wcns4_ct_rk3" % (n, n), "steps": steps,
+            "forward_s": t_fwd, "forward_with_checkpoints_s": t_rec, "backward_s": t_bwd,
+            "ratio": t_bwd / t_fwd, "finite": bool(np.isfinite(g).all()),
+            "note": "backward = per step: 3 recomputed forward stages + 3 stage adjoints; host copies of the "
+                    "8 x nd gradient in and out included"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +180,11 @@ def main():
     ap.add_argument("--workload", default="turb2d")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and adjoint lines")
+    ap.add_argument("--mixed-n", type=int, default=2048)
+    ap.add_argument("--mixed-steps", type=int, default=200)
+    ap.add_argument("--adj-n", type=int, default=256)
+    ap.add_argument("--adj-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the domain is one device-resident block); "
@@ -169,6 +263,19 @@ def main():
         except Exception as e:  # the baseline is reported, not required
             _log("bench: cpu baseline failed: %s" % e)
 
+    extra = {}
+    if not args.no_extra and p.ndim == 2 and args.workload.startswith("turb2d"):
+        try:
+            _log("bench: mixed precision, %d^2, %d steps" % (args.mixed_n, args.mixed_steps))
+            extra["mixed"] = mixed_line(args.mixed_n, args.mixed_steps)
+        except Exception as e:
+            _log("bench: mixed line failed: %s" % e)
+        try:
+            _log("bench: adjoint, %d^2, %d steps" % (args.adj_n, args.adj_steps))
+            extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps)
+        except Exception as e:
+            _log("bench: adjoint line failed: %s" % e)
+
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -178,6 +285,7 @@ def main():
                    "dt_final": dt_now, "finite": finite},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 
