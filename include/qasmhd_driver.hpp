@@ -12,6 +12,11 @@
 // state lives on the device between steps; the host arrays are refreshed
 // only for output (dout_) and on request (download()).
 //
+// Output and restart files follow the reference: dout_ writes params.dat,
+// offsets.dat, t.dat, x/y/z.dat and outdat_%05d.dat (FP64, 8 arrays of nd
+// doubles, ghosts included) into fildir; bkup_save/bkup_load the
+// bkup_stamp/time/data files of common_func.c:281-344.
+//
 // Header-only; link with libqasmhd_amd.so.  Errors throw qasmhd::Error
 // (the C ABI itself never throws).
 #pragma once
@@ -142,6 +147,114 @@ class MHDDevice {
   // re-send the host arrays (after the caller changed them)
   void upload() { dom().upload(val); }
 
+  // Output directory of dout_ and the backup files (empty: no files).
+  std::string fildir;
+
+  // Backup files of common_func.c:318-344 (bkup_save, rank 0):
+  // bkup_stamp.dat "n cnt", bkup_time.dat {tim, dt, trec} FP64,
+  // bkup_data_00000.dat the 8 arrays (nd doubles each, ghosts included).
+  void bkup_save()
+  {
+    download();
+    const double vald[3] = {tim, dt, trec};
+    write_text(path("bkup_stamp.dat"), "w", std::to_string(n) + " " + std::to_string(cnt) + "\n");
+    FILE* f = open_or_throw(path("bkup_time.dat"), "wb");
+    std::fwrite(vald, sizeof(double), 3, f);
+    std::fclose(f);
+    write_state(path("bkup_data_00000.dat"));
+  }
+  // bkup_load (common_func.c:281-316): restores n, cnt, tim, dt, trec and
+  // the state if bkup_stamp.dat exists; returns whether it did.
+  bool bkup_load()
+  {
+    FILE* f = std::fopen(path("bkup_stamp.dat").c_str(), "r");
+    if (!f) return false;
+    int vi[2] = {0, 0};
+    const int got = std::fscanf(f, "%d %d", &vi[0], &vi[1]);
+    std::fclose(f);
+    if (got != 2) throw Error("bkup_load: malformed bkup_stamp.dat");
+    double vd[3] = {0, 0, 0};
+    f = open_or_throw(path("bkup_time.dat"), "rb");
+    const size_t nt = std::fread(vd, sizeof(double), 3, f);
+    std::fclose(f);
+    if (nt != 3) throw Error("bkup_load: short bkup_time.dat");
+    f = open_or_throw(path("bkup_data_00000.dat"), "rb");
+    for (int m = 0; m < nm; m++)
+      if (std::fread(val[m], sizeof(double), store_.size() / nm, f) != store_.size() / nm) {
+        std::fclose(f);
+        throw Error("bkup_load: short bkup_data_00000.dat");
+      }
+    std::fclose(f);
+    n = vi[0];
+    cnt = vi[1];
+    tim = vd[0];
+    dt = vd[1];
+    trec = vd[2];
+    upload();
+    return true;
+  }
+
+ protected:
+  std::string path(const char* name) const { return fildir + "/" + name; }
+  static FILE* open_or_throw(const std::string& p, const char* mode)
+  {
+    FILE* f = std::fopen(p.c_str(), mode);
+    if (!f) throw Error("cannot open " + p);
+    return f;
+  }
+  static void write_text(const std::string& p, const char* mode, const std::string& text)
+  {
+    FILE* f = open_or_throw(p, mode);
+    std::fputs(text.c_str(), f);
+    std::fclose(f);
+  }
+  // the 8 host arrays, nd doubles each, in val[] order (the dout_ format)
+  void write_state(const std::string& p)
+  {
+    FILE* f = open_or_throw(p, "wb");
+    for (int m = 0; m < nm; m++) std::fwrite(val[m], sizeof(double), store_.size() / nm, f);
+    std::fclose(f);
+  }
+  // dout_ of the 2D/1D/3D serial drivers (e.g. 2D/SERIAL/OTvortex/
+  // mhd2d_class.cpp:62-112): params.dat, offsets.dat, t.dat and the
+  // coordinate files at n == 0, t.dat appended afterwards, and
+  // outdat_%05d.dat (FP64 SoA incl. ghosts) at every call.
+  // 1D (mhd1d_class.cpp:55-95) prints gam and t with "%.12f" and the
+  // offset into xoff.dat; 2D with "%f" into offsets.dat.
+  void dout_files(int msg, const std::vector<std::vector<double>>& coords, const std::vector<int>& offs)
+  {
+    if (fildir.empty()) return;
+    const bool one = offs.size() == 1;
+    const char* tf = one ? "%.12f\n" : "%f\n";
+    char buf[64];
+    if (n == 0) {
+      std::snprintf(buf, sizeof(buf), tf, gam);
+      write_text(path("params.dat"), "w", buf);
+      std::string o;
+      for (size_t a = 0; a < offs.size(); a++) o += (a ? " " : "") + std::to_string(offs[a]);
+      write_text(path(one ? "xoff.dat" : "offsets.dat"), "w", o + "\n");
+      std::snprintf(buf, sizeof(buf), tf, tim);
+      write_text(path("t.dat"), "w", buf);
+      const char* names[3] = {"x.dat", "y.dat", "z.dat"};
+      for (size_t a = 0; a < coords.size(); a++) {
+        std::string t;
+        for (double v : coords[a]) {
+          std::snprintf(buf, sizeof(buf), "%.12f\n", v);
+          t += buf;
+        }
+        write_text(path(names[a]), "w", t);
+      }
+    } else {
+      std::snprintf(buf, sizeof(buf), tf, tim);
+      write_text(path("t.dat"), "a", buf);
+    }
+    std::snprintf(buf, sizeof(buf), "outdat_%05d.dat", cnt);
+    write_state(path(buf));
+    if (msg) std::printf("Output data at t=%.4f (%d iterations).\n", tim, n);
+  }
+
+ public:
+
  protected:
   // bound(val) of the reference: ghost fill on the device
   void bound() { dom().bound(); }
@@ -201,8 +314,14 @@ class MHD2DDevice : public MHDDevice {
     return g;
   }
   double cfl_value() const override { return cfl; }
-  // output hook (the reference writes dat/outdat_%05d.dat); default: nothing
-  virtual void dout_(int) {}
+  // output (the reference's dout_ file set, when fildir is set)
+  virtual void dout_(int msg)
+  {
+    std::vector<std::vector<double>> c(2);
+    for (int i = 0; i < nx; i++) c[0].push_back((i - xoff) * dx + xmin);
+    for (int j = 0; j < ny; j++) c[1].push_back((j - yoff) * dy + ymin);
+    dout_files(msg, c, {xoff, yoff});
+  }
 };
 
 // 1D driver (mhd1d_class.hpp:7-38): fixed-dt exec_(0) with nrec output
@@ -254,7 +373,12 @@ class MHD1DDevice : public MHDDevice {
     return g;
   }
   double cfl_value() const override { return cfl; }
-  virtual void dout_() {}
+  virtual void dout_()
+  {
+    std::vector<std::vector<double>> c(1);  // cell centres (mhd1d_paras.cpp:24)
+    for (int i = 0; i < nx; i++) c[0].push_back((i - xoff + 0.5) * dx + xmin);
+    dout_files(0, c, {xoff});
+  }
 };
 
 // 3D driver (mhd3d_class.hpp:8-70) on one device-resident domain.
@@ -307,7 +431,14 @@ class MHD3DDevice : public MHDDevice {
     return g;
   }
   double cfl_value() const override { return cfl; }
-  virtual void dout_(int) {}
+  virtual void dout_(int msg)
+  {
+    std::vector<std::vector<double>> c(3);
+    for (int i = 0; i < nx; i++) c[0].push_back((i - xoff) * dx + xmin);
+    for (int j = 0; j < ny; j++) c[1].push_back((j - yoff) * dy + ymin);
+    for (int k = 0; k < nz; k++) c[2].push_back((k - zoff) * dz + zmin);
+    dout_files(msg, c, {xoff, yoff, zoff});
+  }
 };
 
 }  // namespace qasmhd

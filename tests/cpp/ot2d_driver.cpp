@@ -5,10 +5,14 @@
 // exec_(1).  It dumps the initial and final FP64 states so a test can rerun
 // the oracle on the same initial data.
 //
-// usage: ot2d_driver <xmesh> <ymesh> <nsteps> <rmn> <init.bin> <final.bin>
+// usage: ot2d_driver <xmesh> <ymesh> <nsteps> <rmn> <init.bin> <final.bin> [outdir [restart]]
+// With outdir, the initial output (dout_) and a backup at the end go there;
+// with "restart", the run instead resumes from that backup (bkup_load) and
+// advances nsteps more steps.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "qasmhd_driver.hpp"
 
@@ -56,6 +60,23 @@ class OTVortex : public qasmhd::MHD2DDevice {
     std::fclose(f);
   }
   void set_scheme(int r) { rmn = r; }
+  // exec_ leaves n at nmax + 1 when it stops on the step count (its loop
+  // test increments n once more); the reference saves its backup inside the
+  // loop, after step n.  Save the state after step nmax the same way.
+  void backup_after_run()
+  {
+    n = nmax;
+    bkup_save();
+  }
+  // resume from the backup in fildir and advance `steps_` more steps
+  void restart_steps(int flg)
+  {
+    if (!bkup_load()) throw qasmhd::Error("no backup in " + fildir);
+    nflg_ = 1;
+    nmax = n + steps_;
+    exec_(flg);
+    download();
+  }
 
  private:
   int steps_;
@@ -71,11 +92,18 @@ int main(int argc, char** argv)
     OTVortex ot(std::atoi(argv[1]), std::atoi(argv[2]), std::atoi(argv[3]));
     ot.set_scheme(std::atoi(argv[4]));
     ot.paras();
-    ot.init_();
-    ot.save(argv[5]);
-    ot.setdt(1);
-    ot.run_steps(1);
-    ot.save(argv[6]);
+    if (argc > 7) ot.fildir = argv[7];
+    if (argc > 8 && std::string(argv[8]) == "restart") {
+      ot.restart_steps(1);
+      ot.save(argv[6]);
+    } else {
+      ot.init_();
+      ot.save(argv[5]);
+      ot.setdt(1);
+      ot.run_steps(1);
+      ot.save(argv[6]);
+      if (argc > 7) ot.backup_after_run();
+    }
     std::printf("dt %.17g\n", ot.getdt());
   } catch (const qasmhd::Error& e) {
     std::fprintf(stderr, "error: %s\n", e.what());

@@ -191,6 +191,46 @@ def test_cpp_driver_matches_oracle(oracle, tmp_path):
     assert np.array_equal(out, ref)
 
 
+def test_cpp_driver_output_and_restart(oracle, tmp_path):
+    """dout_ / bkup files of the C++ driver in the reference's formats
+    (2D/SERIAL/OTvortex/mhd2d_class.cpp:62-112, common/common_func.c:281-344),
+    and a restart from the backup continues the run bit for bit: 6 steps,
+    backup, 6 more steps == 12 straight steps of the oracle."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLDEN), "cpp", "_build", "ot2d_driver")
+    if not os.path.exists(exe):
+        pytest.skip("C++ driver not built (run __graft_entry__.build())")
+    out = tmp_path / "dat"
+    out.mkdir()
+    init, mid, fin = str(tmp_path / "init.bin"), str(tmp_path / "mid.bin"), str(tmp_path / "fin.bin")
+    _say("cpp driver restart")
+    r = subprocess.run([exe, "32", "24", "6", "1", init, mid, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    shape = (8, 32, 40)
+    st0 = np.fromfile(init).reshape(shape)
+    # dout_(0) at n == 0: the initial state and the text files
+    assert np.array_equal(np.fromfile(str(out / "outdat_00000.dat")).reshape(shape), st0)
+    assert (out / "offsets.dat").read_text() == "4 4\n"
+    assert (out / "params.dat").read_text() == "%f\n" % (5.0 / 3.0)
+    x = np.loadtxt(str(out / "x.dat"))
+    assert x.size == 40 and abs(x[4]) < 1e-12
+    # backup after 6 steps
+    n, cnt = (int(v) for v in (out / "bkup_stamp.dat").read_text().split())
+    assert n >= 6
+    tdt = np.fromfile(str(out / "bkup_time.dat"))
+    assert tdt.size == 3 and tdt[1] > 0
+    assert np.array_equal(np.fromfile(str(out / "bkup_data_00000.dat")).reshape(shape),
+                          np.fromfile(mid).reshape(shape))
+    r = subprocess.run([exe, "32", "24", "6", "1", init, fin, str(out), "restart"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = 2 * np.pi / 32, 2 * np.pi / 24
+    cfg = oracle.make_cfg(shape[1:], d[0], d[1])
+    ref = st0.copy()
+    oracle.run(cfg, ref, 12, 1)
+    assert np.array_equal(np.fromfile(fin).reshape(shape), ref)
+
+
 @pytest.mark.parametrize("nsteps", [7, 10])
 def test_graph_replay_matches_direct_launches(qas, oracle, nsteps):
     """qas_run with captured 2-step graphs == host-launched steps == oracle,
