@@ -5,8 +5,8 @@
 // reconstructs each cell (:113-141), solves each face (:142-162) and applies
 // the 4th-order difference and RK stage sum to the seven evolved variables
 // (:163-179).  In 1D the cell-centred B is the stored B (cx=bx, cy=by,
-// cz=bz, :89-91) and bx is not evolved (copied into the stage output
-// buffer).  Lane layout as in the 2D sweeps.
+// cz=bz, :89-91) and bx is not evolved (the prim kernel copies it into
+// the stage output buffer).  Lane layout as in the 2D sweeps.
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
 
@@ -28,6 +28,9 @@ __global__ __launch_bounds__(256) void k_prim1d(StagePtrs p, Geo g, double gam)
   p.vx[t] = vx;
   p.vy[t] = vy;
   p.vz[t] = vz;
+  // bx is not evolved in 1D (the reference leaves val[4] in place); the
+  // stage output is another buffer, so carry it over at every cell
+  p.out[4][t] = bx;
 }
 
 template <int RMN, int ODR>
@@ -80,9 +83,6 @@ __global__ __launch_bounds__(256) void k_sweep1d(StagePtrs p, Geo g, double gam,
     const int m = vid[v];
     p.out[m][q] = rk_upd(p.in[m][q], p.u0[m][q], -dtdx * du[v], p.a, p.b);
   }
-  // bx is not evolved in 1D (the reference leaves val[4] in place); the
-  // stage output is another buffer, so carry it over
-  p.out[4][q] = p.in[4][q];
 }
 
 typedef void (*sweep1_fn)(StagePtrs, Geo, double, int);
