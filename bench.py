@@ -181,6 +181,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and adjoint lines")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host (for counters)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy the 8 state arrays to a device buffer (8 copy "
+                         "dispatches of nd doubles: the calibration of the memory counters)")
     ap.add_argument("--mixed-n", type=int, default=2048)
     ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--adj-n", type=int, default=256)
@@ -203,6 +207,8 @@ def main():
     lib.qas_prof_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     lib.qas_kernel_model.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     s = capi.solver_for(p)
+    if args.no_graphs:
+        s.lib.qas_use_graphs(s.ctx, 0)
     s.upload(p.state)
     s.run(args.warmup, flg=1, history=False)
     s.sync()
@@ -220,6 +226,15 @@ def main():
     el = time.perf_counter() - t0
     out = s.download()
     finite = bool(np.isfinite(out).all())
+    if args.calibrate_copy:
+        hip = C.CDLL("libamdhip64.so.7")
+        buf = C.c_void_p()
+        if hip.hipMalloc(C.byref(buf), C.c_size_t(out.nbytes)) != 0:
+            raise SystemExit("bench: hipMalloc for the copy calibration failed")
+        s.download_device(buf.value)
+        s.sync()
+        hip.hipFree(buf)
+        _log("bench: copied 8 x %d bytes device to device (counter calibration)" % (out.nbytes // 8))
     value = zones * args.steps / el
     _log("bench: %.3f s for %d steps -> %.4g zone-cycles/s (finite=%s)" % (el, args.steps, value, finite))
 
