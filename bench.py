@@ -61,6 +61,24 @@ def workload(name):
     raise SystemExit("unknown workload %s" % name)
 
 
+def recorded_traffic(workload_name, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (profiles/*_traffic.md, written from two rocprofv3 --pmc
+    passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration);
+    None when there is none."""
+    import glob
+    import re
+    tag = {"2d_decaying_mhd_1024x1024_hlld_wcns4_ct_rk3": "stage2d_turb1024_fp64"}.get(workload_name)
+    if tag is None:
+        return None
+    for path in glob.glob(os.path.join(ROOT, "profiles", tag + "_traffic.md")):
+        with open(path) as f:
+            m = re.search(r"%s=(\d+)" % kernel, f.read())
+        if m:
+            return float(m.group(1))
+    return None
+
+
 def cpu_baseline(p, nsteps):
     """The reference's own driver on this host (kind 'reference'), else the C
     oracle (kind 'port'); same initial state, first `nsteps` steps."""
@@ -263,7 +281,7 @@ def main():
         else:
             roof = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": tflops / FP64_PEAK_TFLOPS}
-        roof.update({"traffic": None, "kernel": names[best["kid"]], "avg_launch_us": best["avg_s"] * 1e6,
+        roof.update({"traffic": recorded_traffic(wname, names[best["kid"]]), "kernel": names[best["kid"]], "avg_launch_us": best["avg_s"] * 1e6,
                      "launches": best["launches"], "alg_flops_per_launch": best["flops"],
                      "alg_bytes_per_launch": best["bytes"], "fp64_tflops": tflops, "hbm_gbs": gbs,
                      "fp64_frac": tflops / FP64_PEAK_TFLOPS, "hbm_frac": gbs / HBM_PEAK_GBS,
