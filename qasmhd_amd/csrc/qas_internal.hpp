@@ -40,6 +40,27 @@ struct StagePtrs {
   double a, b;           // RK stage coefficients (rk_fac)
 };
 
+// The primitive scratch as the sweeps read it: element type PT = the
+// reconstruction precision (double: FP64 path; float: mixed path, where the
+// prim kernels round the FP64 primitives once on store).
+template <typename PT>
+struct PrimView {
+  const PT *vx, *vy, *vz, *pr, *cx, *cy, *cz;
+};
+template <typename PT>
+__host__ __device__ inline PrimView<PT> prim_view(const StagePtrs& p)
+{
+  return PrimView<PT>{reinterpret_cast<const PT*>(p.vx), reinterpret_cast<const PT*>(p.vy),
+                      reinterpret_cast<const PT*>(p.vz), reinterpret_cast<const PT*>(p.pr),
+                      reinterpret_cast<const PT*>(p.cx), reinterpret_cast<const PT*>(p.cy),
+                      reinterpret_cast<const PT*>(p.cz)};
+}
+template <typename PT>
+__device__ __forceinline__ void store_prim(double* a, long long t, double v)
+{
+  reinterpret_cast<PT*>(a)[t] = (PT)v;
+}
+
 // Ghost-index resolution: bound() writes ghost cells as (factor x) a kept
 // cell of the same array.  Applying x then y then z (bc2d/bc3d order) to a
 // cell gives value = kz*(ky*(kx*f[src])) with the sources below.

@@ -24,7 +24,9 @@
 namespace qas {
 
 // ------------------------------------------------------------------ prim
-template <int ODR>
+// PT: storage type of the primitives (double, or float for the mixed path);
+// the arithmetic is FP64 either way.
+template <int ODR, typename PT>
 __global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, double gam)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,12 +60,19 @@ __global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, 
   const double cz = p.in[6][t];
   const double v2 = vx * vx + vy * vy + vz * vz;
   const double b2 = cx * cx + cy * cy + cz * cz;
-  p.pr[t] = (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2));
-  p.vx[t] = vx;
-  p.vy[t] = vy;
-  p.vz[t] = vz;
-  p.cx[t] = cx;
-  p.cy[t] = cy;
+  store_prim<PT>(p.pr, t, (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2)));
+  store_prim<PT>(p.vx, t, vx);
+  store_prim<PT>(p.vy, t, vy);
+  store_prim<PT>(p.vz, t, vz);
+  store_prim<PT>(p.cx, t, cx);
+  store_prim<PT>(p.cy, t, cy);
+}
+
+// velocity component of cell c from the state: the same expression as the
+// prim kernel (m * (1/ro)), so the FP64 value is bit-identical to it
+__device__ __forceinline__ double vel(const StagePtrs& p, int comp, long long c)
+{
+  return p.in[1 + comp][c] * (1.0 / p.in[0][c]);
 }
 
 // the face sweeps live in qas_sweep2d.hpp (shared with the mixed path)
@@ -81,9 +90,10 @@ __global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
   double ct = 0.5;
   if (ctw) {
     const double* ro = p.in[0];
-    ct = cuct_weight(ro[ss - 1 - nx], ro[ss - nx], ro[ss - 1], ro[ss], p.vx[ss - 1 - nx], p.vx[ss - nx],
-                     p.vx[ss - 1], p.vx[ss], p.vy[ss - 1 - nx], p.vy[ss - nx], p.vy[ss - 1], p.vy[ss],
-                     p.in[4][ss - nx], p.in[4][ss], p.in[5][ss - 1], p.in[5][ss]);
+    const long long c0 = ss - 1 - nx, c1 = ss - nx, c2 = ss - 1, c3 = ss;
+    ct = cuct_weight(ro[c0], ro[c1], ro[c2], ro[c3], vel(p, 0, c0), vel(p, 0, c1), vel(p, 0, c2), vel(p, 0, c3),
+                     vel(p, 1, c0), vel(p, 1, c1), vel(p, 1, c2), vel(p, 1, c3), p.in[4][ss - nx], p.in[4][ss],
+                     p.in[5][ss - 1], p.in[5][ss]);
   }
   double ez = 0.0;
   {  // from the x sweep: By flux interpolated along y to (i-1/2, j-1/2)
@@ -168,9 +178,13 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   const double gam = c->s.gam;
   hipStream_t st = c->stream;
   // 1) cell-centred B and primitives
-  switch (odr) {
-    case 1: case 2: QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<2>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
-    default: QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam); break;
+  const bool f32 = c->s.precision == 1;
+  if (odr <= 2) {
+    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<2, float>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<2, double>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  } else {
+    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4, float>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4, double>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
   }
   // 2) x sweep, 3) y sweep
   const int nsx = (g.nx - 5 + SEG - 1) / SEG;

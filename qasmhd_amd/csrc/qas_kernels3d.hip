@@ -20,7 +20,9 @@
 
 namespace qas {
 
-template <int ODR>
+// PT: storage type of the primitives (double, or float for the mixed path);
+// the arithmetic is FP64 either way.
+template <int ODR, typename PT>
 __global__ __launch_bounds__(256) void k_prim3d(StagePtrs p, Geo g, BcFlags bc, double gam)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -47,14 +49,21 @@ __global__ __launch_bounds__(256) void k_prim3d(StagePtrs p, Geo g, BcFlags bc, 
   const double vz = p.in[3][t] * iro;
   const double v2 = vx * vx + vy * vy + vz * vz;
   const double b2 = c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2];
-  p.pr[t] = (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2));
-  p.vx[t] = vx;
-  p.vy[t] = vy;
-  p.vz[t] = vz;
-  p.cx[t] = c3[0];
-  p.cy[t] = c3[1];
-  p.cz[t] = c3[2];
+  store_prim<PT>(p.pr, t, (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2)));
+  store_prim<PT>(p.vx, t, vx);
+  store_prim<PT>(p.vy, t, vy);
+  store_prim<PT>(p.vz, t, vz);
+  store_prim<PT>(p.cx, t, c3[0]);
+  store_prim<PT>(p.cy, t, c3[1]);
+  store_prim<PT>(p.cz, t, c3[2]);
 }
+
+// velocity of cell c from the state: the prim kernel's expression
+// (m * (1/ro)), so the FP64 value is bit-identical to it
+struct Vel3 {
+  const double *ro, *m;
+  __device__ __forceinline__ double operator[](long long c) const { return m[c] * (1.0 / ro[c]); }
+};
 
 // the face sweeps live in qas_sweep3d.hpp (shared with the mixed path)
 
@@ -92,7 +101,7 @@ __global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
   const long long ss = sz * k + sy * j + i;
   const double* ro = p.in[0];
   const double *bx = p.in[4], *by = p.in[5], *bz = p.in[6];
-  const double *vx = p.vx, *vy = p.vy, *vz = p.vz;
+  const Vel3 vx{p.in[0], p.in[1]}, vy{p.in[0], p.in[2]}, vz{p.in[0], p.in[3]};
   double ctz = 0.5, ctx = 0.5, cty = 0.5;
   if (ctw) {
     // x-y plane for Ez: strides (1, ny)
@@ -182,8 +191,14 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   const int odr = c->s.odr;
   const double gam = c->s.gam;
   hipStream_t st = c->stream;
-  if (odr <= 2) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<2>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-  else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<4>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  const bool f32 = c->s.precision == 1;
+  if (odr <= 2) {
+    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<2, float>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<2, double>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  } else {
+    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<4, float>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<4, double>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
+  }
   const int dims[3] = {g.nx, g.ny, g.nz};
   sweep3_fn f[3] = {pick3<0>(c->s.rmn, odr), pick3<1>(c->s.rmn, odr), pick3<2>(c->s.rmn, odr)};
   if (c->s.precision == 1)

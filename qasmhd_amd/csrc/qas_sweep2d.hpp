@@ -14,8 +14,8 @@ namespace qas {
 
 // shock sensor dv of mhd2d_solve.cpp:105-125 at (i, j) after its bound():
 // DIR 0 -> dvx (flags -dnxs[1], dnys[1]); DIR 1 -> dvy (dnxs[2], -dnys[2])
-template <int DIR>
-__device__ __forceinline__ double dv_at(const double* v, int i, int j, const Geo& g, const BcFlags& bc)
+template <int DIR, typename VT>
+__device__ __forceinline__ double dv_at(const VT* v, int i, int j, const Geo& g, const BcFlags& bc)
 {
   const int dnx = DIR == 0 ? -bc.dn[0][1] : bc.dn[0][2];
   const int dny = DIR == 0 ? bc.dn[1][1] : -bc.dn[1][2];
@@ -23,8 +23,8 @@ __device__ __forceinline__ double dv_at(const double* v, int i, int j, const Geo
   const Src sy = resolve(j, g.ny, 0, dny);
   const long long q = (long long)g.nx * sy.i + sx.i;
   const long long o = DIR == 0 ? 1 : g.nx;
-  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
-  return apply(sy, apply(sx, r));
+  const VT r = tmin((VT)(v[q + o] - v[q]), (VT)(v[q] - v[q - o]));
+  return apply(sy, apply(sx, (double)r));
 }
 
 // XCD-aware tile order: the hardware hands consecutive workgroups to the 8
@@ -63,8 +63,8 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
   const int f0 = 3 + seg * SEG;
   const int n = g.ny;
   const long long sy = g.sy;
+  const PrimView<T> pv = prim_view<T>(p);
   {
-    const double* src[9] = {p.in[0], p.vy, p.vz, p.vx, p.in[6], p.cx, p.pr, p.cy, p.in[5]};
     for (int e = threadIdx.x; e < YC * YR; e += blockDim.x) {
       const int c = e % YC, r = e / YC;
       int row = f0 - 4 + r;
@@ -72,8 +72,15 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
       int col = i0 + c;
       col = col > g.nx - 1 ? g.nx - 1 : col;
       const long long a = sy * row + col;
-#pragma unroll
-      for (int k = 0; k < 9; k++) sh[k][c][r] = src[k][a];
+      sh[0][c][r] = p.in[0][a];
+      sh[1][c][r] = (double)pv.vy[a];
+      sh[2][c][r] = (double)pv.vz[a];
+      sh[3][c][r] = (double)pv.vx[a];
+      sh[4][c][r] = p.in[6][a];
+      sh[5][c][r] = (double)pv.cx[a];
+      sh[6][c][r] = (double)pv.pr[a];
+      sh[7][c][r] = (double)pv.cy[a];
+      sh[8][c][r] = p.in[5][a];
     }
   }
   __syncthreads();
@@ -123,7 +130,7 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
   if constexpr (RMN >= 2) {
     const int lc = line > g.nx - 1 ? g.nx - 1 : line;
     dv0 = (T)(sh[1][w][rf] - sh[1][w][rf - 1]);
-    dv1 = (T)tmin(dv_at<0>(p.vx, lc, sf - 1, g, bc), dv_at<0>(p.vx, lc, sf, g, bc));
+    dv1 = (T)tmin(dv_at<0>(pv.vx, lc, sf - 1, g, bc), dv_at<0>(pv.vx, lc, sf, g, bc));
   }
   riemann<RMN>(Lf, Rf, bnf, (T)gam, dv0, dv1, F);
   const T fc = T(0.5) * (qlf + qr);
@@ -196,24 +203,24 @@ __global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc,
   const long long sc = base + o * qc;
 
   // primitive stencil, cyclic order (vn, vt, vu), (bt, bu)
-  const double* vn = DIR == 0 ? p.vx : p.vy;
-  const double* vt = DIR == 0 ? p.vy : p.vz;
-  const double* vu = DIR == 0 ? p.vz : p.vx;
-  const double* bt = DIR == 0 ? p.cy : p.in[6];
-  const double* bu = DIR == 0 ? p.in[6] : p.cx;
-  const double* cn = DIR == 0 ? p.cx : p.cy;
+  const PrimView<T> pv = prim_view<T>(p);
+  const T* vn = DIR == 0 ? pv.vx : pv.vy;
+  const T* vt = DIR == 0 ? pv.vy : pv.vz;
+  const T* vu = DIR == 0 ? pv.vz : pv.vx;
+  const T* cn = DIR == 0 ? pv.cx : pv.cy;
+  const T* cb = DIR == 0 ? pv.cy : pv.cx;  // the cell-centred tangential B (the other one is bz)
   const double* bf = DIR == 0 ? p.in[4] : p.in[5];  // staggered normal B
   T s[7][5];
 #pragma unroll
   for (int m = 0; m < 5; m++) {
     const long long a = sc + (m - 2) * o;
     s[0][m] = (T)p.in[0][a];
-    s[1][m] = (T)vn[a];
-    s[2][m] = (T)vt[a];
-    s[3][m] = (T)vu[a];
-    s[4][m] = (T)bt[a];
-    s[5][m] = (T)bu[a];
-    s[6][m] = (T)p.pr[a];
+    s[1][m] = vn[a];
+    s[2][m] = vt[a];
+    s[3][m] = vu[a];
+    s[4][m] = DIR == 0 ? cb[a] : (T)p.in[6][a];
+    s[5][m] = DIR == 0 ? (T)p.in[6][a] : cb[a];
+    s[6][m] = pv.pr[a];
   }
   T vl[7], vr[7];
   lrstate<ODR>(s, (T)cn[sc], (T)gam, vl, vr);
@@ -222,14 +229,13 @@ __global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc,
   // component that pairs with this sweep (x: By, y: Bx)
   T ql, qr;
   {
-    const double* bb = DIR == 0 ? p.cy : p.cx;
-    const double* vv = DIR == 0 ? p.vy : p.vx;
+    const T* vv = DIR == 0 ? pv.vy : pv.vx;
     T pp[5], qq[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       const long long a = sc + (m - 2) * o;
-      pp[m] = (T)bb[a] * (T)vn[a];
-      qq[m] = (T)vv[a];
+      pp[m] = cb[a] * vn[a];
+      qq[m] = vv[a];
     }
     lr_fb<ODR>(pp, qq, (T)bf[sc], (T)bf[sc + o], ql, qr);
   }
@@ -249,9 +255,9 @@ __global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc,
     const int qq = q < 3 ? 3 : (q > n - 3 ? n - 3 : q);
     dv0 = (T)(vn[sf] - vn[sf - o]);
     if (DIR == 0) {
-      dv1 = (T)tmin(dv_at<1>(p.vy, qq - 1, line, g, bc), dv_at<1>(p.vy, qq, line, g, bc));
+      dv1 = (T)tmin(dv_at<1>(pv.vy, qq - 1, line, g, bc), dv_at<1>(pv.vy, qq, line, g, bc));
     } else {
-      dv1 = (T)tmin(dv_at<0>(p.vx, line, qq - 1, g, bc), dv_at<0>(p.vx, line, qq, g, bc));
+      dv1 = (T)tmin(dv_at<0>(pv.vx, line, qq - 1, g, bc), dv_at<0>(pv.vx, line, qq, g, bc));
     }
   }
   riemann<RMN>(L, R, bnf, (T)gam, dv0, dv1, F);

@@ -11,8 +11,8 @@ namespace qas {
 // dv of axis A (0 x, 1 y, 2 z) at (i, j, k) after its bound()
 // (mhd3d_solve.cpp:138-164): flags (-dnxs[1], dnys[1], dnzs[1]) for dvx,
 // (dnxs[2], -dnys[2], dnzs[2]) for dvy, (dnxs[3], dnys[3], -dnzs[3]) for dvz.
-template <int A>
-__device__ __forceinline__ double dv3(const double* v, int i, int j, int k, const Geo& g, const BcFlags& bc)
+template <int A, typename VT>
+__device__ __forceinline__ double dv3(const VT* v, int i, int j, int k, const Geo& g, const BcFlags& bc)
 {
   const int dnx = A == 0 ? -bc.dn[0][1] : bc.dn[0][1 + A];
   const int dny = A == 1 ? -bc.dn[1][2] : bc.dn[1][1 + A];
@@ -22,15 +22,15 @@ __device__ __forceinline__ double dv3(const double* v, int i, int j, int k, cons
   const Src sz = resolve(k, g.nz, 0, dnz);
   const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
   const long long o = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
-  const double r = tmin((v[q + o] - v[q]), (v[q] - v[q - o]));
-  return apply(sz, apply(sy, apply(sx, r)));
+  const VT r = tmin((VT)(v[q + o] - v[q]), (VT)(v[q] - v[q - o]));
+  return apply(sz, apply(sy, apply(sx, (double)r)));
 }
 
-template <int A>
-__device__ __forceinline__ double dv3_face(const StagePtrs& p, int i, int j, int k, const Geo& g,
+template <int A, typename VT>
+__device__ __forceinline__ double dv3_face(const PrimView<VT>& pv, int i, int j, int k, const Geo& g,
                                            const BcFlags& bc)
 {
-  const double* v = A == 0 ? p.vx : (A == 1 ? p.vy : p.vz);
+  const VT* v = A == 0 ? pv.vx : (A == 1 ? pv.vy : pv.vz);
   return dv3<A>(v, i, j, k, g, bc);
 }
 
@@ -66,10 +66,11 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
   const int n = dims[DIR];
   const long long o = strd[DIR];
   const long long blb = strd[LB] * lb;
-  const double* vax[3] = {p.vx, p.vy, p.vz};
-  const double* cax[3] = {p.cx, p.cy, p.cz};
+  const PrimView<T> pv = prim_view<T>(p);
+  const T* vax[3] = {pv.vx, pv.vy, pv.vz};
+  const T* cax[3] = {pv.cx, pv.cy, pv.cz};
   {
-    const double* src[9] = {p.in[0], vax[DIR], vax[TA], vax[TB], cax[TA], cax[TB], p.pr, cax[DIR], p.in[4 + DIR]};
+    const T* src[6] = {vax[DIR], vax[TA], vax[TB], cax[TA], cax[TB], pv.pr};
     for (int e = threadIdx.x; e < YC * YR; e += blockDim.x) {
       const int c = e % YC, r = e / YC;
       int row = f0 - 4 + r;
@@ -77,8 +78,11 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
       int col = i0 + c;
       col = col > g.nx - 1 ? g.nx - 1 : col;
       const long long a = blb + o * row + col;
+      sh[0][c][r] = p.in[0][a];
 #pragma unroll
-      for (int k = 0; k < 9; k++) sh[k][c][r] = src[k][a];
+      for (int k = 0; k < 6; k++) sh[1 + k][c][r] = (double)src[k][a];
+      sh[7][c][r] = (double)cax[DIR][a];
+      sh[8][c][r] = p.in[4 + DIR][a];
     }
   }
   __syncthreads();
@@ -130,10 +134,10 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
     int c0[3], c1[3];
     c0[0] = lac; c0[LB] = lb; c0[DIR] = qf - 1;
     c1[0] = lac; c1[LB] = lb; c1[DIR] = qf;
-    const double a0 = dv3_face<TA>(p, c0[0], c0[1], c0[2], g, bc);
-    const double a1 = dv3_face<TA>(p, c1[0], c1[1], c1[2], g, bc);
-    const double b0 = dv3_face<TB>(p, c0[0], c0[1], c0[2], g, bc);
-    const double b1 = dv3_face<TB>(p, c1[0], c1[1], c1[2], g, bc);
+    const double a0 = dv3_face<TA>(pv, c0[0], c0[1], c0[2], g, bc);
+    const double a1 = dv3_face<TA>(pv, c1[0], c1[1], c1[2], g, bc);
+    const double b0 = dv3_face<TB>(pv, c0[0], c0[1], c0[2], g, bc);
+    const double b1 = dv3_face<TB>(pv, c1[0], c1[1], c1[2], g, bc);
     dv1 = (T)tmin(tmin(a0, a1), tmin(b0, b1));
   }
   riemann<RMN>(Lf, vr, bnf, (T)gam, dv0, dv1, F);
@@ -225,25 +229,26 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
   const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
   const long long sc = base + o * qc;
 
-  const double* vax[3] = {p.vx, p.vy, p.vz};
-  const double* cax[3] = {p.cx, p.cy, p.cz};
-  const double* vn = vax[DIR];
-  const double* vt = vax[TA];
-  const double* vu = vax[TB];
-  const double* bt = cax[TA];
-  const double* bu = cax[TB];
+  const PrimView<T> pv = prim_view<T>(p);
+  const T* vax[3] = {pv.vx, pv.vy, pv.vz};
+  const T* cax[3] = {pv.cx, pv.cy, pv.cz};
+  const T* vn = vax[DIR];
+  const T* vt = vax[TA];
+  const T* vu = vax[TB];
+  const T* bt = cax[TA];
+  const T* bu = cax[TB];
   const double* bf = p.in[4 + DIR];
   T s[7][5];
 #pragma unroll
   for (int m = 0; m < 5; m++) {
     const long long a = sc + (m - 2) * o;
     s[0][m] = (T)p.in[0][a];
-    s[1][m] = (T)vn[a];
-    s[2][m] = (T)vt[a];
-    s[3][m] = (T)vu[a];
-    s[4][m] = (T)bt[a];
-    s[5][m] = (T)bu[a];
-    s[6][m] = (T)p.pr[a];
+    s[1][m] = vn[a];
+    s[2][m] = vt[a];
+    s[3][m] = vu[a];
+    s[4][m] = bt[a];
+    s[5][m] = bu[a];
+    s[6][m] = pv.pr[a];
   }
   T vl[7], vr[7];
   lrstate<ODR>(s, (T)cax[DIR][sc], (T)gam, vl, vr);
@@ -255,15 +260,15 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       const long long a = sc + (m - 2) * o;
-      pp[m] = (T)bt[a] * (T)vn[a];
-      qq[m] = (T)vt[a];
+      pp[m] = bt[a] * vn[a];
+      qq[m] = vt[a];
     }
     lr_fb<ODR>(pp, qq, (T)bf[sc], (T)bf[sc + o], qlt, qrt);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       const long long a = sc + (m - 2) * o;
-      pp[m] = (T)bu[a] * (T)vn[a];
-      qq[m] = (T)vu[a];
+      pp[m] = bu[a] * vn[a];
+      qq[m] = vu[a];
     }
     lr_fb<ODR>(pp, qq, (T)bf[sc], (T)bf[sc + o], qlu, qru);
   }
@@ -282,10 +287,10 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
     int c0[3], c1[3];
     c0[LA] = la; c0[LB] = lb; c0[DIR] = qf - 1;
     c1[LA] = la; c1[LB] = lb; c1[DIR] = qf;
-    const double a0 = dv3_face<TA>(p, c0[0], c0[1], c0[2], g, bc);
-    const double a1 = dv3_face<TA>(p, c1[0], c1[1], c1[2], g, bc);
-    const double b0 = dv3_face<TB>(p, c0[0], c0[1], c0[2], g, bc);
-    const double b1 = dv3_face<TB>(p, c1[0], c1[1], c1[2], g, bc);
+    const double a0 = dv3_face<TA>(pv, c0[0], c0[1], c0[2], g, bc);
+    const double a1 = dv3_face<TA>(pv, c1[0], c1[1], c1[2], g, bc);
+    const double b0 = dv3_face<TB>(pv, c0[0], c0[1], c0[2], g, bc);
+    const double b1 = dv3_face<TB>(pv, c1[0], c1[1], c1[2], g, bc);
     dv1 = (T)tmin(tmin(a0, a1), tmin(b0, b1));
   }
   riemann<RMN>(L, vr, bnf, (T)gam, dv0, dv1, F);
