@@ -16,6 +16,9 @@ Also reported, on one JSON line:
                 state, RK and pressure recovery) on the low-Mach
                 Kelvin-Helmholtz workload at 2048^2: both rates and the L1
                 difference of density and pressure
+  batch         64 periodic instances of uneven sizes (128-512 square) in one
+                batch (one launch per stage kernel for all) against the same
+                instances run one after another
   adjoint       time of the hand-written discrete adjoint over 20 steps of a
                 256^2 periodic run against the plain forward of those steps
   roofline      the dominant kernel's algorithmic work per launch (flops and
@@ -152,6 +155,51 @@ def mixed_line(n, steps):
             "l1_p": l1p, "finite": bool(np.isfinite(b).all())}
 
 
+def batch_line(ninst, steps, seed=5):
+    """Batched domains (qas_batch_run): `ninst` periodic 2D instances of
+    square sizes drawn (numpy PCG64(seed)) from {128, 192, 256, 384, 512},
+    rms Mach 0.3, HLLD / WCNS-4 / CUCT / RK3, `steps` steps in one batch
+    (one launch per stage kernel for all instances, two steps per graph
+    replay); against the same instances run one after another with qas_run."""
+    import numpy as np
+    from qasmhd_amd import capi, problems
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = rng.choice([128, 192, 256, 384, 512], size=ninst)
+    ps = [problems.turbulence2d(n=int(n), seed=seed * 1000 + i, vrms=0.3, brms=0.3) for i, n in enumerate(sizes)]
+    zones = int(sum(int(np.prod(p.interior)) for p in ps))
+    solvers = [capi.solver_for(p) for p in ps]
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+    with capi.Batch(solvers) as b:
+        b.run(2, flg=1, history=False)  # graph capture, warm-up
+        for s, p in zip(solvers, ps):
+            s.upload(p.state)
+            s.reset_counter()
+        t0 = time.perf_counter()
+        b.run(steps, flg=1, history=False)
+        for s in solvers:
+            s.sync()
+        t_batch = time.perf_counter() - t0
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+        s.reset_counter()
+        s.run(2, flg=1, history=False)
+        s.upload(p.state)
+        s.reset_counter()
+        s.sync()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run(steps, flg=1, history=False)
+    for s in solvers:
+        s.sync()
+    t_seq = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    return {"workload": "%d_instances_2d_periodic_sizes_128-512_mach0.3_hlld_wcns4_ct_rk3" % ninst, "steps": steps,
+            "zones": zones, "batched_value": zones * steps / t_batch, "sequential_value": zones * steps / t_seq,
+            "speedup": t_seq / t_batch, "sizes": [int(v) for v in sizes]}
+
+
 def adjoint_line(n, steps):
     """Discrete adjoint (qas_adj_forward / qas_adj_backward) on the 2D
     periodic workload at rms Mach 0.3 (HLLD, WCNS-4, CUCT, RK3): time of the
@@ -206,6 +254,8 @@ def main():
     ap.add_argument("--mixed-n", type=int, default=2048)
     ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--adj-n", type=int, default=256)
+    ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adj-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -303,6 +353,11 @@ def main():
             extra["mixed"] = mixed_line(args.mixed_n, args.mixed_steps)
         except Exception as e:
             _log("bench: mixed line failed: %s" % e)
+        try:
+            _log("bench: batched instances, %d x %d steps" % (args.batch_n, args.batch_steps))
+            extra["batch"] = batch_line(args.batch_n, args.batch_steps)
+        except Exception as e:
+            _log("bench: batch line failed: %s" % e)
         try:
             _log("bench: adjoint, %d^2, %d steps" % (args.adj_n, args.adj_steps))
             extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps)

@@ -6,7 +6,7 @@
  * (1D/SERIAL/shock/mhd1d_class.hpp:35-36, 2D/SERIAL/OTvortex/mhd2d_class.hpp:
  * 43-45, 3D/MPI/OTvortex/mhd3d_class.hpp:58-60), reached only from exec_()
  * (mhd2d_class.cpp:35-61).  A driver that swaps this library in overrides
- * those three methods with the calls below (see INTEGRATION.md); its field
+ * those three methods with the calls below; its field
  * layout (common/mhd_class.hpp:32-42) and solver switches (RMN, ODR, R_K,
  * CTW of mymacros.hpp, dnxs/dnys/dnzs and stxs/stys/stzs) carry over as the
  * runtime fields of qas_grid / qas_scheme.
@@ -118,6 +118,23 @@ int qas_sync(qas_ctx *c);
 
 /* Device memory held by the context (bytes). */
 long long qas_device_bytes(const qas_ctx *c);
+
+
+/* ---- batched domains (no counterpart in the reference) -----------------
+ * Advances many independent 2D domains of uneven sizes with one launch per
+ * stage kernel for all of them, two steps per replay of a captured
+ * hipGraph.  Each domain is a qas_ctx with its own grid, state and dt; all
+ * must share rmn, odr, r_k (2 or 3), ctw, precision and gam, live on one
+ * device, hold an uploaded state and have run the same number of steps.
+ * qas_batch_run(b, n, flg, dt, hist) gives every domain i the result of
+ * qas_run(ctx_i, n, flg, &dt[i], hist + i*n) (dt[i] <= 0: from the CFL
+ * condition; hist may be NULL).  The contexts stay owned by the caller and
+ * must outlive the batch; calls on one batch are not thread-safe. */
+typedef struct qas_batch qas_batch;
+int qas_batch_create(qas_batch **b, qas_ctx *const *ctxs, int n);
+int qas_batch_run(qas_batch *b, int nsteps, int flg, double *dt_inout, double *dt_hist);
+const char *qas_batch_last_error(const qas_batch *b);
+void qas_batch_destroy(qas_batch *b);
 
 
 /* ---- discrete adjoint (no counterpart in the reference) ----------------

@@ -37,6 +37,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_upload_packed", "qas_download_packed", "qas_upload_device", "qas_download_device", "qas_bound",
            "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_use_graphs", "qas_sync", "qas_device_bytes",
            "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck",
+           "qas_batch_create", "qas_batch_run", "qas_batch_last_error", "qas_batch_destroy",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct"]
 
@@ -80,6 +81,12 @@ def load(path: str | None = None):
     l.qas_device_bytes.argtypes = [vp]
     l.qas_device_bytes.restype = C.c_longlong
     l.qas_adjoint_selfcheck.argtypes = [C.c_int, C.c_int, C.c_ulonglong, dp]
+    l.qas_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
+    l.qas_batch_run.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    l.qas_batch_last_error.argtypes = [vp]
+    l.qas_batch_last_error.restype = C.c_char_p
+    l.qas_batch_destroy.argtypes = [vp]
+    l.qas_batch_destroy.restype = None
     l.qas_adj_forward.argtypes = [vp, C.c_int, dp]
     l.qas_adj_backward.argtypes = [vp, dp, dp]
     l.qas_adj_backward_device.argtypes = [vp, vp, vp]
@@ -295,6 +302,52 @@ class Solver:
     @property
     def device_bytes(self):
         return int(self.lib.qas_device_bytes(self.ctx))
+
+
+class Batch:
+    """Many 2D domains advanced together (qas_batch_*): one launch per stage
+    kernel for all of them, two steps per graph replay.  `solvers` are
+    Solver objects (same scheme), kept alive by the batch."""
+
+    def __init__(self, solvers):
+        self.lib = load()
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.ctx for s in self.solvers])
+        self.b = C.c_void_p()
+        r = self.lib.qas_batch_create(C.byref(self.b), arr, len(self.solvers))
+        if r != QAS_OK:
+            msg = self.lib.qas_batch_last_error(self.b).decode() if self.b else "qas_batch_create failed"
+            self.lib.qas_batch_destroy(self.b)
+            self.b = None
+            raise QasError("qas_batch_create: %s (%d)" % (msg, r))
+
+    def run(self, nsteps, flg=1, dts=None, history=True):
+        """qas_run on every domain; returns (dt_history[n, nsteps], dt_now[n])."""
+        n = len(self.solvers)
+        d = np.zeros(n) if dts is None else np.array(dts, dtype=np.float64)
+        hist = np.zeros((n, max(nsteps, 1)))
+        hp = hist.ctypes.data_as(C.POINTER(C.c_double)) if history else None
+        r = self.lib.qas_batch_run(self.b, int(nsteps), int(flg), d.ctypes.data_as(C.POINTER(C.c_double)), hp)
+        if r != QAS_OK:
+            raise QasError("qas_batch_run: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
+        return hist[:, :nsteps], d
+
+    def close(self):
+        if self.b:
+            self.lib.qas_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def solver_for(problem, **over):

@@ -1,0 +1,415 @@
+// qas_batch.hip -- batched 2D domains: qas_batch_create / qas_batch_run.
+//
+// A batch advances n independent 2D domains (each a qas_ctx with its own
+// grid size, state and dt) with one launch per stage kernel for all of them
+// (qas_batch.hpp), and captures two consecutive steps of those launches in
+// one hipGraph that the run replays.  Per instance the result is the one
+// qas_run would give: the same kernels bodies, the same dt schedule (CFL
+// every second step when flg), the same buffer rotation.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qas_batch.hpp"
+#include "qas_common.hpp"
+#include "qas_device.hpp"
+#include "qas_internal.hpp"
+#include "qas_stage2d.hpp"
+#include "qas_sweep2d.hpp"
+
+namespace qas {
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ long long batch_thread(const int* off, int i, int b)
+{
+  return (long long)(b - off[i]) * blockDim.x + threadIdx.x;
+}
+
+template <int ODR, typename PT>
+__global__ __launch_bounds__(256) void kb_prim2d(const BatchInst* I, const int* off, int n, int stage, double gam)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  prim2d_at<ODR, PT>(I[i].st[stage], I[i].g, I[i].bc, gam, batch_thread(off, i, b));
+}
+
+template <int ODR>
+__global__ __launch_bounds__(256) void kb_emf2d(const BatchInst* I, const int* off, int n, int stage, int ctw)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  emf2d_at<ODR>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
+}
+
+template <int ODR>
+__global__ __launch_bounds__(256) void kb_ct2d(const BatchInst* I, const int* off, int n, int stage)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  ct2d_at<ODR>(I[i].st[stage], I[i].g, batch_thread(off, i, b));
+}
+
+// ghost fill of stage `stage`'s output, or of the step's state (stage < 0)
+__global__ __launch_bounds__(256) void kb_bound(const BatchInst* I, const int* off, int n, int stage)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const BatchInst& B = I[i];
+  BoundArgs a;
+  for (int m = 0; m < 8; m++) a.f[m] = stage < 0 ? B.cur[m] : B.st[stage].out[m];
+  a.nvar = 8;
+  bound_at(a, B.g, B.bc, B.nghost, NG, NG, 0, batch_thread(off, i, b));
+}
+
+__global__ __launch_bounds__(256) void kb_cfl(const BatchInst* I, const int* off, int n, double gam)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const BatchInst& B = I[i];
+  CflArgs a;
+  for (int m = 0; m < 8; m++) a.u[m] = B.cur[m];
+  a.part = B.cflpart;
+  a.gam = gam;
+  a.stx4 = B.bc.st[0][4];
+  a.sty5 = B.bc.st[1][5];
+  a.stz6 = B.bc.st[2][6];
+  cfl_block(a, B.g, b - off[i], off[i + 1] - off[i]);
+}
+
+// one block per instance
+__global__ __launch_bounds__(256) void kb_cfl_final(const BatchInst* I, int n)
+{
+  const BatchInst& B = I[blockIdx.x];
+  cfl_final_block(B.cflpart, B.cflblocks, B.dt, nullptr, B.cfl_dr);
+}
+
+// one thread per instance
+__global__ void kb_set_dt(const BatchInst* I, int n, const double* dts)
+{
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n && dts[i] > 0.0) *I[i].dt = dts[i];
+}
+__global__ void kb_record(const BatchInst* I, int n)
+{
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const long long k = *I[i].hcount;
+  if (k < I[i].hcap) I[i].hist[k] = *I[i].dt;
+  *I[i].hcount = k + 1;
+}
+__global__ void kb_zero_counts(const BatchInst* I, int n)
+{
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) *I[i].hcount = 0;
+}
+__global__ void kb_get_dt(const BatchInst* I, int n, double* out)
+{
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) out[i] = *I[i].dt;
+}
+
+typedef void (*prim_fn)(const BatchInst*, const int*, int, int, double);
+typedef void (*emf_fn)(const BatchInst*, const int*, int, int, int);
+typedef void (*ct_fn)(const BatchInst*, const int*, int, int);
+
+}  // namespace qas
+
+using namespace qas;
+
+struct qas_batch {
+  std::vector<qas_ctx*> c;
+  int n = 0;
+  qas_scheme s;
+  hipStream_t stream = nullptr;
+  BatchInst* d_inst = nullptr;  // n descriptors
+  int* d_off = nullptr;         // BK_COUNT tables of n + 1 offsets
+  double* d_dts = nullptr;      // n step sizes (in / out)
+  int nblk[BK_COUNT] = {};
+  std::vector<BatchInst> inst;
+  batch_fn sx = nullptr, sy = nullptr;
+  prim_fn prim = nullptr;
+  emf_fn emf = nullptr;
+  ct_fn ct = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_flg = -1, graph_hist = -1;
+  long long nstep = 0;
+  std::string err;
+};
+
+static int bfail(qas_batch* b, int code, const std::string& m)
+{
+  if (b) b->err = m;
+  return code;
+}
+
+#define QAS_BCHECK(b, expr)                                                                            \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess) return bfail((b), QAS_E_HIP, std::string(#expr) + ": " + hipGetErrorName(_e)); \
+  } while (0)
+
+static void rk_ab(int r_k, int rk, double* a, double* b)
+{
+  // rk_fac of mhd2d_solve.cpp:7
+  if (rk == 0) { *a = 0.0; *b = 1.0; }
+  else if (rk == 1) { *a = 0.5 + (r_k - 2) * 0.25; *b = 0.5 - (r_k - 2) * 0.25; }
+  else { *a = 1. / 3.; *b = 2. / 3.; }
+}
+
+static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// descriptors from the contexts' current buffers (rebuilt every run)
+static int fill_table(qas_batch* b)
+{
+  const int n = b->n;
+  const int R = b->s.r_k;
+  b->inst.assign(n, BatchInst());
+  std::vector<int> off((size_t)BK_COUNT * (n + 1), 0);
+  for (int i = 0; i < n; i++) {
+    qas_ctx* c = b->c[i];
+    BatchInst& B = b->inst[i];
+    std::memset(&B, 0, sizeof(B));
+    const Geo& g = c->geo;
+    const int A = c->cur, Bb = (c->cur + 1) % 3, Cc = (c->cur + 2) % 3;
+    int in_[3], out_[3];
+    if (R == 2) { in_[0] = A; out_[0] = Bb; in_[1] = Bb; out_[1] = A; }
+    else { in_[0] = A; out_[0] = Bb; in_[1] = Bb; out_[1] = Cc; in_[2] = Cc; out_[2] = A; }
+    for (int rk = 0; rk < R; rk++) {
+      StagePtrs& p = B.st[rk];
+      for (int m = 0; m < 8; m++) {
+        p.in[m] = c->U[in_[rk]][m];
+        p.out[m] = c->U[out_[rk]][m];
+        p.u0[m] = c->U[A][m];
+      }
+      p.vx = c->vx; p.vy = c->vy; p.vz = c->vz; p.pr = c->pr; p.cx = c->cx; p.cy = c->cy;
+      for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
+      for (int k = 0; k < 8; k++) p.emf[k] = c->emf[k];
+      p.ez = c->ez;
+      p.dt = c->d_dt;
+      rk_ab(R, rk, &p.a, &p.b);
+    }
+    B.g = g;
+    B.bc = c->bc;
+    for (int m = 0; m < 8; m++) B.cur[m] = c->U[A][m];
+    B.cflpart = c->d_cflpart;
+    B.dt = c->d_dt;
+    B.hist = c->d_hist;
+    B.hcount = c->d_hcount;
+    B.hcap = c->hist_cap;
+    B.nghost = (long long)g.nx * g.ny - (long long)(g.nx - 2 * NG) * (g.ny - 2 * NG);
+    B.cfl_dr = c->s.cfl * ((g.dx >= g.dy) ? g.dy : g.dx);
+    B.nsx = (g.nx - 5 + SEG - 1) / SEG;
+    B.nsy = (g.ny - 5 + SEG - 1) / SEG;
+    const long long ninter = (long long)(g.nx - 2 * NG) * (g.ny - 2 * NG);
+    B.cflblocks = std::min(cdiv(ninter, 256), CFL_MAX_BLOCKS);
+    const int cnt[BK_COUNT] = {cdiv(g.nd, 256),
+                               cdiv((long long)B.nsx * g.ny * WAVE, 256),
+                               ((g.nx + YC - 1) / YC) * B.nsy,
+                               cdiv((long long)(g.nx - 5) * (g.ny - 5), 256),
+                               cdiv((long long)(g.nx - 7) * (g.ny - 7), 256),
+                               cdiv(B.nghost * 8, 256),
+                               B.cflblocks};
+    for (int k = 0; k < BK_COUNT; k++) off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
+  }
+  for (int k = 0; k < BK_COUNT; k++) b->nblk[k] = off[(size_t)k * (n + 1) + n];
+  QAS_BCHECK(b, hipMemcpyAsync(b->d_inst, b->inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice, b->stream));
+  QAS_BCHECK(b, hipMemcpyAsync(b->d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, b->stream));
+  QAS_BCHECK(b, hipStreamSynchronize(b->stream));
+  return QAS_OK;
+}
+
+static const int* boff(const qas_batch* b, int k) { return b->d_off + (size_t)k * (b->n + 1); }
+
+static hipError_t launch_bound_all(qas_batch* b, int stage)
+{
+  hipLaunchKernelGGL(kb_bound, (unsigned)b->nblk[BK_BOUND], 256, 0, b->stream, b->d_inst, boff(b, BK_BOUND), b->n,
+                     stage);
+  return hipGetLastError();
+}
+
+static hipError_t launch_cfl_all(qas_batch* b)
+{
+  hipLaunchKernelGGL(kb_cfl, (unsigned)b->nblk[BK_CFL], 256, 0, b->stream, b->d_inst, boff(b, BK_CFL), b->n,
+                     b->s.gam);
+  hipLaunchKernelGGL(kb_cfl_final, (unsigned)b->n, 256, 0, b->stream, b->d_inst, b->n);
+  return hipGetLastError();
+}
+
+// one step of exec_'s loop body for every instance
+static hipError_t batch_step(qas_batch* b, int flg, bool hist, bool even)
+{
+  hipStream_t st = b->stream;
+  const int n = b->n;
+  if (hist) hipLaunchKernelGGL(kb_record, (unsigned)cdiv(n, 64), 64, 0, st, b->d_inst, n);
+  for (int rk = 0; rk < b->s.r_k; rk++) {
+    hipLaunchKernelGGL(b->prim, (unsigned)b->nblk[BK_PRIM], 256, 0, st, b->d_inst, boff(b, BK_PRIM), n, rk, b->s.gam);
+    hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], 256, 0, st, b->d_inst, boff(b, BK_SX), n, rk, b->s.gam);
+    hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, b->d_inst, boff(b, BK_SY), n, rk, b->s.gam);
+    hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], 256, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
+    hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], 256, 0, st, b->d_inst, boff(b, BK_CT), n, rk);
+    hipError_t e = launch_bound_all(b, rk);
+    if (e != hipSuccess) return e;
+  }
+  if (flg && even) return launch_cfl_all(b);
+  return hipGetLastError();
+}
+
+template <int ODR>
+static void pick_stage(qas_batch* b, bool f32)
+{
+  b->prim = f32 ? (prim_fn)kb_prim2d<ODR, float> : (prim_fn)kb_prim2d<ODR, double>;
+  b->emf = kb_emf2d<ODR>;
+  b->ct = kb_ct2d<ODR>;
+}
+
+extern "C" int qas_batch_create(qas_batch** out, qas_ctx* const* ctxs, int n)
+{
+  if (!out) return QAS_E_ARG;
+  *out = nullptr;
+  qas_batch* b = new qas_batch();
+  *out = b;
+  if (!ctxs || n <= 0) return bfail(b, QAS_E_ARG, "qas_batch_create: no contexts");
+  b->n = n;
+  b->c.assign(ctxs, ctxs + n);
+  for (int i = 0; i < n; i++)
+    if (!b->c[i] || !b->c[i]->slab) return bfail(b, QAS_E_ARG, "qas_batch_create: invalid context");
+  b->s = b->c[0]->s;
+  int dev0 = b->c[0]->device;
+  for (int i = 0; i < n; i++) {
+    const qas_ctx* c = b->c[i];
+    const qas_scheme& s = c->s;
+    if (c->g.ndim != 2) return bfail(b, QAS_E_ARG, "qas_batch_create: batches hold 2D domains");
+    if (c->device != dev0) return bfail(b, QAS_E_ARG, "qas_batch_create: all domains on one device");
+    if (s.rmn != b->s.rmn || s.odr != b->s.odr || s.r_k != b->s.r_k || s.ctw != b->s.ctw ||
+        s.precision != b->s.precision || s.gam != b->s.gam)
+      return bfail(b, QAS_E_ARG, "qas_batch_create: every domain needs the same rmn, odr, r_k, ctw, precision, gam");
+    for (int j = 0; j < i; j++)
+      if (b->c[j] == c) return bfail(b, QAS_E_ARG, "qas_batch_create: a context appears twice");
+  }
+  if (b->s.r_k < 2) return bfail(b, QAS_E_ARG, "qas_batch_create: r_k must be 2 or 3");
+  const bool f32 = b->s.precision == 1;
+  b->sx = f32 ? pick_batch_sweep2d_f32(0, b->s.rmn, b->s.odr) : pick_batch_sweep2d(0, b->s.rmn, b->s.odr);
+  b->sy = f32 ? pick_batch_sweep2d_f32(1, b->s.rmn, b->s.odr) : pick_batch_sweep2d(1, b->s.rmn, b->s.odr);
+  if (!b->sx || !b->sy) return bfail(b, QAS_E_ARG, "qas_batch_create: no kernel for this scheme");
+  switch (b->s.odr) {
+    case 1: pick_stage<1>(b, f32); break;
+    case 2: pick_stage<2>(b, f32); break;
+    case 3: pick_stage<3>(b, f32); break;
+    default: pick_stage<4>(b, f32); break;
+  }
+  // the prim kernel's ODR is fcen's (2nd order for ODR 1-2, 4th for 3-4)
+  if (b->s.odr <= 2) b->prim = f32 ? (prim_fn)kb_prim2d<2, float> : (prim_fn)kb_prim2d<2, double>;
+  else b->prim = f32 ? (prim_fn)kb_prim2d<4, float> : (prim_fn)kb_prim2d<4, double>;
+  QAS_BCHECK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  QAS_BCHECK(b, hipMalloc(&b->d_inst, sizeof(BatchInst) * n));
+  QAS_BCHECK(b, hipMalloc(&b->d_off, sizeof(int) * (size_t)BK_COUNT * (n + 1)));
+  QAS_BCHECK(b, hipMalloc(&b->d_dts, sizeof(double) * n));
+  return QAS_OK;
+}
+
+extern "C" void qas_batch_destroy(qas_batch* b)
+{
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->d_off) (void)hipFree(b->d_off);
+  if (b->d_dts) (void)hipFree(b->d_dts);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" const char* qas_batch_last_error(const qas_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+static int build_batch_graph(qas_batch* b, int flg, bool hist)
+{
+  if (b->gexec) { QAS_BCHECK(b, hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+  if (b->graph) { QAS_BCHECK(b, hipGraphDestroy(b->graph)); b->graph = nullptr; }
+  QAS_BCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e1 = batch_step(b, flg, hist, false);
+  hipError_t e2 = batch_step(b, flg, hist, true);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e1 != hipSuccess || e2 != hipSuccess || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(b, QAS_E_HIP, std::string("batch graph capture: ") +
+                                   hipGetErrorName(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e)));
+  }
+  b->graph = g;
+  QAS_BCHECK(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  b->graph_flg = flg;
+  b->graph_hist = hist ? 1 : 0;
+  return QAS_OK;
+}
+
+extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout, double* dt_hist)
+{
+  if (!b || !b->stream) return bfail(b, QAS_E_ARG, "qas_batch_run: invalid batch");
+  if (nsteps < 0 || !dt_inout) return bfail(b, QAS_E_ARG, "qas_batch_run: bad arguments");
+  const int n = b->n;
+  const long long n0 = b->c[0]->nstep;
+  for (int i = 0; i < n; i++) {
+    qas_ctx* c = b->c[i];
+    if (!c->uploaded) return bfail(b, QAS_E_STATE, "qas_batch_run: a domain has no state uploaded");
+    if (c->nstep != n0) return bfail(b, QAS_E_STATE, "qas_batch_run: the domains' step counters differ");
+    QAS_BCHECK(b, hipStreamSynchronize(c->stream));
+  }
+  const bool hist = dt_hist != nullptr;
+  bool table_changed = false;
+  if (hist) {
+    for (int i = 0; i < n; i++) {
+      qas_ctx* c = b->c[i];
+      if (nsteps > c->hist_cap) {
+        if (c->d_hist) QAS_BCHECK(b, hipFree(c->d_hist));
+        c->d_hist = nullptr;
+        QAS_BCHECK(b, hipMalloc(&c->d_hist, sizeof(double) * (size_t)nsteps));
+        c->hist_cap = nsteps;
+        table_changed = true;
+      }
+    }
+  }
+  (void)table_changed;
+  int r = fill_table(b);
+  if (r) return r;
+  if (hist) hipLaunchKernelGGL(kb_zero_counts, (unsigned)cdiv(n, 64), 64, 0, b->stream, b->d_inst, n);
+  // dt: instances with dt_inout[i] > 0 take it, the others the CFL value
+  bool any_cfl = false;
+  for (int i = 0; i < n; i++) any_cfl = any_cfl || !(dt_inout[i] > 0.0);
+  if (any_cfl) QAS_BCHECK(b, launch_cfl_all(b));
+  QAS_BCHECK(b, hipMemcpyAsync(b->d_dts, dt_inout, sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(kb_set_dt, (unsigned)cdiv(n, 64), 64, 0, b->stream, b->d_inst, n, b->d_dts);
+  bool bounded = true;
+  for (int i = 0; i < n; i++) bounded = bounded && b->c[i]->bounded;
+  if (!bounded) QAS_BCHECK(b, launch_bound_all(b, -1));
+  long long ns = n0;
+  int k = 0;
+  if (k < nsteps && (ns % 2) == 1) {
+    ns++;
+    QAS_BCHECK(b, batch_step(b, flg, hist, true));
+    k++;
+  }
+  if (nsteps - k >= 2) {
+    if (!b->gexec || b->graph_flg != flg || b->graph_hist != (hist ? 1 : 0)) {
+      r = build_batch_graph(b, flg, hist);
+      if (r) return r;
+    }
+    for (; k + 2 <= nsteps; k += 2) {
+      QAS_BCHECK(b, hipGraphLaunch(b->gexec, b->stream));
+      ns += 2;
+    }
+  }
+  for (; k < nsteps; k++) {
+    ns++;
+    QAS_BCHECK(b, batch_step(b, flg, hist, ns % 2 == 0));
+  }
+  hipLaunchKernelGGL(kb_get_dt, (unsigned)cdiv(n, 64), 64, 0, b->stream, b->d_inst, n, b->d_dts);
+  QAS_BCHECK(b, hipMemcpyAsync(dt_inout, b->d_dts, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+  QAS_BCHECK(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; i++) {
+    qas_ctx* c = b->c[i];
+    c->nstep = ns;
+    c->bounded = true;
+    if (hist && nsteps > 0)
+      QAS_BCHECK(b, hipMemcpy(dt_hist + (size_t)i * nsteps, c->d_hist, sizeof(double) * (size_t)nsteps,
+                              hipMemcpyDeviceToHost));
+  }
+  return QAS_OK;
+}

@@ -1,0 +1,53 @@
+// qas_batch.hpp -- device side of batched 2D domains (qas_batch_*): many
+// independent domains of uneven sizes advanced by single launches.
+//
+// Each launch of a batched kernel covers every instance: instance i owns
+// the blocks [off[i], off[i+1]) of the grid (a per-kernel-type prefix sum
+// computed on the host), finds itself by a binary search over off[] on its
+// (uniform) workgroup index, and runs the single-domain body with its own
+// descriptor and local block index.
+#pragma once
+#include "qas_internal.hpp"
+
+namespace qas {
+
+// kernel types with their own block-offset table
+enum BatchKernel { BK_PRIM = 0, BK_SX, BK_SY, BK_EMF, BK_CT, BK_BOUND, BK_CFL, BK_COUNT };
+
+struct BatchInst {
+  StagePtrs st[3];   // stage descriptors of one step (buffer rotation, rk_fac, dt)
+  Geo g;
+  BcFlags bc;
+  double* cur[8];    // the state at the start of a step (and after it, R_K 2 and 3)
+  double* cflpart;   // per-block CFL maxima
+  double* dt;        // this instance's device dt
+  double* hist;      // dt history (qas_batch_run with a history)
+  long long* hcount;
+  long long hcap;
+  long long nghost;  // ghost cells per array
+  double cfl_dr;     // cfl * min(dx, dy)
+  int nsx, nsy;      // sweep segments along x and y
+  int cflblocks;     // blocks of this instance's CFL reduction
+  int pad;
+};
+
+// instance owning block b: off[0..n] ascending, off[0] = 0, off[n] = grid
+__device__ __forceinline__ int batch_find(const int* off, int n, int b)
+{
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+typedef void (*batch_fn)(const BatchInst*, const int*, int, int, double);
+
+// batched sweeps (instantiated in qas_kernels2d.hip for FP64 and in
+// qas_mixed.hip for the mixed path)
+batch_fn pick_batch_sweep2d(int dir, int rmn, int odr);
+batch_fn pick_batch_sweep2d_f32(int dir, int rmn, int odr);
+
+}  // namespace qas

@@ -11,60 +11,16 @@
 //               LDS, one partial maximum per block (no atomics).
 //   k_cfl_final dt = cfl*dr/vmax into device memory from the partials,
 //               with the reference's floor vmax >= 1.0.
+#include "qas_common.hpp"
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
 
 namespace qas {
 
-struct BoundArgs {
-  double* f[8];
-  int nvar;
-};
-
 __global__ __launch_bounds__(256) void k_bound(BoundArgs a, Geo g, BcFlags bc, long long nghost, int ox, int oy,
                                                int oz)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nghost * a.nvar) return;
-  const int m = (int)(t / nghost);
-  long long r = t - (long long)m * nghost;
-  // decode ghost cell r: z slabs, then y slabs of the inner z range, then
-  // x slabs of the inner (y, z) range
-  const long long nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long zs = 2LL * oz * ny * nx;
-  const long long ys = 2LL * oy * nx * (nz - 2 * oz);
-  int i, j, k;
-  if (r < zs) {
-    const long long plane = r / (ny * nx);
-    const long long rem = r - plane * ny * nx;
-    k = (int)(plane < oz ? plane : nz - 2 * oz + plane);
-    j = (int)(rem / nx);
-    i = (int)(rem % nx);
-  } else if (r < zs + ys) {
-    r -= zs;
-    const long long row = r / nx;  // 0 .. 2*oy*(nz-2oz)-1
-    i = (int)(r % nx);
-    k = oz + (int)(row / (2 * oy));
-    const int jj = (int)(row % (2 * oy));
-    j = jj < oy ? jj : (int)ny - 2 * oy + jj;
-  } else {
-    r -= zs + ys;
-    const long long col = r % (2 * ox);
-    const long long line = r / (2 * ox);
-    i = (int)(col < ox ? col : nx - 2 * ox + col);
-    j = oy + (int)(line % (ny - 2 * oy));
-    k = oz + (int)(line / (ny - 2 * oy));
-  }
-  const Src sx = ox ? resolve(i, g.nx, bc.st[0][m], bc.dn[0][m]) : Src{i, 1.0, false};
-  const Src sy = oy ? resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]) : Src{j, 1.0, false};
-  const Src sz = oz ? resolve(k, g.nz, bc.st[2][m], bc.dn[2][m]) : Src{k, 1.0, false};
-  if (sx.i == i && sy.i == j && sz.i == k) return;  // a kept cell (staggered boundary face)
-  double* f = a.f[m];
-  double v = f[(long long)nx * (ny * sz.i + sy.i) + sx.i];
-  v = apply(sx, v);
-  v = apply(sy, v);
-  v = apply(sz, v);
-  f[(long long)nx * (ny * k + j) + i] = v;
+  bound_at(a, g, bc, nghost, ox, oy, oz, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar)
@@ -84,98 +40,12 @@ hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar)
 }
 
 // ------------------------------------------------------------------ CFL
-struct CflArgs {
-  const double* u[8];
-  double* part;  // one maximum per block
-  double gam;
-  int stx4, sty5, stz6;
-};
+__global__ __launch_bounds__(256) void k_cfl(CflArgs a, Geo g) { cfl_block(a, g, (int)blockIdx.x, (int)gridDim.x); }
 
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const double w = __shfl_xor(v, o, 64);
-    v = (w > v) ? w : v;
-  }
-  return v;
-}
-
-// max over the block's 4 wavefronts, valid in thread 0
-__device__ __forceinline__ double block_max(double v)
-{
-  __shared__ double wm[4];
-  v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; w++) v = (wm[w] > v) ? wm[w] : v;
-  }
-  return v;
-}
-
-// grid-stride over the interior; each block writes its maximum of
-// |v| + vfast (floor 1.0, the reference's initial vmax) to part[block]
-__global__ __launch_bounds__(256) void k_cfl(CflArgs a, Geo g)
-{
-  const int ox = qas::NG;
-  const int oy = g.ndim >= 2 ? qas::NG : 0;
-  const int oz = g.ndim >= 3 ? qas::NG : 0;
-  const int ni = g.nx - 2 * ox, nj = g.ny - 2 * oy, nk = g.nz - 2 * oz;
-  const long long ntot = (long long)ni * nj * nk;
-  double vm = 1.0;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntot;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = ox + (int)(t % ni);
-    const int j = oy + (int)((t / ni) % nj);
-    const int k = oz + (int)(t / ((long long)ni * nj));
-    const long long ss = (long long)g.nx * ((long long)g.ny * k + j) + i;
-    const double gam = a.gam;
-    const double ro = a.u[0][ss];
-    const double* bx = a.u[4];
-    const double* by = a.u[5];
-    const double* bz = a.u[6];
-    double cx, cy, cz;
-    if (g.ndim == 1) {
-      cx = bx[ss];
-      cy = by[ss];
-      cz = bz[ss];
-    } else {
-      cx = 0.5 * (bx[ss] + bx[ss + a.stx4]);
-      cy = 0.5 * (by[ss] + by[ss + g.sy * a.sty5]);
-      cz = (g.ndim == 3) ? 0.5 * (bz[ss] + bz[ss + g.sz * a.stz6]) : bz[ss];
-    }
-    const double iro = 1.0 / ro;
-    const double vx = a.u[1][ss] * iro;
-    const double vy = a.u[2][ss] * iro;
-    const double vz = a.u[3][ss] * iro;
-    const double v2 = vx * vx + vy * vy + vz * vz;
-    const double b2 = cx * cx + cy * cy + cz * cz;
-    const double pr = (gam - 1) * (a.u[7][ss] - 0.5 * (ro * v2 + b2));
-    const double vf = sqrt((gam * pr + (cx * cx + cy * cy + cz * cz)) / ro);
-    double v;
-    if (g.ndim == 1) v = fabs(vx) + vf;
-    else if (g.ndim == 2) v = sqrt(vx * vx + vy * vy) + vf;
-    else v = sqrt(vx * vx + vy * vy + vz * vz) + vf;
-    if (v > vm) vm = v;
-  }
-  vm = block_max(vm);
-  if (threadIdx.x == 0) a.part[blockIdx.x] = vm;
-}
-
-// dt = cfl * dr / max(part[0..nb)); one block of 256 threads.  max is
-// exact and order-independent, so dt equals the reference's serial loop.
 __global__ __launch_bounds__(256) void k_cfl_final(const double* part, int nb, double* dt, double* hist_slot,
                                                    double cfl_dr)
 {
-  double vm = 1.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) vm = (part[b] > vm) ? part[b] : vm;
-  vm = block_max(vm);
-  if (threadIdx.x == 0) {
-    const double d = cfl_dr / vm;
-    *dt = d;
-    if (hist_slot) *hist_slot = d;
-  }
+  cfl_final_block(part, nb, dt, hist_slot, cfl_dr);
 }
 
 __global__ void k_set_dt(double* dt, double v) { *dt = v; }

@@ -48,4 +48,6 @@ sweep_fn_t pick_sweep_f32(int ndim, int dir, int rmn, int odr)
   return nullptr;
 }
 
+batch_fn pick_batch_sweep2d_f32(int dir, int rmn, int odr) { return pick_batch_sweep2d_t<float>(dir, rmn, odr); }
+
 }  // namespace qas

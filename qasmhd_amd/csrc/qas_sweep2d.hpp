@@ -7,6 +7,7 @@
 // move between lanes by cross-lane shuffles, and the wavefront owns the 60
 // faces/cells [f0, f0+60).
 #pragma once
+#include "qas_batch.hpp"
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
 
@@ -51,12 +52,12 @@ constexpr int YR = WAVE + 4;   // stencil rows per segment
 constexpr int YRP = YR + 1;    // padded LDS column length
 
 template <int RMN, int ODR, typename T>
-__device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam, int nseg)
+__device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam, int nseg,
+                                          int L)
 {
   __shared__ double sh[9][YC][YRP];  // ro vy vz vx bz cx pr cy by
   __shared__ double so[8][YC][WAVE];  // du[6], upwind and central B-flux parts
   const int ncg = (g.nx + YC - 1) / YC;
-  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   if (L >= ncg * nseg) return;  // whole workgroup
   const int cg = L % ncg, seg = L / ncg;
   const int i0 = cg * YC;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
 template <int RMN, int ODR, typename T>
 __global__ __launch_bounds__(256, 2) void k_sweep2d_y(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
 {
-  sweep2d_y<RMN, ODR, T>(p, g, bc, gam, nseg);
+  sweep2d_y<RMN, ODR, T>(p, g, bc, gam, nseg, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // ------------------------------------------------------------------ sweeps
@@ -186,10 +187,10 @@ __global__ __launch_bounds__(256, 2) void k_sweep2d_y(StagePtrs p, Geo g, BcFlag
 // row).  DIR 1, the same per-lane layout along columns with strided loads,
 // is kept for reference and testing; the stage uses k_sweep2d_y.
 template <int DIR, int RMN, int ODR, typename T>
-__global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+__device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam,
+                                             int nseg, int wave)
 {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   const int nline = DIR == 0 ? g.ny : g.nx;
   if (wave >= nseg * nline) return;  // whole wavefront exits together
   const int line = wave / nseg;
@@ -308,6 +309,46 @@ __global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc,
       p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (p.acc[v][c] - dtdy * du[v]), p.a, p.b);
     }
   }
+}
+
+template <int DIR, int RMN, int ODR, typename T>
+__global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+{
+  sweep2d_lane<DIR, RMN, ODR, T>(p, g, bc, gam, nseg, (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE));
+}
+
+// batched versions (qas_batch.hpp): one launch over every instance
+template <int RMN, int ODR, typename T>
+__global__ __launch_bounds__(256) void kb_sweep2d_x(const BatchInst* I, const int* off, int n, int stage, double gam)
+{
+  const int b = (int)blockIdx.x;
+  const int i = batch_find(off, n, b);
+  const BatchInst& B = I[i];
+  sweep2d_lane<0, RMN, ODR, T>(B.st[stage], B.g, B.bc, gam, B.nsx,
+                               (b - off[i]) * (int)(blockDim.x / WAVE) + (int)(threadIdx.x / WAVE));
+}
+
+template <int RMN, int ODR, typename T>
+__global__ __launch_bounds__(256, 2) void kb_sweep2d_y(const BatchInst* I, const int* off, int n, int stage,
+                                                        double gam)
+{
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int i = batch_find(off, n, b);
+  const BatchInst& B = I[i];
+  sweep2d_y<RMN, ODR, T>(B.st[stage], B.g, B.bc, gam, B.nsy, b - off[i]);
+}
+
+template <typename T>
+batch_fn pick_batch_sweep2d_t(int dir, int rmn, int odr)
+{
+#define QAS_S(R, O) \
+  if (rmn == R && odr == O) return dir == 0 ? kb_sweep2d_x<R, O, T> : kb_sweep2d_y<R, O, T>;
+  QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
+  QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
+  QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
+  QAS_S(3, 1) QAS_S(3, 2) QAS_S(3, 3) QAS_S(3, 4)
+#undef QAS_S
+  return nullptr;
 }
 
 }  // namespace qas

@@ -1,0 +1,62 @@
+"""Batched domains (qas_batch_*): every instance of a batch of uneven 2D
+grids ends bit-identical to its own single-domain qas_run -- state and dt
+history -- which in turn equals the reference (test_gpu_parity.py)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    from qasmhd_amd import problems
+    sizes = [(32, 32), (48, 24), (24, 56), (64, 40)]
+    return [problems.turbulence2d(n=nx, ny=ny, seed=10 + i, vrms=0.3 + 0.2 * i) for i, (nx, ny) in enumerate(sizes)]
+
+
+@pytest.mark.parametrize("precision,nsteps", [(0, 7), (0, 10), (1, 6)])
+def test_batch_matches_single_runs(precision, nsteps):
+    from qasmhd_amd import capi
+    print("case: batch precision", precision, "steps", nsteps, flush=True)
+    ps = _problems()
+    ref = []
+    for p in ps:
+        with capi.solver_for(p, precision=precision) as s:
+            s.upload(p.state)
+            h, dt = s.run(nsteps, flg=1)
+            ref.append((h, dt, s.download()))
+    solvers = [capi.solver_for(p, precision=precision) for p in ps]
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+    with capi.Batch(solvers) as b:
+        hist, dts = b.run(nsteps, flg=1)
+        # a second call continues the step counter as qas_run does
+        hist2, dts2 = b.run(3, flg=1)
+    outs = [s.download() for s in solvers]
+    for i, p in enumerate(ps):
+        assert np.array_equal(hist[i], ref[i][0]), i
+        assert dts[i] == ref[i][1], i
+    # compare the continued run with single-domain runs of nsteps + 3
+    for i, p in enumerate(ps):
+        with capi.solver_for(p, precision=precision) as s:
+            s.upload(p.state)
+            s.run(nsteps, flg=1)
+            h3, d3 = s.run(3, flg=1)
+            full = s.download()
+        assert np.array_equal(hist2[i], h3), i
+        assert dts2[i] == d3, i
+        assert np.array_equal(outs[i], full), i
+    for s in solvers:
+        s.close()
+
+
+def test_batch_rejects_mixed_schemes():
+    from qasmhd_amd import capi, problems
+    p = problems.turbulence2d(n=16, seed=1)
+    a = capi.solver_for(p)
+    b = capi.solver_for(p, rmn=3)
+    a.upload(p.state)
+    b.upload(p.state)
+    with pytest.raises(capi.QasError):
+        capi.Batch([a, b])
+    a.close()
+    b.close()
