@@ -126,19 +126,25 @@ def mixed_line(n, steps):
     with capi.solver_for(p) as s:
         s.upload(p.state)
         dt = s.cfl_dt()
-    out, rate = {}, {}
-    for prec in (0, 1):
-        with capi.solver_for(p, precision=prec) as s:
+    out, rate = {}, {0: 0.0, 1: 0.0}
+    solvers = {prec: capi.solver_for(p, precision=prec) for prec in (0, 1)}
+    for s in solvers.values():
+        s.upload(p.state)
+        s.run(4, flg=0, dt=dt, history=False)  # builds the step graph
+    for trial in range(2):  # alternate the two precisions; best of two each
+        for prec in (0, 1):
+            s = solvers[prec]
             s.upload(p.state)
-            s.run(4, flg=0, dt=dt, history=False)  # builds the step graph
-            s.upload(p.state)
+            s.reset_counter()
             s.sync()
             t0 = time.perf_counter()
             s.run(steps, flg=0, dt=dt, history=False)
             s.sync()
             el = time.perf_counter() - t0
             out[prec] = s.download()
-            rate[prec] = zones * steps / el
+            rate[prec] = max(rate[prec], zones * steps / el)
+    for s in solvers.values():
+        s.close()
     gam = p.gam
 
     def pres(st):
@@ -248,6 +254,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and adjoint lines")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host (for counters)")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64 (default), 1 mixed, for the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy the 8 state arrays to a device buffer (8 copy "
                          "dispatches of nd doubles: the calibration of the memory counters)")
@@ -274,7 +281,7 @@ def main():
     lib.qas_prof_enable.argtypes = [C.c_void_p, C.c_uint, C.c_int]
     lib.qas_prof_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     lib.qas_kernel_model.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    s = capi.solver_for(p)
+    s = capi.solver_for(p, precision=args.precision)
     if args.no_graphs:
         s.lib.qas_use_graphs(s.ctx, 0)
     s.upload(p.state)
@@ -367,9 +374,10 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if args.precision == 0 else "f32+f64", "data": "synthetic",
         "config": {"workload": wname, "cells": list(p.interior), "solver": ["roe", "hlld", "lhlld", "mlau"][p.rmn],
-                   "odr": p.odr, "r_k": p.r_k, "ctw": p.ctw, "cfl": p.cfl, "precision": "fp64",
+                   "odr": p.odr, "r_k": p.r_k, "ctw": p.ctw, "cfl": p.cfl,
+                   "precision": "fp64" if args.precision == 0 else "mixed",
                    "dt_final": dt_now, "finite": finite},
         "roofline": roof, "cpu_baseline": cpu,
     }

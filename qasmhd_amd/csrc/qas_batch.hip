@@ -29,28 +29,28 @@ __device__ __forceinline__ long long batch_thread(const int* off, int i, int b)
 template <int ODR, typename PT>
 __global__ __launch_bounds__(256) void kb_prim2d(const BatchInst* I, const int* off, int n, int stage, double gam)
 {
-  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
   prim2d_at<ODR, PT>(I[i].st[stage], I[i].g, I[i].bc, gam, batch_thread(off, i, b));
 }
 
 template <int ODR>
 __global__ __launch_bounds__(256) void kb_emf2d(const BatchInst* I, const int* off, int n, int stage, int ctw)
 {
-  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
   emf2d_at<ODR>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
 }
 
 template <int ODR>
 __global__ __launch_bounds__(256) void kb_ct2d(const BatchInst* I, const int* off, int n, int stage)
 {
-  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
   ct2d_at<ODR>(I[i].st[stage], I[i].g, batch_thread(off, i, b));
 }
 
 // ghost fill of stage `stage`'s output, or of the step's state (stage < 0)
 __global__ __launch_bounds__(256) void kb_bound(const BatchInst* I, const int* off, int n, int stage)
 {
-  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
   const BatchInst& B = I[i];
   BoundArgs a;
   for (int m = 0; m < 8; m++) a.f[m] = stage < 0 ? B.cur[m] : B.st[stage].out[m];
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void kb_bound(const BatchInst* I, const int* o
 
 __global__ __launch_bounds__(256) void kb_cfl(const BatchInst* I, const int* off, int n, double gam)
 {
-  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
   const BatchInst& B = I[i];
   CflArgs a;
   for (int m = 0; m < 8; m++) a.u[m] = B.cur[m];
@@ -179,7 +179,7 @@ static int fill_table(qas_batch* b)
         p.u0[m] = c->U[A][m];
       }
       p.vx = c->vx; p.vy = c->vy; p.vz = c->vz; p.pr = c->pr; p.cx = c->cx; p.cy = c->cy;
-      for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
+      for (int k = 0; k < NACC; k++) p.acc[k] = c->acc[k];
       for (int k = 0; k < 8; k++) p.emf[k] = c->emf[k];
       p.ez = c->ez;
       p.dt = c->d_dt;
@@ -240,8 +240,8 @@ static hipError_t batch_step(qas_batch* b, int flg, bool hist, bool even)
   if (hist) hipLaunchKernelGGL(kb_record, (unsigned)cdiv(n, 64), 64, 0, st, b->d_inst, n);
   for (int rk = 0; rk < b->s.r_k; rk++) {
     hipLaunchKernelGGL(b->prim, (unsigned)b->nblk[BK_PRIM], 256, 0, st, b->d_inst, boff(b, BK_PRIM), n, rk, b->s.gam);
-    hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], 256, 0, st, b->d_inst, boff(b, BK_SX), n, rk, b->s.gam);
     hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, b->d_inst, boff(b, BK_SY), n, rk, b->s.gam);
+    hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], 256, 0, st, b->d_inst, boff(b, BK_SX), n, rk, b->s.gam);
     hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], 256, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
     hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], 256, 0, st, b->d_inst, boff(b, BK_CT), n, rk);
     hipError_t e = launch_bound_all(b, rk);

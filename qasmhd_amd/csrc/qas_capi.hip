@@ -129,7 +129,7 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
   const long long nd = geo.nd;
   const size_t stride = (size_t)((nd + 63) / 64 * 64);
   const int nprim = g->ndim == 3 ? 7 : (g->ndim == 2 ? 6 : 4);
-  const int nacc = g->ndim == 1 ? 0 : 6;
+  const int nacc = g->ndim == 1 ? 0 : (g->ndim == 2 ? 6 : 10);
   const int nemf = g->ndim == 3 ? 12 : (g->ndim == 2 ? 8 : 0);
   const int nedge = g->ndim == 3 ? 3 : (g->ndim == 2 ? 1 : 0);
   const size_t narr = 24 + nprim + nacc + nemf + nedge;
@@ -287,7 +287,7 @@ static int do_step(qas_ctx* c)
     }
     p.vx = c->vx; p.vy = c->vy; p.vz = c->vz; p.pr = c->pr;
     p.cx = c->cx; p.cy = c->cy; p.cz = c->cz;
-    for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
+    for (int k = 0; k < NACC; k++) p.acc[k] = c->acc[k];
     for (int k = 0; k < 12; k++) p.emf[k] = c->emf[k];
     p.ex = c->ex; p.ey = c->ey; p.ez = c->ez;
     p.dt = c->d_dt;

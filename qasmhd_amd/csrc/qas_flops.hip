@@ -177,16 +177,17 @@ extern "C" int qas_kernel_model(const qas_ctx* c, int kid, double* flops, double
     const double lrc = (n[d] - 4) * lines;   // reconstructed cells
     const double fac = (n[d] - 5) * lines;   // solved faces
     *flops = lrc * (lr + ntan * pc.lrfb) + fac * (pc.riem + dvf + 3.0 * ntan);
-    const bool last = d == nd3 - 1;
-    // divergence (+ scaling / accumulation / RK stage sum)
-    const double per_var = pc.df + (d == 0 ? 1.0 : 2.0) + (last ? 4.0 : 0.0);
+    // The transverse sweeps run first and store dt/dy Dy (and dt/dz Dz):
+    // difference + scale.  The x sweep scales its own difference, subtracts
+    // the stored ones and applies the RK stage sum (2 mul + 2 add).
+    const double per_var = d == 0 ? pc.df + 1.0 + (nd3 - 1) + 4.0 : pc.df + 1.0;
     *flops += inter * nupd * per_var;
     // bytes: 7 stencil arrays + cell-centred normal B + staggered B (read),
     // 2*ntan face EMF parts (write), accumulators / state
     double b = cells * (9.0 + 2.0 * ntan);
     if (nd3 == 1) b = cells * 8.0;
-    b += inter * nupd * (d == 0 ? 1.0 : 2.0);            // acc write (x) / read+write (y, z)
-    if (last) b += inter * nupd * (2.0 + 1.0 - 1.0);     // in, u0 read; out write; no acc write
+    if (d == 0) b += inter * nupd * ((nd3 - 1) + 3.0);  // stored divergences + in, u0 read; out write
+    else b += inter * nupd * 1.0;                        // this sweep's divergence written
     *bytes = b * 8.0;
     return QAS_OK;
   }

@@ -12,7 +12,10 @@ namespace qas {
 constexpr int NG = 4;       // ghost layers (mhd2d_class.hpp:10)
 constexpr int SEG = 60;     // faces owned by one wavefront segment (64 lanes - 4 halo)
 constexpr int WAVE = 64;
-constexpr int CFL_MAX_BLOCKS = 1024;  // blocks of the CFL reduction (grid-stride)
+constexpr int CFL_MAX_BLOCKS = 1024;
+// accumulator slots: 2D uses 6 (dt/dy Dy of ro mx my mz bz en); 3D uses
+// 5 + 5 (dt/dy Dy and dt/dz Dz of ro mx my mz en)
+constexpr int NACC = 12;  // blocks of the CFL reduction (grid-stride)
 
 // Boundary flags as kernel arguments (dnxs/dnys/dnzs, stxs/stys/stzs).
 struct BcFlags {
@@ -33,7 +36,7 @@ struct StagePtrs {
   double* out[8];        // stage output state
   const double* u0[8];   // state at the start of the step (RK base)
   double *vx, *vy, *vz, *pr, *cx, *cy, *cz;  // primitive scratch
-  double* acc[6];        // partial flux divergence of the sweeps so far
+  double* acc[NACC];     // flux divergences of the sweeps run before the x sweep
   double* emf[12];       // face parts of the B fluxes: [sweep][comp][upwind/central]
   double *ex, *ey, *ez;  // edge EMFs
   const double* dt;      // device dt
@@ -117,7 +120,7 @@ struct qas_ctx {
   int cur = 0;
   double *vx = nullptr, *vy = nullptr, *vz = nullptr, *pr = nullptr, *cx = nullptr, *cy = nullptr,
          *cz = nullptr;
-  double* acc[6] = {};
+  double* acc[qas::NACC] = {};
   double* emf[12] = {};
   double *ex = nullptr, *ey = nullptr, *ez = nullptr;
   double* d_dt = nullptr;             // current dt

@@ -4,11 +4,13 @@
 // mhd2d_solve.cpp:44-386).  A stage is five launches:
 //   prim2d   cell-centred B (fcen, ghosts by the bound() index map) and
 //            primitives at every cell                       (:51-89)
-//   sweep<X> characteristic WCNS + Riemann flux per x face, for every row;
-//            writes the x flux divergence (times -dt/dx) of the interior
-//            and the upwind/central parts of the By flux    (:127-199)
-//   sweep<Y> same along y; adds the y divergence and applies the RK stage
-//            sum to ro, mx, my, mz, bz, en                  (:234-306, 341-358)
+//   sweep<Y> characteristic WCNS + Riemann flux per y face, for every
+//            column (LDS-staged); writes dt/dy times the y flux divergence
+//            of the interior and the upwind/central parts of the Bx flux
+//                                                           (:234-306)
+//   sweep<X> same along x, for every row; applies the RK stage sum
+//            a u0 + b (u + ((-dt/dx Dx) - dt/dy Dy)) to ro, mx, my, mz, bz,
+//            en (the reference's expression)               (:127-199, 341-358)
 //   emf2d    CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
 //   ct2d     staggered bx, by update                        (:359-380)
 // followed by the bound() kernel (:384-385).
@@ -25,23 +27,25 @@
 namespace qas {
 
 // ------------------------------------------------------------------ kernels
-// bodies in qas_stage2d.hpp (shared with the batched kernels)
+// bodies in qas_stage2d.hpp (shared with the batched kernels).  Blocks are
+// renumbered XCD-aware (xcd_remap): the stencils of neighbouring rows then
+// meet in the same L2.
 template <int ODR, typename PT>
 __global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, double gam)
 {
-  prim2d_at<ODR, PT>(p, g, bc, gam, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  prim2d_at<ODR, PT>(p, g, bc, gam, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 template <int ODR>
 __global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
 {
-  emf2d_at<ODR>(p, g, ctw, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  emf2d_at<ODR>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
 {
-  ct2d_at<ODR>(p, g, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  ct2d_at<ODR>(p, g, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ launch
@@ -84,9 +88,10 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   sweep_fn fx = mixed ? pick_sweep_f32(2, 0, c->s.rmn, odr) : pick_sweep<0>(c->s.rmn, odr);
   sweep_fn fy = mixed ? pick_sweep_f32(2, 1, c->s.rmn, odr) : pick_sweep<1>(c->s.rmn, odr);
   if (!fx || !fy) return hipErrorInvalidValue;
-  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx);
-  // y sweep: one 256-thread workgroup per (4-column group, segment)
+  // y sweep first (one 256-thread workgroup per (4-column group, segment)):
+  // it leaves dt/dy * Dy in the accumulators; the x sweep applies the stage sum
   QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, p, g, c->bc, gam, nsy);
+  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx);
   // 4) corner EMF, 5) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);

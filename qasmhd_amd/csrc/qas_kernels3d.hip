@@ -4,12 +4,11 @@
 // 42-715) on one device-resident domain:
 //   prim3d    cell-centred B (fcen along x, y, z; ghosts by the bound()
 //             index map) and primitives at every cell        (:49-98)
-//   sweep<d>  for d = x, y, z: characteristic WCNS + Riemann flux on every
-//             line; partial flux divergence of ro, mx, my, mz, en
-//             accumulated in the reference's order
-//             ((-dtdx Dx) - dtdy Dy) - dtdz Dz, the z sweep applying the RK
-//             stage sum; upwind/central parts of both tangential B fluxes
-//             stored per face                                (:166-258, 330-422, 494-586, 658-676)
+//   sweep<d>  for d = z, y, x: characteristic WCNS + Riemann flux on every
+//             line; z and y store dt/dz Dz and dt/dy Dy of ro, mx, my, mz,
+//             en, the x sweep applies the RK stage sum with the reference's
+//             ((-dtdx Dx) - dtdy Dy) - dtdz Dz; upwind/central parts of both
+//             tangential B fluxes stored per face          (:166-258, 330-422, 494-586, 658-676)
 //   emf3d     Ex, Ey, Ez at cell edges from the face parts (CUCT)
 //                                                            (:100-136, 259-328, 423-492, 587-656)
 //   ct3d      staggered bx, by, bz update                    (:677-709)
@@ -25,7 +24,7 @@ namespace qas {
 template <int ODR, typename PT>
 __global__ __launch_bounds__(256) void k_prim3d(StagePtrs p, Geo g, BcFlags bc, double gam)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
   const int i = (int)(t % g.nx);
   const int j = (int)((t / g.nx) % g.ny);
@@ -92,7 +91,7 @@ __global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
 {
   // edges on [3, n-3]^3
   const int ni = g.nx - 5, nj = g.ny - 5, nk = g.nz - 5;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= (long long)ni * nj * nk) return;
   const int i = 3 + (int)(t % ni);
   const int j = 3 + (int)((t / ni) % nj);
@@ -139,7 +138,7 @@ template <int ODR>
 __global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
 {
   const int ni = g.nx - 7, nj = g.ny - 7, nk = g.nz - 7;  // [4, n-4]
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= (long long)ni * nj * nk) return;
   const int i = 4 + (int)(t % ni);
   const int j = 4 + (int)((t / ni) % nj);
@@ -203,7 +202,9 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   sweep3_fn f[3] = {pick3<0>(c->s.rmn, odr), pick3<1>(c->s.rmn, odr), pick3<2>(c->s.rmn, odr)};
   if (c->s.precision == 1)
     for (int d = 0; d < 3; d++) f[d] = pick_sweep_f32(3, d, c->s.rmn, odr);
-  for (int d = 0; d < 3; d++) {
+  // z, then y, then x: the transverse sweeps leave dt/dz Dz and dt/dy Dy in
+  // the accumulators, the x sweep applies the stage sum
+  for (int d = 2; d >= 0; d--) {
     if (!f[d]) return hipErrorInvalidValue;
     const int nseg = (dims[d] - 5 + SEG - 1) / SEG;
     if (d == 0) {

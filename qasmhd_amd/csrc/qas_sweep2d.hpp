@@ -39,14 +39,14 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
   return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
-// y sweep, LDS-staged.  One workgroup = 4 wavefronts = 4 adjacent columns
-// of one 60-face segment.  The stencil inputs of the 68 rows f0-4..f0+63 of
+// y sweep, LDS-staged; it runs before the x sweep of the stage.  One
+// workgroup = 4 wavefronts = 4 adjacent columns of one 60-face segment.  The stencil inputs of the 68 rows f0-4..f0+63 of
 // the 4 columns are loaded cooperatively (4 consecutive threads read 4
 // consecutive doubles of a row) into LDS, column-major; each wavefront then
 // runs the same per-lane computation as the x sweep on its column from LDS;
-// the outputs (y divergence of 6 variables, 2 face parts) go back through
-// LDS and are written, and combined into the RK stage sum, by coalesced
-// threads.  Same arithmetic as a direct per-lane sweep: bit-identical.
+// the outputs (dt/dy times the y divergence of 6 variables, 2 face parts)
+// go back through LDS and are written by coalesced threads.  Same
+// arithmetic as a direct per-lane sweep: bit-identical.
 constexpr int YC = 4;          // columns per workgroup
 constexpr int YR = WAVE + 4;   // stencil rows per segment
 constexpr int YRP = YR + 1;    // padded LDS column length
@@ -163,29 +163,28 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
       p.emf[5][a] = so[7][c][r];
       const bool upd = (qq >= NG) && (qq <= n - NG - 1) && (i >= NG) && (i <= g.nx - NG - 1);
       if (upd) {
+        // dt/dy * Dy of (ro, mx, my, mz, bz, en); the x sweep, which runs
+        // next, subtracts it in the stage sum
         const double dtdy = *p.dt / g.dy;
-        const int vid[6] = {0, 1, 2, 3, 6, 7};
 #pragma unroll
-        for (int v = 0; v < 6; v++) {
-          const int m = vid[v];
-          p.out[m][a] = rk_upd(p.in[m][a], p.u0[m][a], (p.acc[v][a] - dtdy * so[v][c][r]), p.a, p.b);
-        }
+        for (int v = 0; v < 6; v++) p.acc[v][a] = dtdy * so[v][c][r];
       }
     }
   }
 }
 
 // y faces for every column (grid: 4-column groups x segments)
+// FP32 (mixed) instantiation: 4 waves per SIMD fit without spills
 template <int RMN, int ODR, typename T>
-__global__ __launch_bounds__(256, 2) void k_sweep2d_y(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
+__global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void k_sweep2d_y(StagePtrs p, Geo g, BcFlags bc,
+                                                                             double gam, int nseg)
 {
   sweep2d_y<RMN, ODR, T>(p, g, bc, gam, nseg, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
-// ------------------------------------------------------------------ sweeps
-// DIR 0: x faces for every row j (one wavefront per 60-face segment of a
-// row).  DIR 1, the same per-lane layout along columns with strided loads,
-// is kept for reference and testing; the stage uses k_sweep2d_y.
+// ------------------------------------------------------------------ x sweep
+// x faces for every row j (one wavefront per 60-face segment of a row); it
+// runs after the y sweep and applies the RK stage sum.
 template <int DIR, int RMN, int ODR, typename T>
 __device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam,
                                              int nseg, int wave)
@@ -294,20 +293,17 @@ __device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, c
   const int ntl = DIR == 0 ? g.ny : g.nx;
   const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (tl >= NG) && (tl <= ntl - NG - 1);
   if (!upd) return;
+  // the y sweep ran first and left acc = dt/dy * Dy; the stage sum
+  // u = a u0 + b (u + ((-dt/dx Dx) - dt/dy Dy)) is the reference's
+  // expression in its order (mhd2d_solve.cpp:199, 341-358)
+  static_assert(DIR == 0, "the per-lane sweep applies the stage sum along x");
   const long long c = base + o * q;
-  const double dtv = *p.dt;
-  if (DIR == 0) {
-    const double dtdx = dtv / g.dx;
+  const double dtdx = *p.dt / g.dx;
+  const int vid[6] = {0, 1, 2, 3, 6, 7};
 #pragma unroll
-    for (int v = 0; v < 6; v++) p.acc[v][c] = -dtdx * du[v];
-  } else {
-    const double dtdy = dtv / g.dy;
-    const int vid[6] = {0, 1, 2, 3, 6, 7};
-#pragma unroll
-    for (int v = 0; v < 6; v++) {
-      const int m = vid[v];
-      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (p.acc[v][c] - dtdy * du[v]), p.a, p.b);
-    }
+  for (int v = 0; v < 6; v++) {
+    const int m = vid[v];
+    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (-dtdx * du[v]) - p.acc[v][c], p.a, p.b);
   }
 }
 
@@ -329,8 +325,8 @@ __global__ __launch_bounds__(256) void kb_sweep2d_x(const BatchInst* I, const in
 }
 
 template <int RMN, int ODR, typename T>
-__global__ __launch_bounds__(256, 2) void kb_sweep2d_y(const BatchInst* I, const int* off, int n, int stage,
-                                                        double gam)
+__global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void kb_sweep2d_y(const BatchInst* I, const int* off,
+                                                                              int n, int stage, double gam)
 {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int i = batch_find(off, n, b);

@@ -173,20 +173,12 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
       const bool upd = (qq >= NG) && (qq <= n - NG - 1) && (i >= NG) && (i <= g.nx - NG - 1) && (lb >= NG) &&
                        (lb <= nlb - NG - 1);
       if (upd) {
-        const double dtv = *p.dt;
-        if (DIR == 1) {
-          const double dtdy = dtv / g.dy;
+        // dt/dy Dy (slots 0-4) or dt/dz Dz (slots 5-9) of (ro, mx, my, mz,
+        // en); the x sweep, which runs last, subtracts both
+        const double dtd = *p.dt / (DIR == 1 ? g.dy : g.dz);
+        double* const* acc = p.acc + (DIR == 1 ? 0 : 5);
 #pragma unroll
-          for (int v = 0; v < 5; v++) p.acc[v][a] = p.acc[v][a] - dtdy * so[v][c][r];
-        } else {
-          const double dtdz = dtv / g.dz;
-          const int vid[5] = {0, 1, 2, 3, 7};
-#pragma unroll
-          for (int v = 0; v < 5; v++) {
-            const int m = vid[v];
-            p.out[m][a] = rk_upd(p.in[m][a], p.u0[m][a], p.acc[v][a] - dtdz * so[v][c][r], p.a, p.b);
-          }
-        }
+        for (int v = 0; v < 5; v++) acc[v][a] = dtd * so[v][c][r];
       }
     }
   }
@@ -198,10 +190,9 @@ __global__ __launch_bounds__(256, 2) void k_sweep3d_t(StagePtrs p, Geo g, BcFlag
   sweep3d_t<DIR, RMN, ODR, T>(p, g, bc, gam, nseg);
 }
 
-// ------------------------------------------------------------------ sweeps
-// One wavefront per 60-face segment of a line along DIR (the x sweep; the
-// same layout along y and z, with strided loads, is kept for reference and
-// testing -- the stage uses k_sweep3d_t there).
+// ------------------------------------------------------------------ x sweep
+// One wavefront per 60-face segment of an x line; runs after the z and y
+// sweeps and applies the RK stage sum.
 template <int DIR, int RMN, int ODR, typename T>
 __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
 {
@@ -322,24 +313,17 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
   const bool upd = own && (q >= NG) && (q <= n - NG - 1) && (la >= NG) && (la <= nla - NG - 1) && (lb >= NG) &&
                    (lb <= nlb - NG - 1);
   if (!upd) return;
+  // the z and y sweeps ran first (dt/dz Dz in slots 5-9, dt/dy Dy in 0-4);
+  // the stage sum u = a u0 + b (u + (((-dt/dx Dx) - dt/dy Dy) - dt/dz Dz))
+  // is the reference's expression in its order (mhd3d_solve.cpp:658-676)
+  static_assert(DIR == 0, "the per-lane sweep applies the stage sum along x");
   const long long c = base + o * q;
-  const double dtv = *p.dt;
+  const double dtdx = *p.dt / g.dx;
   const int vid[5] = {0, 1, 2, 3, 7};
-  if (DIR == 0) {
-    const double dtdx = dtv / g.dx;
 #pragma unroll
-    for (int v = 0; v < 5; v++) p.acc[v][c] = -dtdx * du[v];
-  } else if (DIR == 1) {
-    const double dtdy = dtv / g.dy;
-#pragma unroll
-    for (int v = 0; v < 5; v++) p.acc[v][c] = p.acc[v][c] - dtdy * du[v];
-  } else {
-    const double dtdz = dtv / g.dz;
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-      const int m = vid[v];
-      p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], p.acc[v][c] - dtdz * du[v], p.a, p.b);
-    }
+  for (int v = 0; v < 5; v++) {
+    const int m = vid[v];
+    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], ((-dtdx * du[v]) - p.acc[v][c]) - p.acc[5 + v][c], p.a, p.b);
   }
 }
 
