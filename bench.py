@@ -21,6 +21,12 @@ Also reported, on one JSON line:
                 instances run one after another
   adjoint       time of the hand-written discrete adjoint over 20 steps of a
                 256^2 periodic run against the plain forward of those steps
+  inverse       64 batched instances of the velocity-mode inversion: the
+                batched forward, the adjoint gradient of all 512 amplitudes
+                and a finite-difference check
+  turb3d        3D periodic MHD turbulence at 512^3 (HLLD, WCNS-4, 3D CT,
+                RK3; initial state built on the device with torch): rate over
+                20 steps and the roofline of its dominant sweep
   roofline      the dominant kernel's algorithmic work per launch (flops and
                 compulsory bytes from the library's work model) over its
                 average launch time, measured with HIP events on the
@@ -36,6 +42,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -57,8 +65,9 @@ def workload(name):
         return problems.turbulence2d(n=512, seed=2), "2d_decaying_mhd_512x512_hlld_wcns4_ct_rk3"
     if name == "khi":
         return problems.kelvin_helmholtz(n=2048), "2d_lowmach_khi_2048x2048_mlau_wcns4_ct_rk3"
-    if name == "turb3d":
-        return problems.turbulence3d(n=512), "3d_mhd_turbulence_512x512x512_hlld_wcns4_ct_rk3"
+    if name.startswith("turb3d"):
+        n = int(name[6:] or 512)
+        return problems.turbulence3d(n=n, device="cuda:0"), "3d_mhd_turbulence_%dx%dx%d_hlld_wcns4_ct_rk3" % (n, n, n)
     if name == "brio_wu":
         return problems.brio_wu(1024), "1d_brio_wu_1024_hlld_wcns4_rk3"
     raise SystemExit("unknown workload %s" % name)
@@ -244,6 +253,172 @@ def adjoint_line(n, steps):
                     "8 x nd gradient in and out included"}
 
 
+def kernel_roofline(s, lib, sweeps, wname):
+    """Roofline object of the dominant sweep kernel: its algorithmic flops and
+    compulsory bytes per launch (qas_kernel_model) over its average launch
+    time from the HIP events recorded on the library's stream."""
+    import ctypes as C
+    best = None
+    for k in sweeps:
+        tot, n = C.c_double(), C.c_longlong()
+        s._check(lib.qas_prof_query(s.ctx, k, C.byref(tot), C.byref(n)), "qas_prof_query")
+        fl, by = C.c_double(), C.c_double()
+        s._check(lib.qas_kernel_model(s.ctx, k, C.byref(fl), C.byref(by)), "qas_kernel_model")
+        if n.value == 0:
+            continue
+        avg_s = tot.value / n.value * 1e-3
+        rec = dict(kid=k, total_ms=tot.value, launches=n.value, avg_s=avg_s, flops=fl.value, bytes=by.value)
+        if best is None or tot.value > best["total_ms"]:
+            best = rec
+    names = {1: "sweep_x", 2: "sweep_y", 3: "sweep_z"}
+    if best is None:
+        return None
+    t_flop = best["flops"] / (FP64_PEAK_TFLOPS * 1e12)
+    t_hbm = best["bytes"] / (HBM_PEAK_GBS * 1e9)
+    tflops = best["flops"] / best["avg_s"] / 1e12
+    gbs = best["bytes"] / best["avg_s"] / 1e9
+    if t_hbm > t_flop:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    else:
+        roof = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tflops / FP64_PEAK_TFLOPS}
+    roof.update({"traffic": recorded_traffic(wname, names[best["kid"]]), "kernel": names[best["kid"]],
+                 "avg_launch_us": best["avg_s"] * 1e6, "launches": best["launches"],
+                 "alg_flops_per_launch": best["flops"], "alg_bytes_per_launch": best["bytes"],
+                 "fp64_tflops": tflops, "hbm_gbs": gbs, "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+                 "hbm_frac": gbs / HBM_PEAK_GBS,
+                 "compute": "FP64 VALU (no MFMA on this stencil path); peak = MI355X FP64 78.6 TFLOP/s"})
+    return roof
+
+
+def upload_problem(s, p):
+    """Copy a problem's state in: a numpy array from the host, a torch tensor
+    (device-built initial condition) device to device."""
+    if isinstance(p.state, np.ndarray):
+        s.upload(p.state)
+        return
+    import torch
+    torch.cuda.synchronize()
+    s.upload_device(p.state.data_ptr())
+    s.sync()
+
+
+def state_finite(s, p):
+    """All state values finite (checked on the device for a device problem)."""
+    if isinstance(p.state, np.ndarray):
+        return bool(np.isfinite(s.download()).all())
+    import torch
+    out = torch.empty(tuple(p.state.shape), dtype=torch.float64, device="cuda:0")
+    s.download_device(out.data_ptr())
+    s.sync()
+    return bool(torch.isfinite(out).all())
+
+
+def enable_sweep_prof(s, lib, ndim, nstages):
+    sweeps = [1, 2, 3][:ndim]
+    mask = 0
+    for k in sweeps:
+        mask |= 1 << k
+    s._check(lib.qas_prof_enable(s.ctx, mask, nstages * len(sweeps) + 16), "qas_prof_enable")
+    return sweeps
+
+
+def inverse_line(ninst, steps, seed=5, nfd=2):
+    """Velocity-mode inversion on `ninst` batched 2D instances (sizes drawn
+    from {128, 192, 256, 384, 512}, M = 0.3, 8 vy-mode amplitudes each,
+    qasmhd_amd/inverse.py): the batched forward that fixes the dt schedule,
+    the loss gradient of all instances by the discrete adjoint, and a
+    central-difference check (h = 1e-6) of `nfd` amplitudes of the first
+    instance."""
+    from qasmhd_amd import inverse
+    sizes = inverse.config_sizes(ninst, seed=seed)
+    inv = inverse.VyModeInversion(sizes, steps=steps, seed=seed)
+    zones = int(sum(n * n for n in sizes))
+    inv.schedule(warm=True)  # fixes the schedule
+    t_fwd = inv.batch_seconds
+    inv.targets()
+    t0 = time.perf_counter()
+    L, G = inv.loss_and_grad(inv.a0)
+    t_grad = time.perf_counter() - t0
+    errs = []
+    scale = float(np.abs(G[0]).max())
+    for m in range(nfd):
+        fd = inv.fd_grad(0, inv.a0[0], m, h=1e-6)
+        errs.append(abs(fd - G[0, m]) / scale)
+    return {"workload": "%d_instances_2d_vy_mode_inversion_sizes_128-512_mach0.3_hlld_wcns4_ct_rk3" % ninst,
+            "steps": steps, "zones": zones, "params": int(G.size), "loss": L,
+            "batched_forward_s": t_fwd, "batched_forward_value": zones * steps / t_fwd,
+            "gradient_s": t_grad, "ratio": t_grad / t_fwd, "fd_max_rel_err": max(errs) if errs else None,
+            "fd_checked": nfd, "finite": bool(np.isfinite(G).all()),
+            "note": "gradient = per instance: forward with checkpoints + backward (3 recomputed stages and 3 stage "
+                    "adjoints per step) + host copies; forward = one batched run of all instances"}
+
+
+def mixed3d_line(p, steps):
+    """FP64 vs mixed precision on a device-built 3D problem: both rates over
+    `steps` steps at the same fixed dt, and the L1 difference of density and
+    pressure of the final states (computed on the device)."""
+    import torch
+    from qasmhd_amd import capi
+    zones = int(np.prod(p.interior))
+    shape = tuple(p.state.shape)
+    rate, fin, dt = {}, {}, None
+    for prec in (0, 1):
+        with capi.solver_for(p, precision=prec) as s:
+            upload_problem(s, p)
+            if dt is None:
+                dt = s.cfl_dt()
+            s.run(2, flg=0, dt=dt, history=False)  # builds the step graph
+            upload_problem(s, p)
+            s.reset_counter()
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(steps, flg=0, dt=dt, history=False)
+            s.sync()
+            rate[prec] = zones * steps / (time.perf_counter() - t0)
+            f = torch.empty(shape, dtype=torch.float64, device="cuda:0")
+            s.download_device(f.data_ptr())
+            s.sync()
+            fin[prec] = f
+    gam = p.gam
+    I = (slice(4, -4),) * 3
+
+    def pres(st):
+        ro = st[0]
+        v2 = (st[1] ** 2 + st[2] ** 2 + st[3] ** 2) / ro
+        bx = 0.5 * (st[4] + torch.roll(st[4], -1, dims=2))
+        by = 0.5 * (st[5] + torch.roll(st[5], -1, dims=1))
+        bz = 0.5 * (st[6] + torch.roll(st[6], -1, dims=0))
+        return (gam - 1) * (st[7] - 0.5 * (v2 + bx ** 2 + by ** 2 + bz ** 2))
+    a, b = fin[0], fin[1]
+    l1r = float(torch.mean(torch.abs(b[0][I] - a[0][I])))
+    l1p = float(torch.mean(torch.abs(pres(b)[I] - pres(a)[I])))
+    finite = bool(torch.isfinite(b).all())
+    del fin, a, b
+    torch.cuda.empty_cache()
+    return {"steps": steps, "dt": dt, "fp64_value": rate[0], "mixed_value": rate[1], "speedup": rate[1] / rate[0],
+            "l1_rho": l1r, "l1_p": l1p, "finite": finite}
+
+
+def turb3d_child(n, steps, warmup=2):
+    """3D periodic MHD turbulence at n^3 (HLLD, WCNS-4, 3D CT, RK3, CFL 0.4;
+    initial state built on the device with torch): this script's own main
+    line for that workload, run as a child process so that this process
+    keeps its HIP runtime free of torch's."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "turb3d%d" % n, "--steps", str(steps),
+           "--warmup", str(warmup), "--no-cpu", "--no-extra", "--mixed3d-steps", str(steps)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("3D child exited with %d" % r.returncode)
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "roofline", "mixed")
+    out = {"workload": line["config"]["workload"]}
+    out.update({k: line[k] for k in keep if k in line})
+    out.update({k: line["config"][k] for k in ("dt_final", "finite")})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +439,12 @@ def main():
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adj-steps", type=int, default=20)
+    ap.add_argument("--inverse-n", type=int, default=64, help="instances of the inversion line (0: skip it)")
+    ap.add_argument("--inverse-steps", type=int, default=200)
+    ap.add_argument("--turb3d-n", type=int, default=512, help="3D turbulence line size (0: skip it)")
+    ap.add_argument("--turb3d-steps", type=int, default=20)
+    ap.add_argument("--mixed3d-steps", type=int, default=0,
+                    help="with a 3D workload: also FP64 vs mixed over this many fixed-dt steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the domain is one device-resident block); "
@@ -271,7 +452,9 @@ def main():
         sys.exit(2)
 
     import ctypes as C
-    import numpy as np
+    if args.workload.startswith("turb3d"):
+        import torch  # before the library: both then bind torch's HIP runtime
+        torch.cuda.init()
     from qasmhd_amd import capi
 
     p, wname = workload(args.workload)
@@ -284,24 +467,19 @@ def main():
     s = capi.solver_for(p, precision=args.precision)
     if args.no_graphs:
         s.lib.qas_use_graphs(s.ctx, 0)
-    s.upload(p.state)
+    upload_problem(s, p)
     s.run(args.warmup, flg=1, history=False)
     s.sync()
     # timing hooks on the sweep kernels (the dominant ones)
-    sweeps = [1, 2, 3][:p.ndim]
-    mask = 0
-    for k in sweeps:
-        mask |= 1 << k
-    stages = p.r_k * args.steps
-    s._check(lib.qas_prof_enable(s.ctx, mask, stages * len(sweeps) + 16), "qas_prof_enable")
+    sweeps = enable_sweep_prof(s, lib, p.ndim, p.r_k * args.steps)
     s.sync()
     t0 = time.perf_counter()
     hist, dt_now = s.run(args.steps, flg=1, history=False)
     s.sync()
     el = time.perf_counter() - t0
-    out = s.download()
-    finite = bool(np.isfinite(out).all())
+    finite = state_finite(s, p)
     if args.calibrate_copy:
+        out = s.download()
         hip = C.CDLL("libamdhip64.so.7")
         buf = C.c_void_p()
         if hip.hipMalloc(C.byref(buf), C.c_size_t(out.nbytes)) != 0:
@@ -313,36 +491,7 @@ def main():
     value = zones * args.steps / el
     _log("bench: %.3f s for %d steps -> %.4g zone-cycles/s (finite=%s)" % (el, args.steps, value, finite))
 
-    # roofline of the dominant kernel
-    best = None
-    for k in sweeps:
-        tot, n = C.c_double(), C.c_longlong()
-        s._check(lib.qas_prof_query(s.ctx, k, C.byref(tot), C.byref(n)), "qas_prof_query")
-        fl, by = C.c_double(), C.c_double()
-        s._check(lib.qas_kernel_model(s.ctx, k, C.byref(fl), C.byref(by)), "qas_kernel_model")
-        if n.value == 0:
-            continue
-        avg_s = tot.value / n.value * 1e-3
-        rec = dict(kid=k, total_ms=tot.value, launches=n.value, avg_s=avg_s, flops=fl.value, bytes=by.value)
-        if best is None or tot.value > best["total_ms"]:
-            best = rec
-    names = {1: "sweep_x", 2: "sweep_y", 3: "sweep_z"}
-    roof = None
-    if best:
-        t_flop = best["flops"] / (FP64_PEAK_TFLOPS * 1e12)
-        t_hbm = best["bytes"] / (HBM_PEAK_GBS * 1e9)
-        tflops = best["flops"] / best["avg_s"] / 1e12
-        gbs = best["bytes"] / best["avg_s"] / 1e9
-        if t_hbm > t_flop:
-            roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
-        else:
-            roof = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": tflops / FP64_PEAK_TFLOPS}
-        roof.update({"traffic": recorded_traffic(wname, names[best["kid"]]), "kernel": names[best["kid"]], "avg_launch_us": best["avg_s"] * 1e6,
-                     "launches": best["launches"], "alg_flops_per_launch": best["flops"],
-                     "alg_bytes_per_launch": best["bytes"], "fp64_tflops": tflops, "hbm_gbs": gbs,
-                     "fp64_frac": tflops / FP64_PEAK_TFLOPS, "hbm_frac": gbs / HBM_PEAK_GBS,
-                     "compute": "FP64 VALU (no MFMA on this stencil path); peak = MI355X FP64 78.6 TFLOP/s"})
+    roof = kernel_roofline(s, lib, sweeps, wname)
     s.close()
 
     cpu = None
@@ -354,6 +503,18 @@ def main():
             _log("bench: cpu baseline failed: %s" % e)
 
     extra = {}
+    if p.ndim == 3 and args.mixed3d_steps > 0 and not isinstance(p.state, np.ndarray):
+        try:
+            _log("bench: 3D FP64 vs mixed, %d steps" % args.mixed3d_steps)
+            extra["mixed"] = mixed3d_line(p, args.mixed3d_steps)
+        except Exception as e:
+            _log("bench: 3D mixed line failed: %s" % e)
+    if not args.no_extra and p.ndim == 2 and args.workload.startswith("turb2d") and args.turb3d_n > 0:
+        try:
+            _log("bench: 3D turbulence, %d^3, %d steps" % (args.turb3d_n, args.turb3d_steps))
+            extra["turb3d"] = turb3d_child(args.turb3d_n, args.turb3d_steps)
+        except Exception as e:
+            _log("bench: 3D line failed: %s" % e)
     if not args.no_extra and p.ndim == 2 and args.workload.startswith("turb2d"):
         try:
             _log("bench: mixed precision, %d^2, %d steps" % (args.mixed_n, args.mixed_steps))
@@ -365,6 +526,12 @@ def main():
             extra["batch"] = batch_line(args.batch_n, args.batch_steps)
         except Exception as e:
             _log("bench: batch line failed: %s" % e)
+        if args.inverse_n > 0:
+            try:
+                _log("bench: velocity-mode inversion, %d instances x %d steps" % (args.inverse_n, args.inverse_steps))
+                extra["inverse"] = inverse_line(args.inverse_n, args.inverse_steps)
+            except Exception as e:
+                _log("bench: inverse line failed: %s" % e)
         try:
             _log("bench: adjoint, %d^2, %d steps" % (args.adj_n, args.adj_steps))
             extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps)

@@ -1,0 +1,189 @@
+"""Gradient-based inversion of initial velocity modes on many 2D domains.
+
+Each instance is a periodic 2D decaying-MHD domain on [0, 2pi]^2 (the
+turbulence2d set-up, rho = gam^2, p = gam so c_s = 1, rms|v| = rms|B| = M)
+whose initial y velocity carries `nmodes` extra Fourier modes
+
+    vy(a) = vy_base + sum_m a_m cos(kx_m x + ky_m y + phi_m),
+
+0 < |k| <= kmax, wavevectors and phases drawn per instance from numpy
+PCG64(seed * 1000 + i).  The parameters are the amplitudes a (nmodes per
+instance).  With the dt schedule frozen (the schedule of the forward run at
+the starting amplitudes a0, dt from setdt every second step as exec_ does,
+2D/SERIAL/OTvortex/mhd2d_class.cpp:35-61), the loss is
+
+    L(a) = sum_i sum_interior (rho_i(T; a_i) - rho_target_i)^2,
+
+with rho_target_i the final density of the same run at amplitudes a_target
+= a0 + a seeded perturbation.  dL/da comes from the hand-written discrete
+adjoint of the step (qas_adj_forward / qas_adj_backward) chained through
+the initial-condition map:
+
+    my(a) = my_base + rho * D,   en(a) = en_base + D * (my_base + rho * D / 2),
+    D = sum_m a_m psi_m,
+    dL/da_m = sum_cells psi_m * (g_my * rho + g_en * my(a)).
+
+The starting-amplitude forward that fixes the schedule runs all instances in
+one batch (qas_batch_run: one launch per stage kernel for all domains, two
+steps per graph replay).  Losses at other amplitudes replay the frozen
+schedule step by step (qas_step), so a central finite difference compares
+two runs with identical step sizes.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import capi, problems
+
+NG = problems.NG
+
+
+class VyModeInversion:
+    def __init__(self, sizes, steps=200, seed=5, nmodes=8, kmax=4, mach=0.3, amp=0.05, perturb=0.02):
+        self.sizes = [int(n) for n in sizes]
+        self.steps = int(steps)
+        self.nmodes = int(nmodes)
+        self.base = []
+        self.psi = []
+        a0 = []
+        at = []
+        for i, n in enumerate(self.sizes):
+            p = problems.turbulence2d(n=n, seed=seed * 1000 + i, vrms=mach, brms=mach)
+            rng = np.random.Generator(np.random.PCG64(seed * 1000 + i))
+            ks, _, ph = problems._modes2d(rng, nmodes, kmax)
+            ny, nx = p.state.shape[1:]
+            x = (np.arange(nx) - NG + 0.5) * p.dx
+            y = (np.arange(ny) - NG + 0.5) * p.dy
+            X, Y = np.meshgrid(x, y)
+            self.psi.append(np.stack([np.cos(kx * X + ky * Y + f) for (kx, ky), f in zip(ks, ph)]))
+            self.base.append(p)
+            a = amp * rng.uniform(-1.0, 1.0, size=nmodes)
+            a0.append(a)
+            at.append(a + perturb * rng.standard_normal(nmodes))
+        self.a0 = np.array(a0)
+        self.a_target = np.array(at)
+        self.dts = None
+        self.rho_target = None
+
+    @property
+    def ninst(self):
+        return len(self.sizes)
+
+    def state(self, i, a):
+        """Initial state (8, ny, nx) of instance i at amplitudes a."""
+        p = self.base[i]
+        st = p.state.copy()
+        ro = st[0]
+        D = np.tensordot(np.asarray(a, dtype=np.float64), self.psi[i], axes=1)
+        my0 = st[2].copy()
+        st[2] = my0 + ro * D
+        st[7] = st[7] + D * (my0 + ro * D * 0.5)
+        return st
+
+    def _solver(self, i):
+        return capi.solver_for(self.base[i])
+
+    # ------------------------------------------------------------ forward runs
+    def schedule(self, warm=False):
+        """Batched forward at a0 (dt from setdt every 2nd step): fixes each
+        instance's dt schedule; returns the final states at a0.  With `warm`,
+        two steps run first (graph capture) and the states are re-uploaded;
+        `self.batch_seconds` is the wall time of the batched run."""
+        solvers = [self._solver(i) for i in range(self.ninst)]
+        try:
+            for i, s in enumerate(solvers):
+                s.upload(self.state(i, self.a0[i]))
+            with capi.Batch(solvers) as b:
+                if warm:
+                    b.run(2, flg=1, history=True)
+                    for i, s in enumerate(solvers):
+                        s.upload(self.state(i, self.a0[i]))
+                        s.reset_counter()
+                    for s in solvers:
+                        s.sync()
+                t0 = time.perf_counter()
+                hist, _ = b.run(self.steps, flg=1, history=True)
+                self.batch_seconds = time.perf_counter() - t0
+            finals = [s.download() for s in solvers]
+        finally:
+            for s in solvers:
+                s.close()
+        self.dts = [np.array(h) for h in hist]
+        return finals
+
+    def final_state(self, i, a, solver=None):
+        """Final state of instance i at amplitudes a over the frozen schedule."""
+        if self.dts is None:
+            self.schedule()
+        s = solver or self._solver(i)
+        try:
+            s.upload(self.state(i, a))
+            s.bound()
+            for dt in self.dts[i]:
+                s.step(dt)
+            return s.download()
+        finally:
+            if solver is None:
+                s.close()
+
+    def targets(self):
+        if self.rho_target is None:
+            self.rho_target = [self.final_state(i, self.a_target[i])[0] for i in range(self.ninst)]
+        return self.rho_target
+
+    def instance_loss(self, i, a, solver=None):
+        r = self.final_state(i, a, solver)[0] - self.targets()[i]
+        return float(np.sum(r[NG:-NG, NG:-NG] ** 2))
+
+    def loss(self, a):
+        return sum(self.instance_loss(i, a[i]) for i in range(self.ninst))
+
+    # ------------------------------------------------------------ gradient
+    def instance_grad(self, i, a):
+        """(loss_i, dloss_i/da_i) by the discrete adjoint."""
+        tgt = self.targets()[i]
+        u0 = self.state(i, a)
+        with self._solver(i) as s:
+            s.upload(u0)
+            s.adj_forward(self.dts[i])
+            uf = s.download()
+            r = uf[0] - tgt
+            g = np.zeros_like(uf)
+            g[0, NG:-NG, NG:-NG] = 2.0 * r[NG:-NG, NG:-NG]
+            gi = s.adj_backward(g)
+        w = gi[2] * u0[0] + gi[7] * u0[2]
+        grad = np.tensordot(self.psi[i], w, axes=([1, 2], [0, 1]))
+        return float(np.sum(r[NG:-NG, NG:-NG] ** 2)), grad
+
+    def loss_and_grad(self, a):
+        if self.dts is None:
+            self.schedule()
+        L = 0.0
+        G = np.zeros_like(np.asarray(a, dtype=np.float64))
+        for i in range(self.ninst):
+            li, gi = self.instance_grad(i, a[i])
+            L += li
+            G[i] = gi
+        return L, G
+
+    def fd_grad(self, i, a, m, h=1e-6, richardson=True):
+        """Central difference of instance i's loss in a_m over the frozen
+        schedule (Richardson-extrapolated from steps h and h/2)."""
+        with self._solver(i) as s:
+            def D(hh):
+                ap = np.array(a, dtype=np.float64)
+                am = ap.copy()
+                ap[m] += hh
+                am[m] -= hh
+                return (self.instance_loss(i, ap, s) - self.instance_loss(i, am, s)) / (2.0 * hh)
+            if not richardson:
+                return D(h)
+            return (4.0 * D(h / 2) - D(h)) / 3.0
+
+
+def config_sizes(ninst=64, seed=5, choices=(128, 192, 256, 384, 512)):
+    """Square domain sizes drawn uniformly (numpy PCG64(seed)) from `choices`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return [int(v) for v in rng.choice(list(choices), size=ninst)]

@@ -243,54 +243,115 @@ def _modes3d(rng, nmodes, kmax):
     return [ks[i] for i in idx], amp, ph
 
 
-def _vec_modes(k, amp, ph, X, Y, Z):
+class _Numpy:
+    """Array operations of the 3D initial condition on the host."""
+    sin = staticmethod(np.sin)
+    cos = staticmethod(np.cos)
+
+    @staticmethod
+    def zeros(shape):
+        return np.zeros(shape)
+
+    @staticmethod
+    def empty(shape):
+        return np.empty(shape)
+
+    @staticmethod
+    def full(shape, v):
+        return np.full(shape, v)
+
+    @staticmethod
+    def arange(n):
+        return np.arange(n, dtype=np.float64)
+
+    @staticmethod
+    def mean(a):
+        return float(np.mean(a))
+
+
+class _Torch:
+    """The same operations on a torch device (float64), for sizes where the
+    host would take minutes (512^3: 8 arrays of 1.1 GB)."""
+
+    def __init__(self, device):
+        import torch
+        self.t = torch
+        self.dev = torch.device(device)
+        self.sin = torch.sin
+        self.cos = torch.cos
+
+    def zeros(self, shape):
+        return self.t.zeros(shape, dtype=self.t.float64, device=self.dev)
+
+    def empty(self, shape):
+        return self.t.empty(shape, dtype=self.t.float64, device=self.dev)
+
+    def full(self, shape, v):
+        return self.t.full(shape, v, dtype=self.t.float64, device=self.dev)
+
+    def arange(self, n):
+        return self.t.arange(n, dtype=self.t.float64, device=self.dev)
+
+    def mean(self, a):
+        return float(self.t.mean(a))
+
+
+def _vec_modes(xp, shape, k, amp, ph, X, Y, Z):
     """Sum of modes a_m cos(k.x + p_m) for each of 3 components."""
-    out = [np.zeros(X.shape) for _ in range(3)]
+    out = [xp.zeros(shape) for _ in range(3)]
     for (kx, ky, kz), a, p in zip(k, amp, ph):
         arg = kx * X + ky * Y + kz * Z
         kk = math.sqrt(kx * kx + ky * ky + kz * kz)
         for c in range(3):
-            out[c] += (a[c] / kk) * np.cos(arg + p[c])
+            out[c] += (a[c] / kk) * xp.cos(arg + p[c])
     return out
 
 
 def turbulence3d(n: int = 512, seed: int = 4, nmodes: int = 32, kmax: int = 4, gam: float = 5.0 / 3.0,
-                 vrms: float = 1.0, brms: float = 0.5, bz0: float = 0.5, dtype_chunks: int = 0) -> Problem:
+                 vrms: float = 1.0, brms: float = 0.5, bz0: float = 0.5, device: str | None = None) -> Problem:
     """3D MHD turbulence on [0, 2pi]^3, periodic, rho = 1, p = 1/gam.
 
     Velocity = curl(psi) and delta-B = curl(A) with psi, A sums of `nmodes`
     random modes 0 < |k| <= kmax (PCG64(seed)); A lives at cell edges
     (Ax at (i, j-1/2, k-1/2), Ay at (i-1/2, j, k-1/2), Az at (i-1/2, j-1/2, k))
     and face B is its 4th-order discrete curl, as in
-    3D/MPI/loop/mhd3d_init_.cpp:36-75; plus a mean Bz = bz0."""
+    3D/MPI/loop/mhd3d_init_.cpp:36-75; plus a mean Bz = bz0.
+
+    With `device` (a torch device string such as "cuda:0") the fields are
+    evaluated there and `state` is a float64 torch tensor on that device;
+    the mode draws are the same, the values agree with the host ones to
+    rounding of sin/cos."""
+    xp = _Numpy() if device is None else _Torch(device)
     rng = np.random.Generator(np.random.PCG64(seed))
     L = 2.0 * math.pi
     d = L / n
     N = n + 2 * NG
-    c = (np.arange(N) - NG + 0.5) * d
-    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    shape = (N, N, N)
+    c = (xp.arange(N) - NG + 0.5) * d
+    X = c[None, None, :]
+    Y = c[None, :, None]
+    Z = c[:, None, None]
     h = 0.5 * d
     # velocity: analytic curl of psi at cell centres
     kv, av, pv = _modes3d(rng, nmodes, kmax)
-    vx = np.zeros(X.shape)
-    vy = np.zeros(X.shape)
-    vz = np.zeros(X.shape)
+    vx = xp.zeros(shape)
+    vy = xp.zeros(shape)
+    vz = xp.zeros(shape)
     for (kx, ky, kz), a, p in zip(kv, av, pv):
         arg = kx * X + ky * Y + kz * Z
         kk = math.sqrt(kx * kx + ky * ky + kz * kz)
-        s = [-(a[q] / kk) * np.sin(arg + p[q]) for q in range(3)]  # d/dx of cos = -k sin
+        s = [-(a[q] / kk) * xp.sin(arg + p[q]) for q in range(3)]  # d/dx of cos = -k sin
         vx += ky * s[2] - kz * s[1]
         vy += kz * s[0] - kx * s[2]
         vz += kx * s[1] - ky * s[0]
-    del s
+        del s, arg
     ka, aa, pa = _modes3d(rng, nmodes, kmax)
-    Ax = _vec_modes(ka, aa, pa, X, Y - h, Z - h)[0]
-    Ay = _vec_modes(ka, aa, pa, X - h, Y, Z - h)[1]
-    Az = _vec_modes(ka, aa, pa, X - h, Y - h, Z)[2]
-    del X, Y, Z
-    bx = np.zeros((N, N, N))
-    by = np.zeros((N, N, N))
-    bz = np.zeros((N, N, N))
+    Ax = _vec_modes(xp, shape, ka, aa, pa, X, Y - h, Z - h)[0]
+    Ay = _vec_modes(xp, shape, ka, aa, pa, X - h, Y, Z - h)[1]
+    Az = _vec_modes(xp, shape, ka, aa, pa, X - h, Y - h, Z)[2]
+    bx = xp.zeros(shape)
+    by = xp.zeros(shape)
+    bz = xp.zeros(shape)
     # bx(i-1/2,j,k) = dAz/dy - dAy/dz
     bx[1:-2, 1:-2, :] = (_df4(Az[1:-2, 0:-3, :], Az[1:-2, 1:-2, :], Az[1:-2, 2:-1, :], Az[1:-2, 3:, :]) / d
                          - _df4(Ay[0:-3, 1:-2, :], Ay[1:-2, 1:-2, :], Ay[2:-1, 1:-2, :], Ay[3:, 1:-2, :]) / d)
@@ -304,37 +365,37 @@ def turbulence3d(n: int = 512, seed: int = 4, nmodes: int = 32, kmax: int = 4, g
     for b in (bx, by, bz):
         periodic_fill(b, (0, 1, 2))
     I = (slice(NG, N - NG),) * 3
-    cx = _fcen4(bx[..., 0:-3], bx[..., 1:-2], bx[..., 2:-1], bx[..., 3:])[I[0], I[1], NG - 1:N - NG - 1]
-    cy = _fcen4(by[:, 0:-3, :], by[:, 1:-2, :], by[:, 2:-1, :], by[:, 3:, :])[I[0], NG - 1:N - NG - 1, I[2]]
-    cz = _fcen4(bz[0:-3], bz[1:-2], bz[2:-1], bz[3:])[NG - 1:N - NG - 1, I[1], I[2]]
-    sv = vrms / math.sqrt(np.mean(vx[I] ** 2 + vy[I] ** 2 + vz[I] ** 2))
-    sb = brms / math.sqrt(np.mean(cx ** 2 + cy ** 2 + cz ** 2))
-    del cx, cy, cz
+    # energy and the rms of B use cell-centred B (fcen of the face values)
+    CX = xp.zeros(shape)
+    CY = xp.zeros(shape)
+    CZ = xp.zeros(shape)
+
+    def centre():
+        CX[..., 1:-2] = _fcen4(bx[..., 0:-3], bx[..., 1:-2], bx[..., 2:-1], bx[..., 3:])
+        CY[:, 1:-2, :] = _fcen4(by[:, 0:-3, :], by[:, 1:-2, :], by[:, 2:-1, :], by[:, 3:, :])
+        CZ[1:-2] = _fcen4(bz[0:-3], bz[1:-2], bz[2:-1], bz[3:])
+        for q in (CX, CY, CZ):
+            periodic_fill(q, (0, 1, 2))
+    centre()
+    sv = vrms / math.sqrt(xp.mean(vx[I] ** 2 + vy[I] ** 2 + vz[I] ** 2))
+    sb = brms / math.sqrt(xp.mean(CX[I] ** 2 + CY[I] ** 2 + CZ[I] ** 2))
     for v in (vx, vy, vz):
         v *= sv
     for b in (bx, by, bz):
         b *= sb
     bz += bz0
-    ro = np.ones((N, N, N))
-    pr = np.full((N, N, N), 1.0 / gam)
-    st = np.empty((8, N, N, N))
-    st[0] = ro
+    centre()  # of the scaled field, for the energy
+    st = xp.empty((8,) + shape)
+    st[0] = 1.0
     st[1] = vx
     st[2] = vy
     st[3] = vz
     st[4] = bx
     st[5] = by
     st[6] = bz
-    # energy with cell-centred B (fcen of the face values)
-    CX = np.zeros((N, N, N))
-    CY = np.zeros((N, N, N))
-    CZ = np.zeros((N, N, N))
-    CX[..., 1:-2] = _fcen4(bx[..., 0:-3], bx[..., 1:-2], bx[..., 2:-1], bx[..., 3:])
-    CY[:, 1:-2, :] = _fcen4(by[:, 0:-3, :], by[:, 1:-2, :], by[:, 2:-1, :], by[:, 3:, :])
-    CZ[1:-2] = _fcen4(bz[0:-3], bz[1:-2], bz[2:-1], bz[3:])
-    for q in (CX, CY, CZ):
-        periodic_fill(q, (0, 1, 2))
-    st[7] = energy(ro, vx, vy, vz, CX, CY, CZ, pr, gam)
-    del CX, CY, CZ
+    del bx, by, bz
+    ro = st[0]
+    st[7] = energy(ro, vx, vy, vz, CX, CY, CZ, 1.0 / gam, gam)
+    del CX, CY, CZ, vx, vy, vz
     return Problem(state=st, dx=d, dy=d, dz=d, gam=gam, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1,
                    name="turb3d_%d" % n)
