@@ -516,27 +516,30 @@ def main():
         except Exception as e:
             _log("bench: 3D line failed: %s" % e)
     if not args.no_extra and p.ndim == 2 and args.workload.startswith("turb2d"):
-        try:
-            _log("bench: mixed precision, %d^2, %d steps" % (args.mixed_n, args.mixed_steps))
-            extra["mixed"] = mixed_line(args.mixed_n, args.mixed_steps)
-        except Exception as e:
-            _log("bench: mixed line failed: %s" % e)
-        try:
-            _log("bench: batched instances, %d x %d steps" % (args.batch_n, args.batch_steps))
-            extra["batch"] = batch_line(args.batch_n, args.batch_steps)
-        except Exception as e:
-            _log("bench: batch line failed: %s" % e)
+        if args.mixed_n > 0:
+            try:
+                _log("bench: mixed precision, %d^2, %d steps" % (args.mixed_n, args.mixed_steps))
+                extra["mixed"] = mixed_line(args.mixed_n, args.mixed_steps)
+            except Exception as e:
+                _log("bench: mixed line failed: %s" % e)
+        if args.batch_n > 0:
+            try:
+                _log("bench: batched instances, %d x %d steps" % (args.batch_n, args.batch_steps))
+                extra["batch"] = batch_line(args.batch_n, args.batch_steps)
+            except Exception as e:
+                _log("bench: batch line failed: %s" % e)
         if args.inverse_n > 0:
             try:
                 _log("bench: velocity-mode inversion, %d instances x %d steps" % (args.inverse_n, args.inverse_steps))
                 extra["inverse"] = inverse_line(args.inverse_n, args.inverse_steps)
             except Exception as e:
                 _log("bench: inverse line failed: %s" % e)
-        try:
-            _log("bench: adjoint, %d^2, %d steps" % (args.adj_n, args.adj_steps))
-            extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps)
-        except Exception as e:
-            _log("bench: adjoint line failed: %s" % e)
+        if args.adj_n > 0:
+            try:
+                _log("bench: adjoint, %d^2, %d steps" % (args.adj_n, args.adj_steps))
+                extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps)
+            except Exception as e:
+                _log("bench: adjoint line failed: %s" % e)
 
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,

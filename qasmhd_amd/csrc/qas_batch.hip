@@ -33,11 +33,11 @@ __global__ __launch_bounds__(256) void kb_prim2d(const BatchInst* I, const int* 
   prim2d_at<ODR, PT>(I[i].st[stage], I[i].g, I[i].bc, gam, batch_thread(off, i, b));
 }
 
-template <int ODR>
+template <int ODR, typename FT>
 __global__ __launch_bounds__(256) void kb_emf2d(const BatchInst* I, const int* off, int n, int stage, int ctw)
 {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
-  emf2d_at<ODR>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
+  emf2d_at<ODR, FT>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
 }
 
 template <int ODR>
@@ -255,7 +255,7 @@ template <int ODR>
 static void pick_stage(qas_batch* b, bool f32)
 {
   b->prim = f32 ? (prim_fn)kb_prim2d<ODR, float> : (prim_fn)kb_prim2d<ODR, double>;
-  b->emf = kb_emf2d<ODR>;
+  b->emf = f32 ? (emf_fn)kb_emf2d<ODR, float> : (emf_fn)kb_emf2d<ODR, double>;
   b->ct = kb_ct2d<ODR>;
 }
 

@@ -58,6 +58,21 @@ __host__ __device__ inline PrimView<PT> prim_view(const StagePtrs& p)
                       reinterpret_cast<const PT*>(p.cx), reinterpret_cast<const PT*>(p.cy),
                       reinterpret_cast<const PT*>(p.cz)};
 }
+// The face parts of the B fluxes (emf) and the stored transverse
+// divergences (acc) in element type ST = the sweep precision: double on the
+// FP64 path; float on the mixed path, where the face parts are FP32 values
+// already and the divergences are rounded once on store.
+template <typename ST>
+__device__ __forceinline__ void store_as(double* a, long long i, double v)
+{
+  reinterpret_cast<ST*>(a)[i] = (ST)v;
+}
+template <typename ST>
+__device__ __forceinline__ double load_as(const double* a, long long i)
+{
+  return (double)reinterpret_cast<const ST*>(a)[i];
+}
+
 template <typename PT>
 __device__ __forceinline__ void store_prim(double* a, long long t, double v)
 {

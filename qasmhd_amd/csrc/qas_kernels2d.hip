@@ -36,10 +36,10 @@ __global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, 
   prim2d_at<ODR, PT>(p, g, bc, gam, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
-template <int ODR>
+template <int ODR, typename FT>
 __global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
 {
-  emf2d_at<ODR>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
+  emf2d_at<ODR, FT>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 template <int ODR>
@@ -81,7 +81,7 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
     if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4, float>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
     else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4, double>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
   }
-  // 2) x sweep, 3) y sweep
+  // 2) y sweep, 3) x sweep
   const int nsx = (g.nx - 5 + SEG - 1) / SEG;
   const int nsy = (g.ny - 5 + SEG - 1) / SEG;
   const bool mixed = c->s.precision == 1;
@@ -95,16 +95,18 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   // 4) corner EMF, 5) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
+  // (the face parts are stored in the sweep precision)
+#define QAS_EMF_CT(O)                                                                                      \
+  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);      \
+  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);          \
+  QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<O>), nblk(nc, 256), 256, 0, st, p, g);
   switch (odr) {
-    case 1: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<1>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<1>), nblk(nc, 256), 256, 0, st, p, g); break;
-    case 2: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<2>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<2>), nblk(nc, 256), 256, 0, st, p, g); break;
-    case 3: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<3>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<3>), nblk(nc, 256), 256, 0, st, p, g); break;
-    default: QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<4>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);
-             QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<4>), nblk(nc, 256), 256, 0, st, p, g); break;
+    case 1: QAS_EMF_CT(1) break;
+    case 2: QAS_EMF_CT(2) break;
+    case 3: QAS_EMF_CT(3) break;
+    default: QAS_EMF_CT(4) break;
   }
+#undef QAS_EMF_CT
   return hipGetLastError();
 }
 

@@ -70,23 +70,24 @@ struct Vel3 {
 // One contribution: interpolate the face parts (upwind fu, central fc) of a
 // sweep along axis o to the edge and combine with weight w:
 //   sgn * ((uL + uR) + w (cL + cR))
-template <int ODR>
+// (FT: element type of the face parts, the sweep precision)
+template <int ODR, typename FT>
 __device__ __forceinline__ double emf_part(const double* fu, const double* fc, long long ss, long long o, double w,
                                            double sgn)
 {
   double a[5], b[5], ul0, ur0, ul1, ur1, dum;
 #pragma unroll
-  for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 3) * o]; b[m] = fc[ss + (m - 3) * o]; }
+  for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(fu, ss + (m - 3) * o); b[m] = load_as<FT>(fc, ss + (m - 3) * o); }
   lin5<ODR>(a, ul0, dum);
   lin5<ODR>(b, ul1, dum);
 #pragma unroll
-  for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 2) * o]; b[m] = fc[ss + (m - 2) * o]; }
+  for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(fu, ss + (m - 2) * o); b[m] = load_as<FT>(fc, ss + (m - 2) * o); }
   lin5<ODR>(a, dum, ur0);
   lin5<ODR>(b, dum, ur1);
   return sgn * ((ul0 + ur0) + w * (ul1 + ur1));
 }
 
-template <int ODR>
+template <int ODR, typename FT>
 __global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
 {
   // edges on [3, n-3]^3
@@ -118,16 +119,16 @@ __global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
   }
   // Ez: x sweep B_y (t) along y, then y sweep B_x (u) along x
   double ez = 0.0;
-  ez += emf_part<ODR>(p.emf[0], p.emf[1], ss, sy, 1.0 - ctz, -0.5);
-  ez += emf_part<ODR>(p.emf[6], p.emf[7], ss, 1, ctz, +0.5);
+  ez += emf_part<ODR, FT>(p.emf[0], p.emf[1], ss, sy, 1.0 - ctz, -0.5);
+  ez += emf_part<ODR, FT>(p.emf[6], p.emf[7], ss, 1, ctz, +0.5);
   // Ey: x sweep B_z (u) along z, then z sweep B_x (t) along x
   double ey = 0.0;
-  ey += emf_part<ODR>(p.emf[2], p.emf[3], ss, sz, cty, +0.5);
-  ey += emf_part<ODR>(p.emf[8], p.emf[9], ss, 1, 1.0 - cty, -0.5);
+  ey += emf_part<ODR, FT>(p.emf[2], p.emf[3], ss, sz, cty, +0.5);
+  ey += emf_part<ODR, FT>(p.emf[8], p.emf[9], ss, 1, 1.0 - cty, -0.5);
   // Ex: y sweep B_z (t) along z, then z sweep B_y (u) along y
   double ex = 0.0;
-  ex += emf_part<ODR>(p.emf[4], p.emf[5], ss, sz, 1.0 - ctx, -0.5);
-  ex += emf_part<ODR>(p.emf[10], p.emf[11], ss, sy, ctx, +0.5);
+  ex += emf_part<ODR, FT>(p.emf[4], p.emf[5], ss, sz, 1.0 - ctx, -0.5);
+  ex += emf_part<ODR, FT>(p.emf[10], p.emf[11], ss, sy, ctx, +0.5);
   p.ex[ss] = ex;
   p.ey[ss] = ey;
   p.ez[ss] = ez;
@@ -218,16 +219,18 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   }
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5) * (g.nz - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7) * (g.nz - 7);
+  // (the face parts are stored in the sweep precision)
+#define QAS_EMF_CT(O)                                                                                      \
+  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<O, float>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);     \
+  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<O, double>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);         \
+  QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O>), nblk3(nc, 256), 256, 0, st, p, g);
   switch (odr) {
-    case 1: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<1>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<1>), nblk3(nc, 256), 256, 0, st, p, g); break;
-    case 2: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<2>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<2>), nblk3(nc, 256), 256, 0, st, p, g); break;
-    case 3: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<3>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-            QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<3>), nblk3(nc, 256), 256, 0, st, p, g); break;
-    default: QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<4>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);
-             QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<4>), nblk3(nc, 256), 256, 0, st, p, g); break;
+    case 1: QAS_EMF_CT(1) break;
+    case 2: QAS_EMF_CT(2) break;
+    case 3: QAS_EMF_CT(3) break;
+    default: QAS_EMF_CT(4) break;
   }
+#undef QAS_EMF_CT
   return hipGetLastError();
 }
 

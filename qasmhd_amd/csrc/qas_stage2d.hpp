@@ -60,7 +60,8 @@ __device__ __forceinline__ double vel(const StagePtrs& p, int comp, long long c)
 }
 
 // ------------------------------------------------------------------ EMF
-template <int ODR>
+// FT: element type of the face parts (the sweep precision)
+template <int ODR, typename FT>
 __device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int ctw, long long t)
 {
   const int ni = g.nx - 5, nj = g.ny - 5;   // corners [3, n-3]
@@ -83,11 +84,11 @@ __device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int c
     const double* fcn = p.emf[1];
     double a[5], b[5], ul0, ur0, ul1, ur1, dum;
 #pragma unroll
-    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 3) * nx]; b[m] = fcn[ss + (m - 3) * nx]; }
+    for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(fu, ss + (m - 3) * nx); b[m] = load_as<FT>(fcn, ss + (m - 3) * nx); }
     lin5<ODR>(a, ul0, dum);
     lin5<ODR>(b, ul1, dum);
 #pragma unroll
-    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 2) * nx]; b[m] = fcn[ss + (m - 2) * nx]; }
+    for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(fu, ss + (m - 2) * nx); b[m] = load_as<FT>(fcn, ss + (m - 2) * nx); }
     lin5<ODR>(a, dum, ur0);
     lin5<ODR>(b, dum, ur1);
     ez += -0.5 * ((ul0 + ur0) + (1.0 - ct) * (ul1 + ur1));
@@ -97,11 +98,11 @@ __device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int c
     const double* fcn = p.emf[5];
     double a[5], b[5], ul0, ur0, ul1, ur1, dum;
 #pragma unroll
-    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 3)]; b[m] = fcn[ss + (m - 3)]; }
+    for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(fu, ss + (m - 3)); b[m] = load_as<FT>(fcn, ss + (m - 3)); }
     lin5<ODR>(a, ul0, dum);
     lin5<ODR>(b, ul1, dum);
 #pragma unroll
-    for (int m = 0; m < 5; m++) { a[m] = fu[ss + (m - 2)]; b[m] = fcn[ss + (m - 2)]; }
+    for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(fu, ss + (m - 2)); b[m] = load_as<FT>(fcn, ss + (m - 2)); }
     lin5<ODR>(a, dum, ur0);
     lin5<ODR>(b, dum, ur1);
     ez += +0.5 * ((ul0 + ur0) + ct * (ul1 + ur1));

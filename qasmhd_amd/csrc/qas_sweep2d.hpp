@@ -159,15 +159,15 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
     const bool ownr = (r >= 2) && (r < 2 + SEG) && (qq <= n - 3) && (i < g.nx);
     if (ownr) {
       const long long a = sy * qq + i;
-      p.emf[4][a] = so[6][c][r];
-      p.emf[5][a] = so[7][c][r];
+      store_as<T>(p.emf[4], a, so[6][c][r]);
+      store_as<T>(p.emf[5], a, so[7][c][r]);
       const bool upd = (qq >= NG) && (qq <= n - NG - 1) && (i >= NG) && (i <= g.nx - NG - 1);
       if (upd) {
         // dt/dy * Dy of (ro, mx, my, mz, bz, en); the x sweep, which runs
         // next, subtracts it in the stage sum
         const double dtdy = *p.dt / g.dy;
 #pragma unroll
-        for (int v = 0; v < 6; v++) p.acc[v][a] = dtdy * so[v][c][r];
+        for (int v = 0; v < 6; v++) store_as<T>(p.acc[v], a, dtdy * so[v][c][r]);
       }
     }
   }
@@ -268,8 +268,8 @@ __device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, c
   const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
   if (own) {
     const long long fi = base + o * q;
-    p.emf[DIR * 4 + 0][fi] = (double)fbu;
-    p.emf[DIR * 4 + 1][fi] = (double)fc;
+    store_as<T>(p.emf[DIR * 4 + 0], fi, (double)fbu);
+    store_as<T>(p.emf[DIR * 4 + 1], fi, (double)fc);
   }
 
   // flux divergence of ro, mx, my, mz, bz, en at cell q: faces q-1 .. q+2
@@ -303,7 +303,7 @@ __device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, c
 #pragma unroll
   for (int v = 0; v < 6; v++) {
     const int m = vid[v];
-    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (-dtdx * du[v]) - p.acc[v][c], p.a, p.b);
+    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (-dtdx * du[v]) - load_as<T>(p.acc[v], c), p.a, p.b);
   }
 }
 

@@ -169,7 +169,7 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
     if (ownr) {
       const long long a = blb + o * qq + i;
 #pragma unroll
-      for (int k = 0; k < 4; k++) p.emf[DIR * 4 + k][a] = so[5 + k][c][r];
+      for (int k = 0; k < 4; k++) store_as<T>(p.emf[DIR * 4 + k], a, so[5 + k][c][r]);
       const bool upd = (qq >= NG) && (qq <= n - NG - 1) && (i >= NG) && (i <= g.nx - NG - 1) && (lb >= NG) &&
                        (lb <= nlb - NG - 1);
       if (upd) {
@@ -178,7 +178,7 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
         const double dtd = *p.dt / (DIR == 1 ? g.dy : g.dz);
         double* const* acc = p.acc + (DIR == 1 ? 0 : 5);
 #pragma unroll
-        for (int v = 0; v < 5; v++) acc[v][a] = dtd * so[v][c][r];
+        for (int v = 0; v < 5; v++) store_as<T>(acc[v], a, dtd * so[v][c][r]);
       }
     }
   }
@@ -290,10 +290,10 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
   const bool own = (lane >= 2) && (lane < 2 + SEG) && (q <= n - 3);
   if (own) {
     const long long fi = base + o * q;
-    p.emf[DIR * 4 + 0][fi] = (double)(F[4] - fct);  // B_t flux, upwind part
-    p.emf[DIR * 4 + 1][fi] = (double)fct;           // B_t flux, central part
-    p.emf[DIR * 4 + 2][fi] = (double)(F[5] - fcu);  // B_u flux, upwind part
-    p.emf[DIR * 4 + 3][fi] = (double)fcu;           // B_u flux, central part
+    store_as<T>(p.emf[DIR * 4 + 0], fi, (double)(F[4] - fct));  // B_t flux, upwind part
+    store_as<T>(p.emf[DIR * 4 + 1], fi, (double)fct);           // B_t flux, central part
+    store_as<T>(p.emf[DIR * 4 + 2], fi, (double)(F[5] - fcu));  // B_u flux, upwind part
+    store_as<T>(p.emf[DIR * 4 + 3], fi, (double)fcu);           // B_u flux, central part
   }
   // divergence of ro, mx, my, mz, en: (ro, mn, mt, mu) permuted back
   double fv[5];
@@ -323,7 +323,8 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
 #pragma unroll
   for (int v = 0; v < 5; v++) {
     const int m = vid[v];
-    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], ((-dtdx * du[v]) - p.acc[v][c]) - p.acc[5 + v][c], p.a, p.b);
+    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], ((-dtdx * du[v]) - load_as<T>(p.acc[v], c)) - load_as<T>(p.acc[5 + v], c),
+                         p.a, p.b);
   }
 }
 
