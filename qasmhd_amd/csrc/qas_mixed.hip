@@ -19,7 +19,7 @@ template <int DIR>
 static sweep_fn_t pick2(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return DIR == 0 ? k_sweep2d<0, R, O, float> : k_sweep2d_y<R, O, float>;
+  if (rmn == R && odr == O) return DIR == 0 ? nullptr : k_sweep2d_y<R, O, float>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
@@ -48,6 +48,7 @@ sweep_fn_t pick_sweep_f32(int ndim, int dir, int rmn, int odr)
   return nullptr;
 }
 
-batch_fn pick_batch_sweep2d_f32(int dir, int rmn, int odr) { return pick_batch_sweep2d_t<float>(dir, rmn, odr); }
+batch_fn pick_batch_sweep2d_y_f32(int rmn, int odr) { return pick_batch_sweep2d_y_t<float>(rmn, odr); }
+xsweep2d_fn pick_xsweep2d_f32(int rmn, int odr, bool batch) { return pick_xsweep2d_t<float>(rmn, odr, batch); }
 
 }  // namespace qas

@@ -184,11 +184,37 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void k_sweep2d_y(Sta
 
 // ------------------------------------------------------------------ x sweep
 // x faces for every row j (one wavefront per 60-face segment of a row); it
-// runs after the y sweep and applies the RK stage sum.
-template <int DIR, int RMN, int ODR, typename T>
-__device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam,
-                                             int nseg, int wave)
+// runs after the y sweep and applies the RK stage sum.  One kernel serves a
+// single domain (BATCH false: the stage descriptor is the kernel argument)
+// and a batch of domains (BATCH true: the instance table, as kb_* kernels).
+// The body is the kernel itself: as a force-inlined device function called
+// from thin kernels it compiled to 246 VGPRs (2 waves per SIMD) instead of
+// 158 (3 waves) for FP64 HLLD / ODR 4.  The FP64 Roe and MLAU instances ask
+// for 2 waves per SIMD, which they otherwise miss by a few registers.
+template <int DIR, int RMN, int ODR, typename T, bool BATCH>
+__global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_sweep2d(StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam,
+                                                 int nseg_arg, const BatchInst* I, const int* off, int ninst,
+                                                 int stage)
 {
+  StagePtrs p;
+  Geo g;
+  BcFlags bc;
+  int nseg, wave;
+  if constexpr (BATCH) {
+    const int b = (int)blockIdx.x;
+    const int i = batch_find(off, ninst, b);
+    p = I[i].st[stage];
+    g = I[i].g;
+    bc = I[i].bc;
+    nseg = I[i].nsx;
+    wave = (b - off[i]) * (int)(blockDim.x / WAVE) + (int)(threadIdx.x / WAVE);
+  } else {
+    p = p_arg;
+    g = g_arg;
+    bc = bc_arg;
+    nseg = nseg_arg;
+    wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  }
   const int lane = threadIdx.x & (WAVE - 1);
   const int nline = DIR == 0 ? g.ny : g.nx;
   if (wave >= nseg * nline) return;  // whole wavefront exits together
@@ -307,23 +333,7 @@ __device__ __forceinline__ void sweep2d_lane(const StagePtrs& p, const Geo& g, c
   }
 }
 
-template <int DIR, int RMN, int ODR, typename T>
-__global__ __launch_bounds__(256) void k_sweep2d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
-{
-  sweep2d_lane<DIR, RMN, ODR, T>(p, g, bc, gam, nseg, (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE));
-}
-
-// batched versions (qas_batch.hpp): one launch over every instance
-template <int RMN, int ODR, typename T>
-__global__ __launch_bounds__(256) void kb_sweep2d_x(const BatchInst* I, const int* off, int n, int stage, double gam)
-{
-  const int b = (int)blockIdx.x;
-  const int i = batch_find(off, n, b);
-  const BatchInst& B = I[i];
-  sweep2d_lane<0, RMN, ODR, T>(B.st[stage], B.g, B.bc, gam, B.nsx,
-                               (b - off[i]) * (int)(blockDim.x / WAVE) + (int)(threadIdx.x / WAVE));
-}
-
+// batched y sweep (qas_batch.hpp): one launch over every instance
 template <int RMN, int ODR, typename T>
 __global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void kb_sweep2d_y(const BatchInst* I, const int* off,
                                                                               int n, int stage, double gam)
@@ -334,11 +344,25 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void kb_sweep2d_y(co
   sweep2d_y<RMN, ODR, T>(B.st[stage], B.g, B.bc, gam, B.nsy, b - off[i]);
 }
 
+// kernel tables: batched y sweep, x sweep (single domain or batch)
 template <typename T>
-batch_fn pick_batch_sweep2d_t(int dir, int rmn, int odr)
+batch_fn pick_batch_sweep2d_y_t(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return dir == 0 ? kb_sweep2d_x<R, O, T> : kb_sweep2d_y<R, O, T>;
+  if (rmn == R && odr == O) return kb_sweep2d_y<R, O, T>;
+  QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
+  QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
+  QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
+  QAS_S(3, 1) QAS_S(3, 2) QAS_S(3, 3) QAS_S(3, 4)
+#undef QAS_S
+  return nullptr;
+}
+
+template <typename T>
+xsweep2d_fn pick_xsweep2d_t(int rmn, int odr, bool batch)
+{
+#define QAS_S(R, O) \
+  if (rmn == R && odr == O) return batch ? k_sweep2d<0, R, O, T, true> : k_sweep2d<0, R, O, T, false>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
