@@ -203,6 +203,21 @@ __global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
 }
 
 // ---------------------------------------------------------------- sweeps
+// Addressing of the line passes.  x lines are rows (stride 1); y lines are
+// columns, read either in place (stride nx, 64 cache lines per wave-wide
+// load) or from transposed copies (TR: element (i, j) at i * ny + j, so a
+// column is contiguous and the y pass loads like the x pass).
+template <int DIR, bool TR>
+__device__ __forceinline__ long long line_stride(const Geo& g)
+{
+  return (DIR == 0 || TR) ? 1 : g.sy;
+}
+template <int DIR, bool TR>
+__device__ __forceinline__ long long line_base(const Geo& g, int line)
+{
+  return DIR == 0 ? g.sy * line : (TR ? (long long)g.ny * line : (long long)line);
+}
+
 // Forward of k_sweep2d (RMN 1, ODR 4) up to the face states, for the cell
 // of this lane; shared by the two adjoint sweep kernels.
 template <int DIR>
@@ -250,9 +265,10 @@ constexpr int NGF = 17;
 // Face pass: lane l reconstructs cell q = seg*63 - 1 + l and, for l >= 1,
 // owns face q (between cells q-1 and q): flux adjoint (HLLD) from the
 // divergence and EMF-part adjoints, written to p.gfc[.][face].
-template <int DIR>
+template <int DIR, bool TR>
 __global__ __launch_bounds__(256) void k_adj_sweep_face(AdjPtrs p, Geo g, double gam, int nseg)
 {
+  static_assert(!TR || DIR == 1, "only the y pass runs on transposed arrays");
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   const int nline = DIR == 0 ? g.ny : g.nx;
@@ -260,8 +276,8 @@ __global__ __launch_bounds__(256) void k_adj_sweep_face(AdjPtrs p, Geo g, double
   const int line = wave / nseg;
   const int seg = wave - line * nseg;
   const int n = DIR == 0 ? g.nx : g.ny;
-  const long long o = DIR == 0 ? 1 : g.sy;
-  const long long base = DIR == 0 ? g.sy * line : (long long)line;
+  const long long o = line_stride<DIR, TR>(g);
+  const long long base = line_base<DIR, TR>(g, line);
   const int q = seg * (WAVE - 1) - 1 + lane;
   const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
   SweepFwd<DIR> f;
@@ -319,9 +335,10 @@ __global__ __launch_bounds__(256) void k_adj_sweep_face(AdjPtrs p, Geo g, double
 // Cell pass: lane l handles cell q = seg*SEGA - 3 + l; lanes 3..60 own
 // their cell and face.  Reconstruction adjoint from the face adjoints of
 // faces q (right state) and q+1 (left state), gathered onto cells.
-template <int DIR>
+template <int DIR, bool TR>
 __global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
 {
+  static_assert(!TR || DIR == 1, "only the y pass runs on transposed arrays");
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   const int nline = DIR == 0 ? g.ny : g.nx;
@@ -329,8 +346,8 @@ __global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double
   const int line = wave / nseg;
   const int seg = wave - line * nseg;
   const int n = DIR == 0 ? g.nx : g.ny;
-  const long long o = DIR == 0 ? 1 : g.sy;
-  const long long base = DIR == 0 ? g.sy * line : (long long)line;
+  const long long o = line_stride<DIR, TR>(g);
+  const long long base = line_base<DIR, TR>(g, line);
   const int q = seg * SEGA - 3 + lane;
   double gs[7][5] = {};
   double gcn = 0.0, gbm = 0.0, gbp = 0.0;
@@ -379,10 +396,17 @@ __global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double
   const double gface = ((q >= 3 && q <= n - 3) ? p.gfc[16][c] : 0.0) + gbm + gbpl;
   // stencil rows -> adjoint arrays (ro vx vy vz cx cy bz pr)
   const int gi[7] = {0, DIR == 0 ? 1 : 2, DIR == 0 ? 2 : 3, DIR == 0 ? 3 : 1, DIR == 0 ? 5 : 6, DIR == 0 ? 6 : 4, 7};
+  if (TR) {  // transposed scratch, written once per cell, added back by k_adj_untranspose_add
 #pragma unroll
-  for (int k = 0; k < 7; k++) p.gp[gi[k]][c] += gc7[k];
-  p.gp[DIR == 0 ? 4 : 5][c] += gcn;
-  p.gbf[DIR][c] += gface;
+    for (int k = 0; k < 7; k++) p.gp[gi[k]][c] = gc7[k];
+    p.gp[5][c] = gcn;
+    p.gbf[1][c] = gface;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; k++) p.gp[gi[k]][c] += gc7[k];
+    p.gp[DIR == 0 ? 4 : 5][c] += gcn;
+    p.gbf[DIR][c] += gface;
+  }
 }
 
 // ---------------------------------------------------------------- prim
@@ -532,6 +556,52 @@ __global__ __launch_bounds__(256) void k_adj_clear8(FoldArgs f, long long nd)
   for (int m = 0; m < 8; m++) f.dst[m][t] = 0.0;
 }
 
+// ---------------------------------------------------------------- transposes
+// dst[a][i * ny + j] = src[a][j * nx + i] for array a = blockIdx.z, through
+// a 32 x 32 LDS tile (both global accesses run along a contiguous axis);
+// the add-back form accumulates dst[a][j * nx + i] += src[a][i * ny + j].
+constexpr int NTR = 17;
+struct TrArgs {
+  const double* src[NTR];
+  double* dst[NTR];
+};
+
+__global__ __launch_bounds__(256) void k_adj_transpose(TrArgs a, int nx, int ny)
+{
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const double* s = a.src[blockIdx.z];
+  double* d = a.dst[blockIdx.z];
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    if (i < nx && j < ny) tile[r][tx] = s[(long long)j * nx + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + tx, i = i0 + r;
+    if (i < nx && j < ny) d[(long long)i * ny + j] = tile[tx][r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_untranspose_add(TrArgs a, int nx, int ny)
+{
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const double* s = a.src[blockIdx.z];
+  double* d = a.dst[blockIdx.z];
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + tx, i = i0 + r;
+    if (i < nx && j < ny) tile[r][tx] = s[(long long)i * ny + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    if (i < nx && j < ny) d[(long long)j * nx + i] += tile[tx][r];
+  }
+}
+
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace qas
@@ -554,6 +624,8 @@ struct qas_adj_work {
   double* gemf[4] = {};
   double *gez = nullptr, *gct = nullptr, *ctv = nullptr;
   double* gfc[17] = {};
+  double* tin[NTR] = {};      // transposed inputs of the y pass
+  double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
   int ck_cap = 0;
   int nrec = 0;               // steps recorded
@@ -603,7 +675,7 @@ static int adj_workspace(qas_ctx* c)
   c->adj = w;
   const size_t stride = (size_t)((c->geo.nd + 63) / 64 * 64);
   w->stride = stride;
-  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 3 + 17;
+  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 3 + 17 + NTR + 9;
   hipError_t e = hipMalloc(&w->slab, narr * stride * sizeof(double));
   if (e != hipSuccess) {
     adj_release(c);
@@ -628,6 +700,8 @@ static int adj_workspace(qas_ctx* c)
   w->gct = take();
   w->ctv = take();
   for (int k = 0; k < 17; k++) w->gfc[k] = take();
+  for (int k = 0; k < NTR; k++) w->tin[k] = take();
+  for (int k = 0; k < 9; k++) w->tout[k] = take();
   QAS_ACHECK(c, hipMemsetAsync(w->slab, 0, narr * stride * sizeof(double), c->stream));
   return QAS_OK;
 }
@@ -696,10 +770,56 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
   const int fsx = (g.nx + WAVE - 2) / (WAVE - 1), fsy = (g.ny + WAVE - 2) / (WAVE - 1);
   const int csx = (g.nx + SEGA - 1) / SEGA, csy = (g.ny + SEGA - 1) / SEGA;
-  hipLaunchKernelGGL((k_adj_sweep_face<0>), nblk((long long)fsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam, fsx);
-  hipLaunchKernelGGL((k_adj_sweep_cell<0>), nblk((long long)csx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam, csx);
-  hipLaunchKernelGGL((k_adj_sweep_face<1>), nblk((long long)fsy * g.nx * WAVE, 256), 256, 0, st, p, g, c->s.gam, fsy);
-  hipLaunchKernelGGL((k_adj_sweep_cell<1>), nblk((long long)csy * g.nx * WAVE, 256), 256, 0, st, p, g, c->s.gam, csy);
+  hipLaunchKernelGGL((k_adj_sweep_face<0, false>), nblk((long long)fsx * g.ny * WAVE, 256), 256, 0, st, p, g,
+                     c->s.gam, fsx);
+  hipLaunchKernelGGL((k_adj_sweep_cell<0, false>), nblk((long long)csx * g.ny * WAVE, 256), 256, 0, st, p, g,
+                     c->s.gam, csx);
+  {
+    // y pass on transposed copies of what it reads (columns contiguous), its
+    // outputs added back onto gp, gbf[1]
+    const double* src[NTR] = {p.in[0], p.in[5], p.in[6], p.vx, p.vy, p.vz, p.cx, p.cy, p.pr,
+                              p.gO[0], p.gO[1], p.gO[2], p.gO[3], p.gO[6], p.gO[7], p.gemf[2], p.gemf[3]};
+    TrArgs ti;
+    for (int k = 0; k < NTR; k++) {
+      ti.src[k] = src[k];
+      ti.dst[k] = w->tin[k];
+    }
+    const dim3 tg((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), NTR);
+    hipLaunchKernelGGL(k_adj_transpose, tg, 256, 0, st, ti, g.nx, g.ny);
+    AdjPtrs pt = p;
+    pt.in[0] = w->tin[0];
+    pt.in[5] = w->tin[1];
+    pt.in[6] = w->tin[2];
+    pt.vx = w->tin[3];
+    pt.vy = w->tin[4];
+    pt.vz = w->tin[5];
+    pt.cx = w->tin[6];
+    pt.cy = w->tin[7];
+    pt.pr = w->tin[8];
+    const int gom[6] = {0, 1, 2, 3, 6, 7};
+    for (int v = 0; v < 6; v++) pt.gO[gom[v]] = w->tin[9 + v];
+    pt.gemf[2] = w->tin[15];
+    pt.gemf[3] = w->tin[16];
+    for (int m = 0; m < 8; m++) pt.gp[m] = w->tout[m];
+    pt.gbf[1] = w->tout[8];
+    hipLaunchKernelGGL((k_adj_sweep_face<1, true>), nblk((long long)fsy * g.nx * WAVE, 256), 256, 0, st, pt, g,
+                       c->s.gam, fsy);
+    hipLaunchKernelGGL((k_adj_sweep_cell<1, true>), nblk((long long)csy * g.nx * WAVE, 256), 256, 0, st, pt, g,
+                       c->s.gam, csy);
+    TrArgs to;
+    for (int k = 0; k < NTR; k++) {
+      to.src[k] = nullptr;
+      to.dst[k] = nullptr;
+    }
+    for (int m = 0; m < 8; m++) {
+      to.src[m] = w->tout[m];
+      to.dst[m] = w->gp[m];
+    }
+    to.src[8] = w->tout[8];
+    to.dst[8] = w->gbf[1];
+    const dim3 tb((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), 9);
+    hipLaunchKernelGGL(k_adj_untranspose_add, tb, 256, 0, st, to, g.nx, g.ny);
+  }
   if (ctw) {
     hipLaunchKernelGGL((k_adj_prim<1>), nb, 256, 0, st, p, g, c->s.gam);
     hipLaunchKernelGGL((k_adj_face<1>), nb, 256, 0, st, p, g);
