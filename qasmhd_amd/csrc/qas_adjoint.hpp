@@ -100,10 +100,17 @@ QAS_HD void lin4_adj(T gl, T gr, T* gf)
 }
 
 // ---------------------------------------------------------------- lrstate
-// gs[7][5] += adjoint of the stencil, *gbn += adjoint of the cell-centred bn
-template <typename T>
-QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl_in)[7], const T (&gvr_in)[7],
-                        T (&gs)[7][5], T& gbn)
+// Adjoint of lrstate<4> (characteristic WCNS-4 reconstruction of one cell,
+// mhd_func.c:108-161 with mhd_eigen.c's projections).  The stencil comes
+// through a loader, ld(j, m) = s[j][m] (ld.fence() marks where values read
+// so far need not be kept), and its adjoint leaves through an
+// accumulator, acc(j, m, v) adds v to the adjoint of s[j][m]; *gbn gets the
+// adjoint of the cell-centred bn.  The seven characteristic fields are
+// reversed one group at a time (fields 0-2, 3-4, 5-6), so no 7 x 5 array
+// of characteristic values or of their adjoints is ever held at once.
+template <typename T, typename Ld, typename Acc>
+QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const T (&gvr_in)[7], Acc& acc,
+                          T& gbn)
 {
   T gvl[7], gvr[7];
   for (int k = 0; k < 7; k++) {
@@ -111,14 +118,14 @@ QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl_in)[7], c
     gvr[k] = gvr_in[k];
   }
   // forward recompute (the projection branch of mhd_func.c:143)
-  const T ro0 = s[0][2], bt0 = s[4][2], bu0 = s[5][2], pr0 = s[6][2];
-  const bool fast = (s[0][2] * (s[1][2] * s[1][2] + s[2][2] * s[2][2] + s[3][2] * s[3][2]) > gam * s[6][2]);
+  const T ro0 = ld(0, 2), bt0 = ld(4, 2), bu0 = ld(5, 2), pr0 = ld(6, 2);
+  const bool fast = (ld(0, 2) * (ld(1, 2) * ld(1, 2) + ld(2, 2) * ld(2, 2) + ld(3, 2) * ld(3, 2)) > gam * ld(6, 2));
   const T eps = T(1e-12);
   // exact-zero flags
   bool zf[5];
   for (int n = 1; n <= 5; n++)
-    zf[n - 1] = (s[n][0] == T(0.0)) && (s[n][1] == T(0.0)) && (s[n][2] == T(0.0)) && (s[n][3] == T(0.0)) &&
-                (s[n][4] == T(0.0));
+    zf[n - 1] = (ld(n, 0) == T(0.0)) && (ld(n, 1) == T(0.0)) && (ld(n, 2) == T(0.0)) && (ld(n, 3) == T(0.0)) &&
+                (ld(n, 4) == T(0.0));
   // eigen quantities
   const T iro = T(1.0) / ro0;
   const T cs2 = gam * pr0 * iro;
@@ -137,45 +144,51 @@ QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl_in)[7], c
   const T ircfh = T(0.5) / rcf;
   const T byh = T(0.5) * bt0;
   const T bzh = T(0.5) * bu0;
-  // characteristic stencils and face values (pre-fallback)
-  T w[7][5], wl[7], wr[7];
-  for (int m = 0; m < 5; m++) {
-    w[0][m] = fast ? (-rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m]) : s[1][m];
-    w[1][m] = cf2 * s[0][m] - bt0 * s[4][m] - bu0 * s[5][m] - s[6][m];
-    w[2][m] = fast ? (rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m])
-                   : (bt0 * s[4][m] + bu0 * s[5][m] + s[6][m]);
-    w[3][m] = sro * s[2][m] + s[4][m];
-    w[4][m] = sro * s[2][m] - s[4][m];
-    w[5][m] = sro * s[3][m] + s[5][m];
-    w[6][m] = sro * s[3][m] - s[5][m];
+  // characteristic value of field k at stencil point m (w = L s)
+  auto proj = [&](int k, int m) -> T {
+    switch (k) {
+      case 0: return fast ? (-rcf * ld(1, m) + bt0 * ld(4, m) + bu0 * ld(5, m) + ld(6, m)) : ld(1, m);
+      case 1: return cf2 * ld(0, m) - bt0 * ld(4, m) - bu0 * ld(5, m) - ld(6, m);
+      case 2:
+        return fast ? (rcf * ld(1, m) + bt0 * ld(4, m) + bu0 * ld(5, m) + ld(6, m))
+                    : (bt0 * ld(4, m) + bu0 * ld(5, m) + ld(6, m));
+      case 3: return sro * ld(2, m) + ld(4, m);
+      case 4: return sro * ld(2, m) - ld(4, m);
+      case 5: return sro * ld(3, m) + ld(5, m);
+      default: return sro * ld(3, m) - ld(5, m);
+    }
+  };
+  // face values of the characteristic fields (pre-fallback)
+  T wl[7], wr[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    T wk[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) wk[m] = proj(k, m);
+    interp5<4>(wk, wl[k], wr[k]);
   }
-  for (int k = 0; k < 7; k++) interp5<4>(w[k], wl[k], wr[k]);
-  T ul[7], ur[7];
+  ld.fence();
+  // back projection, only as far as the positivity test needs it
+  bool fb[2];
   for (int side = 0; side < 2; side++) {
     const T* q = side ? wr : wl;
-    T* u = side ? ur : ul;
-    u[0] = fast ? (icf2h * q[0] + icf2 * q[1] + icf2h * q[2]) : (icf2 * q[1] + icf2 * q[2]);
-    u[1] = fast ? (-ircfh * q[0] + ircfh * q[2]) : q[0];
-    u[2] = isroh * q[3] + isroh * q[4];
-    u[3] = isroh * q[5] + isroh * q[6];
-    u[4] = T(0.5) * q[3] - T(0.5) * q[4];
-    u[5] = T(0.5) * q[5] - T(0.5) * q[6];
-    u[6] = fast ? (T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6])
-                : (q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6]);
+    const T u0 = fast ? (icf2h * q[0] + icf2 * q[1] + icf2h * q[2]) : (icf2 * q[1] + icf2 * q[2]);
+    const T u6 = fast ? (T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6])
+                      : (q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6]);
+    fb[side] = (u0 <= eps || u6 <= eps);
   }
-  for (int n = 0; n < 5; n++)
-    if (zf[n]) ul[n + 1] = ur[n + 1] = T(0.0);
-  // reverse: positivity fallback
-  if (ul[0] <= eps || ul[6] <= eps) {
-    gs[0][2] += gvl[0];
-    gs[1][2] += gvl[1];
-    gs[6][2] += gvl[6];
+  // reverse: positivity fallback (first order for ro, vn, p)
+  T gc0 = T(0), gc1 = T(0), gc6 = T(0);  // adjoints of the centre values
+  if (fb[0]) {
+    gc0 += gvl[0];
+    gc1 += gvl[1];
+    gc6 += gvl[6];
     gvl[0] = gvl[1] = gvl[6] = T(0.0);
   }
-  if (ur[0] <= eps || ur[6] <= eps) {
-    gs[0][2] += gvr[0];
-    gs[1][2] += gvr[1];
-    gs[6][2] += gvr[6];
+  if (fb[1]) {
+    gc0 += gvr[0];
+    gc1 += gvr[1];
+    gc6 += gvr[6];
     gvr[0] = gvr[1] = gvr[6] = T(0.0);
   }
   for (int n = 0; n < 5; n++)
@@ -208,41 +221,72 @@ QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl_in)[7], c
     gbyh += (q[4] - q[3]) * g[6];
     gbzh += (q[6] - q[5]) * g[6];
   }
-  // reverse: WCNS per characteristic field
-  T gw[7][5] = {};
-  for (int k = 0; k < 7; k++) wcns4_adj(w[k], gwl[k], gwr[k], gw[k]);
-  // reverse: projection w = L s
+  // reverse: WCNS of field k, then the projection w = L s, by groups
   T grcf = T(0), gcf2 = T(0), gsro = T(0), gbt0 = T(0), gbu0 = T(0);
-  for (int m = 0; m < 5; m++) {
-    const T g0 = gw[0][m], g1 = gw[1][m], g2 = gw[2][m], g3 = gw[3][m], g4 = gw[4][m], g5 = gw[5][m],
-            g6 = gw[6][m];
-    if (fast) {
-      gs[1][m] += -rcf * g0 + rcf * g2;
-      gs[4][m] += bt0 * (g0 + g2);
-      gs[5][m] += bu0 * (g0 + g2);
-      gs[6][m] += g0 + g2;
-      grcf += s[1][m] * (g2 - g0);
-      gbt0 += s[4][m] * (g0 + g2);
-      gbu0 += s[5][m] * (g0 + g2);
-    } else {
-      gs[1][m] += g0;
-      gs[4][m] += bt0 * g2;
-      gs[5][m] += bu0 * g2;
-      gs[6][m] += g2;
-      gbt0 += s[4][m] * g2;
-      gbu0 += s[5][m] * g2;
+  auto field_adj = [&](int k, T (&gk)[5]) {
+    T wk[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      wk[m] = proj(k, m);
+      gk[m] = T(0);
     }
-    gs[0][m] += cf2 * g1;
-    gs[4][m] += -bt0 * g1 + g3 - g4;
-    gs[5][m] += -bu0 * g1 + g5 - g6;
-    gs[6][m] += -g1;
-    gcf2 += s[0][m] * g1;
-    gbt0 += -s[4][m] * g1;
-    gbu0 += -s[5][m] * g1;
-    gs[2][m] += sro * (g3 + g4);
-    gs[3][m] += sro * (g5 + g6);
-    gsro += s[2][m] * (g3 + g4) + s[3][m] * (g5 + g6);
+    wcns4_adj(wk, gwl[k], gwr[k], gk);
+  };
+  {  // fields 0-2 (fast pair / slow and entropy)
+    ld.fence();
+    T g0[5], g1[5], g2[5];
+    field_adj(0, g0);
+    field_adj(1, g1);
+    field_adj(2, g2);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const T s0 = ld(0, m), s1 = ld(1, m), s4 = ld(4, m), s5 = ld(5, m);
+      if (fast) {
+        const T t = g0[m] + g2[m];
+        acc(1, m, -rcf * g0[m] + rcf * g2[m]);
+        acc(4, m, bt0 * t - bt0 * g1[m]);
+        acc(5, m, bu0 * t - bu0 * g1[m]);
+        acc(6, m, t - g1[m]);
+        grcf += s1 * (g2[m] - g0[m]);
+        gbt0 += s4 * (t - g1[m]);
+        gbu0 += s5 * (t - g1[m]);
+      } else {
+        acc(1, m, g0[m]);
+        acc(4, m, bt0 * g2[m] - bt0 * g1[m]);
+        acc(5, m, bu0 * g2[m] - bu0 * g1[m]);
+        acc(6, m, g2[m] - g1[m]);
+        gbt0 += s4 * (g2[m] - g1[m]);
+        gbu0 += s5 * (g2[m] - g1[m]);
+      }
+      acc(0, m, cf2 * g1[m]);
+      gcf2 += s0 * g1[m];
+    }
   }
+  {  // fields 3-4 (Alfven, t component)
+    ld.fence();
+    T g3[5], g4[5];
+    field_adj(3, g3);
+    field_adj(4, g4);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      acc(2, m, sro * (g3[m] + g4[m]));
+      acc(4, m, g3[m] - g4[m]);
+      gsro += ld(2, m) * (g3[m] + g4[m]);
+    }
+  }
+  {  // fields 5-6 (Alfven, u component)
+    ld.fence();
+    T g5[5], g6[5];
+    field_adj(5, g5);
+    field_adj(6, g6);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      acc(3, m, sro * (g5[m] + g6[m]));
+      acc(5, m, g5[m] - g6[m]);
+      gsro += ld(3, m) * (g5[m] + g6[m]);
+    }
+  }
+  ld.fence();
   // reverse: eigen quantities
   gbt0 += T(0.5) * gbyh;
   gbu0 += T(0.5) * gbzh;
@@ -272,10 +316,32 @@ QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl_in)[7], c
   gbt0 += T(2.0) * bt0 * iro * gct2;
   gbu0 += T(2.0) * bu0 * iro * gct2;
   gro0 += -iro * iro * giro;
-  gs[0][2] += gro0;
-  gs[4][2] += gbt0;
-  gs[5][2] += gbu0;
-  gs[6][2] += gpr0;
+  acc(0, 2, gro0 + gc0);
+  acc(1, 2, gc1);
+  acc(4, 2, gbt0);
+  acc(5, 2, gbu0);
+  acc(6, 2, gpr0 + gc6);
+}
+
+// the same with the stencil and its adjoint as 7 x 5 arrays (gs += ...)
+template <typename T>
+struct StencilArr {
+  const T (&s)[7][5];
+  QAS_HD T operator()(int j, int m) const { return s[j][m]; }
+  QAS_HD void fence() const {}
+};
+template <typename T>
+struct StencilAcc {
+  T (&gs)[7][5];
+  QAS_HD void operator()(int j, int m, T v) { gs[j][m] += v; }
+};
+template <typename T>
+QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl)[7], const T (&gvr)[7], T (&gs)[7][5],
+                        T& gbn)
+{
+  StencilArr<T> ld{s};
+  StencilAcc<T> acc{gs};
+  lrstate_adj_g(ld, bn, gam, gvl, gvr, acc, gbn);
 }
 
 // ---------------------------------------------------------------- lr_fb

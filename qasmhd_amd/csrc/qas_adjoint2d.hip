@@ -266,7 +266,7 @@ constexpr int NGF = 17;
 // owns face q (between cells q-1 and q): flux adjoint (HLLD) from the
 // divergence and EMF-part adjoints, written to p.gfc[.][face].
 template <int DIR, bool TR>
-__global__ __launch_bounds__(256) void k_adj_sweep_face(AdjPtrs p, Geo g, double gam, int nseg)
+__global__ __launch_bounds__(256, 2) void k_adj_sweep_face(AdjPtrs p, Geo g, double gam, int nseg)
 {
   static_assert(!TR || DIR == 1, "only the y pass runs on transposed arrays");
   const int lane = threadIdx.x & (WAVE - 1);
@@ -334,11 +334,34 @@ __global__ __launch_bounds__(256) void k_adj_sweep_face(AdjPtrs p, Geo g, double
 
 // Cell pass: lane l handles cell q = seg*SEGA - 3 + l; lanes 3..60 own
 // their cell and face.  Reconstruction adjoint from the face adjoints of
-// faces q (right state) and q+1 (left state), gathered onto cells.
+// faces q (right state) and q+1 (left state).  The stencil adjoints are
+// added into a per-wavefront LDS row indexed by cell (lane l's stencil point
+// m is cell slot l + m), which is the gather onto cells; the stencil itself
+// is read from memory where it is used (loads re-issued after each fence
+// rather than held in registers across the whole reverse pass).
+template <int DIR>
+struct StencilLd {
+  const double* a[7];  // ro vn vt vu bt bu pr
+  long long sc, o;
+  __device__ __forceinline__ double operator()(int j, int m) const { return a[j][sc + (m - 2) * o]; }
+  // a compiler barrier: loads after it are issued again instead of reusing
+  // registers that would otherwise stay live across the whole reverse pass
+  __device__ __forceinline__ void fence() const { asm volatile("" ::: "memory"); }
+};
+struct SlotAcc {
+  double* row;  // this wavefront's [7][WAVE + 4] accumulators
+  int lane;
+  __device__ __forceinline__ void operator()(int j, int m, double v) const
+  {
+    atomicAdd(&row[j * (WAVE + 4) + lane + m], v);  // one add per lane and (j, m): no two lanes share a slot
+  }
+};
+
 template <int DIR, bool TR>
-__global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
+__global__ __launch_bounds__(256, 2) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
 {
   static_assert(!TR || DIR == 1, "only the y pass runs on transposed arrays");
+  __shared__ double gacc[256 / WAVE][7 * (WAVE + 4)];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   const int nline = DIR == 0 ? g.ny : g.nx;
@@ -349,13 +372,29 @@ __global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double
   const long long o = line_stride<DIR, TR>(g);
   const long long base = line_base<DIR, TR>(g, line);
   const int q = seg * SEGA - 3 + lane;
-  double gs[7][5] = {};
+  double* row = gacc[threadIdx.x / WAVE];
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    row[j * (WAVE + 4) + lane] = 0.0;
+    if (lane < 4) row[j * (WAVE + 4) + WAVE + lane] = 0.0;
+  }
+  SlotAcc acc{row, lane};
   double gcn = 0.0, gbm = 0.0, gbp = 0.0;
   const bool cell_ok = q >= 2 && q <= n - 3;
   if (cell_ok) {
     const long long sc = base + o * q;
-    SweepFwd<DIR> f;
-    f.load(p, sc, o);
+    StencilLd<DIR> ld;
+    ld.a[0] = p.in[0];
+    ld.a[1] = DIR == 0 ? p.vx : p.vy;
+    ld.a[2] = DIR == 0 ? p.vy : p.vz;
+    ld.a[3] = DIR == 0 ? p.vz : p.vx;
+    ld.a[4] = DIR == 0 ? p.cy : p.in[6];
+    ld.a[5] = DIR == 0 ? p.in[6] : p.cx;
+    ld.a[6] = p.pr;
+    ld.sc = sc;
+    ld.o = o;
+    const double* cn = DIR == 0 ? p.cx : p.cy;
+    const double* bf = DIR == 0 ? p.in[4] : p.in[5];
     double gvl[7], gvr[7];
     const bool fr_ok = q >= 3, fl_ok = q + 1 <= n - 3;
 #pragma unroll
@@ -365,30 +404,34 @@ __global__ __launch_bounds__(256) void k_adj_sweep_cell(AdjPtrs p, Geo g, double
     }
     const double gql = fl_ok ? p.gfc[14][sc + o] : 0.0;
     const double gqr = fr_ok ? p.gfc[15][sc] : 0.0;
-    lrstate_adj(f.s, f.bnc, gam, gvl, gvr, gs, gcn);
-    double gpp[5] = {}, gqq[5] = {};
-    lr_fb_adj(f.pp, f.qq, f.bnm, f.bnp, gql, gqr, gpp, gqq, gbm, gbp);
+    lrstate_adj_g(ld, cn[sc], gam, gvl, gvr, acc, gcn);
+    // lr_fb: p = b_pair * v_n, q = v_pair (x: cy, vy; y: cx, vx)
+    double pp[5], qq[5], gpp[5] = {}, gqq[5] = {};
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      pp[m] = ld(DIR == 0 ? 4 : 5, m) * ld(1, m);
+      qq[m] = ld(DIR == 0 ? 2 : 3, m);
+    }
+    lr_fb_adj(pp, qq, bf[sc], bf[sc + o], gql, gqr, gpp, gqq, gbm, gbp);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       if (DIR == 0) {
-        gs[1][m] += gpp[m] * f.s[4][m];
-        gs[4][m] += gpp[m] * f.s[1][m];
-        gs[2][m] += gqq[m];
+        acc(1, m, gpp[m] * ld(4, m));
+        acc(4, m, gpp[m] * ld(1, m));
+        acc(2, m, gqq[m]);
       } else {
-        gs[1][m] += gpp[m] * f.s[5][m];
-        gs[5][m] += gpp[m] * f.s[1][m];
-        gs[3][m] += gqq[m];
+        acc(1, m, gpp[m] * ld(5, m));
+        acc(5, m, gpp[m] * ld(1, m));
+        acc(3, m, gqq[m]);
       }
     }
   }
-  // gather the stencil adjoints onto cells: cell of lane l gets entry m of
-  // lane l - (m - 2)
+  // every lane's adds are in the row (one wavefront, LDS operations in order)
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   double gc7[7];
 #pragma unroll
-  for (int k = 0; k < 7; k++) {
-    gc7[k] = __shfl_down(gs[k][0], 2, WAVE) + __shfl_down(gs[k][1], 1, WAVE) + gs[k][2] +
-             __shfl_up(gs[k][3], 1, WAVE) + __shfl_up(gs[k][4], 2, WAVE);
-  }
+  for (int k = 0; k < 7; k++) gc7[k] = row[k * (WAVE + 4) + lane + 2];
   const double gbpl = __shfl_up(gbp, 1, WAVE);
   const bool own = lane >= 3 && lane < 3 + SEGA && q >= 0 && q < n;
   if (!own) return;
