@@ -120,6 +120,55 @@ def build_ref(ndim: int, xmesh: int, ymesh: int = 1, rmn: int = 1, odr: int = 4,
     return out
 
 
+def ref_dsptv_name(nx: int, ny: int, rmn: int, odr: int, r_k: int, ctw: int) -> str:
+    return "refdsp2d_%dx%d_r%do%dk%dw%d" % (nx, ny, rmn, odr, r_k, ctw)
+
+
+def build_ref_dsptv(xmesh: int, ymesh: int, rmn: int = 1, odr: int = 4, r_k: int = 3, ctw: int = 1,
+                    force: bool = False) -> str:
+    """Build (once) the reference's dissipative 2D driver (2D/SERIAL/MRX:
+    DMHD2D::dsptv with MHD2D::ideal/bound) with the dsptv harness."""
+    os.makedirs(REFOUT, exist_ok=True)
+    out = os.path.join(REFOUT, ref_dsptv_name(xmesh, ymesh, rmn, odr, r_k, ctw))
+    if os.path.exists(out) and not force:
+        return out
+    objs = _ref_objects()
+    ddir = os.path.join(REF, "2D", "SERIAL", "MRX")
+    macros = ["-D_MYMACROS_", "-DXMESH=(%d)" % xmesh, "-DYMESH=(%d)" % ymesh, "-DN_OUT=(1)", "-DDTREC=(1.0)",
+              "-DCFL=(0.4)", "-DRMN=(%d)" % rmn, "-DODR=(%d)" % odr, "-DR_K=(%d)" % r_k, "-DCTW=(%d)" % ctw,
+              "-DRANDOM=(0)", "-DREV=(2e2)", "-DREM=(2e2)"]
+    srcs = [os.path.join(HERE, "ref", "harness_dmhd2d.cpp")] + [
+        os.path.join(ddir, f) for f in ("dmhd2d_class.cpp", "dmhd2d_solve.cpp", "dmhd2d_init_.cpp",
+                                        "mhd2d_class.cpp", "mhd2d_solve.cpp")]
+    tmp = out + ".tmp%d" % os.getpid()
+    _run(REF_CXX + macros + ["-I", os.path.join(REF, "common"), "-I", ddir, "-I", os.path.join(HERE, "ref")]
+         + srcs + objs + ["-lm", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def run_ref_dsptv(binary: str, state, nu, eta, dx: float, dy: float, gam: float, dt: float, nsub: int = 1,
+                  mode: int = 0, nsteps: int = 0, dcmax: float = 0.0, dnx=(0,) * 8, dny=(0,) * 8):
+    """Run the dsptv harness; returns the final state."""
+    import numpy as np
+    state = np.ascontiguousarray(state, dtype=np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, fnu, feta, fpar = (os.path.join(td, n) for n in ("in.bin", "out.bin", "nu.bin", "eta.bin",
+                                                                      "p.txt"))
+        state.tofile(fin)
+        np.ascontiguousarray(nu, dtype=np.float64).tofile(fnu)
+        np.ascontiguousarray(eta, dtype=np.float64).tofile(feta)
+        with open(fpar, "w") as f:
+            f.write("in %s\nout %s\nnu %s\neta %s\n" % (fin, fout, fnu, feta))
+            f.write("dx %s\ndy %s\ngam %s\ndt %s\n" % (_hex(dx), _hex(dy), _hex(gam), _hex(dt)))
+            f.write("nsub %d\nmode %d\nnsteps %d\ndcmax %s\n" % (nsub, mode, nsteps, _hex(dcmax)))
+            f.write("dnx %s\ndny %s\n" % (" ".join(map(str, dnx)), " ".join(map(str, dny))))
+        r = subprocess.run([binary, fpar], capture_output=True, text=True, cwd=td)
+        if r.returncode != 0:
+            raise RuntimeError("reference dsptv harness failed (%d): %s %s" % (r.returncode, r.stdout, r.stderr))
+        return np.fromfile(fout, dtype=np.float64).reshape(state.shape)
+
+
 def _hex(x: float) -> str:
     return float(x).hex()
 

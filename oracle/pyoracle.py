@@ -45,6 +45,9 @@ def lib():
         l.qor_lrstate_batch.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
         l.qor_flux_batch.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, dp, dp]
         l.qor_cuct_batch.argtypes = [C.c_int, dp, dp, dp]
+        l.qor_dsptv.argtypes = [C.POINTER(QorCfg), C.POINTER(dp), dp, dp, C.c_double]
+        l.qor_run_dissipative.argtypes = [C.POINTER(QorCfg), C.POINTER(dp), dp, dp, C.c_double, C.c_int, C.c_int,
+                                          dp, dp]
         _lib = l
     return _lib
 
@@ -99,6 +102,27 @@ def step(cfg, state, dt):
 def cfl_dt(cfg, state):
     v = _Val(state)
     return lib().qor_cfl_dt(C.byref(cfg), v.ptrs)
+
+
+def dsptv(cfg, state, nu, eta, dt):
+    """DMHD2D::dsptv(dt) on state (2D), nu / eta per cell (ghosts included)."""
+    v = _Val(state)
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    lib().qor_dsptv(C.byref(cfg), v.ptrs, _dp(nu), _dp(eta), dt)
+
+
+def run_dissipative(cfg, state, nu, eta, dcmax, nsteps, flg=1, dt=None):
+    """DMHD2D::exec_ loop body nsteps times; returns the dt history."""
+    v = _Val(state)
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    if dt is None:
+        dt = lib().qor_cfl_dt(C.byref(cfg), v.ptrs)
+    d = C.c_double(dt)
+    hist = np.zeros(max(nsteps, 1))
+    lib().qor_run_dissipative(C.byref(cfg), v.ptrs, _dp(nu), _dp(eta), dcmax, nsteps, flg, C.byref(d), _dp(hist))
+    return hist[:nsteps]
 
 
 def run(cfg, state, nsteps, flg=1, dt=None):

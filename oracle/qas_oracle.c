@@ -1583,6 +1583,224 @@ void qor_run(const qor_cfg *c, double *const val[8], int nsteps, int flg, double
 }
 
 /* ---------------------------------------------------------------------- */
+/* Dissipative step: DMHD2D::dsptv (2D/SERIAL/MRX/dmhd2d_solve.cpp:3-151).   */
+
+/* calc_flux_viscous, common/mhd_flux.c:700-753: viscous fluxes of momentum
+ * and energy at face i-1/2 of the cell at v*, offsets as the reference's. */
+static void visc_flux(const double *vx, const double *vy, const double *vz, double rnu, double idx,
+                      double idy, double idz, long xo, long yo, long zo, c_fn fc, c_fn df,
+                      double *fmx, double *fmy, double *fmz, double *fen)
+{
+  const long ss[4] = {-2 * xo, -xo, 0, +xo};
+  const double c2 = 2. / 3., c4 = 4. / 3.;
+  double vxi[4], vyi[4], vzi[4], dv[4];
+  int k;
+  for (k = 0; k < 4; k++) {
+    vxi[k] = vx[ss[k]];
+    vyi[k] = vy[ss[k]];
+    vzi[k] = vz[ss[k]];
+  }
+  const double vx0 = fc(&vxi[1]), vy0 = fc(&vyi[1]), vz0 = fc(&vzi[1]);
+  const double dvxdx = df(&vxi[1]) * idx;
+  const double dvydx = df(&vyi[1]) * idx;
+  const double dvzdx = df(&vzi[1]) * idx;
+  for (k = 0; k < 4; k++) dv[k] = 0.5 * (vx[ss[k] + yo] - vx[ss[k] - yo]);
+  const double dvxdy = fc(&dv[1]) * idy;
+  for (k = 0; k < 4; k++) dv[k] = 0.5 * (vy[ss[k] + yo] - vy[ss[k] - yo]);
+  const double dvydy = fc(&dv[1]) * idy;
+  for (k = 0; k < 4; k++) dv[k] = 0.5 * (vx[ss[k] + zo] - vx[ss[k] - zo]);
+  const double dvxdz = fc(&dv[1]) * idz;
+  for (k = 0; k < 4; k++) dv[k] = 0.5 * (vz[ss[k] + zo] - vz[ss[k] - zo]);
+  const double dvzdz = fc(&dv[1]) * idz;
+  *fmx = -rnu * (c4 * dvxdx - c2 * (dvydy + dvzdz));
+  *fmy = -rnu * (dvydx + dvxdy);
+  *fmz = -rnu * (dvzdx + dvxdz);
+  *fen = (*fmx) * vx0 + (*fmy) * vy0 + (*fmz) * vz0;
+}
+
+/* calc_flux_resistive, common/mhd_flux.c:755-782 */
+static void res_flux(const double *by, const double *bz, const double *ey, const double *ez, long xo, long yo,
+                     long zo, c_fn fc, double *fby, double *fbz, double *fen)
+{
+  const long ss[4] = {-2 * xo, -xo, 0, +xo};
+  const double ey0 = 0.5 * (ey[0] + ey[zo]);
+  const double ez0 = 0.5 * (ez[0] + ez[yo]);
+  double byi[4], bzi[4];
+  int k;
+  for (k = 0; k < 4; k++) {
+    byi[k] = 0.5 * (by[ss[k]] + by[ss[k] + yo]);
+    bzi[k] = 0.5 * (bz[ss[k]] + bz[ss[k] + zo]);
+  }
+  const double by0 = fc(&byi[1]);
+  const double bz0 = fc(&bzi[1]);
+  *fby = -ez0;
+  *fbz = +ey0;
+  *fen = ey0 * bz0 - ez0 * by0;
+}
+
+/* mhd_ct_eres, common/mhd_func.c:78-106: eta j at the CT edges of cell b*. */
+static void ct_eres(const double *bx, const double *by, const double *bz, const double *eta, double idx,
+                    double idy, double idz, long xo, long yo, long zo, c_fn df, double *ex, double *ey,
+                    double *ez)
+{
+  const long si[4] = {-2 * xo, -xo, 0, +xo}, sj[4] = {-2 * yo, -yo, 0, +yo}, sk[4] = {-2 * zo, -zo, 0, +zo};
+  double bxj[4], bxk[4], byk[4], byi[4], bzi[4], bzj[4];
+  int k;
+  for (k = 0; k < 4; k++) {
+    bxj[k] = bx[sj[k]];
+    bxk[k] = bx[sk[k]];
+    byk[k] = by[sk[k]];
+    byi[k] = by[si[k]];
+    bzi[k] = bz[si[k]];
+    bzj[k] = bz[sj[k]];
+  }
+  const double jx = df(&bzj[1]) * idy - df(&byk[1]) * idz;
+  const double jy = df(&bxk[1]) * idz - df(&bzi[1]) * idx;
+  const double jz = df(&byi[1]) * idx - df(&bxj[1]) * idy;
+  *ex = 0.25 * (eta[-yo - zo] + eta[-zo] + eta[-yo] + eta[0]) * jx;
+  *ey = 0.25 * (eta[-zo - xo] + eta[-xo] + eta[-zo] + eta[0]) * jy;
+  *ez = 0.25 * (eta[-xo - yo] + eta[-yo] + eta[-xo] + eta[0]) * jz;
+}
+
+void qor_dsptv(const qor_cfg *c, double *const val[8], const double *nu, const double *eta, double dt)
+{
+  const int nx = c->nx, ny = c->ny;
+  const long nxy = (long)nx * ny;
+  const double idx = 1.0 / c->dx, idy = 1.0 / c->dy;
+  const double dtdx = dt * idx, dtdy = dt * idy;
+  c_fn fc = fcen_tab[c->odr - 1];
+  c_fn df = df_tab[c->odr - 1];
+  double *ut = (double *)malloc(sizeof(double) * 8 * nxy);
+  double *fx = (double *)calloc(8 * nxy, sizeof(double));
+  double *fy = (double *)calloc(8 * nxy, sizeof(double));
+  double *ex = (double *)calloc(nxy, sizeof(double));
+  double *ey = (double *)calloc(nxy, sizeof(double));
+  double *ez = (double *)calloc(nxy, sizeof(double));
+  double *vx = (double *)malloc(sizeof(double) * nxy);
+  double *vy = (double *)malloc(sizeof(double) * nxy);
+  double *vz = (double *)malloc(sizeof(double) * nxy);
+  const double *ro = val[0];
+  double *bx = val[4], *by = val[5], *bz = val[6];
+  int m, rk;
+  long ss;
+  for (m = 0; m < 8; m++) memcpy(ut + m * nxy, val[m], sizeof(double) * nxy);
+  for (rk = 0; rk < c->r_k; rk++) {
+    const double *rf = rk_coef(c->r_k, rk);
+    int i, j;
+    for (ss = 0; ss < nxy; ss++) { /* vlcty, common/mhd_class.hpp:59-66 */
+      double iro = 1.0 / ro[ss];
+      vx[ss] = val[1][ss] * iro;
+      vy[ss] = val[2][ss] * iro;
+      vz[ss] = val[3][ss] * iro;
+    }
+    for (j = 2; j < ny - 1; j++)
+      for (i = 2; i < nx - 1; i++) {
+        ss = (long)nx * j + i;
+        ct_eres(&bx[ss], &by[ss], &bz[ss], &eta[ss], idx, idy, 0.0, 1, nx, 0, df, &ex[ss], &ey[ss], &ez[ss]);
+      }
+    /* faces whose fluxes the updates read (the reference computes j, i in
+     * [2, n-2]; the rest are never used) */
+    for (j = 2; j < ny - 1; j++)
+      for (i = 2; i < nx - 1; i++) {
+        double rnu, flux[2][8] = {{0.0}, {0.0}};
+        ss = (long)nx * j + i;
+        if (j >= c->yoff && j < ny - c->yoff && i >= 3 && i <= nx - 3) {
+          rnu = 0.5 * (ro[ss - 1] * nu[ss - 1] + ro[ss] * nu[ss]);
+          visc_flux(&vx[ss], &vy[ss], &vz[ss], rnu, idx, idy, 0.0, 1, nx, 0, fc, df, &flux[0][1], &flux[0][2],
+                    &flux[0][3], &flux[0][7]);
+          res_flux(&by[ss], &bz[ss], &ey[ss], &ez[ss], 1, nx, 0, fc, &flux[1][5], &flux[1][6], &flux[1][7]);
+          for (m = 0; m < 8; m++) fx[8 * ss + m] = (m == 4) ? 0 : flux[0][m] + flux[1][m];
+        }
+        if (i >= c->xoff && i < nx - c->xoff && j >= 3 && j <= ny - 3) {
+          double g[2][8] = {{0.0}, {0.0}};
+          rnu = 0.5 * (ro[ss - nx] * nu[ss - nx] + ro[ss] * nu[ss]);
+          visc_flux(&vy[ss], &vz[ss], &vx[ss], rnu, idy, 0.0, idx, nx, 0, 1, fc, df, &g[0][1], &g[0][2], &g[0][3],
+                    &g[0][7]);
+          res_flux(&bz[ss], &bx[ss], &ez[ss], &ex[ss], nx, 0, 1, fc, &g[1][5], &g[1][6], &g[1][7]);
+          fy[8 * ss + 0] = g[0][0] + g[1][0];
+          fy[8 * ss + 2] = g[0][1] + g[1][1];
+          fy[8 * ss + 3] = g[0][2] + g[1][2];
+          fy[8 * ss + 1] = g[0][3] + g[1][3];
+          fy[8 * ss + 5] = 0;
+          fy[8 * ss + 6] = g[0][5] + g[1][5];
+          fy[8 * ss + 4] = g[0][6] + g[1][6];
+          fy[8 * ss + 7] = g[0][7] + g[1][7];
+        }
+      }
+    /* cell update (mhd_updt2d, mhd_func.c:222-248) of ro mx my mz bz en */
+    for (j = c->yoff; j < ny - c->yoff; j++)
+      for (i = c->xoff; i < nx - c->xoff; i++) {
+        static const int vm[6] = {0, 1, 2, 3, 6, 7};
+        int v;
+        ss = (long)nx * j + i;
+        for (v = 0; v < 6; v++) {
+          const int q = vm[v];
+          double a4[4], b4[4];
+          int k;
+          for (k = 0; k < 4; k++) {
+            a4[k] = fx[8 * (ss + (k - 1)) + q];
+            b4[k] = fy[8 * (ss + (long)(k - 1) * nx) + q];
+          }
+          const double du0 = df(&a4[1]), du1 = df(&b4[1]);
+          rk_upd(&val[q][ss], ut[q * nxy + ss], (-dtdx * du0 - dtdy * du1), rf[0], rf[1]);
+        }
+      }
+    /* CT (mhd_updt2d_ctb, mhd_func.c:286-300) */
+    for (j = c->yoff; j < ny - c->yoff; j++)
+      for (i = c->xoff; i < nx - c->xoff + 1; i++) {
+        double e4[4];
+        int k;
+        ss = (long)nx * j + i;
+        for (k = 0; k < 4; k++) e4[k] = ez[ss + (long)(k - 1) * nx];
+        const double dtdy_ct = +dtdy;
+        rk_upd(&bx[ss], ut[4 * nxy + ss], -dtdy_ct * df(&e4[1]), rf[0], rf[1]);
+      }
+    for (j = c->yoff; j < ny - c->yoff + 1; j++)
+      for (i = c->xoff; i < nx - c->xoff; i++) {
+        double e4[4];
+        int k;
+        ss = (long)nx * j + i;
+        for (k = 0; k < 4; k++) e4[k] = ez[ss + (k - 1)];
+        const double dtdy_ct = -dtdx;
+        rk_upd(&by[ss], ut[5 * nxy + ss], -dtdy_ct * df(&e4[1]), rf[0], rf[1]);
+      }
+    qor_bound(c, val);
+  }
+  free(ut);
+  free(fx);
+  free(fy);
+  free(ex);
+  free(ey);
+  free(ez);
+  free(vx);
+  free(vy);
+  free(vz);
+}
+
+/* DMHD2D::exec_ step body (dmhd2d_class.cpp:15-33) without the output:
+ * bound; ideal(dt); if dcmax > 1e-15: ncmax = 1 + (int)(2 (dcmax dt) / dr^2)
+ * times {bound; dsptv(dt / ncmax)}; then setdt when flg and n even.       */
+void qor_run_dissipative(const qor_cfg *c, double *const val[8], const double *nu, const double *eta,
+                         double dcmax, int nsteps, int flg, double *dt, double *dt_hist)
+{
+  const double dr = omin(c->dx, c->dy);
+  int n, nc;
+  for (n = 1; n <= nsteps; n++) {
+    if (dt_hist) dt_hist[n - 1] = *dt;
+    qor_bound(c, val);
+    qor_step(c, val, *dt);
+    if (dcmax > 1e-15) {
+      const int ncmax = 1 + (int)(2 * (dcmax * (*dt)) / (dr * dr));
+      for (nc = 0; nc < ncmax; nc++) {
+        qor_bound(c, val);
+        qor_dsptv(c, val, nu, eta, (*dt) / ncmax);
+      }
+    }
+    if (flg && (n % 2 == 0)) *dt = qor_cfl_dt(c, val);
+  }
+}
+
+/* ---------------------------------------------------------------------- */
 void qor_interp_batch(int kind, int n, const double *f5, double *fl, double *fr)
 {
   lr_fn f = (kind >= 10) ? linear_tab[kind - 10] : interp_tab[kind];

@@ -44,6 +44,12 @@ double qor_cfl_dt(const qor_cfg *c, double *const val[8]);
 void qor_run(const qor_cfg *c, double *const val[8], int nsteps, int flg,
              double *dt, double *dt_hist);
 
+/* dsptv(dt): one visco-resistive SSP-RK step (2D), nu / eta per cell.      */
+void qor_dsptv(const qor_cfg *c, double *const val[8], const double *nu, const double *eta, double dt);
+/* DMHD2D::exec_ loop: ideal(dt) then ncmax sub-steps of dsptv(dt/ncmax).    */
+void qor_run_dissipative(const qor_cfg *c, double *const val[8], const double *nu, const double *eta,
+                         double dcmax, int nsteps, int flg, double *dt, double *dt_hist);
+
 /* ---- per-function entry points (batched, for golden-vector tests) ------- */
 /* kind 0..3: interpol[ODR-1] (1st, MUSCL-mm, WCNS3, WCNS4);
  * kind 10..13: l_interp[ODR-1] (linear 1st..4th).  f5: n x 5 stencil.       */

@@ -399,3 +399,62 @@ def turbulence3d(n: int = 512, seed: int = 4, nmodes: int = 32, kmax: int = 4, g
     del CX, CY, CZ, vx, vy, vz
     return Problem(state=st, dx=d, dy=d, dz=d, gam=gam, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1,
                    name="turb3d_%d" % n)
+
+
+# ---------------------------------------------------------------- 2D reconnection
+def reconnection2d(nx: int = 1024, ny: int = 128, gam: float = 5.0 / 3.0, beta: float = 0.2, ro0: float = 1.0,
+                   ro1: float = 0.2, b0: float = 1.0, b1: float = 0.05, bg: float = 0.0, lam: float = 1.0,
+                   rev: float = 2e2, rem: float = 2e2, xlen: float = 64.0):
+    """Harris current sheet with a localised resistivity and viscosity, as
+    2D/SERIAL/MRX/dmhd2d_init_.cpp:7-103 and mhd2d_paras.cpp:6-40: x in
+    [-xlen/2, xlen/2], y in the same aspect as ny/nx, ro = ro0 sech^2(y/lam)
+    + ro1, Bx = b0 tanh(y/lam), pressure balance with plasma beta `beta` in
+    the lobe, Zenitani's B perturbation of amplitude b1; nu = nu0 exp(-r/lam),
+    eta = eta0 exp(-r/lam) with nu0 = vA lam / rev, eta0 = vA lam / rem.
+    Boundaries: periodic in x; in y mirror with sign flips of my and by.
+    Returns (Problem, nu, eta, dcmax) with dcmax = max(2 nu, eta) (setdc)."""
+    xmin, xmax = -0.5 * xlen, 0.5 * xlen
+    ymin = -0.5 * (xmax - xmin) * ny / nx
+    ymax = -ymin
+    dx = (xmax - xmin) / nx
+    dy = (ymax - ymin) / ny
+    Nx, Ny = nx + 2 * NG, ny + 2 * NG
+    x = (np.arange(Nx) - NG + 0.5) * dx + xmin
+    y = (np.arange(Ny) - NG + 0.5) * dy + ymin
+    X, Y = np.meshgrid(x, y)
+    w = lam + 1e-15
+    ro = ro0 / np.cosh(Y / w) ** 2 + ro1
+    bx = b0 * np.tanh(Y / w)
+    by = np.zeros_like(X)
+    bz = np.full_like(X, bg)
+    cx = bx.copy()
+    cy = by.copy()
+    pr = 0.5 * ((1.0 + beta) * (b0 * b0 + bg * bg) - (cx * cx + cy * cy + bz * bz))
+    xm = X - 0.5 * dx
+    ym = Y - 0.5 * dy
+    bx = bx - b1 * (Y / lam) * np.exp(-(xm * xm + Y * Y) / (4 * lam * lam))
+    by = by + b1 * (X / lam) * np.exp(-(X * X + ym * ym) / (4 * lam * lam))
+    g = np.exp(-(X * X + Y * Y) / (4 * lam * lam))
+    cx = cx - b1 * (Y / lam) * g
+    cy = cy + b1 * (X / lam) * g
+    z = np.zeros_like(X)
+    st = np.zeros((8, Ny, Nx))
+    st[0] = ro
+    st[1] = z
+    st[2] = z
+    st[3] = z
+    st[4] = bx
+    st[5] = by
+    st[6] = bz
+    st[7] = energy(ro, z, z, z, cx, cy, bz, pr, gam)
+    dn = np.zeros((3, 8), dtype=int)
+    dn[1] = [1, 1, -1, 1, 1, -1, 1, 1]
+    al = math.sqrt((b0 * b0 + bg * bg) / ro0)
+    nu0, eta0 = al * lam / rev, al * lam / rem
+    r = np.sqrt(X * X + Y * Y)
+    nu = nu0 * np.exp(-r / lam)
+    eta = eta0 * np.exp(-r / lam)
+    dcmax = float(np.max(np.maximum(2 * nu, eta)))
+    p = Problem(state=st, dx=dx, dy=dy, gam=gam, cfl=0.4, rmn=2, odr=2, r_k=3, ctw=1, dn=dn,
+                name="mrx_%dx%d" % (nx, ny))
+    return p, nu, eta, dcmax
