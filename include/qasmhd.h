@@ -119,6 +119,20 @@ int qas_sync(qas_ctx *c);
 /* Device memory held by the context (bytes). */
 long long qas_device_bytes(const qas_ctx *c);
 
+/* ---- dissipative step (2D, FP64) ---------------------------------------
+ * qas_set_dissipation replaces the arrays DMHD2D keeps and DMHD2D::setdc
+ * fills (common/mhd_class.hpp:38 nu, eta; 2D/SERIAL/MRX/dmhd2d_init_.cpp:86-103):
+ * per-cell kinematic viscosity and resistivity, nx*ny doubles each (ghosts
+ * included, host memory), and dcmax = max over cells of max(2 nu, eta), the
+ * value setdc returns.  nu = NULL switches the dissipative step off.
+ * With it set, qas_run follows DMHD2D::exec_ (dmhd2d_class.cpp:15-33): after
+ * ideal(dt), ncmax = 1 + (int)(2 (dcmax dt) / dr^2) sub-steps of
+ * {bound; dsptv(dt / ncmax)} (dt is read back each step; no graph replay).
+ * qas_dsptv is one DMHD2D::dsptv(dt) call (dmhd2d_solve.cpp:3-151), after
+ * bound() when the state is not bounded. */
+int qas_set_dissipation(qas_ctx *c, const double *nu, const double *eta, double dcmax);
+int qas_dsptv(qas_ctx *c, double dt);
+
 
 /* ---- batched domains (no counterpart in the reference) -----------------
  * Advances many independent 2D domains of uneven sizes with one launch per

@@ -39,7 +39,8 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck",
            "qas_batch_create", "qas_batch_run", "qas_batch_last_error", "qas_batch_destroy",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
-           "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct"]
+           "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct",
+           "qas_set_dissipation", "qas_dsptv"]
 
 _lib = None
 
@@ -95,6 +96,8 @@ def load(path: str | None = None):
     l.qas_test_lrstate.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
     l.qas_test_flux.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, dp, dp]
     l.qas_test_cuct.argtypes = [C.c_int, dp, dp, dp]
+    l.qas_set_dissipation.argtypes = [vp, dp, dp, C.c_double]
+    l.qas_dsptv.argtypes = [vp, C.c_double]
     _lib = l
     return l
 
@@ -283,6 +286,22 @@ class Solver:
 
     def sync(self):
         self._check(self.lib.qas_sync(self.ctx), "qas_sync")
+
+    def set_dissipation(self, nu, eta, dcmax):
+        """Per-cell viscosity / resistivity (arrays of the domain shape, ghosts
+        included) and max(2 nu, eta); None switches the dissipative step off."""
+        if nu is None:
+            self._check(self.lib.qas_set_dissipation(self.ctx, None, None, 0.0), "qas_set_dissipation")
+            return
+        self._nu = np.ascontiguousarray(nu, dtype=np.float64)
+        self._eta = np.ascontiguousarray(eta, dtype=np.float64)
+        assert self._nu.shape == self.shape and self._eta.shape == self.shape
+        self._check(self.lib.qas_set_dissipation(self.ctx, _p(self._nu), _p(self._eta), float(dcmax)),
+                    "qas_set_dissipation")
+
+    def dsptv(self, dt):
+        """One visco-resistive step dsptv(dt)."""
+        self._check(self.lib.qas_dsptv(self.ctx, float(dt)), "qas_dsptv")
 
     def adj_forward(self, dts):
         """Advance len(dts) steps at the given dt values, keeping checkpoints

@@ -168,6 +168,7 @@ extern "C" void qas_destroy(qas_ctx* c)
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   adj_release(c);
+  dsp_release(c);
   if (c->slab) hipFree(c->slab);
   if (c->d_hist) hipFree(c->d_hist);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -334,6 +335,9 @@ static int loop_step(qas_ctx* c, int flg, bool hist, bool even)
   if (!c->bounded) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
   int r = do_step(c);
   if (r) return r;
+  // DMHD2D::exec_: the sub-cycled dissipative step after ideal(dt)
+  r = dsp_after_ideal(c);
+  if (r) return r;
   if (flg && even) {
     QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
     QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
@@ -398,7 +402,8 @@ extern "C" int qas_run(qas_ctx* c, int nsteps, int flg, double* dt_inout, double
     if (r) return r;
     n++;
   }
-  const bool graphs = c->use_graph && c->s.r_k >= 2 && c->prof_mask == 0;
+  // (the dissipative sub-cycle count depends on dt: no graph then)
+  const bool graphs = c->use_graph && c->s.r_k >= 2 && c->prof_mask == 0 && !c->d_nu;
   if (graphs && nsteps - n >= 2) {
     if (!c->bounded) {
       QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));

@@ -58,6 +58,17 @@ __host__ __device__ inline PrimView<PT> prim_view(const StagePtrs& p)
                       reinterpret_cast<const PT*>(p.cx), reinterpret_cast<const PT*>(p.cy),
                       reinterpret_cast<const PT*>(p.cz)};
 }
+// XCD-aware tile order: the hardware hands consecutive workgroups to the 8
+// XCDs in turn; remap block b of nb so that each XCD runs one contiguous
+// range of logical tiles (neighbouring column groups then share that XCD's
+// L2 lines).
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+  const int x = b % 8, slot = b / 8;
+  const int q = nb / 8, r = nb % 8;
+  return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // The face parts of the B fluxes (emf) and the stored transverse
 // divergences (acc) in element type ST = the sweep precision: double on the
 // FP64 path; float on the mixed path, where the face parts are FP32 values
@@ -161,6 +172,10 @@ struct qas_ctx {
   size_t prof_n = 0;
   // discrete-adjoint workspace (qas_adjoint2d.hip), created on first use
   struct qas_adj_work* adj = nullptr;
+  // dissipative step (qas_dissipative.hip): per-cell viscosity and
+  // resistivity (one allocation, eta follows nu) and max(2 nu, eta)
+  double *d_nu = nullptr, *d_eta = nullptr;
+  double dcmax = 0.0;
 };
 
 namespace qas {
@@ -199,6 +214,10 @@ hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
 void adj_release(qas_ctx* c);  // frees the adjoint workspace (qas_adjoint2d.hip)
+// dissipative step (qas_dissipative.hip)
+void dsp_release(qas_ctx* c);
+int dsp_step(qas_ctx* c, double dt);
+int dsp_after_ideal(qas_ctx* c);
 // FP32 face sweeps of the mixed path (qas_mixed.hip)
 void (*pick_sweep_f32(int ndim, int dir, int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
 }  // namespace qas

@@ -28,17 +28,6 @@ __device__ __forceinline__ double dv_at(const VT* v, int i, int j, const Geo& g,
   return apply(sy, apply(sx, (double)r));
 }
 
-// XCD-aware tile order: the hardware hands consecutive workgroups to the 8
-// XCDs in turn; remap block b of nb so that each XCD runs one contiguous
-// range of logical tiles (neighbouring column groups then share that XCD's
-// L2 lines).
-__device__ __forceinline__ int xcd_remap(int b, int nb)
-{
-  const int x = b % 8, slot = b / 8;
-  const int q = nb / 8, r = nb % 8;
-  return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-}
-
 // y sweep, LDS-staged; it runs before the x sweep of the stage.  One
 // workgroup = 4 wavefronts = 4 adjacent columns of one 60-face segment.  The stencil inputs of the 68 rows f0-4..f0+63 of
 // the 4 columns are loaded cooperatively (4 consecutive threads read 4
