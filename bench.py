@@ -21,6 +21,9 @@ Also reported, on one JSON line:
                 instances run one after another
   adjoint       time of the hand-written discrete adjoint over 20 steps of a
                 256^2 periodic run against the plain forward of those steps
+  dissipative   2D reconnection 4096 x 512 with localised viscosity and
+                resistivity: ideal step plus sub-cycled visco-resistive step,
+                one dissipative call alone, and the reference's own one
   inverse       64 batched instances of the velocity-mode inversion: the
                 batched forward, the adjoint gradient of all 512 amplitudes
                 and a finite-difference check
@@ -354,6 +357,61 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
                     "adjoints per step) + host copies; forward = one batched run of all instances"}
 
 
+def dissipative_line(nx, ny, steps, cpu=True):
+    """2D Harris-sheet reconnection with localised viscosity and resistivity
+    (the reference's dissipative MRX set-up: LHLLD, ODR 2, RK3, CUCT,
+    Reynolds numbers 200) at nx x ny: zone-cycles/s of DMHD2D::exec_ steps
+    (ideal(dt), then ncmax sub-steps of the visco-resistive dsptv), the time
+    of one dsptv call alone, and the reference's own dsptv on this host over
+    a bounded sample."""
+    from qasmhd_amd import capi, problems
+    p, nu, eta, dcmax = problems.reconnection2d(nx=nx, ny=ny)
+    zones = nx * ny
+    dr = min(p.dx, p.dy)
+    with capi.solver_for(p) as s:
+        s.upload(p.state)
+        s.set_dissipation(nu, eta, dcmax)
+        s.run(2, flg=1, history=False)  # warm-up
+        s.upload(p.state)
+        s.reset_counter()
+        s.sync()
+        t0 = time.perf_counter()
+        hist, dt_now = s.run(steps, flg=1)
+        s.sync()
+        el = time.perf_counter() - t0
+        ncmax = [1 + int(2 * (dcmax * d) / (dr * dr)) for d in hist]
+        dt_sub = float(hist[-1]) / ncmax[-1]
+        nd = 20
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(nd):
+            s.dsptv(dt_sub)
+        s.sync()
+        t_dsp = (time.perf_counter() - t0) / nd
+        finite = bool(np.isfinite(s.download()).all())
+    out = {"workload": "2d_reconnection_%dx%d_lhlld_odr2_ct_rk3_visco_resistive" % (nx, ny), "steps": steps,
+           "value": zones * steps / el, "unit": "zone-cycles/s", "ms_per_step": el / steps * 1e3,
+           "ncmax": [min(ncmax), max(ncmax)], "dsptv_ms": t_dsp * 1e3, "dsptv_zones_per_s": zones / t_dsp,
+           "dcmax": dcmax, "finite": finite}
+    if cpu:
+        try:
+            from oracle import build as obuild
+            binary = os.path.join(obuild.REFOUT, obuild.ref_dsptv_name(nx, ny, p.rmn, p.odr, p.r_k, p.ctw))
+            if os.path.exists(binary):
+                threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+                k = 4
+                _, el_ref = obuild.run_ref_dsptv(binary, p.state, nu, eta, p.dx, p.dy, p.gam, dt_sub, nsub=k,
+                                                 dnx=list(p.dn[0]), dny=list(p.dn[1]), threads=threads,
+                                                 timing=True)
+                out["cpu_reference"] = {"dsptv_zones_per_s": zones * k / el_ref, "cores": threads,
+                                        "kind": "reference",
+                                        "sample": "%d calls of bound + DMHD2D::dsptv (g++ -O2 -fopenmp), %.2f s"
+                                                  % (k, el_ref)}
+        except Exception as e:  # reported, not required
+            _log("bench: reference dsptv timing failed: %s" % e)
+    return out
+
+
 def mixed3d_line(p, steps):
     """FP64 vs mixed precision on a device-built 3D problem: both rates over
     `steps` steps at the same fixed dt, and the L1 difference of density and
@@ -440,6 +498,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adj-steps", type=int, default=20)
     ap.add_argument("--inverse-n", type=int, default=64, help="instances of the inversion line (0: skip it)")
+    ap.add_argument("--dsp-nx", type=int, default=4096, help="dissipative line size along x (0: skip it)")
+    ap.add_argument("--dsp-ny", type=int, default=512)
+    ap.add_argument("--dsp-steps", type=int, default=100)
     ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--turb3d-n", type=int, default=512, help="3D turbulence line size (0: skip it)")
     ap.add_argument("--turb3d-steps", type=int, default=20)
@@ -528,6 +589,13 @@ def main():
                 extra["batch"] = batch_line(args.batch_n, args.batch_steps)
             except Exception as e:
                 _log("bench: batch line failed: %s" % e)
+        if args.dsp_nx > 0:
+            try:
+                _log("bench: dissipative reconnection, %dx%d, %d steps" % (args.dsp_nx, args.dsp_ny, args.dsp_steps))
+                extra["dissipative"] = dissipative_line(args.dsp_nx, args.dsp_ny, args.dsp_steps,
+                                                        cpu=not args.no_cpu)
+            except Exception as e:
+                _log("bench: dissipative line failed: %s" % e)
         if args.inverse_n > 0:
             try:
                 _log("bench: velocity-mode inversion, %d instances x %d steps" % (args.inverse_n, args.inverse_steps))

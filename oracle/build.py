@@ -148,8 +148,10 @@ def build_ref_dsptv(xmesh: int, ymesh: int, rmn: int = 1, odr: int = 4, r_k: int
 
 
 def run_ref_dsptv(binary: str, state, nu, eta, dx: float, dy: float, gam: float, dt: float, nsub: int = 1,
-                  mode: int = 0, nsteps: int = 0, dcmax: float = 0.0, dnx=(0,) * 8, dny=(0,) * 8):
-    """Run the dsptv harness; returns the final state."""
+                  mode: int = 0, nsteps: int = 0, dcmax: float = 0.0, dnx=(0,) * 8, dny=(0,) * 8,
+                  threads: int | None = None, timing: bool = False):
+    """Run the dsptv harness; returns the final state (and the wall time of
+    its loop in seconds when timing is set)."""
     import numpy as np
     state = np.ascontiguousarray(state, dtype=np.float64)
     with tempfile.TemporaryDirectory() as td:
@@ -163,10 +165,20 @@ def run_ref_dsptv(binary: str, state, nu, eta, dx: float, dy: float, gam: float,
             f.write("dx %s\ndy %s\ngam %s\ndt %s\n" % (_hex(dx), _hex(dy), _hex(gam), _hex(dt)))
             f.write("nsub %d\nmode %d\nnsteps %d\ndcmax %s\n" % (nsub, mode, nsteps, _hex(dcmax)))
             f.write("dnx %s\ndny %s\n" % (" ".join(map(str, dnx)), " ".join(map(str, dny))))
-        r = subprocess.run([binary, fpar], capture_output=True, text=True, cwd=td)
+        env = dict(os.environ)
+        if threads is not None:
+            env["OMP_NUM_THREADS"] = str(threads)
+        r = subprocess.run([binary, fpar], capture_output=True, text=True, cwd=td, env=env)
         if r.returncode != 0:
             raise RuntimeError("reference dsptv harness failed (%d): %s %s" % (r.returncode, r.stdout, r.stderr))
-        return np.fromfile(fout, dtype=np.float64).reshape(state.shape)
+        out = np.fromfile(fout, dtype=np.float64).reshape(state.shape)
+    if timing:
+        el = float("nan")
+        for line in r.stdout.splitlines():
+            if line.startswith("elapsed_s"):
+                el = float(line.split()[1])
+        return out, el
+    return out
 
 
 def _hex(x: float) -> str:

@@ -19,6 +19,7 @@
  */
 #include "dmhd2d_class.hpp"
 #include "harness_common.hpp"
+#include <chrono>
 
 class RefDsp2D : public DMHD2D {
  public:
@@ -35,6 +36,7 @@ class RefDsp2D : public DMHD2D {
     if (!read_raw(p.s("nu").c_str(), pn, 1, nd)) return 2;
     if (!read_raw(p.s("eta").c_str(), pe, 1, nd)) return 2;
     dt = p.d("dt");
+    const auto t0 = std::chrono::steady_clock::now();
     if (p.i("mode") == 0) {
       const int nsub = p.i("nsub");
       for (int k = 0; k < nsub; k++) {
@@ -56,6 +58,8 @@ class RefDsp2D : public DMHD2D {
         }
       }
     }
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("elapsed_s %.9f\n", el);
     if (!write_raw(p.s("out").c_str(), val, nm, nd)) return 3;
     return 0;
   }
