@@ -85,19 +85,21 @@ def _cfl_tag(cfl: float) -> str:
     return ("%g" % cfl).replace(".", "p").replace("-", "m")
 
 
-def ref_binary_name(ndim: int, nx: int, ny: int, rmn: int, odr: int, r_k: int, ctw: int, cfl: float) -> str:
+def ref_binary_name(ndim: int, nx: int, ny: int, rmn: int, odr: int, r_k: int, ctw: int, cfl: float,
+                    gravity: bool = False) -> str:
     if ndim == 1:
         return "ref1d_%d_r%do%dk%d_c%s" % (nx, rmn, odr, r_k, _cfl_tag(cfl))
-    return "ref2d_%dx%d_r%do%dk%dw%d_c%s" % (nx, ny, rmn, odr, r_k, ctw, _cfl_tag(cfl))
+    return "ref2d%s_%dx%d_r%do%dk%dw%d_c%s" % ("g" if gravity else "", nx, ny, rmn, odr, r_k, ctw, _cfl_tag(cfl))
 
 
 def build_ref(ndim: int, xmesh: int, ymesh: int = 1, rmn: int = 1, odr: int = 4, r_k: int = 3, ctw: int = 1,
-              cfl: float = 0.4, force: bool = False) -> str:
+              cfl: float = 0.4, force: bool = False, gravity: bool = False) -> str:
     """Build (once) the reference 1D or 2D driver for one macro set.
 
-    xmesh/ymesh are the reference's XMESH/YMESH (interior cells)."""
+    xmesh/ymesh are the reference's XMESH/YMESH (interior cells); gravity
+    builds the 2D driver with gravity (2D/SERIAL/RTI) instead of OTvortex."""
     os.makedirs(REFOUT, exist_ok=True)
-    name = ref_binary_name(ndim, xmesh, ymesh, rmn, odr, r_k, ctw, cfl)
+    name = ref_binary_name(ndim, xmesh, ymesh, rmn, odr, r_k, ctw, cfl, gravity)
     out = os.path.join(REFOUT, name)
     if os.path.exists(out) and not force:
         return out
@@ -109,8 +111,10 @@ def build_ref(ndim: int, xmesh: int, ymesh: int = 1, rmn: int = 1, odr: int = 4,
         srcs = [os.path.join(HERE, "ref", "harness1d.cpp"), os.path.join(ddir, "mhd1d_class.cpp"),
                 os.path.join(ddir, "mhd1d_solve.cpp")]
     else:
-        ddir = os.path.join(REF, "2D", "SERIAL", "OTvortex")
+        ddir = os.path.join(REF, "2D", "SERIAL", "RTI" if gravity else "OTvortex")
         macros += ["-DYMESH=(%d)" % ymesh, "-DCTW=(%d)" % ctw]
+        if gravity:
+            macros += ["-DRANDOM=(0)", "-DQAS_HARNESS_GRAVITY"]
         srcs = [os.path.join(HERE, "ref", "harness2d.cpp"), os.path.join(ddir, "mhd2d_class.cpp"),
                 os.path.join(ddir, "mhd2d_solve.cpp")]
     tmp = out + ".tmp%d" % os.getpid()
@@ -186,7 +190,8 @@ def _hex(x: float) -> str:
 
 
 def run_ref(binary: str, state, dx: float, gam: float, nsteps: int, flg: int, dt: float = 0.0,
-            dnx=(0,) * 8, dny=(0,) * 8, dy: float = 1.0, threads: int | None = None, timing: bool = False):
+            dnx=(0,) * 8, dny=(0,) * 8, dy: float = 1.0, threads: int | None = None, timing: bool = False,
+            phi=None):
     """Run a reference harness on `state` (numpy (8, ...) float64, ghosts
     included).  Returns (final_state, dt_history), plus the wall time of the
     step loop in seconds when timing is set."""
@@ -197,6 +202,10 @@ def run_ref(binary: str, state, dx: float, gam: float, nsteps: int, flg: int, dt
         state.tofile(fin)
         with open(fpar, "w") as f:
             f.write("in %s\nout %s\ndthist %s\n" % (fin, fout, fdt))
+            if phi is not None:
+                fphi = os.path.join(td, "phi.bin")
+                np.ascontiguousarray(phi, dtype=np.float64).tofile(fphi)
+                f.write("phi %s\n" % fphi)
             f.write("dx %s\ndy %s\ngam %s\n" % (_hex(dx), _hex(dy), _hex(gam)))
             f.write("nsteps %d\nflg %d\ndt %s\n" % (nsteps, flg, _hex(dt)))
             f.write("dnx %s\ndny %s\n" % (" ".join(map(str, dnx)), " ".join(map(str, dny))))

@@ -23,7 +23,8 @@ class QorCfg(C.Structure):
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
                 ("gam", C.c_double), ("cfl", C.c_double),
                 ("rmn", C.c_int), ("odr", C.c_int), ("r_k", C.c_int), ("ctw", C.c_int),
-                ("dn", (C.c_int * 8) * 3), ("st", (C.c_int * 8) * 3)]
+                ("dn", (C.c_int * 8) * 3), ("st", (C.c_int * 8) * 3),
+                ("phi", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -56,9 +57,14 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def make_cfg(shape, dx, dy=1.0, dz=1.0, gam=5.0 / 3.0, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1, dn=None, st=None):
-    """shape = per-variable array shape (nx,), (ny, nx) or (nz, ny, nx)."""
+def make_cfg(shape, dx, dy=1.0, dz=1.0, gam=5.0 / 3.0, cfl=0.4, rmn=1, odr=4, r_k=3, ctw=1, dn=None, st=None,
+             phi=None):
+    """shape = per-variable array shape (nx,), (ny, nx) or (nz, ny, nx);
+    phi: 2D gravitational potential per cell (the RTI driver's phi_g)."""
     c = QorCfg()
+    if phi is not None:
+        c._phi = np.ascontiguousarray(phi, dtype=np.float64)
+        c.phi = _dp(c._phi)
     c.ndim = len(shape)
     dims = list(reversed(shape)) + [1] * (3 - len(shape))
     c.nx, c.ny, c.nz = dims

@@ -870,6 +870,25 @@ void qor_bound(const qor_cfg *c, double *const val[8])
     int dn[3] = {c->dn[0][m], c->dn[1][m], c->dn[2][m]};
     bc_array(c, val[m], st, dn);
   }
+  if (c->phi && c->ndim == 2) {
+    /* with gravity along y the energy's y ghosts hold the hydrostatic
+     * extrapolation (2D/SERIAL/RTI/mhd2d_class.cpp:10-32) */
+    const int nx = c->nx, ny = c->ny, yoff = c->yoff;
+    const double fac = (2.0 - c->gam) / (c->gam - 1.0);
+    const double *phi = c->phi, *ro = val[0];
+    double *en = val[7];
+    int i, j;
+    for (j = 0; j < yoff; j++)
+      for (i = 0; i < nx; i++) {
+        long ss = (long)nx * j + i, sb = (long)nx * yoff + i;
+        en[ss] = en[sb] - fac * ro[sb] * (phi[ss] - phi[sb]);
+      }
+    for (j = ny - yoff; j < ny; j++)
+      for (i = 0; i < nx; i++) {
+        long ss = (long)nx * j + i, sb = (long)nx * (ny - yoff - 1) + i;
+        en[ss] = en[sb] - fac * ro[sb] * (phi[ss] - phi[sb]);
+      }
+  }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1030,7 +1049,13 @@ static void step2d(const qor_cfg *c, double *const val[8], double dt)
       bc_array(c, cx, s0, dcx);
       bc_array(c, cy, s0, dcy);
     }
+    /* gravity: the potential energy is taken out of en for the pressure and
+     * put back, in place (2D/SERIAL/RTI/mhd2d_solve.cpp:86-88) */
+    if (c->phi)
+      for (long ss = 0; ss < nxy; ss++) en[ss] -= ro[ss] * c->phi[ss];
     prim_all(nxy, ro, mx, my, mz, en, cx, cy, cz, c->gam, vx, vy, vz, pr);
+    if (c->phi)
+      for (long ss = 0; ss < nxy; ss++) en[ss] += ro[ss] * c->phi[ss];
     for (long ss = 0; ss < nxy; ss++) {
       ez[ss] = 0.0;
       ct[ss] = 0.5;
@@ -1091,6 +1116,7 @@ static void step2d(const qor_cfg *c, double *const val[8], double dt)
         riemann(riem, L, R, bn, c->gam, dv, F);
         fx[nm * ss + 0] = F[0]; fx[nm * ss + 1] = F[1]; fx[nm * ss + 2] = F[2]; fx[nm * ss + 3] = F[3];
         fx[nm * ss + 4] = 0; fx[nm * ss + 5] = F[4]; fx[nm * ss + 6] = F[5]; fx[nm * ss + 7] = F[6];
+        if (c->phi) fx[nm * ss + 7] += F[0] * 0.5 * (c->phi[ss - 1] + c->phi[ss]); /* work done by gravity */
         fc[ss] = 0.5 * (ql[ss] + qr[ss]);
         fx[nm * ss + 5] -= fc[ss];
       }
@@ -1148,6 +1174,7 @@ static void step2d(const qor_cfg *c, double *const val[8], double dt)
         riemann(riem, L, R, bn, c->gam, dv, F);
         fy[nm * ss + 0] = F[0]; fy[nm * ss + 2] = F[1]; fy[nm * ss + 3] = F[2]; fy[nm * ss + 1] = F[3];
         fy[nm * ss + 5] = 0; fy[nm * ss + 6] = F[4]; fy[nm * ss + 4] = F[5]; fy[nm * ss + 7] = F[6];
+        if (c->phi) fy[nm * ss + 7] += F[0] * 0.5 * (c->phi[ss - nx] + c->phi[ss]);
         fc[ss] = 0.5 * (ql[ss] + qr[ss]);
         fy[nm * ss + 4] -= fc[ss];
       }
@@ -1184,6 +1211,11 @@ static void step2d(const qor_cfg *c, double *const val[8], double dt)
           double du0 = fdf(&fxs[1]);
           double du1 = fdf(&fys[1]);
           rk_upd(&val[v][ss], ut[v * nxy + ss], (-dtdx * du0 - dtdy * du1), rf[0], rf[1]);
+        }
+        if (c->phi) { /* gravity on the momentum, with the updated density (RTI/mhd2d_solve.cpp:361-362) */
+          const double *phi = c->phi;
+          mx[ss] += -rf[1] * ro[ss] * 0.5 * (phi[ss + 1] - phi[ss - 1]) * dtdx;
+          my[ss] += -rf[1] * ro[ss] * 0.5 * (phi[ss + nx] - phi[ss - nx]) * dtdy;
         }
       }
     /* CT: bx (one extra face in x), by (one extra face in y) */
@@ -1544,7 +1576,12 @@ double qor_cfl_dt(const qor_cfg *c, double *const val[8])
         double cz = bz[ss];
         double iro = 1.0 / ro[ss];
         double vx = mx[ss] * iro, vy = my[ss] * iro, vz = mz[ss] * iro;
-        double pr = (gam - 1) * (en[ss] - 0.5 * (ro[ss] * (vx * vx + vy * vy + vz * vz) + (cx * cx + cy * cy + cz * cz)));
+        double eng = en[ss];
+        if (c->phi) { /* the potential round trip of setdt, in place (RTI/mhd2d_class.cpp:46-48) */
+          eng = en[ss] - ro[ss] * c->phi[ss];
+          val[7][ss] = eng + ro[ss] * c->phi[ss];
+        }
+        double pr = (gam - 1) * (eng - 0.5 * (ro[ss] * (vx * vx + vy * vy + vz * vz) + (cx * cx + cy * cy + cz * cz)));
         double vf = sqrt((gam * pr + (cx * cx + cy * cy + cz * cz)) / ro[ss]);
         double v = sqrt(vx * vx + vy * vy) + vf;
         if (v > vmax) vmax = v;

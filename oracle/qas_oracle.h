@@ -30,6 +30,7 @@ typedef struct {
   int ctw;                /* CUCT upwind weight on/off          (CTW)      */
   int dn[3][8];           /* dnxs/dnys/dnzs boundary flags                 */
   int st[3][8];           /* stxs/stys/stzs staggering flags               */
+  const double *phi;      /* 2D gravitational potential per cell, or NULL  */
 } qor_cfg;
 
 /* bound(val): ghost fill of the 8 variables (bc1d/bc2d/bc3d).               */

@@ -29,6 +29,12 @@ class RefRun2D : public MHD2D {
     p.ints("dnx", dnxs, nm);
     p.ints("dny", dnys, nm);
     if (!read_raw(p.s("in").c_str(), val, nm, nd)) return 2;
+#ifdef QAS_HARNESS_GRAVITY
+    {  // the RTI driver's gravitational potential (allocated by its MHD2D())
+      double *const pg[1] = {phi_g};
+      if (!read_raw(p.s("phi").c_str(), pg, 1, nd)) return 2;
+    }
+#endif
     const int nsteps = p.i("nsteps");
     const int flg = p.i("flg");
     std::vector<double> hist;

@@ -458,3 +458,69 @@ def reconnection2d(nx: int = 1024, ny: int = 128, gam: float = 5.0 / 3.0, beta: 
     p = Problem(state=st, dx=dx, dy=dy, gam=gam, cfl=0.4, rmn=2, odr=2, r_k=3, ctw=1, dn=dn,
                 name="mrx_%dx%d" % (nx, ny))
     return p, nu, eta, dcmax
+
+
+# ---------------------------------------------------------------- 2D Rayleigh-Taylor
+def _sqrwave2(val_u, val_l, dx_u, dx_l):
+    return 0.5 * (2.0 + np.tanh(dx_u) - np.tanh(dx_l)) * (val_u - val_l) + val_l
+
+
+def rayleigh_taylor2d(nx: int = 128, ny: int = 1024, gam: float = 5.0 / 3.0, beta: float = 1e3, ro_u: float = 1.0,
+                      ro_l: float = 0.1, pr0: float = 500.0, g0: float = 1.0, dv: float = 0.01, lam: float = 1.0,
+                      xlen: float = 20.0):
+    """Magnetised Rayleigh-Taylor set-up with gravity along y, as
+    2D/SERIAL/RTI/mhd2d_init_.cpp:8-130 and mhd2d_paras.cpp:5-40: x in
+    [0, xlen], y in the aspect ny/nx around 0; heavy fluid (ro_u) above light
+    (ro_l) at y = +-0.35 Ly (tanh layers of width lam); potential
+    phi = g0 lg log(cosh(y/lg)), lg = 8 dy; pressure from the hydrostatic
+    balance integrated by the reference's midpoint rule (floor 1e-4); uniform
+    Bz = sqrt(2 pr0/beta); single-mode vy perturbation of amplitude dv.
+    Energy includes ro phi.  Boundaries: periodic in x, open (+2) in y.
+    Returns (Problem, phi)."""
+    xmin, xmax = 0.0, xlen
+    ymin = -0.5 * (xmax - xmin) * ny / nx
+    ymax = -ymin
+    dx = (xmax - xmin) / nx
+    dy = (ymax - ymin) / ny
+    Nx, Ny = nx + 2 * NG, ny + 2 * NG
+    x = (np.arange(Nx) - NG + 0.5) * dx + xmin
+    y = (np.arange(Ny) - NG + 0.5) * dy + ymin
+    ly = ymax - ymin
+    s0 = 0.35 * ly
+    lg = 8 * dy
+    b0 = math.sqrt(2.0 * pr0 / beta)
+    wlen = (xmax - xmin) / 1
+    dvy = dv * np.cos(2 * math.pi * x / wlen)
+    ro1 = _sqrwave2(ro_u, ro_l, (y - s0) / lam, (y + s0) / lam)
+    phi1 = g0 * lg * np.log(np.cosh(y / lg))
+    pr1 = np.empty(Ny)
+    for j in range(Ny):  # cal_pressure, mhd2d_init_.cpp:105-120
+        n = 1 + int(abs(y[j]) / dy + 0.5)
+        dyc = (y[j] - 0.0) / n
+        ans = pr0
+        for k in range(n):
+            yc = 0.0 + (k + 0.5) * dyc
+            roc = _sqrwave2(ro_u, ro_l, (yc - s0) / lam, (yc + s0) / lam)
+            ans += -roc * (g0 * math.tanh(yc / lg)) * dyc
+        pr1[j] = max(ans, 1e-4)
+    X, Y = np.meshgrid(x, y)
+    ro = np.repeat(ro1[:, None], Nx, axis=1)
+    vy = dvy[None, :] * (np.exp(-((Y - s0) ** 2) / (4 * lam * lam)) - np.exp(-((Y + s0) ** 2) / (4 * lam * lam)))
+    pr = np.repeat(pr1[:, None], Nx, axis=1)
+    phi = np.repeat(phi1[:, None], Nx, axis=1)
+    z = np.zeros_like(X)
+    bz = np.full_like(X, b0)
+    st = np.zeros((8, Ny, Nx))
+    st[0] = ro
+    st[1] = ro * z
+    st[2] = ro * vy
+    st[3] = ro * z
+    st[4] = z
+    st[5] = z
+    st[6] = bz
+    st[7] = energy(ro, z, vy, z, z, z, bz, pr, gam) + ro * phi
+    dn = np.zeros((3, 8), dtype=int)
+    dn[1] = 2
+    p = Problem(state=st, dx=dx, dy=dy, gam=gam, cfl=0.4, rmn=2, odr=4, r_k=3, ctw=1, dn=dn,
+                name="rti_%dx%d" % (nx, ny))
+    return p, phi
