@@ -133,6 +133,17 @@ long long qas_device_bytes(const qas_ctx *c);
 int qas_set_dissipation(qas_ctx *c, const double *nu, const double *eta, double dcmax);
 int qas_dsptv(qas_ctx *c, double dt);
 
+/* ---- gravity (2D, FP64) -------------------------------------------------
+ * qas_set_gravity gives the step the gravitational potential of the
+ * reference's RTI driver (common/mhd_class.hpp:39 phi_g, filled by
+ * 2D/SERIAL/RTI/mhd2d_init_.cpp:55): nx*ny doubles, ghosts included, host
+ * memory; the state's energy includes ro phi.  With it, ideal(), bound()
+ * and setdt() are those of 2D/SERIAL/RTI/mhd2d_{solve,class}.cpp: potential
+ * round trips around the pressure recovery, work done by gravity in the
+ * energy fluxes, momentum sources, hydrostatic energy ghosts along y.
+ * phi = NULL switches gravity off. */
+int qas_set_gravity(qas_ctx *c, const double *phi);
+
 
 /* ---- batched domains (no counterpart in the reference) -----------------
  * Advances many independent 2D domains of uneven sizes with one launch per

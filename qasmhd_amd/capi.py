@@ -40,7 +40,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_batch_create", "qas_batch_run", "qas_batch_last_error", "qas_batch_destroy",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct",
-           "qas_set_dissipation", "qas_dsptv"]
+           "qas_set_dissipation", "qas_dsptv", "qas_set_gravity"]
 
 _lib = None
 
@@ -98,6 +98,7 @@ def load(path: str | None = None):
     l.qas_test_cuct.argtypes = [C.c_int, dp, dp, dp]
     l.qas_set_dissipation.argtypes = [vp, dp, dp, C.c_double]
     l.qas_dsptv.argtypes = [vp, C.c_double]
+    l.qas_set_gravity.argtypes = [vp, dp]
     _lib = l
     return l
 
@@ -298,6 +299,16 @@ class Solver:
         assert self._nu.shape == self.shape and self._eta.shape == self.shape
         self._check(self.lib.qas_set_dissipation(self.ctx, _p(self._nu), _p(self._eta), float(dcmax)),
                     "qas_set_dissipation")
+
+    def set_gravity(self, phi):
+        """Gravitational potential per cell (domain shape, ghosts included);
+        None switches gravity off."""
+        if phi is None:
+            self._check(self.lib.qas_set_gravity(self.ctx, None), "qas_set_gravity")
+            return
+        self._phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert self._phi.shape == self.shape
+        self._check(self.lib.qas_set_gravity(self.ctx, _p(self._phi)), "qas_set_gravity")
 
     def dsptv(self, dt):
         """One visco-resistive step dsptv(dt)."""

@@ -705,6 +705,7 @@ static std::string adj_supported(const qas_ctx* c)
   if (c->s.rmn != 1) return "the adjoint covers the HLLD solver (rmn 1) only";
   if (c->s.odr != 4) return "the adjoint covers odr 4 only";
   if (c->s.r_k != 3) return "the adjoint covers r_k 3 only";
+  if (c->d_phi || c->d_nu) return "the adjoint covers the ideal step without gravity or dissipation";
   for (int a = 0; a < 2; a++)
     for (int m = 0; m < 8; m++)
       if (c->s.dn[a][m] != 0) return "the adjoint covers periodic boundaries only";

@@ -279,6 +279,8 @@ extern "C" int qas_batch_create(qas_batch** out, qas_ctx* const* ctxs, int n)
     const qas_scheme& s = c->s;
     if (c->g.ndim != 2) return bfail(b, QAS_E_ARG, "qas_batch_create: batches hold 2D domains");
     if (c->device != dev0) return bfail(b, QAS_E_ARG, "qas_batch_create: all domains on one device");
+    if (c->d_phi || c->d_nu)
+      return bfail(b, QAS_E_ARG, "qas_batch_create: batches run the ideal step (no gravity, no dissipation)");
     if (s.rmn != b->s.rmn || s.odr != b->s.odr || s.r_k != b->s.r_k || s.ctw != b->s.ctw ||
         s.precision != b->s.precision || s.gam != b->s.gam)
       return bfail(b, QAS_E_ARG, "qas_batch_create: every domain needs the same rmn, odr, r_k, ctw, precision, gam");

@@ -54,5 +54,8 @@ batch_fn pick_batch_sweep2d_y(int rmn, int odr);
 batch_fn pick_batch_sweep2d_y_f32(int rmn, int odr);
 xsweep2d_fn pick_xsweep2d(int rmn, int odr, bool batch);
 xsweep2d_fn pick_xsweep2d_f32(int rmn, int odr, bool batch);
+// gravity instantiations (qas_gravity.hip)
+xsweep2d_fn pick_grav_xsweep(int rmn, int odr);
+void (*pick_grav_ysweep(int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
 
 }  // namespace qas

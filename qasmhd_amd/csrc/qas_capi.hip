@@ -169,6 +169,7 @@ extern "C" void qas_destroy(qas_ctx* c)
   if (c->stream) hipStreamSynchronize(c->stream);
   adj_release(c);
   dsp_release(c);
+  grav_release(c);
   if (c->slab) hipFree(c->slab);
   if (c->d_hist) hipFree(c->d_hist);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -292,6 +293,8 @@ static int do_step(qas_ctx* c)
     for (int k = 0; k < 12; k++) p.emf[k] = c->emf[k];
     p.ex = c->ex; p.ey = c->ey; p.ez = c->ez;
     p.dt = c->d_dt;
+    p.phi = c->d_phi;
+    p.enrt = c->d_enrt;
     rk_coef(R, rk, &p.a, &p.b);
     hipError_t e;
     if (c->g.ndim == 1) e = launch_stage1d(c, p);
@@ -332,7 +335,10 @@ extern "C" int qas_cfl_dt(qas_ctx* c, double* dt_out)
 static int loop_step(qas_ctx* c, int flg, bool hist, bool even)
 {
   if (hist) QAS_CHECK(c, launch_record_counted(c));
-  if (!c->bounded) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+  // exec_ bounds every step; the bound is skipped when nothing touched the
+  // state since the last one -- not with gravity, whose setdt rewrites the
+  // interior energy (so a captured step pair always carries it)
+  if (!c->bounded || c->d_phi) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
   int r = do_step(c);
   if (r) return r;
   // DMHD2D::exec_: the sub-cycled dissipative step after ideal(dt)

@@ -34,6 +34,10 @@ hipError_t launch_bound(qas_ctx* c, double* const f[8], int nvar)
   BoundArgs a;
   for (int m = 0; m < 8; m++) a.f[m] = f[m];
   a.nvar = nvar;
+  if (c->d_phi && nvar == 8) {
+    a.phi = c->d_phi;
+    a.gfac = (2.0 - c->s.gam) / (c->s.gam - 1.0);
+  }
   const long long tot = nghost * nvar;
   QAS_LAUNCH(c, QAS_K_BOUND, k_bound, (unsigned)((tot + 255) / 256), 256, 0, c->stream, a, g, c->bc, nghost, ox, oy, oz);
   return hipGetLastError();
@@ -68,6 +72,11 @@ hipError_t launch_cfl(qas_ctx* c, const double* const U[8])
   a.stx4 = c->s.st[0][4];
   a.sty5 = c->s.st[1][5];
   a.stz6 = c->s.st[2][6];
+  if (c->d_phi) {
+    a.phi = c->d_phi;
+    a.enw = const_cast<double*>(U[7]);
+    c->bounded = false;  // the interior energy is rewritten: its ghosts are stale
+  }
   const int ox = qas::NG, oy = g.ndim >= 2 ? qas::NG : 0, oz = g.ndim >= 3 ? qas::NG : 0;
   const long long n = (long long)(g.nx - 2 * ox) * (g.ny - 2 * oy) * (g.nz - 2 * oz);
   long long nb = (n + 255) / 256;

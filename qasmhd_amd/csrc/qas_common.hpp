@@ -10,6 +10,8 @@ namespace qas {
 struct BoundArgs {
   double* f[8];
   int nvar;
+  const double* phi = nullptr;  // 2D gravity: hydrostatic energy ghosts along y
+  double gfac = 0.0;            // (2 - gam) / (gam - 1)
 };
 
 __device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const BcFlags& bc, long long nghost,
@@ -45,6 +47,20 @@ __device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const
     j = oy + (int)(line % (ny - 2 * oy));
     k = oz + (int)(line / (ny - 2 * oy));
   }
+  if (a.phi && m == 7 && a.nvar == 8 && oz == 0 && (j < oy || j >= ny - oy)) {
+    // with gravity along y the energy's y ghosts (x ghosts included) are
+    // the hydrostatic extrapolation from the first / last interior row, as
+    // 2D/SERIAL/RTI/mhd2d_class.cpp:10-32 sets them after bc2d; that row's
+    // ro and en are taken through their x boundary maps
+    const int jb = j < oy ? oy : (int)ny - oy - 1;
+    const Src ex = resolve(i, g.nx, bc.st[0][7], bc.dn[0][7]);
+    const Src rx = resolve(i, g.nx, bc.st[0][0], bc.dn[0][0]);
+    const double enb = apply(ex, a.f[7][nx * jb + ex.i]);
+    const double rob = apply(rx, a.f[0][nx * jb + rx.i]);
+    const long long ss = nx * j + i, sb = nx * jb + i;
+    a.f[7][ss] = enb - a.gfac * rob * (a.phi[ss] - a.phi[sb]);
+    return;
+  }
   const Src sx = ox ? resolve(i, g.nx, bc.st[0][m], bc.dn[0][m]) : Src{i, 1.0, false};
   const Src sy = oy ? resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]) : Src{j, 1.0, false};
   const Src sz = oz ? resolve(k, g.nz, bc.st[2][m], bc.dn[2][m]) : Src{k, 1.0, false};
@@ -62,6 +78,8 @@ struct CflArgs {
   double* part;  // one maximum per block
   double gam;
   int stx4, sty5, stz6;
+  const double* phi = nullptr;  // 2D gravity: potential round trip of the energy
+  double* enw = nullptr;        // the energy array it is written back to (u[7])
 };
 
 __device__ __forceinline__ double wave_max(double v)
@@ -123,7 +141,13 @@ __device__ __forceinline__ void cfl_block(const CflArgs& a, const Geo& g, int bl
     const double vz = a.u[3][ss] * iro;
     const double v2 = vx * vx + vy * vy + vz * vz;
     const double b2 = cx * cx + cy * cy + cz * cz;
-    const double pr = (gam - 1) * (a.u[7][ss] - 0.5 * (ro * v2 + b2));
+    double en = a.u[7][ss];
+    if (a.phi) {  // setdt's potential round trip, in place (RTI/mhd2d_class.cpp:46-48)
+      const double rp = ro * a.phi[ss];
+      en = en - rp;
+      a.enw[ss] = en + rp;
+    }
+    const double pr = (gam - 1) * (en - 0.5 * (ro * v2 + b2));
     const double vf = sqrt((gam * pr + (cx * cx + cy * cy + cz * cz)) / ro);
     double v;
     if (g.ndim == 1) v = fabs(vx) + vf;

@@ -41,6 +41,8 @@ struct StagePtrs {
   double *ex, *ey, *ez;  // edge EMFs
   const double* dt;      // device dt
   double a, b;           // RK stage coefficients (rk_fac)
+  const double* phi;     // 2D gravitational potential per cell (gravity runs only)
+  double* enrt;          // the stage energy after the potential round trip (gravity)
 };
 
 // The primitive scratch as the sweeps read it: element type PT = the
@@ -176,6 +178,9 @@ struct qas_ctx {
   // resistivity (one allocation, eta follows nu) and max(2 nu, eta)
   double *d_nu = nullptr, *d_eta = nullptr;
   double dcmax = 0.0;
+  // gravity (2D): potential per cell and the stage energy after its round
+  // trip (one allocation, d_enrt follows d_phi)
+  double *d_phi = nullptr, *d_enrt = nullptr;
 };
 
 namespace qas {
@@ -216,6 +221,8 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
 void adj_release(qas_ctx* c);  // frees the adjoint workspace (qas_adjoint2d.hip)
 // dissipative step (qas_dissipative.hip)
 void dsp_release(qas_ctx* c);
+// gravity (qas_gravity.hip)
+void grav_release(qas_ctx* c);
 int dsp_step(qas_ctx* c, double dt);
 int dsp_after_ideal(qas_ctx* c);
 // FP32 face sweeps of the mixed path (qas_mixed.hip)

@@ -87,11 +87,18 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   const bool mixed = c->s.precision == 1;
   xsweep2d_fn fx = mixed ? pick_xsweep2d_f32(c->s.rmn, odr, false) : pick_xsweep2d(c->s.rmn, odr, false);
   sweep_fn fy = mixed ? pick_sweep_f32(2, 1, c->s.rmn, odr) : pick_sweep<1>(c->s.rmn, odr);
+  // with gravity the sweeps take the round-tripped stage energy
+  StagePtrs ps = p;
+  if (p.phi) {
+    fx = pick_grav_xsweep(c->s.rmn, odr);
+    fy = pick_grav_ysweep(c->s.rmn, odr);
+    ps.in[7] = p.enrt;
+  }
   if (!fx || !fy) return hipErrorInvalidValue;
   // y sweep first (one 256-thread workgroup per (4-column group, segment)):
   // it leaves dt/dy * Dy in the accumulators; the x sweep applies the stage sum
-  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, p, g, c->bc, gam, nsy);
-  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, p, g, c->bc, gam, nsx,
+  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, ps, g, c->bc, gam, nsy);
+  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   // 4) corner EMF, 5) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);

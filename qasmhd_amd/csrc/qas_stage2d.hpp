@@ -44,7 +44,15 @@ __device__ __forceinline__ void prim2d_at(const StagePtrs& p, const Geo& g, cons
   const double cz = p.in[6][t];
   const double v2 = vx * vx + vy * vy + vz * vz;
   const double b2 = cx * cx + cy * cy + cz * cz;
-  store_prim<PT>(p.pr, t, (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2)));
+  double en = p.in[7][t];
+  if (p.phi) {
+    // gravity: the potential energy is taken out for the pressure and put
+    // back in the stage's energy (2D/SERIAL/RTI/mhd2d_solve.cpp:86-88)
+    const double rp = ro * p.phi[t];
+    en = en - rp;
+    p.enrt[t] = en + rp;
+  }
+  store_prim<PT>(p.pr, t, (gam - 1) * (en - 0.5 * (ro * v2 + b2)));
   store_prim<PT>(p.vx, t, vx);
   store_prim<PT>(p.vy, t, vy);
   store_prim<PT>(p.vz, t, vz);

@@ -40,7 +40,7 @@ constexpr int YC = 4;          // columns per workgroup
 constexpr int YR = WAVE + 4;   // stencil rows per segment
 constexpr int YRP = YR + 1;    // padded LDS column length
 
-template <int RMN, int ODR, typename T>
+template <int RMN, int ODR, typename T, bool GRAV = false>
 __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam, int nseg,
                                           int L)
 {
@@ -129,6 +129,10 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
   // y: (F0, F3, F1, F2, F4, F6) feed (ro, mx, my, mz, bz, en)
   double fv[6];
   fv[0] = F[0]; fv[1] = F[3]; fv[2] = F[1]; fv[3] = F[2]; fv[4] = F[4]; fv[5] = F[6];
+  if constexpr (GRAV) {  // work done by gravity (2D/SERIAL/RTI/mhd2d_solve.cpp:305)
+    const long long lc = line > g.nx - 1 ? g.nx - 1 : line;
+    fv[5] += F[0] * 0.5 * (p.phi[lc + sy * (sf - 1)] + p.phi[lc + sy * sf]);
+  }
 #pragma unroll
   for (int v = 0; v < 6; v++) {
     const double fm1 = __shfl_up(fv[v], 1, WAVE);
@@ -164,11 +168,11 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
 
 // y faces for every column (grid: 4-column groups x segments)
 // FP32 (mixed) instantiation: 4 waves per SIMD fit without spills
-template <int RMN, int ODR, typename T>
+template <int RMN, int ODR, typename T, bool GRAV = false>
 __global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void k_sweep2d_y(StagePtrs p, Geo g, BcFlags bc,
                                                                              double gam, int nseg)
 {
-  sweep2d_y<RMN, ODR, T>(p, g, bc, gam, nseg, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  sweep2d_y<RMN, ODR, T, GRAV>(p, g, bc, gam, nseg, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // ------------------------------------------------------------------ x sweep
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void k_sweep2d_y(Sta
 // from thin kernels it compiled to 246 VGPRs (2 waves per SIMD) instead of
 // 158 (3 waves) for FP64 HLLD / ODR 4.  The FP64 Roe and MLAU instances ask
 // for 2 waves per SIMD, which they otherwise miss by a few registers.
-template <int DIR, int RMN, int ODR, typename T, bool BATCH>
+template <int DIR, int RMN, int ODR, typename T, bool BATCH, bool GRAV = false>
 __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_sweep2d(StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam,
                                                  int nseg_arg, const BatchInst* I, const int* off, int ninst,
                                                  int stage)
@@ -296,6 +300,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
   } else {
     fv[0] = F[0]; fv[1] = F[3]; fv[2] = F[1]; fv[3] = F[2]; fv[4] = F[4]; fv[5] = F[6];
   }
+  if constexpr (GRAV) fv[5] += F[0] * 0.5 * (p.phi[sf - o] + p.phi[sf]);  // work done by gravity (RTI:197)
   double du[6];
 #pragma unroll
   for (int v = 0; v < 6; v++) {
@@ -315,11 +320,20 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
   const long long c = base + o * q;
   const double dtdx = *p.dt / g.dx;
   const int vid[6] = {0, 1, 2, 3, 6, 7};
+  double un[6];
 #pragma unroll
   for (int v = 0; v < 6; v++) {
     const int m = vid[v];
-    p.out[m][c] = rk_upd(p.in[m][c], p.u0[m][c], (-dtdx * du[v]) - load_as<T>(p.acc[v], c), p.a, p.b);
+    un[v] = rk_upd(p.in[m][c], p.u0[m][c], (-dtdx * du[v]) - load_as<T>(p.acc[v], c), p.a, p.b);
   }
+  if constexpr (GRAV) {
+    // gravity on the momentum with the updated density (RTI/mhd2d_solve.cpp:361-362)
+    const double dtdy = *p.dt / g.dy;
+    un[1] += -p.b * un[0] * 0.5 * (p.phi[c + 1] - p.phi[c - 1]) * dtdx;
+    un[2] += -p.b * un[0] * 0.5 * (p.phi[c + g.sy] - p.phi[c - g.sy]) * dtdy;
+  }
+#pragma unroll
+  for (int v = 0; v < 6; v++) p.out[vid[v]][c] = un[v];
 }
 
 // batched y sweep (qas_batch.hpp): one launch over every instance
