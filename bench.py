@@ -24,6 +24,8 @@ Also reported, on one JSON line:
   dissipative   2D reconnection 4096 x 512 with localised viscosity and
                 resistivity: ideal step plus sub-cycled visco-resistive step,
                 one dissipative call alone, and the reference's own one
+  gravity       2D Rayleigh-Taylor 512 x 4096 with gravity, and the
+                reference's own RTI driver on this host
   inverse       64 batched instances of the velocity-mode inversion: the
                 batched forward, the adjoint gradient of all 512 amplitudes
                 and a finite-difference check
@@ -357,6 +359,48 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
                     "adjoints per step) + host copies; forward = one batched run of all instances"}
 
 
+def gravity_line(nx, ny, steps, cpu=True):
+    """2D magnetised Rayleigh-Taylor with gravity along y (the reference's
+    RTI set-up: LHLLD, WCNS-4, RK3, CUCT, heavy over light fluid) at
+    nx x ny: zone-cycles/s of exec_ steps with the gravity terms, and the
+    reference's own RTI driver on this host over a bounded sample."""
+    from qasmhd_amd import capi, problems
+    p, phi = problems.rayleigh_taylor2d(nx=nx, ny=ny)
+    zones = nx * ny
+    with capi.solver_for(p) as s:
+        s.upload(p.state)
+        s.set_gravity(phi)
+        s.run(2, flg=1, history=False)  # warm-up (graph capture)
+        s.upload(p.state)
+        s.reset_counter()
+        s.sync()
+        t0 = time.perf_counter()
+        _, dt_now = s.run(steps, flg=1, history=False)
+        s.sync()
+        el = time.perf_counter() - t0
+        finite = bool(np.isfinite(s.download()).all())
+    out = {"workload": "2d_rayleigh_taylor_%dx%d_lhlld_wcns4_ct_rk3_gravity" % (nx, ny), "steps": steps,
+           "value": zones * steps / el, "unit": "zone-cycles/s", "ms_per_step": el / steps * 1e3,
+           "dt_final": dt_now, "finite": finite}
+    if cpu:
+        try:
+            from oracle import build as obuild
+            binary = os.path.join(obuild.REFOUT, obuild.ref_binary_name(2, nx, ny, p.rmn, p.odr, p.r_k, p.ctw, p.cfl,
+                                                                        gravity=True))
+            if os.path.exists(binary):
+                threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+                k = 4
+                _, _, el_ref = obuild.run_ref(binary, p.state, p.dx, p.gam, k, 1, dnx=list(p.dn[0]),
+                                              dny=list(p.dn[1]), dy=p.dy, threads=threads, timing=True, phi=phi)
+                out["cpu_reference"] = {"value": zones * k / el_ref, "unit": "zone-cycles/s", "cores": threads,
+                                        "kind": "reference",
+                                        "sample": "first %d steps of the same run (reference RTI MHD2D, g++ -O2 "
+                                                  "-fopenmp), %.2f s" % (k, el_ref)}
+        except Exception as e:  # reported, not required
+            _log("bench: reference RTI timing failed: %s" % e)
+    return out
+
+
 def dissipative_line(nx, ny, steps, cpu=True):
     """2D Harris-sheet reconnection with localised viscosity and resistivity
     (the reference's dissipative MRX set-up: LHLLD, ODR 2, RK3, CUCT,
@@ -501,6 +545,9 @@ def main():
     ap.add_argument("--dsp-nx", type=int, default=4096, help="dissipative line size along x (0: skip it)")
     ap.add_argument("--dsp-ny", type=int, default=512)
     ap.add_argument("--dsp-steps", type=int, default=100)
+    ap.add_argument("--grav-nx", type=int, default=512, help="gravity line size along x (0: skip it)")
+    ap.add_argument("--grav-ny", type=int, default=4096)
+    ap.add_argument("--grav-steps", type=int, default=100)
     ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--turb3d-n", type=int, default=512, help="3D turbulence line size (0: skip it)")
     ap.add_argument("--turb3d-steps", type=int, default=20)
@@ -596,6 +643,13 @@ def main():
                                                         cpu=not args.no_cpu)
             except Exception as e:
                 _log("bench: dissipative line failed: %s" % e)
+        if args.grav_nx > 0:
+            try:
+                _log("bench: gravity (Rayleigh-Taylor), %dx%d, %d steps" % (args.grav_nx, args.grav_ny,
+                                                                           args.grav_steps))
+                extra["gravity"] = gravity_line(args.grav_nx, args.grav_ny, args.grav_steps, cpu=not args.no_cpu)
+            except Exception as e:
+                _log("bench: gravity line failed: %s" % e)
         if args.inverse_n > 0:
             try:
                 _log("bench: velocity-mode inversion, %d instances x %d steps" % (args.inverse_n, args.inverse_steps))
