@@ -134,6 +134,136 @@ __global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
   p.ez[ss] = ez;
 }
 
+// The same, marching along z: one thread per (i, j) edge column of a chunk
+// of EMF_KC planes.  The four z-interpolated face-part arrays (Ey's and
+// Ex's contributions from the x and y sweeps' partner sweeps along z) are
+// kept as 6-point register windows shifted one plane per step, and the cell
+// values of plane k-1 the CUCT weights of Ex and Ey need are carried from
+// the previous step, so each is read from memory once per point instead of
+// 6 (resp. 2) times from planes far apart.  Same values, same expressions:
+// bit-identical to k_emf3d.
+constexpr int EMF_KC = 32;     // planes per thread
+constexpr int EMF_TX = 64;     // tile: 64 x 4 columns per 256-thread workgroup
+constexpr int EMF_TY = 4;
+
+template <int ODR>
+__device__ __forceinline__ double emf_part_w(const double (&u)[6], const double (&c)[6], double w, double sgn)
+{
+  double a[5], b[5], ul0, ur0, ul1, ur1, dum;
+#pragma unroll
+  for (int m = 0; m < 5; m++) { a[m] = u[m]; b[m] = c[m]; }
+  lin5<ODR>(a, ul0, dum);
+  lin5<ODR>(b, ul1, dum);
+#pragma unroll
+  for (int m = 0; m < 5; m++) { a[m] = u[m + 1]; b[m] = c[m + 1]; }
+  lin5<ODR>(a, dum, ur0);
+  lin5<ODR>(b, dum, ur1);
+  return sgn * ((ul0 + ur0) + w * (ul1 + ur1));
+}
+
+template <int ODR, typename FT>
+__global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
+{
+  const int ni = g.nx - 5, nj = g.ny - 5, nk = g.nz - 5;  // edges on [3, n-3]^3
+  const int ntx = (ni + EMF_TX - 1) / EMF_TX, nty = (nj + EMF_TY - 1) / EMF_TY;
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int tile = L % (ntx * nty), chunk = L / (ntx * nty);
+  const int i = 3 + (tile % ntx) * EMF_TX + (int)(threadIdx.x % EMF_TX);
+  const int j = 3 + (tile / ntx) * EMF_TY + (int)(threadIdx.x / EMF_TX);
+  const int k0 = 3 + chunk * EMF_KC;
+  if (i > g.nx - 3 || j > g.ny - 3 || k0 > g.nz - 3) return;  // no barriers below
+  const int k1 = (k0 + EMF_KC < 3 + nk) ? k0 + EMF_KC : 3 + nk;
+  const long long sy = g.sy, sz = g.sz;
+  const double* ro = p.in[0];
+  const double *mx = p.in[1], *my = p.in[2], *mz = p.in[3];
+  const double *bx = p.in[4], *by = p.in[5], *bz = p.in[6];
+  auto vel = [&](const double* m, long long c) { return m[c] * (1.0 / ro[c]); };
+  // z windows at planes k-3 .. k+2
+  double u2[6], c2[6], u4[6], c4[6];
+  {
+    const long long s0 = sz * k0 + sy * j + i;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const long long a = s0 + (m - 3) * sz;
+      u2[m] = load_as<FT>(p.emf[2], a);
+      c2[m] = load_as<FT>(p.emf[3], a);
+      u4[m] = load_as<FT>(p.emf[4], a);
+      c4[m] = load_as<FT>(p.emf[5], a);
+    }
+  }
+  // plane k0-1 cell values used by the Ex / Ey weights
+  double ro_p = 0, ro_pi = 0, ro_pj = 0, vx_p = 0, vx_pi = 0, vy_p = 0, vy_pj = 0, vz_p = 0, vz_pi = 0, vz_pj = 0,
+         bx_p = 0, by_p = 0;
+  if (ctw) {
+    const long long s = sz * (k0 - 1) + sy * j + i;
+    ro_p = ro[s]; ro_pi = ro[s - 1]; ro_pj = ro[s - sy];
+    vx_p = vel(mx, s); vx_pi = vel(mx, s - 1);
+    vy_p = vel(my, s); vy_pj = vel(my, s - sy);
+    vz_p = vel(mz, s); vz_pi = vel(mz, s - 1); vz_pj = vel(mz, s - sy);
+    bx_p = bx[s]; by_p = by[s];
+  }
+  for (int k = k0; k < k1; k++) {
+    const long long ss = sz * k + sy * j + i;
+    {
+      const long long a = ss + 2 * sz;
+      u2[5] = load_as<FT>(p.emf[2], a);
+      c2[5] = load_as<FT>(p.emf[3], a);
+      u4[5] = load_as<FT>(p.emf[4], a);
+      c4[5] = load_as<FT>(p.emf[5], a);
+    }
+    double ctz = 0.5, ctx = 0.5, cty = 0.5;
+    double ro0 = 0, roi = 0, roj = 0, vx0 = 0, vxi = 0, vy0 = 0, vyj = 0, vz0 = 0, vzi = 0, vzj = 0, bx0 = 0,
+           by0 = 0;
+    if (ctw) {
+      ro0 = ro[ss]; roi = ro[ss - 1]; roj = ro[ss - sy];
+      const double roij = ro[ss - 1 - sy];
+      vx0 = vel(mx, ss); vxi = vel(mx, ss - 1);
+      const double vxj = vel(mx, ss - sy), vxij = vel(mx, ss - 1 - sy);
+      vy0 = vel(my, ss); vyj = vel(my, ss - sy);
+      const double vyi = vel(my, ss - 1), vyij = vel(my, ss - 1 - sy);
+      vz0 = vel(mz, ss); vzi = vel(mz, ss - 1); vzj = vel(mz, ss - sy);
+      bx0 = bx[ss]; by0 = by[ss];
+      // x-y plane for Ez: strides (1, ny)
+      ctz = cuct_weight(roij, roj, roi, ro0, vxij, vxj, vxi, vx0, vyij, vyj, vyi, vy0, bx[ss - sy], bx0, by[ss - 1],
+                        by0);
+      // y-z plane for Ex: strides (sy, sz)
+      ctx = cuct_weight(ro_pj, ro_p, roj, ro0, vy_pj, vy_p, vyj, vy0, vz_pj, vz_p, vzj, vz0, by_p, by0, bz[ss - sy],
+                        bz[ss]);
+      // z-x plane for Ey: strides (sz, 1)
+      cty = cuct_weight(ro_pi, roi, ro_p, ro0, vz_pi, vzi, vz_p, vz0, vx_pi, vxi, vx_p, vx0, bz[ss - 1], bz[ss], bx_p,
+                        bx0);
+    }
+    // Ez: x sweep B_y (t) along y, then y sweep B_x (u) along x
+    double ez = 0.0;
+    ez += emf_part<ODR, FT>(p.emf[0], p.emf[1], ss, sy, 1.0 - ctz, -0.5);
+    ez += emf_part<ODR, FT>(p.emf[6], p.emf[7], ss, 1, ctz, +0.5);
+    // Ey: x sweep B_z (u) along z, then z sweep B_x (t) along x
+    double ey = 0.0;
+    ey += emf_part_w<ODR>(u2, c2, cty, +0.5);
+    ey += emf_part<ODR, FT>(p.emf[8], p.emf[9], ss, 1, 1.0 - cty, -0.5);
+    // Ex: y sweep B_z (t) along z, then z sweep B_y (u) along y
+    double ex = 0.0;
+    ex += emf_part_w<ODR>(u4, c4, 1.0 - ctx, -0.5);
+    ex += emf_part<ODR, FT>(p.emf[10], p.emf[11], ss, sy, ctx, +0.5);
+    p.ex[ss] = ex;
+    p.ey[ss] = ey;
+    p.ez[ss] = ez;
+    // next plane: shift the windows, carry this plane's values
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      u2[m] = u2[m + 1];
+      c2[m] = c2[m + 1];
+      u4[m] = u4[m + 1];
+      c4[m] = c4[m + 1];
+    }
+    ro_p = ro0; ro_pi = roi; ro_pj = roj;
+    vx_p = vx0; vx_pi = vxi;
+    vy_p = vy0; vy_pj = vyj;
+    vz_p = vz0; vz_pi = vzi; vz_pj = vzj;
+    bx_p = bx0; by_p = by0;
+  }
+}
+
 // ------------------------------------------------------------------ CT
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
@@ -220,9 +350,12 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5) * (g.nz - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7) * (g.nz - 7);
   // (the face parts are stored in the sweep precision)
+  const long long nemf = (long long)((g.nx - 5 + EMF_TX - 1) / EMF_TX) * ((g.ny - 5 + EMF_TY - 1) / EMF_TY) *
+                         ((g.nz - 5 + EMF_KC - 1) / EMF_KC);
+  (void)ne;
 #define QAS_EMF_CT(O)                                                                                      \
-  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<O, float>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);     \
-  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d<O, double>), nblk3(ne, 256), 256, 0, st, p, g, c->s.ctw);         \
+  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d_z<O, float>), (unsigned)nemf, 256, 0, st, p, g, c->s.ctw);   \
+  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d_z<O, double>), (unsigned)nemf, 256, 0, st, p, g, c->s.ctw);       \
   QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O>), nblk3(nc, 256), 256, 0, st, p, g);
   switch (odr) {
     case 1: QAS_EMF_CT(1) break;
