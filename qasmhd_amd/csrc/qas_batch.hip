@@ -123,7 +123,7 @@ struct qas_batch {
   int nblk[BK_COUNT] = {};
   std::vector<BatchInst> inst;
   xsweep2d_fn sx = nullptr;
-  batch_fn sy = nullptr;
+  xsweep2d_fn sy = nullptr;
   prim_fn prim = nullptr;
   emf_fn emf = nullptr;
   ct_fn ct = nullptr;
@@ -241,7 +241,8 @@ static hipError_t batch_step(qas_batch* b, int flg, bool hist, bool even)
   if (hist) hipLaunchKernelGGL(kb_record, (unsigned)cdiv(n, 64), 64, 0, st, b->d_inst, n);
   for (int rk = 0; rk < b->s.r_k; rk++) {
     hipLaunchKernelGGL(b->prim, (unsigned)b->nblk[BK_PRIM], 256, 0, st, b->d_inst, boff(b, BK_PRIM), n, rk, b->s.gam);
-    hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, b->d_inst, boff(b, BK_SY), n, rk, b->s.gam);
+    hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
+                       (const BatchInst*)b->d_inst, boff(b, BK_SY), n, rk);
     hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
                        (const BatchInst*)b->d_inst, boff(b, BK_SX), n, rk);
     hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], 256, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
@@ -290,7 +291,7 @@ extern "C" int qas_batch_create(qas_batch** out, qas_ctx* const* ctxs, int n)
   if (b->s.r_k < 2) return bfail(b, QAS_E_ARG, "qas_batch_create: r_k must be 2 or 3");
   const bool f32 = b->s.precision == 1;
   b->sx = f32 ? pick_xsweep2d_f32(b->s.rmn, b->s.odr, true) : pick_xsweep2d(b->s.rmn, b->s.odr, true);
-  b->sy = f32 ? pick_batch_sweep2d_y_f32(b->s.rmn, b->s.odr) : pick_batch_sweep2d_y(b->s.rmn, b->s.odr);
+  b->sy = f32 ? pick_ysweep2d_f32(b->s.rmn, b->s.odr, true) : pick_ysweep2d(b->s.rmn, b->s.odr, true);
   if (!b->sx || !b->sy) return bfail(b, QAS_E_ARG, "qas_batch_create: no kernel for this scheme");
   switch (b->s.odr) {
     case 1: pick_stage<1>(b, f32); break;

@@ -44,18 +44,18 @@ __device__ __forceinline__ int batch_find(const int* off, int n, int b)
 }
 
 typedef void (*batch_fn)(const BatchInst*, const int*, int, int, double);
-// the 2D x sweep: (stage descriptor, geometry, boundary flags, gam, segments)
+// the 2D sweeps: (stage descriptor, geometry, boundary flags, gam, segments)
 // for one domain, (instance table, block offsets, instances, stage) for a batch
 typedef void (*xsweep2d_fn)(StagePtrs, Geo, BcFlags, double, int, const BatchInst*, const int*, int, int);
 
 // kernel tables (instantiated in qas_kernels2d.hip for FP64 and in
 // qas_mixed.hip for the mixed path)
-batch_fn pick_batch_sweep2d_y(int rmn, int odr);
-batch_fn pick_batch_sweep2d_y_f32(int rmn, int odr);
+xsweep2d_fn pick_ysweep2d(int rmn, int odr, bool batch);
+xsweep2d_fn pick_ysweep2d_f32(int rmn, int odr, bool batch);
 xsweep2d_fn pick_xsweep2d(int rmn, int odr, bool batch);
 xsweep2d_fn pick_xsweep2d_f32(int rmn, int odr, bool batch);
 // gravity instantiations (qas_gravity.hip)
 xsweep2d_fn pick_grav_xsweep(int rmn, int odr);
-void (*pick_grav_ysweep(int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
+xsweep2d_fn pick_grav_ysweep(int rmn, int odr);
 
 }  // namespace qas

@@ -21,12 +21,10 @@
 
 namespace qas {
 
-typedef void (*ysweep_fn)(StagePtrs, Geo, BcFlags, double, int);
-
-ysweep_fn pick_grav_ysweep(int rmn, int odr)
+xsweep2d_fn pick_grav_ysweep(int rmn, int odr)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return k_sweep2d_y<R, O, double, true>;
+  if (rmn == R && odr == O) return k_sweep2d_y<R, O, double, false, true>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)

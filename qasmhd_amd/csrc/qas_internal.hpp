@@ -225,6 +225,6 @@ void dsp_release(qas_ctx* c);
 void grav_release(qas_ctx* c);
 int dsp_step(qas_ctx* c, double dt);
 int dsp_after_ideal(qas_ctx* c);
-// FP32 face sweeps of the mixed path (qas_mixed.hip)
+// FP32 face sweeps of the 3D mixed path (qas_mixed.hip)
 void (*pick_sweep_f32(int ndim, int dir, int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
 }  // namespace qas

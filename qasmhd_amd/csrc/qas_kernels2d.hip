@@ -49,21 +49,6 @@ __global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
 }
 
 // ------------------------------------------------------------------ launch
-typedef void (*sweep_fn)(StagePtrs, Geo, BcFlags, double, int);
-
-template <int DIR>
-static sweep_fn pick_sweep(int rmn, int odr)
-{
-#define QAS_S(R, O) \
-  if (rmn == R && odr == O) return DIR == 0 ? nullptr : k_sweep2d_y<R, O, double>;
-  QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
-  QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
-  QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
-  QAS_S(3, 1) QAS_S(3, 2) QAS_S(3, 3) QAS_S(3, 4)
-#undef QAS_S
-  return nullptr;
-}
-
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
@@ -86,7 +71,7 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   const int nsy = (g.ny - 5 + SEG - 1) / SEG;
   const bool mixed = c->s.precision == 1;
   xsweep2d_fn fx = mixed ? pick_xsweep2d_f32(c->s.rmn, odr, false) : pick_xsweep2d(c->s.rmn, odr, false);
-  sweep_fn fy = mixed ? pick_sweep_f32(2, 1, c->s.rmn, odr) : pick_sweep<1>(c->s.rmn, odr);
+  xsweep2d_fn fy = mixed ? pick_ysweep2d_f32(c->s.rmn, odr, false) : pick_ysweep2d(c->s.rmn, odr, false);
   // with gravity the sweeps take the round-tripped stage energy
   StagePtrs ps = p;
   if (p.phi) {
@@ -97,7 +82,8 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   if (!fx || !fy) return hipErrorInvalidValue;
   // y sweep first (one 256-thread workgroup per (4-column group, segment)):
   // it leaves dt/dy * Dy in the accumulators; the x sweep applies the stage sum
-  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, ps, g, c->bc, gam, nsy);
+  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, ps, g, c->bc, gam, nsy,
+             (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   // 4) corner EMF, 5) CT
@@ -118,7 +104,7 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
   return hipGetLastError();
 }
 
-batch_fn pick_batch_sweep2d_y(int rmn, int odr) { return pick_batch_sweep2d_y_t<double>(rmn, odr); }
+xsweep2d_fn pick_ysweep2d(int rmn, int odr, bool batch) { return pick_ysweep2d_t<double>(rmn, odr, batch); }
 xsweep2d_fn pick_xsweep2d(int rmn, int odr, bool batch) { return pick_xsweep2d_t<double>(rmn, odr, batch); }
 
 }  // namespace qas

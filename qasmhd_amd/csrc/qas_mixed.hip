@@ -16,19 +16,6 @@ namespace qas {
 typedef void (*sweep_fn_t)(StagePtrs, Geo, BcFlags, double, int);
 
 template <int DIR>
-static sweep_fn_t pick2(int rmn, int odr)
-{
-#define QAS_S(R, O) \
-  if (rmn == R && odr == O) return DIR == 0 ? nullptr : k_sweep2d_y<R, O, float>;
-  QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
-  QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
-  QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
-  QAS_S(3, 1) QAS_S(3, 2) QAS_S(3, 3) QAS_S(3, 4)
-#undef QAS_S
-  return nullptr;
-}
-
-template <int DIR>
 static sweep_fn_t pick3(int rmn, int odr)
 {
 #define QAS_S(R, O) \
@@ -43,12 +30,11 @@ static sweep_fn_t pick3(int rmn, int odr)
 
 sweep_fn_t pick_sweep_f32(int ndim, int dir, int rmn, int odr)
 {
-  if (ndim == 2) return dir == 0 ? pick2<0>(rmn, odr) : pick2<1>(rmn, odr);
   if (ndim == 3) return dir == 0 ? pick3<0>(rmn, odr) : (dir == 1 ? pick3<1>(rmn, odr) : pick3<2>(rmn, odr));
   return nullptr;
 }
 
-batch_fn pick_batch_sweep2d_y_f32(int rmn, int odr) { return pick_batch_sweep2d_y_t<float>(rmn, odr); }
+xsweep2d_fn pick_ysweep2d_f32(int rmn, int odr, bool batch) { return pick_ysweep2d_t<float>(rmn, odr, batch); }
 xsweep2d_fn pick_xsweep2d_f32(int rmn, int odr, bool batch) { return pick_xsweep2d_t<float>(rmn, odr, batch); }
 
 }  // namespace qas

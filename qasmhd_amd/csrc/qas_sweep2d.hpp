@@ -40,10 +40,35 @@ constexpr int YC = 4;          // columns per workgroup
 constexpr int YR = WAVE + 4;   // stencil rows per segment
 constexpr int YRP = YR + 1;    // padded LDS column length
 
-template <int RMN, int ODR, typename T, bool GRAV = false>
-__device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam, int nseg,
-                                          int L)
+//
+// One kernel serves a single domain (BATCH false) and a batch of domains
+// (BATCH true), as the x sweep below; with the body as a device function
+// called from thin kernels the FP64 HLLD / ODR 4 instance compiled to 250
+// VGPRs (2 waves per SIMD) instead of 163 (3 waves).
+template <int RMN, int ODR, typename T, bool BATCH, bool GRAV = false>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : (RMN != 1 ? 2 : 1)) void k_sweep2d_y(
+    StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam, int nseg_arg, const BatchInst* I, const int* off,
+    int ninst, int stage)
 {
+  StagePtrs p;
+  Geo g;
+  BcFlags bc;
+  int nseg, L;
+  if constexpr (BATCH) {
+    const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int i = batch_find(off, ninst, b);
+    p = I[i].st[stage];
+    g = I[i].g;
+    bc = I[i].bc;
+    nseg = I[i].nsy;
+    L = b - off[i];
+  } else {
+    p = p_arg;
+    g = g_arg;
+    bc = bc_arg;
+    nseg = nseg_arg;
+    L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  }
   __shared__ double sh[9][YC][YRP];  // ro vy vz vx bz cx pr cy by
   __shared__ double so[8][YC][WAVE];  // du[6], upwind and central B-flux parts
   const int ncg = (g.nx + YC - 1) / YC;
@@ -164,15 +189,6 @@ __device__ __forceinline__ void sweep2d_y(const StagePtrs& p, const Geo& g, cons
       }
     }
   }
-}
-
-// y faces for every column (grid: 4-column groups x segments)
-// FP32 (mixed) instantiation: 4 waves per SIMD fit without spills
-template <int RMN, int ODR, typename T, bool GRAV = false>
-__global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void k_sweep2d_y(StagePtrs p, Geo g, BcFlags bc,
-                                                                             double gam, int nseg)
-{
-  sweep2d_y<RMN, ODR, T, GRAV>(p, g, bc, gam, nseg, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // ------------------------------------------------------------------ x sweep
@@ -336,23 +352,12 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
   for (int v = 0; v < 6; v++) p.out[vid[v]][c] = un[v];
 }
 
-// batched y sweep (qas_batch.hpp): one launch over every instance
-template <int RMN, int ODR, typename T>
-__global__ __launch_bounds__(256, (sizeof(T) == 4 ? 4 : 2)) void kb_sweep2d_y(const BatchInst* I, const int* off,
-                                                                              int n, int stage, double gam)
-{
-  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const int i = batch_find(off, n, b);
-  const BatchInst& B = I[i];
-  sweep2d_y<RMN, ODR, T>(B.st[stage], B.g, B.bc, gam, B.nsy, b - off[i]);
-}
-
-// kernel tables: batched y sweep, x sweep (single domain or batch)
+// kernel tables: y sweep and x sweep (single domain or batch)
 template <typename T>
-batch_fn pick_batch_sweep2d_y_t(int rmn, int odr)
+xsweep2d_fn pick_ysweep2d_t(int rmn, int odr, bool batch)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return kb_sweep2d_y<R, O, T>;
+  if (rmn == R && odr == O) return batch ? k_sweep2d_y<R, O, T, true> : k_sweep2d_y<R, O, T, false>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
