@@ -329,9 +329,70 @@ class Solver:
         self._check(self.lib.qas_adj_backward(self.ctx, _p(g), _p(out)), "qas_adj_backward")
         return out
 
+    def adj_backward_device(self, g_final, g_init):
+        """adj_backward on device-resident buffers (DeviceArray of shape
+        (8,) + shape each): no host copies (qas_adj_backward_device)."""
+        for a in (g_final, g_init):
+            assert a.shape == (8,) + self.shape
+        self._check(self.lib.qas_adj_backward_device(self.ctx, g_final.ptr, g_init.ptr), "qas_adj_backward_device")
+
     @property
     def device_bytes(self):
         return int(self.lib.qas_device_bytes(self.ctx))
+
+
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime libqasmhd_amd is linked against (dlopen of the same
+    soname returns the loaded copy)."""
+    global _hip
+    if _hip is None:
+        load()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A float64 array in device memory (for the *_device entry points
+    without torch)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(n) for n in shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self._h = hip_runtime()
+        p = C.c_void_p()
+        if self._h.hipMalloc(C.byref(p), self.nbytes) != 0:
+            raise QasError("hipMalloc of %d bytes failed" % self.nbytes)
+        self.ptr = p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape
+        if self._h.hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, 1) != 0:
+            raise QasError("hipMemcpy to the device failed")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        if self._h.hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) != 0:
+            raise QasError("hipMemcpy from the device failed")
+        return out
+
+    def close(self):
+        if self.ptr:
+            self._h.hipFree(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Batch:

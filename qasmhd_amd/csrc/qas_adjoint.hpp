@@ -18,9 +18,10 @@ namespace qas {
 template <typename T> QAS_HD T sgnd(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
 
 // ---------------------------------------------------------------- WCNS-4 / lin4
-// gf[0..4] += d(fl, fr)/df ^T (gl, gr)
+// gf[0..4] += d(fl, fr)/df ^T (gl, gr); fl, fr themselves (the forward
+// face values) go to *flo, *fro when given
 template <typename T>
-QAS_HD void wcns4_adj(const T* f, T gl, T gr, T* gf)
+QAS_HD void wcns4_adj(const T* f, T gl, T gr, T* gf, T* flo = nullptr, T* fro = nullptr)
 {
   const T eps = T(1e-40);
   const T c13 = T(13. / 12.);
@@ -48,6 +49,8 @@ QAS_HD void wcns4_adj(const T* f, T gl, T gr, T* gf)
   const T v0 = T(0.275) * S0, v1 = T(0.650) * S1, v2 = T(0.075) * S2;
   const T E = T(1.0) / (v0 + v1 + v2);
   const T z0 = v0 * E, z1 = v1 * E, z2 = T(1.0) - z0 - z1;
+  if (flo) *flo = T(0.125) * (w0 * qa + w1 * qb + w2 * qc);
+  if (fro) *fro = T(0.125) * (z2 * ra + z1 * rb + z0 * rc);
   // reverse: fl = 0.125 (w0 qa + w1 qb + w2 qc)
   T gw0 = T(0.125) * qa * gl, gw1 = T(0.125) * qb * gl, gw2 = T(0.125) * qc * gl;
   const T gqa = T(0.125) * w0 * gl, gqb = T(0.125) * w1 * gl, gqc = T(0.125) * w2 * gl;
@@ -107,10 +110,15 @@ QAS_HD void lin4_adj(T gl, T gr, T* gf)
 // accumulator, acc(j, m, v) adds v to the adjoint of s[j][m]; *gbn gets the
 // adjoint of the cell-centred bn.  The seven characteristic fields are
 // reversed one group at a time (fields 0-2, 3-4, 5-6), so no 7 x 5 array
-// of characteristic values or of their adjoints is ever held at once.
-template <typename T, typename Ld, typename Acc>
+// of characteristic values or of their adjoints is ever held at once; the
+// face values of a group's fields (which the adjoint of the back projection
+// needs) come out of the WCNS adjoints of that group.  FB_IN: fbc holds
+// the forward's positivity-fallback flags (lrstate's fbc), kept by the
+// caller; otherwise they are recomputed by a forward reconstruction of the
+// seven fields.
+template <bool FB_IN = false, typename T, typename Ld, typename Acc>
 QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const T (&gvr_in)[7], Acc& acc,
-                          T& gbn)
+                          T& gbn, int fbc = 0)
 {
   T gvl[7], gvr[7];
   for (int k = 0; k < 7; k++) {
@@ -158,24 +166,29 @@ QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const
       default: return sro * ld(3, m) - ld(5, m);
     }
   };
-  // face values of the characteristic fields (pre-fallback)
-  T wl[7], wr[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) {
-    T wk[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) wk[m] = proj(k, m);
-    interp5<4>(wk, wl[k], wr[k]);
-  }
-  ld.fence();
-  // back projection, only as far as the positivity test needs it
   bool fb[2];
-  for (int side = 0; side < 2; side++) {
-    const T* q = side ? wr : wl;
-    const T u0 = fast ? (icf2h * q[0] + icf2 * q[1] + icf2h * q[2]) : (icf2 * q[1] + icf2 * q[2]);
-    const T u6 = fast ? (T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6])
-                      : (q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6]);
-    fb[side] = (u0 <= eps || u6 <= eps);
+  if constexpr (!FB_IN) {
+    // face values of the characteristic fields (pre-fallback), back
+    // projection only as far as the positivity test needs it
+    T wl[7], wr[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      T wk[5];
+#pragma unroll
+      for (int m = 0; m < 5; m++) wk[m] = proj(k, m);
+      interp5<4>(wk, wl[k], wr[k]);
+    }
+    ld.fence();
+    for (int side = 0; side < 2; side++) {
+      const T* q = side ? wr : wl;
+      const T u0 = fast ? (icf2h * q[0] + icf2 * q[1] + icf2h * q[2]) : (icf2 * q[1] + icf2 * q[2]);
+      const T u6 = fast ? (T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6])
+                        : (q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6]);
+      fb[side] = (u0 <= eps || u6 <= eps);
+    }
+  } else {
+    fb[0] = (fbc & 1) != 0;
+    fb[1] = (fbc & 2) != 0;
   }
   // reverse: positivity fallback (first order for ro, vn, p)
   T gc0 = T(0), gc1 = T(0), gc6 = T(0);  // adjoints of the centre values
@@ -193,51 +206,53 @@ QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const
   }
   for (int n = 0; n < 5; n++)
     if (zf[n]) gvl[n + 1] = gvr[n + 1] = T(0.0);
-  // reverse: back projection (both sides)
+  // reverse: back projection (both sides), the part that does not need the
+  // face values: the adjoints of the characteristic face values
   T gwl[7] = {}, gwr[7] = {};
-  T gicf2 = T(0), gicf2h = T(0), gircfh = T(0), gisroh = T(0), gbyh = T(0), gbzh = T(0);
   for (int side = 0; side < 2; side++) {
-    const T* q = side ? wr : wl;
     const T* g = side ? gvr : gvl;
     T* gq = side ? gwr : gwl;
     if (fast) {
       gq[0] += icf2h * g[0] - ircfh * g[1] + T(0.5) * g[6];
       gq[1] += icf2 * g[0];
       gq[2] += icf2h * g[0] + ircfh * g[1] + T(0.5) * g[6];
-      gicf2h += (q[0] + q[2]) * g[0];
-      gicf2 += q[1] * g[0];
-      gircfh += (q[2] - q[0]) * g[1];
     } else {
       gq[0] += g[1];
       gq[1] += icf2 * g[0];
       gq[2] += icf2 * g[0] + g[6];
-      gicf2 += (q[1] + q[2]) * g[0];
     }
     gq[3] += isroh * g[2] + T(0.5) * g[4] - byh * g[6];
     gq[4] += isroh * g[2] - T(0.5) * g[4] + byh * g[6];
     gq[5] += isroh * g[3] + T(0.5) * g[5] - bzh * g[6];
     gq[6] += isroh * g[3] - T(0.5) * g[5] + bzh * g[6];
-    gisroh += (q[3] + q[4]) * g[2] + (q[5] + q[6]) * g[3];
-    gbyh += (q[4] - q[3]) * g[6];
-    gbzh += (q[6] - q[5]) * g[6];
   }
-  // reverse: WCNS of field k, then the projection w = L s, by groups
+  // reverse: WCNS of field k, then the projection w = L s, by groups; the
+  // rest of the back-projection adjoint (that of its coefficients) per group
+  T gicf2 = T(0), gicf2h = T(0), gircfh = T(0), gisroh = T(0), gbyh = T(0), gbzh = T(0);
   T grcf = T(0), gcf2 = T(0), gsro = T(0), gbt0 = T(0), gbu0 = T(0);
-  auto field_adj = [&](int k, T (&gk)[5]) {
+  auto field_adj = [&](int k, T (&gk)[5], T& ql, T& qr) {
     T wk[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       wk[m] = proj(k, m);
       gk[m] = T(0);
     }
-    wcns4_adj(wk, gwl[k], gwr[k], gk);
+    wcns4_adj(wk, gwl[k], gwr[k], gk, &ql, &qr);
   };
   {  // fields 0-2 (fast pair / slow and entropy)
     ld.fence();
     T g0[5], g1[5], g2[5];
-    field_adj(0, g0);
-    field_adj(1, g1);
-    field_adj(2, g2);
+    T q0l, q0r, q1l, q1r, q2l, q2r;
+    field_adj(0, g0, q0l, q0r);
+    field_adj(1, g1, q1l, q1r);
+    field_adj(2, g2, q2l, q2r);
+    if (fast) {
+      gicf2h += (q0l + q2l) * gvl[0] + (q0r + q2r) * gvr[0];
+      gicf2 += q1l * gvl[0] + q1r * gvr[0];
+      gircfh += (q2l - q0l) * gvl[1] + (q2r - q0r) * gvr[1];
+    } else {
+      gicf2 += (q1l + q2l) * gvl[0] + (q1r + q2r) * gvr[0];
+    }
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       const T s0 = ld(0, m), s1 = ld(1, m), s4 = ld(4, m), s5 = ld(5, m);
@@ -265,8 +280,11 @@ QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const
   {  // fields 3-4 (Alfven, t component)
     ld.fence();
     T g3[5], g4[5];
-    field_adj(3, g3);
-    field_adj(4, g4);
+    T q3l, q3r, q4l, q4r;
+    field_adj(3, g3, q3l, q3r);
+    field_adj(4, g4, q4l, q4r);
+    gisroh += (q3l + q4l) * gvl[2] + (q3r + q4r) * gvr[2];
+    gbyh += (q4l - q3l) * gvl[6] + (q4r - q3r) * gvr[6];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       acc(2, m, sro * (g3[m] + g4[m]));
@@ -277,8 +295,11 @@ QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const
   {  // fields 5-6 (Alfven, u component)
     ld.fence();
     T g5[5], g6[5];
-    field_adj(5, g5);
-    field_adj(6, g6);
+    T q5l, q5r, q6l, q6r;
+    field_adj(5, g5, q5l, q5r);
+    field_adj(6, g6, q6l, q6r);
+    gisroh += (q5l + q6l) * gvl[3] + (q5r + q6r) * gvr[3];
+    gbzh += (q6l - q5l) * gvl[6] + (q6r - q5r) * gvr[6];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       acc(3, m, sro * (g5[m] + g6[m]));

@@ -12,19 +12,22 @@
 // device.  qas_adj_backward walks the steps in reverse; for each step it
 // recomputes the three forward stages from the checkpoint, keeping each
 // stage's primitives and face EMF parts, and then runs the stage adjoints
-// 3, 2, 1.  A stage adjoint is seven launches, all in gather form (every
-// output element is written by one thread; no atomics):
+// 3, 2, 1.  A stage adjoint is these kernels, all in gather form (every
+// output element is written by one thread; no global atomics):
 //   corner    adjoint of the CT update -> g_ez at corners; forward CUCT
-//             weight and g_ct
+//             weight and the adjoint of its inputs (once per corner)
 //   faceparts adjoint of the corner EMF interpolation -> g of the four face
 //             parts (x/y sweep, upwind/central)
 //   sweep     per direction two passes over line segments (one wavefront
 //             each): the face pass reconstructs the face states and runs
 //             the flux adjoint (HLLD) per face, writing the adjoints of the
-//             two face states; the cell pass runs the adjoint of the
-//             reconstruction per cell and gathers the stencil adjoints onto
-//             cells by cross-lane shuffles (58 owned cells per wavefront)
-//             -> g of ro, vx, vy, vz, cx, cy, bz, pr and face B
+//             two face states and the cell's positivity-fallback flags; the
+//             cell pass runs the adjoint of the reconstruction per cell and
+//             gathers the stencil adjoints onto cells through a per-
+//             wavefront LDS row (58 owned cells per wavefront) -> g of ro,
+//             vx, vy, vz, cx, cy, bz, pr and face B.  The x pass sets these
+//             arrays; the y pass runs on transposed copies of its inputs and
+//             its outputs are added back (two transpose kernels)
 //   prim      adjoint of the primitive recovery + CUCT corner terms of
 //             ro, vx, vy + the RK pass-through b * g_out -> g_in of
 //             ro, mx, my, mz, bz, en; a * g_out into the step-base adjoint
@@ -44,6 +47,11 @@
 namespace qas {
 
 constexpr int SEGA = 58;  // owned cells per wavefront segment of the adjoint sweeps (lanes 3..60)
+// per-face arrays of the sweep face pass, read by the cell pass: gL[7] (left
+// state of the face, i.e. the right face state of cell q-1), gR[7],
+// g ql(q-1), g qr(q), g bn(face), and per cell the positivity-fallback
+// flags of its forward reconstruction (lrstate's fbc, as a double)
+constexpr int NGF = 18;
 
 struct AdjPtrs {
   const double* in[8];   // stage input state (ghosts filled)
@@ -55,8 +63,10 @@ struct AdjPtrs {
   double* gp[8];         // adjoint of ro, vx, vy, vz, cx, cy, bz, pr as the sweeps read them
   double* gbf[2];        // adjoint of face bx (x sweep), by (y sweep)
   double* gemf[4];       // adjoint of the face parts: x upwind, x central, y upwind, y central
-  double *gez, *gct, *ctv;  // corner adjoints of ez and of the CUCT weight; forward weight
-  double* gfc[17];       // per-face adjoints of the face states (sweep face pass -> cell pass)
+  double *gez, *ctv;     // corner adjoint of ez; forward CUCT weight
+  double* gcc[5];        // per corner, the CUCT weight's input adjoints: ro, vx, vy (each of the
+                         // four cells), bx, by (each of the two faces)
+  double* gfc[NGF];      // sweep face pass -> cell pass (see NGF)
   const double* dt;
   double a, b;
 };
@@ -79,12 +89,22 @@ __device__ __forceinline__ Img images(int q, int n)
   return r;
 }
 
+// f at (i, j) plus f at its ghost images.  The four loads are issued
+// unconditionally (a missing image re-reads the kept cell) and masked by
+// selects: with per-image branches, successive calls could not overlap
+// their memory latencies.
 __device__ __forceinline__ double fold_at(const double* f, int i, int j, const Geo& g)
 {
   const Img ix = images(i, g.nx), iy = images(j, g.ny);
-  double v = 0.0;
-  for (int b = 0; b < iy.n; b++)
-    for (int a = 0; a < ix.n; a++) v += f[(long long)g.nx * iy.i[b] + ix.i[a]];
+  const long long nx = g.nx;
+  const int i1 = ix.n > 1 ? ix.i[1] : ix.i[0];
+  const int j1 = iy.n > 1 ? iy.i[1] : iy.i[0];
+  const double a = f[nx * iy.i[0] + ix.i[0]], b = f[nx * iy.i[0] + i1];
+  const double c = f[nx * j1 + ix.i[0]], d = f[nx * j1 + i1];
+  double v = a;
+  v += ix.n > 1 ? b : 0.0;
+  v += iy.n > 1 ? c : 0.0;
+  v += (ix.n > 1 && iy.n > 1) ? d : 0.0;
   return v;
 }
 
@@ -111,19 +131,12 @@ __device__ __forceinline__ bool corner_ok(int i, int j, const Geo& g)
   return i >= 3 && i <= g.nx - 3 && j >= 3 && j <= g.ny - 3;
 }
 
-// ---------------------------------------------------------------- zero
-__global__ __launch_bounds__(256) void k_adj_zero(AdjPtrs p, long long nd)
-{
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nd) return;
-  for (int m = 0; m < 8; m++) p.gp[m][t] = 0.0;
-  p.gbf[0][t] = 0.0;
-  p.gbf[1][t] = 0.0;
-}
-
 // ---------------------------------------------------------------- corner
 // k_ct2d: bx(i,j) uses dfc(ez(i,j-1..j+2)) scaled by -dt/dy; by(i,j) uses
-// dfc(ez(i-1..i+2,j)) scaled by +dt/dx; out = a u0 + b (in + ...).
+// dfc(ez(i-1..i+2,j)) scaled by +dt/dx; out = a u0 + b (in + ...).  With
+// CUCT the corner also runs the adjoint of its weight once: the weight's
+// inputs enter through four-cell and two-face means, so one value per input
+// kind (gcc) serves every cell and face of the corner.
 template <int CTW>
 __global__ __launch_bounds__(256) void k_adj_corner(AdjPtrs p, Geo g)
 {
@@ -131,16 +144,17 @@ __global__ __launch_bounds__(256) void k_adj_corner(AdjPtrs p, Geo g)
   if (t >= g.nd) return;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
   const long long nx = g.nx;
-  double gez = 0.0, gct = 0.0, ctv = 0.5;
+  double gez = 0.0, ctv = 0.5;
+  double gc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (corner_ok(i, j, g)) {
     const double dtv = *p.dt;
     const double fy = -p.b * (dtv / g.dy), fx = p.b * (dtv / g.dx);
     const double w[4] = {1.0 / 24.0, -27.0 / 24.0, 27.0 / 24.0, -1.0 / 24.0};  // offsets -1..2
     for (int kk = 0; kk < 4; kk++) {
       const int k = kk - 1;
-      const int jf = j - k, iff = i - k;
-      if (jf >= 0 && jf < g.ny) gez += w[kk] * fy * p.gO[4][nx * jf + i];
-      if (iff >= 0 && iff < g.nx) gez += w[kk] * fx * p.gO[5][nx * j + iff];
+      const int jf = j - k, iff = i - k;  // in [1, n-2] at a corner in [3, n-3]
+      gez += w[kk] * fy * p.gO[4][nx * jf + i];
+      gez += w[kk] * fx * p.gO[5][nx * j + iff];
     }
     if (CTW) {
       double x[16];
@@ -160,12 +174,23 @@ __global__ __launch_bounds__(256) void k_adj_corner(AdjPtrs p, Geo g)
       }
       lin5<4>(a, l1, dum);
       lin5<4>(bq, dum, r1);
-      gct = gez * (0.5 * (l0 + r0) + 0.5 * (l1 + r1));
+      const double gct = gez * (0.5 * (l0 + r0) + 0.5 * (l1 + r1));
+      if (gct != 0.0) {
+        double gx[16] = {};
+        cuct_weight_adj(x, gct, gx);
+        gc[0] = gx[0];   // = gx[1..3]
+        gc[1] = gx[4];   // = gx[5..7]
+        gc[2] = gx[8];   // = gx[9..11]
+        gc[3] = gx[12];  // = gx[13]
+        gc[4] = gx[14];  // = gx[15]
+      }
     }
   }
   p.gez[t] = gez;
-  p.gct[t] = gct;
   p.ctv[t] = ctv;
+  if (CTW)
+#pragma unroll
+    for (int k = 0; k < 5; k++) p.gcc[k][t] = gc[k];
 }
 
 // ---------------------------------------------------------------- face parts
@@ -180,21 +205,18 @@ __global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
   const double cl[5] = {0.003125 * 9.0, 0.003125 * -56.0, 0.003125 * 234.0, 0.003125 * 144.0, 0.003125 * -11.0};
   const double cr[5] = {0.003125 * -11.0, 0.003125 * 144.0, 0.003125 * 234.0, 0.003125 * -56.0, 0.003125 * 9.0};
   double xu = 0.0, xc = 0.0, yu = 0.0, yc = 0.0;
+#pragma unroll
   for (int s = -3; s <= 2; s++) {
     const double coef = (s <= 1 ? cl[s + 3] : 0.0) + (s >= -2 ? cr[s + 2] : 0.0);
     const int jc = j - s, ic = i - s;
-    if (corner_ok(i, jc, g)) {
-      const long long c = nx * jc + i;
-      const double ge = p.gez[c];
-      xu += -0.5 * coef * ge;
-      xc += -0.5 * (1.0 - p.ctv[c]) * coef * ge;
-    }
-    if (corner_ok(ic, j, g)) {
-      const long long c = nx * j + ic;
-      const double ge = p.gez[c];
-      yu += 0.5 * coef * ge;
-      yc += 0.5 * p.ctv[c] * coef * ge;
-    }
+    // loads unconditional (an absent corner re-reads cell t), terms selected
+    const bool okx = corner_ok(i, jc, g), oky = corner_ok(ic, j, g);
+    const long long cx = okx ? nx * jc + i : t, cy = oky ? nx * j + ic : t;
+    const double gex = p.gez[cx], ctx = p.ctv[cx], gey = p.gez[cy], cty = p.ctv[cy];
+    xu += okx ? -0.5 * coef * gex : 0.0;
+    xc += okx ? -0.5 * (1.0 - ctx) * coef * gex : 0.0;
+    yu += oky ? 0.5 * coef * gey : 0.0;
+    yc += oky ? 0.5 * cty * coef * gey : 0.0;
   }
   p.gemf[0][t] = xu;
   p.gemf[1][t] = xc;
@@ -258,10 +280,6 @@ struct SweepFwd {
   }
 };
 
-// number of adjoint-face arrays: gL[7] (left state of the face, i.e. the
-// right face state of cell q-1), gR[7], g ql(q-1), g qr(q), g bn(face)
-constexpr int NGF = 17;
-
 // Face pass: lane l reconstructs cell q = seg*63 - 1 + l and, for l >= 1,
 // owns face q (between cells q-1 and q): flux adjoint (HLLD) from the
 // divergence and EMF-part adjoints, written to p.gfc[.][face].
@@ -283,7 +301,8 @@ __global__ __launch_bounds__(256, 2) void k_adj_sweep_face(AdjPtrs p, Geo g, dou
   SweepFwd<DIR> f;
   f.load(p, base + o * qc, o);
   double vl[7], vr[7], ql, qr;
-  lrstate<4>(f.s, f.bnc, gam, vl, vr);
+  int fbc;
+  lrstate<4>(f.s, f.bnc, gam, vl, vr, &fbc);
   lr_fb<4>(f.pp, f.qq, f.bnm, f.bnp, ql, qr);
   double L[7];
 #pragma unroll
@@ -291,6 +310,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_sweep_face(AdjPtrs p, Geo g, dou
   (void)ql;
   if (lane < 1 || q < 0 || q >= n) return;
   const long long fq = base + o * q;
+  p.gfc[17][fq] = (double)fbc;  // cell q (read for q in [2, n-3], where qc == q)
   double gL[7] = {}, gR[7] = {};
   double gbn = 0.0, gqlf = 0.0, gqr = 0.0;
   if (q >= 3 && q <= n - 3) {
@@ -357,10 +377,13 @@ struct SlotAcc {
   }
 };
 
+// One wave per SIMD: the pass needs about 350 registers (256 VGPRs + 92
+// AGPRs); held to 256 for two waves it spills ~380 bytes per lane and runs
+// slower (measured).
 template <int DIR, bool TR>
-__global__ __launch_bounds__(256, 2) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
+__global__ __launch_bounds__(256, 1) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
 {
-  static_assert(!TR || DIR == 1, "only the y pass runs on transposed arrays");
+  static_assert(TR == (DIR == 1), "the y pass runs on transposed arrays, the x pass in place");
   __shared__ double gacc[256 / WAVE][7 * (WAVE + 4)];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
@@ -404,7 +427,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_sweep_cell(AdjPtrs p, Geo g, dou
     }
     const double gql = fl_ok ? p.gfc[14][sc + o] : 0.0;
     const double gqr = fr_ok ? p.gfc[15][sc] : 0.0;
-    lrstate_adj_g(ld, cn[sc], gam, gvl, gvr, acc, gcn);
+    lrstate_adj_g<true>(ld, cn[sc], gam, gvl, gvr, acc, gcn, (int)p.gfc[17][sc]);
     // lr_fb: p = b_pair * v_n, q = v_pair (x: cy, vy; y: cx, vx)
     double pp[5], qq[5], gpp[5] = {}, gqq[5] = {};
 #pragma unroll
@@ -444,11 +467,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_sweep_cell(AdjPtrs p, Geo g, dou
     for (int k = 0; k < 7; k++) p.gp[gi[k]][c] = gc7[k];
     p.gp[5][c] = gcn;
     p.gbf[1][c] = gface;
-  } else {
+  } else {  // the x pass runs first and writes every cell: it sets the arrays
 #pragma unroll
-    for (int k = 0; k < 7; k++) p.gp[gi[k]][c] += gc7[k];
-    p.gp[DIR == 0 ? 4 : 5][c] += gcn;
-    p.gbf[DIR][c] += gface;
+    for (int k = 0; k < 7; k++) p.gp[gi[k]][c] = gc7[k];
+    p.gp[4][c] = gcn;
+    p.gbf[0][c] = gface;
   }
 }
 
@@ -467,16 +490,12 @@ __global__ __launch_bounds__(256) void k_adj_prim(AdjPtrs p, Geo g, double gam)
     for (int role = 0; role < 4; role++) {
       const int ic = i + ((role == 0 || role == 2) ? 1 : 0);
       const int jc = j + (role <= 1 ? 1 : 0);
-      if (!corner_ok(ic, jc, g)) continue;
-      const long long ss = nx * jc + ic;
-      const double gct = p.gct[ss];
-      if (gct == 0.0) continue;
-      double x[16], gx[16] = {};
-      cuct_inputs(p, ss, nx, x);
-      cuct_weight_adj(x, gct, gx);
-      cro += gx[role];
-      cvx += gx[4 + role];
-      cvy += gx[8 + role];
+      const bool ok = corner_ok(ic, jc, g);
+      const long long ss = ok ? nx * jc + ic : t;  // loads unconditional, terms selected
+      const double a0 = p.gcc[0][ss], a1 = p.gcc[1][ss], a2 = p.gcc[2][ss];
+      cro += ok ? a0 : 0.0;
+      cvx += ok ? a1 : 0.0;
+      cvy += ok ? a2 : 0.0;
     }
   }
   double gro = p.gp[0][t] + cro;
@@ -504,16 +523,26 @@ __global__ __launch_bounds__(256) void k_adj_prim(AdjPtrs p, Geo g, double gam)
   const double giro = mx * gvx + my * gvy + mz * gvz;
   gro += -iro * iro * giro;
   const double b = p.b;
-  p.gI[0][t] = b * p.gO[0][t] + gro;
-  p.gI[1][t] = b * p.gO[1][t] + iro * gvx;
-  p.gI[2][t] = b * p.gO[2][t] + iro * gvy;
-  p.gI[3][t] = b * p.gO[3][t] + iro * gvz;
-  p.gI[6][t] = b * p.gO[6][t] + gbz;
-  p.gI[7][t] = b * p.gO[7][t] + gen;
+  // every load before the first store (the arrays could alias as far as the
+  // compiler knows, and interleaving would serialise the memory latencies)
+  double go[8], gu[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) go[m] = p.gO[m][t];
+  const bool upd = p.a != 0.0;
+  if (upd)
+#pragma unroll
+    for (int m = 0; m < 8; m++) gu[m] = p.gU0[m][t];
+  p.gI[0][t] = b * go[0] + gro;
+  p.gI[1][t] = b * go[1] + iro * gvx;
+  p.gI[2][t] = b * go[2] + iro * gvy;
+  p.gI[3][t] = b * go[3] + iro * gvz;
+  p.gI[6][t] = b * go[6] + gbz;
+  p.gI[7][t] = b * go[7] + gen;
   p.gp[4][t] = gcx;  // totals of the cell-centred B adjoint, for k_adj_face
   p.gp[5][t] = gcy;
-  if (p.a != 0.0)
-    for (int m = 0; m < 8; m++) p.gU0[m][t] += p.a * p.gO[m][t];
+  if (upd)
+#pragma unroll
+    for (int m = 0; m < 8; m++) p.gU0[m][t] = gu[m] + p.a * go[m];
 }
 
 // ---------------------------------------------------------------- face
@@ -527,43 +556,35 @@ __global__ __launch_bounds__(256) void k_adj_face(AdjPtrs p, Geo g)
   double gbx = p.b * p.gO[4][t] + p.gbf[0][t];
   double gby = p.b * p.gO[5][t] + p.gbf[1][t];
   if (CTW) {
-    // bx(i,j) is entry 13 of corner (i,j) and entry 12 of corner (i,j+1);
-    // by(i,j) is entry 15 of corner (i,j) and entry 14 of corner (i+1,j)
-    const int ci[3] = {i, i, i + 1}, cj[3] = {j, j + 1, j};
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-      if (!corner_ok(ci[r], cj[r], g)) continue;
-      const long long ss = nx * cj[r] + ci[r];
-      const double gct = p.gct[ss];
-      if (gct == 0.0) continue;
-      double x[16], gx[16] = {};
-      cuct_inputs(p, ss, nx, x);
-      cuct_weight_adj(x, gct, gx);
-      if (r == 0) {
-        gbx += gx[13];
-        gby += gx[15];
-      } else if (r == 1) {
-        gbx += gx[12];
-      } else {
-        gby += gx[14];
-      }
-    }
+    // bx(i,j) enters the corners (i,j) and (i,j+1); by(i,j) the corners
+    // (i,j) and (i+1,j)
+    const bool ok0 = corner_ok(i, j, g), ok1 = corner_ok(i, j + 1, g), ok2 = corner_ok(i + 1, j, g);
+    const long long s1 = ok1 ? nx * (j + 1) + i : t, s2 = ok2 ? nx * j + i + 1 : t;
+    const double b0 = p.gcc[3][t], c0 = p.gcc[4][t], b1 = p.gcc[3][s1], c2 = p.gcc[4][s2];
+    gbx += ok0 ? b0 : 0.0;
+    gby += ok0 ? c0 : 0.0;
+    gbx += ok1 ? b1 : 0.0;
+    gby += ok2 ? c2 : 0.0;
   }
   // cx(q) = fcen(bx(q-1..q+2)) at kept q; ghost copies of cx fold onto q
   const double fw[4] = {-0.0625, 0.5625, 0.5625, -0.0625};  // offsets -1..2
-  if (kept(j, g.ny)) {
+  // (straight-line: out-of-range terms read a clamped kept cell and are
+  // dropped by selects, so all 32 loads can be in flight together)
+  const int jk = j < NG ? NG : (j > g.ny - NG - 1 ? g.ny - NG - 1 : j);
+  const int ik = i < NG ? NG : (i > g.nx - NG - 1 ? g.nx - NG - 1 : i);
 #pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-      const int iq = i - (kk - 1);
-      if (kept(iq, g.nx)) gbx += fw[kk] * fold_at(p.gp[4], iq, j, g);
-    }
+  for (int kk = 0; kk < 4; kk++) {
+    const int iq = i - (kk - 1);
+    const int iqk = iq < NG ? NG : (iq > g.nx - NG - 1 ? g.nx - NG - 1 : iq);
+    const double v = fold_at(p.gp[4], iqk, jk, g);
+    gbx += (kept(j, g.ny) && kept(iq, g.nx)) ? fw[kk] * v : 0.0;
   }
-  if (kept(i, g.nx)) {
 #pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-      const int jq = j - (kk - 1);
-      if (kept(jq, g.ny)) gby += fw[kk] * fold_at(p.gp[5], i, jq, g);
-    }
+  for (int kk = 0; kk < 4; kk++) {
+    const int jq = j - (kk - 1);
+    const int jqk = jq < NG ? NG : (jq > g.ny - NG - 1 ? g.ny - NG - 1 : jq);
+    const double v = fold_at(p.gp[5], ik, jqk, g);
+    gby += (kept(i, g.nx) && kept(jq, g.ny)) ? fw[kk] * v : 0.0;
   }
   p.gI[4][t] = gbx;
   p.gI[5][t] = gby;
@@ -576,37 +597,38 @@ struct FoldArgs {
   const double* extra[8];  // added at kept cells when non-null
 };
 
+// grid: (cells, 8 variables)
 __global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
+  const int m = blockIdx.y;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
-  const bool k = kept(i, g.nx) && kept(j, g.ny);
-  for (int m = 0; m < 8; m++) {
-    double v = 0.0;
-    if (k) {
-      v = fold_at(f.src[m], i, j, g);
-      if (f.extra[m]) v += f.extra[m][t];
-    }
-    f.dst[m][t] = v;
+  double v = 0.0;
+  if (kept(i, g.nx) && kept(j, g.ny)) {
+    v = fold_at(f.src[m], i, j, g);
+    if (f.extra[m]) v += f.extra[m][t];
   }
+  f.dst[m][t] = v;
 }
 
 __global__ __launch_bounds__(256) void k_adj_clear8(FoldArgs f, long long nd)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nd) return;
-  for (int m = 0; m < 8; m++) f.dst[m][t] = 0.0;
+  f.dst[blockIdx.y][t] = 0.0;
 }
 
 // ---------------------------------------------------------------- transposes
 // dst[a][i * ny + j] = src[a][j * nx + i] for array a = blockIdx.z, through
 // a 32 x 32 LDS tile (both global accesses run along a contiguous axis);
-// the add-back form accumulates dst[a][j * nx + i] += src[a][i * ny + j].
+// the back form accumulates dst[a][j * nx + i] += src[a][i * ny + j] for
+// a < nadd and stores it for the others.
 constexpr int NTR = 17;
 struct TrArgs {
   const double* src[NTR];
   double* dst[NTR];
+  int nadd;
 };
 
 __global__ __launch_bounds__(256) void k_adj_transpose(TrArgs a, int nx, int ny)
@@ -641,7 +663,10 @@ __global__ __launch_bounds__(256) void k_adj_untranspose_add(TrArgs a, int nx, i
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
-    if (i < nx && j < ny) d[(long long)j * nx + i] += tile[tx][r];
+    if (i < nx && j < ny) {
+      if ((int)blockIdx.z < a.nadd) d[(long long)j * nx + i] += tile[tx][r];
+      else d[(long long)j * nx + i] = tile[tx][r];
+    }
   }
 }
 
@@ -665,8 +690,9 @@ struct qas_adj_work {
   double* gp[8] = {};
   double* gbf[2] = {};
   double* gemf[4] = {};
-  double *gez = nullptr, *gct = nullptr, *ctv = nullptr;
-  double* gfc[17] = {};
+  double *gez = nullptr, *ctv = nullptr;
+  double* gcc[5] = {};
+  double* gfc[NGF] = {};
   double* tin[NTR] = {};      // transposed inputs of the y pass
   double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
@@ -719,7 +745,7 @@ static int adj_workspace(qas_ctx* c)
   c->adj = w;
   const size_t stride = (size_t)((c->geo.nd + 63) / 64 * 64);
   w->stride = stride;
-  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 3 + 17 + NTR + 9;
+  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 2 + 5 + NGF + NTR + 9;
   hipError_t e = hipMalloc(&w->slab, narr * stride * sizeof(double));
   if (e != hipSuccess) {
     adj_release(c);
@@ -741,9 +767,9 @@ static int adj_workspace(qas_ctx* c)
   for (int k = 0; k < 2; k++) w->gbf[k] = take();
   for (int k = 0; k < 4; k++) w->gemf[k] = take();
   w->gez = take();
-  w->gct = take();
   w->ctv = take();
-  for (int k = 0; k < 17; k++) w->gfc[k] = take();
+  for (int k = 0; k < 5; k++) w->gcc[k] = take();
+  for (int k = 0; k < NGF; k++) w->gfc[k] = take();
   for (int k = 0; k < NTR; k++) w->tin[k] = take();
   for (int k = 0; k < 9; k++) w->tout[k] = take();
   QAS_ACHECK(c, hipMemsetAsync(w->slab, 0, narr * stride * sizeof(double), c->stream));
@@ -799,8 +825,8 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   for (int k = 0; k < 2; k++) p.gbf[k] = w->gbf[k];
   for (int k = 0; k < 4; k++) p.gemf[k] = w->gemf[k];
   p.gez = w->gez;
-  p.gct = w->gct;
   p.ctv = w->ctv;
+  for (int k = 0; k < 5; k++) p.gcc[k] = w->gcc[k];
   for (int k = 0; k < NGF; k++) p.gfc[k] = w->gfc[k];
   p.dt = c->d_dt;
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
@@ -808,7 +834,6 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   p.b = rb[rk];
   const unsigned nb = nblk(g.nd, 256);
   const bool ctw = c->s.ctw != 0;
-  hipLaunchKernelGGL(k_adj_zero, nb, 256, 0, st, p, g.nd);
   if (ctw) hipLaunchKernelGGL((k_adj_corner<1>), nb, 256, 0, st, p, g);
   else hipLaunchKernelGGL((k_adj_corner<0>), nb, 256, 0, st, p, g);
   hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
@@ -828,6 +853,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
       ti.src[k] = src[k];
       ti.dst[k] = w->tin[k];
     }
+    ti.nadd = 0;
     const dim3 tg((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), NTR);
     hipLaunchKernelGGL(k_adj_transpose, tg, 256, 0, st, ti, g.nx, g.ny);
     AdjPtrs pt = p;
@@ -861,6 +887,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     }
     to.src[8] = w->tout[8];
     to.dst[8] = w->gbf[1];
+    to.nadd = 8;  // gp gets the y terms added, gbf[1] only has them
     const dim3 tb((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), 9);
     hipLaunchKernelGGL(k_adj_untranspose_add, tb, 256, 0, st, to, g.nx, g.ny);
   }
@@ -877,7 +904,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     f.dst[m] = w->gO[m];
     f.extra[m] = last ? w->gU0[m] : nullptr;
   }
-  hipLaunchKernelGGL(k_adj_fold, nb, 256, 0, st, f, g);
+  hipLaunchKernelGGL(k_adj_fold, dim3(nb, 8), 256, 0, st, f, g);
   return hipGetLastError();
 }
 
@@ -937,7 +964,7 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
       f.dst[m] = w->gO[m];
       f.extra[m] = nullptr;
     }
-    hipLaunchKernelGGL(k_adj_fold, nb, 256, 0, c->stream, f, g);
+    hipLaunchKernelGGL(k_adj_fold, dim3(nb, 8), 256, 0, c->stream, f, g);
     QAS_ACHECK(c, hipGetLastError());
   }
   for (int n = w->nrec - 1; n >= 0; n--) {
@@ -951,7 +978,7 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
     {
       FoldArgs f;
       for (int m = 0; m < 8; m++) f.dst[m] = w->gU0[m];
-      hipLaunchKernelGGL(k_adj_clear8, nb, 256, 0, c->stream, f, g.nd);
+      hipLaunchKernelGGL(k_adj_clear8, dim3(nb, 8), 256, 0, c->stream, f, g.nd);
     }
     QAS_ACHECK(c, adj_stage(c, 2, w->S[1], false));
     QAS_ACHECK(c, adj_stage(c, 1, w->S[0], false));

@@ -160,9 +160,10 @@ QAS_HD T dfc(T fm1, T f0, T f1, T f2)
 // bn: cell-centred normal B.  vl: face i+1/2 (left state), vr: face i-1/2.
 // MODE 0: both projections formed, selected per lane (the kernels);
 // MODE 1 / 2: only the fast/Alfven resp. Alfven/entropy projection is
-// evaluated (used by the flop model to count one branch).
+// evaluated (used by the flop model to count one branch).  fbc, when given,
+// receives the positivity-fallback flags (1: left state, 2: right state).
 template <int ODR, typename T, int MODE = 0>
-QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7])
+QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int* fbc = nullptr)
 {
   const T ro0 = s[0][2], bt0 = s[4][2], bu0 = s[5][2], pr0 = s[6][2];
   // exact-zero flags of mhd_eigen.c:575-587
@@ -274,16 +275,19 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7])
   }
   // positivity fallback (mhd_func.c:150-160)
   const T eps = T(1e-12);
-  if (vl[0] <= eps || vl[6] <= eps) {
+  const bool fbl = vl[0] <= eps || vl[6] <= eps;
+  const bool fbr = vr[0] <= eps || vr[6] <= eps;
+  if (fbl) {
     vl[0] = s[0][2];
     vl[1] = s[1][2];
     vl[6] = s[6][2];
   }
-  if (vr[0] <= eps || vr[6] <= eps) {
+  if (fbr) {
     vr[0] = s[0][2];
     vr[1] = s[1][2];
     vr[6] = s[6][2];
   }
+  if (fbc) *fbc = (fbl ? 1 : 0) | (fbr ? 2 : 0);
 }
 
 // face states of the B-flux b_t v_n - b_n v_t (mhd_func.c:163-188); the
