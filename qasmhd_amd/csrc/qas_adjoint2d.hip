@@ -18,16 +18,16 @@
 //             weight and the adjoint of its inputs (once per corner)
 //   faceparts adjoint of the corner EMF interpolation -> g of the four face
 //             parts (x/y sweep, upwind/central)
-//   sweep     per direction two passes over line segments (one wavefront
-//             each): the face pass reconstructs the face states and runs
-//             the flux adjoint (HLLD) per face, writing the adjoints of the
-//             two face states and the cell's positivity-fallback flags; the
-//             cell pass runs the adjoint of the reconstruction per cell and
-//             gathers the stencil adjoints onto cells through a per-
-//             wavefront LDS row (58 owned cells per wavefront) -> g of ro,
-//             vx, vy, vz, cx, cy, bz, pr and face B.  The x pass sets these
-//             arrays; the y pass runs on transposed copies of its inputs and
-//             its outputs are added back (two transpose kernels)
+//   sweep     per direction one pass over line segments (one wavefront
+//             each): it reconstructs the face states, runs the flux adjoint
+//             (HLLD) per face, hands the face-state adjoints to the lanes of
+//             the two cells by cross-lane shuffles and runs the adjoint of
+//             the reconstruction per cell, gathering the stencil adjoints
+//             onto cells through a per-wavefront LDS row (58 owned cells per
+//             wavefront) -> g of ro, vx, vy, vz, cx, cy, bz, pr and face B.
+//             The x pass sets these arrays; the y pass runs on transposed
+//             copies of its inputs and its outputs are added back (two
+//             transpose kernels)
 //   prim      adjoint of the primitive recovery + CUCT corner terms of
 //             ro, vx, vy + the RK pass-through b * g_out -> g_in of
 //             ro, mx, my, mz, bz, en; a * g_out into the step-base adjoint
@@ -47,11 +47,6 @@
 namespace qas {
 
 constexpr int SEGA = 58;  // owned cells per wavefront segment of the adjoint sweeps (lanes 3..60)
-// per-face arrays of the sweep face pass, read by the cell pass: gL[7] (left
-// state of the face, i.e. the right face state of cell q-1), gR[7],
-// g ql(q-1), g qr(q), g bn(face), and per cell the positivity-fallback
-// flags of its forward reconstruction (lrstate's fbc, as a double)
-constexpr int NGF = 18;
 
 struct AdjPtrs {
   const double* in[8];   // stage input state (ghosts filled)
@@ -66,7 +61,6 @@ struct AdjPtrs {
   double *gez, *ctv;     // corner adjoint of ez; forward CUCT weight
   double* gcc[5];        // per corner, the CUCT weight's input adjoints: ro, vx, vy (each of the
                          // four cells), bx, by (each of the two faces)
-  double* gfc[NGF];      // sweep face pass -> cell pass (see NGF)
   const double* dt;
   double a, b;
 };
@@ -280,85 +274,22 @@ struct SweepFwd {
   }
 };
 
-// Face pass: lane l reconstructs cell q = seg*63 - 1 + l and, for l >= 1,
-// owns face q (between cells q-1 and q): flux adjoint (HLLD) from the
-// divergence and EMF-part adjoints, written to p.gfc[.][face].
-template <int DIR, bool TR>
-__global__ __launch_bounds__(256, 2) void k_adj_sweep_face(AdjPtrs p, Geo g, double gam, int nseg)
-{
-  static_assert(!TR || DIR == 1, "only the y pass runs on transposed arrays");
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-  const int nline = DIR == 0 ? g.ny : g.nx;
-  if (wave >= nseg * nline) return;  // whole wavefront exits together
-  const int line = wave / nseg;
-  const int seg = wave - line * nseg;
-  const int n = DIR == 0 ? g.nx : g.ny;
-  const long long o = line_stride<DIR, TR>(g);
-  const long long base = line_base<DIR, TR>(g, line);
-  const int q = seg * (WAVE - 1) - 1 + lane;
-  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
-  SweepFwd<DIR> f;
-  f.load(p, base + o * qc, o);
-  double vl[7], vr[7], ql, qr;
-  int fbc;
-  lrstate<4>(f.s, f.bnc, gam, vl, vr, &fbc);
-  lr_fb<4>(f.pp, f.qq, f.bnm, f.bnp, ql, qr);
-  double L[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
-  (void)ql;
-  if (lane < 1 || q < 0 || q >= n) return;
-  const long long fq = base + o * q;
-  p.gfc[17][fq] = (double)fbc;  // cell q (read for q in [2, n-3], where qc == q)
-  double gL[7] = {}, gR[7] = {};
-  double gbn = 0.0, gqlf = 0.0, gqr = 0.0;
-  if (q >= 3 && q <= n - 3) {
-    const int ntl = DIR == 0 ? g.ny : g.nx;
-    double gF[7] = {};
-    if (line >= NG && line <= ntl - NG - 1) {
-      const double sfac = -p.b * (*p.dt / (DIR == 0 ? g.dx : g.dy));
-      // flux component fed into each updated variable (k_sweep2d fv[])
-      const int fk[6] = {0, DIR == 0 ? 1 : 3, DIR == 0 ? 2 : 1, DIR == 0 ? 3 : 2, DIR == 0 ? 5 : 4, 6};
-      const int vm[6] = {0, 1, 2, 3, 6, 7};
-      const double w[4] = {1.0 / 24.0, -27.0 / 24.0, 27.0 / 24.0, -1.0 / 24.0};  // F(c+k), k = -1..2
-#pragma unroll
-      for (int v = 0; v < 6; v++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) {
-          const int c = q - (kk - 1);
-          if (c >= NG && c <= n - NG - 1) acc += w[kk] * p.gO[vm[v]][base + o * c];
-        }
-        gF[fk[v]] = sfac * acc;
-      }
-    }
-    // face parts of the B flux: fbu = F_B - fc, fc = (ql(q-1) + qr(q)) / 2
-    const double gu = p.gemf[DIR * 2 + 0][fq], gc = p.gemf[DIR * 2 + 1][fq];
-    gF[DIR == 0 ? 4 : 5] += gu;
-    const double gfc = gc - gu;
-    gqlf = 0.5 * gfc;
-    gqr = 0.5 * gfc;
-    const double bnf = 0.5 * (f.bf[fq] + f.bf[fq]);
-    hlld_adj(L, vr, bnf, gam, gF, gL, gR, gbn);  // d(0.5 (b + b))/db = 1
-  }
-#pragma unroll
-  for (int k = 0; k < 7; k++) {
-    p.gfc[k][fq] = gL[k];
-    p.gfc[7 + k][fq] = gR[k];
-  }
-  p.gfc[14][fq] = gqlf;
-  p.gfc[15][fq] = gqr;
-  p.gfc[16][fq] = gbn;
-}
-
-// Cell pass: lane l handles cell q = seg*SEGA - 3 + l; lanes 3..60 own
-// their cell and face.  Reconstruction adjoint from the face adjoints of
-// faces q (right state) and q+1 (left state).  The stencil adjoints are
-// added into a per-wavefront LDS row indexed by cell (lane l's stencil point
-// m is cell slot l + m), which is the gather onto cells; the stencil itself
-// is read from memory where it is used (loads re-issued after each fence
-// rather than held in registers across the whole reverse pass).
+// Line pass (one per direction): one wavefront per segment of a line, lane
+// l on cell q = seg*SEGA - 3 + l.
+//   1. forward reconstruction of cell q (face states, positivity-fallback
+//      flags), as k_sweep2d computes it;
+//   2. lanes l >= 1: flux adjoint (HLLD) of face q (between cells q-1 and
+//      q; the left state comes from lane l-1) from the divergence and
+//      EMF-part adjoints -> adjoints of the two face states and of bn;
+//   3. reconstruction adjoint of cell q from faces q (right state, this
+//      lane) and q+1 (left state, lane l+1).  Its stencil adjoints are
+//      added into a per-wavefront LDS row indexed by cell (lane l's stencil
+//      point m is cell slot l + m), which is the gather onto cells; the
+//      stencil is read from memory where it is used (loads re-issued after
+//      each fence rather than held in registers across the reverse pass).
+// Lanes 1..62 have both faces; lanes 3..60 own their cell and face (all
+// five of their stencil contributors are complete).  Lanes 0 and 63 only
+// add into slots of cells no lane owns.
 template <int DIR>
 struct StencilLd {
   const double* a[7];  // ro vn vt vu bt bu pr
@@ -377,11 +308,10 @@ struct SlotAcc {
   }
 };
 
-// One wave per SIMD: the pass needs about 350 registers (256 VGPRs + 92
-// AGPRs); held to 256 for two waves it spills ~380 bytes per lane and runs
-// slower (measured).
+// One wave per SIMD: the reverse reconstruction needs about 350 registers
+// (held to 256 for two waves it spills and runs slower, measured).
 template <int DIR, bool TR>
-__global__ __launch_bounds__(256, 1) void k_adj_sweep_cell(AdjPtrs p, Geo g, double gam, int nseg)
+__global__ __launch_bounds__(256, 1) void k_adj_sweep(AdjPtrs p, Geo g, double gam, int nseg)
 {
   static_assert(TR == (DIR == 1), "the y pass runs on transposed arrays, the x pass in place");
   __shared__ double gacc[256 / WAVE][7 * (WAVE + 4)];
@@ -395,17 +325,67 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_cell(AdjPtrs p, Geo g, dou
   const long long o = line_stride<DIR, TR>(g);
   const long long base = line_base<DIR, TR>(g, line);
   const int q = seg * SEGA - 3 + lane;
+  const int qc = q < 2 ? 2 : (q > n - 3 ? n - 3 : q);
+  const long long fq = base + o * qc;
   double* row = gacc[threadIdx.x / WAVE];
 #pragma unroll
   for (int j = 0; j < 7; j++) {
     row[j * (WAVE + 4) + lane] = 0.0;
     if (lane < 4) row[j * (WAVE + 4) + WAVE + lane] = 0.0;
   }
+  // ---- 1, 2: forward reconstruction, flux adjoint of face q
+  double gvl[7], gvr[7], gbn = 0.0, gql, gqr = 0.0;
+  int fbc;
+  {
+    SweepFwd<DIR> f;
+    f.load(p, fq, o);
+    double vl[7], vr[7];
+    lrstate<4>(f.s, f.bnc, gam, vl, vr, &fbc);
+    double L[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
+    double gL[7] = {};
+    double gqlf = 0.0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) gvr[k] = 0.0;
+    if (lane >= 1 && q >= 3 && q <= n - 3) {
+      const int ntl = DIR == 0 ? g.ny : g.nx;
+      double gF[7] = {};
+      if (line >= NG && line <= ntl - NG - 1) {
+        const double sfac = -p.b * (*p.dt / (DIR == 0 ? g.dx : g.dy));
+        // flux component fed into each updated variable (k_sweep2d fv[])
+        const int fk[6] = {0, DIR == 0 ? 1 : 3, DIR == 0 ? 2 : 1, DIR == 0 ? 3 : 2, DIR == 0 ? 5 : 4, 6};
+        const int vm[6] = {0, 1, 2, 3, 6, 7};
+        const double w[4] = {1.0 / 24.0, -27.0 / 24.0, 27.0 / 24.0, -1.0 / 24.0};  // F(c+k), k = -1..2
+#pragma unroll
+        for (int v = 0; v < 6; v++) {
+          double acc = 0.0;
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++) {
+            const int c = q - (kk - 1);
+            if (c >= NG && c <= n - NG - 1) acc += w[kk] * p.gO[vm[v]][base + o * c];
+          }
+          gF[fk[v]] = sfac * acc;
+        }
+      }
+      // face parts of the B flux: fbu = F_B - fc, fc = (ql(q-1) + qr(q)) / 2
+      const double gu = p.gemf[DIR * 2 + 0][fq], gc = p.gemf[DIR * 2 + 1][fq];
+      gF[DIR == 0 ? 4 : 5] += gu;
+      const double gfc = gc - gu;
+      gqlf = 0.5 * gfc;
+      gqr = 0.5 * gfc;
+      const double bnf = 0.5 * (f.bf[fq] + f.bf[fq]);
+      hlld_adj(L, vr, bnf, gam, gF, gL, gvr, gbn);  // d(0.5 (b + b))/db = 1
+    }
+    // this cell's left state is the left state of face q+1 (lane l+1)
+#pragma unroll
+    for (int k = 0; k < 7; k++) gvl[k] = __shfl_down(gL[k], 1, WAVE);
+    gql = __shfl_down(gqlf, 1, WAVE);
+  }
+  // ---- 3: reconstruction adjoint of cell q
   SlotAcc acc{row, lane};
   double gcn = 0.0, gbm = 0.0, gbp = 0.0;
-  const bool cell_ok = q >= 2 && q <= n - 3;
-  if (cell_ok) {
-    const long long sc = base + o * q;
+  if (q >= 2 && q <= n - 3) {
     StencilLd<DIR> ld;
     ld.a[0] = p.in[0];
     ld.a[1] = DIR == 0 ? p.vx : p.vy;
@@ -414,20 +394,11 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_cell(AdjPtrs p, Geo g, dou
     ld.a[4] = DIR == 0 ? p.cy : p.in[6];
     ld.a[5] = DIR == 0 ? p.in[6] : p.cx;
     ld.a[6] = p.pr;
-    ld.sc = sc;
+    ld.sc = fq;
     ld.o = o;
     const double* cn = DIR == 0 ? p.cx : p.cy;
     const double* bf = DIR == 0 ? p.in[4] : p.in[5];
-    double gvl[7], gvr[7];
-    const bool fr_ok = q >= 3, fl_ok = q + 1 <= n - 3;
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-      gvl[k] = fl_ok ? p.gfc[k][sc + o] : 0.0;
-      gvr[k] = fr_ok ? p.gfc[7 + k][sc] : 0.0;
-    }
-    const double gql = fl_ok ? p.gfc[14][sc + o] : 0.0;
-    const double gqr = fr_ok ? p.gfc[15][sc] : 0.0;
-    lrstate_adj_g<true>(ld, cn[sc], gam, gvl, gvr, acc, gcn, (int)p.gfc[17][sc]);
+    lrstate_adj_g<true>(ld, cn[fq], gam, gvl, gvr, acc, gcn, fbc);
     // lr_fb: p = b_pair * v_n, q = v_pair (x: cy, vy; y: cx, vx)
     double pp[5], qq[5], gpp[5] = {}, gqq[5] = {};
 #pragma unroll
@@ -435,7 +406,7 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_cell(AdjPtrs p, Geo g, dou
       pp[m] = ld(DIR == 0 ? 4 : 5, m) * ld(1, m);
       qq[m] = ld(DIR == 0 ? 2 : 3, m);
     }
-    lr_fb_adj(pp, qq, bf[sc], bf[sc + o], gql, gqr, gpp, gqq, gbm, gbp);
+    lr_fb_adj(pp, qq, bf[fq], bf[fq + o], gql, gqr, gpp, gqq, gbm, gbp);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       if (DIR == 0) {
@@ -459,7 +430,7 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_cell(AdjPtrs p, Geo g, dou
   const bool own = lane >= 3 && lane < 3 + SEGA && q >= 0 && q < n;
   if (!own) return;
   const long long c = base + o * q;
-  const double gface = ((q >= 3 && q <= n - 3) ? p.gfc[16][c] : 0.0) + gbm + gbpl;
+  const double gface = gbn + gbm + gbpl;
   // stencil rows -> adjoint arrays (ro vx vy vz cx cy bz pr)
   const int gi[7] = {0, DIR == 0 ? 1 : 2, DIR == 0 ? 2 : 3, DIR == 0 ? 3 : 1, DIR == 0 ? 5 : 6, DIR == 0 ? 6 : 4, 7};
   if (TR) {  // transposed scratch, written once per cell, added back by k_adj_untranspose_add
@@ -692,7 +663,6 @@ struct qas_adj_work {
   double* gemf[4] = {};
   double *gez = nullptr, *ctv = nullptr;
   double* gcc[5] = {};
-  double* gfc[NGF] = {};
   double* tin[NTR] = {};      // transposed inputs of the y pass
   double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
@@ -745,7 +715,7 @@ static int adj_workspace(qas_ctx* c)
   c->adj = w;
   const size_t stride = (size_t)((c->geo.nd + 63) / 64 * 64);
   w->stride = stride;
-  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 2 + 5 + NGF + NTR + 9;
+  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 2 + 5 + NTR + 9;
   hipError_t e = hipMalloc(&w->slab, narr * stride * sizeof(double));
   if (e != hipSuccess) {
     adj_release(c);
@@ -769,14 +739,15 @@ static int adj_workspace(qas_ctx* c)
   w->gez = take();
   w->ctv = take();
   for (int k = 0; k < 5; k++) w->gcc[k] = take();
-  for (int k = 0; k < NGF; k++) w->gfc[k] = take();
   for (int k = 0; k < NTR; k++) w->tin[k] = take();
   for (int k = 0; k < 9; k++) w->tout[k] = take();
   QAS_ACHECK(c, hipMemsetAsync(w->slab, 0, narr * stride * sizeof(double), c->stream));
   return QAS_OK;
 }
 
-// forward stage with its primitives and face parts kept in stage slot s
+// forward stage with its primitives and face parts kept in stage slot s;
+// the last stage's output is not needed (only its primitives and face
+// parts are), so its corner EMF, CT and ghost fill are skipped
 static hipError_t fwd_stage(qas_ctx* c, int s, double* const in[8], double* const u0[8], double* const out[8],
                             int rk)
 {
@@ -797,8 +768,9 @@ static hipError_t fwd_stage(qas_ctx* c, int s, double* const in[8], double* cons
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
   p.b = rb[rk];
-  hipError_t e = launch_stage2d(c, p);
-  if (e != hipSuccess) return e;
+  const bool last = rk == 2;
+  hipError_t e = launch_stage2d(c, p, last);
+  if (e != hipSuccess || last) return e;
   return launch_bound(c, out, 8);
 }
 
@@ -827,7 +799,6 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   p.gez = w->gez;
   p.ctv = w->ctv;
   for (int k = 0; k < 5; k++) p.gcc[k] = w->gcc[k];
-  for (int k = 0; k < NGF; k++) p.gfc[k] = w->gfc[k];
   p.dt = c->d_dt;
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
@@ -837,12 +808,9 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   if (ctw) hipLaunchKernelGGL((k_adj_corner<1>), nb, 256, 0, st, p, g);
   else hipLaunchKernelGGL((k_adj_corner<0>), nb, 256, 0, st, p, g);
   hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
-  const int fsx = (g.nx + WAVE - 2) / (WAVE - 1), fsy = (g.ny + WAVE - 2) / (WAVE - 1);
   const int csx = (g.nx + SEGA - 1) / SEGA, csy = (g.ny + SEGA - 1) / SEGA;
-  hipLaunchKernelGGL((k_adj_sweep_face<0, false>), nblk((long long)fsx * g.ny * WAVE, 256), 256, 0, st, p, g,
-                     c->s.gam, fsx);
-  hipLaunchKernelGGL((k_adj_sweep_cell<0, false>), nblk((long long)csx * g.ny * WAVE, 256), 256, 0, st, p, g,
-                     c->s.gam, csx);
+  hipLaunchKernelGGL((k_adj_sweep<0, false>), nblk((long long)csx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam,
+                     csx);
   {
     // y pass on transposed copies of what it reads (columns contiguous), its
     // outputs added back onto gp, gbf[1]
@@ -872,9 +840,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     pt.gemf[3] = w->tin[16];
     for (int m = 0; m < 8; m++) pt.gp[m] = w->tout[m];
     pt.gbf[1] = w->tout[8];
-    hipLaunchKernelGGL((k_adj_sweep_face<1, true>), nblk((long long)fsy * g.nx * WAVE, 256), 256, 0, st, pt, g,
-                       c->s.gam, fsy);
-    hipLaunchKernelGGL((k_adj_sweep_cell<1, true>), nblk((long long)csy * g.nx * WAVE, 256), 256, 0, st, pt, g,
+    hipLaunchKernelGGL((k_adj_sweep<1, true>), nblk((long long)csy * g.nx * WAVE, 256), 256, 0, st, pt, g,
                        c->s.gam, csy);
     TrArgs to;
     for (int k = 0; k < NTR; k++) {

@@ -216,7 +216,9 @@ hipError_t launch_set_dt(qas_ctx* c, double dt);
 hipError_t launch_record_counted(qas_ctx* c);
 hipError_t launch_zero_counter(qas_ctx* c);
 hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
-hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p);
+// faces_only: primitives and both sweeps (face parts, stage sum) but no
+// corner EMF / CT (the adjoint's recompute of a step's last stage)
+hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only = false);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
 void adj_release(qas_ctx* c);  // frees the adjoint workspace (qas_adjoint2d.hip)
 // dissipative step (qas_dissipative.hip)

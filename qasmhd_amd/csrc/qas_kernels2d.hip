@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
 // ------------------------------------------------------------------ launch
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
+hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
 {
   const Geo& g = c->geo;
   const int odr = c->s.odr;
@@ -86,6 +86,7 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p)
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
+  if (faces_only) return hipGetLastError();
   // 4) corner EMF, 5) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
