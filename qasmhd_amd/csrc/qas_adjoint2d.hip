@@ -308,15 +308,14 @@ struct SlotAcc {
   }
 };
 
-// One wave per SIMD: the reverse reconstruction needs about 350 registers
-// (held to 256 for two waves it spills and runs slower, measured).
+// blk: this workgroup's index within the pass
 template <int DIR, bool TR>
-__global__ __launch_bounds__(256, 1) void k_adj_sweep(AdjPtrs p, Geo g, double gam, int nseg)
+__device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, double gam, int nseg, int blk)
 {
   static_assert(TR == (DIR == 1), "the y pass runs on transposed arrays, the x pass in place");
   __shared__ double gacc[256 / WAVE][7 * (WAVE + 4)];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const int wave = (int)(((long long)blk * blockDim.x + threadIdx.x) / WAVE);
   const int nline = DIR == 0 ? g.ny : g.nx;
   if (wave >= nseg * nline) return;  // whole wavefront exits together
   const int line = wave / nseg;
@@ -444,6 +443,21 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep(AdjPtrs p, Geo g, double g
     p.gp[4][c] = gcn;
     p.gbf[0][c] = gface;
   }
+}
+
+// Both directions in one launch: workgroups [0, nbx) run the x pass, the
+// rest the y pass (the two are independent: the x pass sets the adjoint
+// arrays, the y pass writes transposed scratch).  On small domains one
+// pass fills less than two rounds of the 1024 wave slots (one wave per
+// SIMD), so running them side by side saves a partly empty round.  One
+// wave per SIMD: the reverse reconstruction needs about 350 registers (held
+// to 256 for two waves it spills and runs slower, measured).
+__global__ __launch_bounds__(256, 1) void k_adj_sweep_xy(AdjPtrs px, AdjPtrs py, Geo g, double gam, int nsx,
+                                                         int nsy, int nbx)
+{
+  const int b = (int)blockIdx.x;
+  if (b < nbx) adj_sweep_body<0, false>(px, g, gam, nsx, b);
+  else adj_sweep_body<1, true>(py, g, gam, nsy, b - nbx);
 }
 
 // ---------------------------------------------------------------- prim
@@ -809,8 +823,6 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   else hipLaunchKernelGGL((k_adj_corner<0>), nb, 256, 0, st, p, g);
   hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
   const int csx = (g.nx + SEGA - 1) / SEGA, csy = (g.ny + SEGA - 1) / SEGA;
-  hipLaunchKernelGGL((k_adj_sweep<0, false>), nblk((long long)csx * g.ny * WAVE, 256), 256, 0, st, p, g, c->s.gam,
-                     csx);
   {
     // y pass on transposed copies of what it reads (columns contiguous), its
     // outputs added back onto gp, gbf[1]
@@ -840,8 +852,9 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     pt.gemf[3] = w->tin[16];
     for (int m = 0; m < 8; m++) pt.gp[m] = w->tout[m];
     pt.gbf[1] = w->tout[8];
-    hipLaunchKernelGGL((k_adj_sweep<1, true>), nblk((long long)csy * g.nx * WAVE, 256), 256, 0, st, pt, g,
-                       c->s.gam, csy);
+    // x pass in place, y pass on the transposed copies, one launch
+    const unsigned nbx = nblk((long long)csx * g.ny * WAVE, 256), nby = nblk((long long)csy * g.nx * WAVE, 256);
+    hipLaunchKernelGGL(k_adj_sweep_xy, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
     TrArgs to;
     for (int k = 0; k < NTR; k++) {
       to.src[k] = nullptr;
