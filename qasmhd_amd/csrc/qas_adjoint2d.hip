@@ -52,7 +52,8 @@ struct AdjPtrs {
   const double* in[8];   // stage input state (ghosts filled)
   const double *vx, *vy, *vz, *pr, *cx, *cy;  // forward primitives of `in`
   const double* emf[8];  // forward face parts ([0] x upwind, [1] x central, [4] y upwind, [5] y central)
-  const double* gO[8];   // adjoint of the stage output (ghost entries zero)
+  double* gO[8];         // adjoint of the stage output (ghost entries zero); k_adj_foldstage
+                         // overwrites it with the previous stage's
   double* gI[8];         // adjoint of the stage input (before the ghost fold)
   double* gU0[8];        // adjoint of the step base (accumulated over the stages)
   double* gp[8];         // adjoint of ro, vx, vy, vz, cx, cy, bz, pr as the sweeps read them
@@ -87,19 +88,24 @@ __device__ __forceinline__ Img images(int q, int n)
 // unconditionally (a missing image re-reads the kept cell) and masked by
 // selects: with per-image branches, successive calls could not overlap
 // their memory latencies.
-__device__ __forceinline__ double fold_at(const double* f, int i, int j, const Geo& g)
+template <typename F>
+__device__ __forceinline__ double fold_with(const F& f, int i, int j, const Geo& g)
 {
   const Img ix = images(i, g.nx), iy = images(j, g.ny);
   const long long nx = g.nx;
   const int i1 = ix.n > 1 ? ix.i[1] : ix.i[0];
   const int j1 = iy.n > 1 ? iy.i[1] : iy.i[0];
-  const double a = f[nx * iy.i[0] + ix.i[0]], b = f[nx * iy.i[0] + i1];
-  const double c = f[nx * j1 + ix.i[0]], d = f[nx * j1 + i1];
+  const double a = f(nx * iy.i[0] + ix.i[0]), b = f(nx * iy.i[0] + i1);
+  const double c = f(nx * j1 + ix.i[0]), d = f(nx * j1 + i1);
   double v = a;
   v += ix.n > 1 ? b : 0.0;
   v += iy.n > 1 ? c : 0.0;
   v += (ix.n > 1 && iy.n > 1) ? d : 0.0;
   return v;
+}
+__device__ __forceinline__ double fold_at(const double* f, int i, int j, const Geo& g)
+{
+  return fold_with([f](long long q) { return f[q]; }, i, j, g);
 }
 
 __device__ __forceinline__ bool kept(int i, int n) { return i >= NG && i <= n - NG - 1; }
@@ -460,119 +466,140 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_xy(AdjPtrs px, AdjPtrs py,
   else adj_sweep_body<1, true>(py, g, gam, nsy, b - nbx);
 }
 
-// ---------------------------------------------------------------- prim
+// ---------------------------------------------------------------- cell
+// Adjoint of the stage input at cell s before the ghost fold (gi[0..7]):
+//   ro, mx, my, mz, bz, en: the primitive recovery (k_prim2d) with the CUCT
+//     cell terms, plus the RK pass-through b * g_out;
+//   bx, by: the face-B terms of the sweeps and the CUCT corners, the adjoint
+//     of fcen (cx, cy of the kept cells, with their ghost copies folded
+//     on), plus b * g_out.
+// The cell-centred B adjoint totals (sweep terms + the pressure term of
+// the primitive recovery) are formed where fcen's adjoint reads them.
 template <int CTW>
-__global__ __launch_bounds__(256) void k_adj_prim(AdjPtrs p, Geo g, double gam)
+__device__ __forceinline__ void gin_at(const AdjPtrs& p, const Geo& g, double gam, long long s, double (&gi)[8])
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.nd) return;
-  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  const int i = (int)(s % g.nx), j = (int)(s / g.nx);
   const long long nx = g.nx;
-  double cro = 0.0, cvx = 0.0, cvy = 0.0;
-  if (CTW) {
-    // this cell is entry `role` of the corners (i+1,j+1), (i,j+1), (i+1,j), (i,j)
-#pragma unroll
-    for (int role = 0; role < 4; role++) {
-      const int ic = i + ((role == 0 || role == 2) ? 1 : 0);
-      const int jc = j + (role <= 1 ? 1 : 0);
-      const bool ok = corner_ok(ic, jc, g);
-      const long long ss = ok ? nx * jc + ic : t;  // loads unconditional, terms selected
-      const double a0 = p.gcc[0][ss], a1 = p.gcc[1][ss], a2 = p.gcc[2][ss];
-      cro += ok ? a0 : 0.0;
-      cvx += ok ? a1 : 0.0;
-      cvy += ok ? a2 : 0.0;
-    }
-  }
-  double gro = p.gp[0][t] + cro;
-  double gvx = p.gp[1][t] + cvx, gvy = p.gp[2][t] + cvy, gvz = p.gp[3][t];
-  double gcx = p.gp[4][t], gcy = p.gp[5][t], gbz = p.gp[6][t];
-  const double gpr = p.gp[7][t];
-  // forward of k_prim2d
-  const double ro = p.in[0][t];
-  const double iro = 1.0 / ro;
-  const double mx = p.in[1][t], my = p.in[2][t], mz = p.in[3][t];
-  const double vx = mx * iro, vy = my * iro, vz = mz * iro;
-  const double cx = p.cx[t], cy = p.cy[t], cz = p.in[6][t];
-  const double v2 = vx * vx + vy * vy + vz * vz;
   const double gm1 = gam - 1.0;
-  // pr = (gam-1) (en - (ro v2 + b2) / 2)
-  const double gen = gm1 * gpr;
-  gro += -gm1 * 0.5 * v2 * gpr;
-  gvx += -gm1 * ro * vx * gpr;
-  gvy += -gm1 * ro * vy * gpr;
-  gvz += -gm1 * ro * vz * gpr;
-  gcx += -gm1 * cx * gpr;
-  gcy += -gm1 * cy * gpr;
-  gbz += -gm1 * cz * gpr;
-  // v = m / ro
-  const double giro = mx * gvx + my * gvy + mz * gvz;
-  gro += -iro * iro * giro;
   const double b = p.b;
-  // every load before the first store (the arrays could alias as far as the
-  // compiler knows, and interleaving would serialise the memory latencies)
-  double go[8], gu[8];
+  {  // primitive recovery
+    double cro = 0.0, cvx = 0.0, cvy = 0.0;
+    if (CTW) {
+      // this cell is entry `role` of the corners (i+1,j+1), (i,j+1), (i+1,j), (i,j)
 #pragma unroll
-  for (int m = 0; m < 8; m++) go[m] = p.gO[m][t];
-  const bool upd = p.a != 0.0;
-  if (upd)
-#pragma unroll
-    for (int m = 0; m < 8; m++) gu[m] = p.gU0[m][t];
-  p.gI[0][t] = b * go[0] + gro;
-  p.gI[1][t] = b * go[1] + iro * gvx;
-  p.gI[2][t] = b * go[2] + iro * gvy;
-  p.gI[3][t] = b * go[3] + iro * gvz;
-  p.gI[6][t] = b * go[6] + gbz;
-  p.gI[7][t] = b * go[7] + gen;
-  p.gp[4][t] = gcx;  // totals of the cell-centred B adjoint, for k_adj_face
-  p.gp[5][t] = gcy;
-  if (upd)
-#pragma unroll
-    for (int m = 0; m < 8; m++) p.gU0[m][t] = gu[m] + p.a * go[m];
-}
-
-// ---------------------------------------------------------------- face
-template <int CTW>
-__global__ __launch_bounds__(256) void k_adj_face(AdjPtrs p, Geo g)
-{
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.nd) return;
-  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
-  const long long nx = g.nx;
-  double gbx = p.b * p.gO[4][t] + p.gbf[0][t];
-  double gby = p.b * p.gO[5][t] + p.gbf[1][t];
+      for (int role = 0; role < 4; role++) {
+        const int ic = i + ((role == 0 || role == 2) ? 1 : 0);
+        const int jc = j + (role <= 1 ? 1 : 0);
+        const bool ok = corner_ok(ic, jc, g);
+        const long long ss = ok ? nx * jc + ic : s;  // loads unconditional, terms selected
+        const double a0 = p.gcc[0][ss], a1 = p.gcc[1][ss], a2 = p.gcc[2][ss];
+        cro += ok ? a0 : 0.0;
+        cvx += ok ? a1 : 0.0;
+        cvy += ok ? a2 : 0.0;
+      }
+    }
+    double gro = p.gp[0][s] + cro;
+    double gvx = p.gp[1][s] + cvx, gvy = p.gp[2][s] + cvy, gvz = p.gp[3][s];
+    double gbz = p.gp[6][s];
+    const double gpr = p.gp[7][s];
+    // forward of k_prim2d
+    const double ro = p.in[0][s];
+    const double iro = 1.0 / ro;
+    const double mx = p.in[1][s], my = p.in[2][s], mz = p.in[3][s];
+    const double vx = mx * iro, vy = my * iro, vz = mz * iro;
+    const double cz = p.in[6][s];
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    // pr = (gam-1) (en - (ro v2 + b2) / 2)
+    const double gen = gm1 * gpr;
+    gro += -gm1 * 0.5 * v2 * gpr;
+    gvx += -gm1 * ro * vx * gpr;
+    gvy += -gm1 * ro * vy * gpr;
+    gvz += -gm1 * ro * vz * gpr;
+    gbz += -gm1 * cz * gpr;
+    // v = m / ro
+    const double giro = mx * gvx + my * gvy + mz * gvz;
+    gro += -iro * iro * giro;
+    gi[0] = b * p.gO[0][s] + gro;
+    gi[1] = b * p.gO[1][s] + iro * gvx;
+    gi[2] = b * p.gO[2][s] + iro * gvy;
+    gi[3] = b * p.gO[3][s] + iro * gvz;
+    gi[6] = b * p.gO[6][s] + gbz;
+    gi[7] = b * p.gO[7][s] + gen;
+  }
+  // face B
+  double gbx = b * p.gO[4][s] + p.gbf[0][s];
+  double gby = b * p.gO[5][s] + p.gbf[1][s];
   if (CTW) {
     // bx(i,j) enters the corners (i,j) and (i,j+1); by(i,j) the corners
     // (i,j) and (i+1,j)
     const bool ok0 = corner_ok(i, j, g), ok1 = corner_ok(i, j + 1, g), ok2 = corner_ok(i + 1, j, g);
-    const long long s1 = ok1 ? nx * (j + 1) + i : t, s2 = ok2 ? nx * j + i + 1 : t;
-    const double b0 = p.gcc[3][t], c0 = p.gcc[4][t], b1 = p.gcc[3][s1], c2 = p.gcc[4][s2];
+    const long long s1 = ok1 ? nx * (j + 1) + i : s, s2 = ok2 ? nx * j + i + 1 : s;
+    const double b0 = p.gcc[3][s], c0 = p.gcc[4][s], b1 = p.gcc[3][s1], c2 = p.gcc[4][s2];
     gbx += ok0 ? b0 : 0.0;
     gby += ok0 ? c0 : 0.0;
     gbx += ok1 ? b1 : 0.0;
     gby += ok2 ? c2 : 0.0;
   }
-  // cx(q) = fcen(bx(q-1..q+2)) at kept q; ghost copies of cx fold onto q
+  // cx(q) = fcen(bx(q-1..q+2)) at kept q; ghost copies of cx fold onto q.
+  // Adjoint total of cx at q: sweep terms + pressure term -gm1 cx gpr.
+  auto tcx = [&](long long q) { return p.gp[4][q] + -gm1 * p.cx[q] * p.gp[7][q]; };
+  auto tcy = [&](long long q) { return p.gp[5][q] + -gm1 * p.cy[q] * p.gp[7][q]; };
   const double fw[4] = {-0.0625, 0.5625, 0.5625, -0.0625};  // offsets -1..2
   // (straight-line: out-of-range terms read a clamped kept cell and are
-  // dropped by selects, so all 32 loads can be in flight together)
+  // dropped by selects, so all the loads can be in flight together)
   const int jk = j < NG ? NG : (j > g.ny - NG - 1 ? g.ny - NG - 1 : j);
   const int ik = i < NG ? NG : (i > g.nx - NG - 1 ? g.nx - NG - 1 : i);
 #pragma unroll
   for (int kk = 0; kk < 4; kk++) {
     const int iq = i - (kk - 1);
     const int iqk = iq < NG ? NG : (iq > g.nx - NG - 1 ? g.nx - NG - 1 : iq);
-    const double v = fold_at(p.gp[4], iqk, jk, g);
+    const double v = fold_with(tcx, iqk, jk, g);
     gbx += (kept(j, g.ny) && kept(iq, g.nx)) ? fw[kk] * v : 0.0;
   }
 #pragma unroll
   for (int kk = 0; kk < 4; kk++) {
     const int jq = j - (kk - 1);
     const int jqk = jq < NG ? NG : (jq > g.ny - NG - 1 ? g.ny - NG - 1 : jq);
-    const double v = fold_at(p.gp[5], ik, jqk, g);
+    const double v = fold_with(tcy, ik, jqk, g);
     gby += (kept(i, g.nx) && kept(jq, g.ny)) ? fw[kk] * v : 0.0;
   }
-  p.gI[4][t] = gbx;
-  p.gI[5][t] = gby;
+  gi[4] = gbx;
+  gi[5] = gby;
+}
+
+// the stage input adjoint at every cell, ghosts included (gI)
+template <int CTW>
+__global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  double v[8];
+  gin_at<CTW>(p, g, gam, t, v);
+#pragma unroll
+  for (int m = 0; m < 8; m++) p.gI[m][t] = v[m];
+}
+
+// grid (cells, 8 variables): per kept cell, gI folded over the cell's
+// ghost images (the adjoint of bound()) becomes the output adjoint of the
+// previous stage, written in place of this stage's (each thread reads its
+// own entry first; ghost entries stay zero).  The step-base adjoint takes
+// a * g_out (set by the first stage processed, added by the others); the
+// last one processed adds it to its result.
+__global__ __launch_bounds__(256) void k_adj_foldstage(AdjPtrs p, Geo g, int first, int last)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nd) return;
+  const int m = blockIdx.y;
+  const int i = (int)(t % g.nx), j = (int)(t / g.nx);
+  if (!kept(i, g.nx) || !kept(j, g.ny)) return;
+  double v = fold_at(p.gI[m], i, j, g);
+  const double go = p.gO[m][t];
+  if (p.a != 0.0) {
+    const double gu = first ? 0.0 : p.gU0[m][t];
+    p.gU0[m][t] = first ? p.a * go : gu + p.a * go;
+  }
+  if (last) v += p.gU0[m][t];
+  p.gO[m][t] = v;
 }
 
 // ---------------------------------------------------------------- fold
@@ -595,13 +622,6 @@ __global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
     if (f.extra[m]) v += f.extra[m][t];
   }
   f.dst[m][t] = v;
-}
-
-__global__ __launch_bounds__(256) void k_adj_clear8(FoldArgs f, long long nd)
-{
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nd) return;
-  f.dst[blockIdx.y][t] = 0.0;
 }
 
 // ---------------------------------------------------------------- transposes
@@ -670,8 +690,8 @@ struct qas_adj_work {
   double* emf[3][8] = {};     // per stage: face parts (slots 0, 1, 4, 5 used)
   double* ez[3] = {};
   double* gO[8] = {};
-  double* gI[8] = {};
   double* gU0[8] = {};
+  double* gI[8] = {};
   double* gp[8] = {};
   double* gbf[2] = {};
   double* gemf[4] = {};
@@ -745,8 +765,8 @@ static int adj_workspace(qas_ctx* c)
     for (int k = 0; k < 8; k++) w->emf[s][k] = take();
   for (int s = 0; s < 3; s++) w->ez[s] = take();
   for (int m = 0; m < 8; m++) w->gO[m] = take();
-  for (int m = 0; m < 8; m++) w->gI[m] = take();
   for (int m = 0; m < 8; m++) w->gU0[m] = take();
+  for (int m = 0; m < 8; m++) w->gI[m] = take();
   for (int m = 0; m < 8; m++) w->gp[m] = take();
   for (int k = 0; k < 2; k++) w->gbf[k] = take();
   for (int k = 0; k < 4; k++) w->gemf[k] = take();
@@ -801,8 +821,8 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   for (int m = 0; m < 8; m++) {
     p.in[m] = in[m];
     p.gO[m] = w->gO[m];
-    p.gI[m] = w->gI[m];
     p.gU0[m] = w->gU0[m];
+    p.gI[m] = w->gI[m];
     p.gp[m] = w->gp[m];
     p.emf[m] = w->emf[rk][m];
   }
@@ -870,20 +890,10 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     const dim3 tb((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), 9);
     hipLaunchKernelGGL(k_adj_untranspose_add, tb, 256, 0, st, to, g.nx, g.ny);
   }
-  if (ctw) {
-    hipLaunchKernelGGL((k_adj_prim<1>), nb, 256, 0, st, p, g, c->s.gam);
-    hipLaunchKernelGGL((k_adj_face<1>), nb, 256, 0, st, p, g);
-  } else {
-    hipLaunchKernelGGL((k_adj_prim<0>), nb, 256, 0, st, p, g, c->s.gam);
-    hipLaunchKernelGGL((k_adj_face<0>), nb, 256, 0, st, p, g);
-  }
-  FoldArgs f;
-  for (int m = 0; m < 8; m++) {
-    f.src[m] = w->gI[m];
-    f.dst[m] = w->gO[m];
-    f.extra[m] = last ? w->gU0[m] : nullptr;
-  }
-  hipLaunchKernelGGL(k_adj_fold, dim3(nb, 8), 256, 0, st, f, g);
+  if (ctw) hipLaunchKernelGGL((k_adj_gin<1>), nb, 256, 0, st, p, g, c->s.gam);
+  else hipLaunchKernelGGL((k_adj_gin<0>), nb, 256, 0, st, p, g, c->s.gam);
+  // stage 3 is the first one processed, stage 1 the last
+  hipLaunchKernelGGL(k_adj_foldstage, dim3(nb, 8), 256, 0, st, p, g, rk == 2 ? 1 : 0, last ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -954,11 +964,6 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
     QAS_ACHECK(c, fwd_stage(c, 0, U, U, w->S[0], 0));
     QAS_ACHECK(c, fwd_stage(c, 1, w->S[0], U, w->S[1], 1));
     QAS_ACHECK(c, fwd_stage(c, 2, w->S[1], U, w->S[2], 2));
-    {
-      FoldArgs f;
-      for (int m = 0; m < 8; m++) f.dst[m] = w->gU0[m];
-      hipLaunchKernelGGL(k_adj_clear8, dim3(nb, 8), 256, 0, c->stream, f, g.nd);
-    }
     QAS_ACHECK(c, adj_stage(c, 2, w->S[1], false));
     QAS_ACHECK(c, adj_stage(c, 1, w->S[0], false));
     QAS_ACHECK(c, adj_stage(c, 0, U, true));
