@@ -138,9 +138,8 @@ __device__ __forceinline__ bool corner_ok(int i, int j, const Geo& g)
 // inputs enter through four-cell and two-face means, so one value per input
 // kind (gcc) serves every cell and face of the corner.
 template <int CTW>
-__global__ __launch_bounds__(256) void k_adj_corner(AdjPtrs p, Geo g)
+__device__ __forceinline__ void corner_at(const AdjPtrs& p, const Geo& g, long long t)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
   const long long nx = g.nx;
@@ -374,7 +373,9 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
         }
       }
       // face parts of the B flux: fbu = F_B - fc, fc = (ql(q-1) + qr(q)) / 2
-      const double gu = p.gemf[DIR * 2 + 0][fq], gc = p.gemf[DIR * 2 + 1][fq];
+      // (their adjoints are read in place also by the y pass: two loads)
+      const long long fe = TR ? (long long)g.nx * qc + line : fq;
+      const double gu = p.gemf[DIR * 2 + 0][fe], gc = p.gemf[DIR * 2 + 1][fe];
       gF[DIR == 0 ? 4 : 5] += gu;
       const double gfc = gc - gu;
       gqlf = 0.5 * gfc;
@@ -629,20 +630,21 @@ __global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
 // a 32 x 32 LDS tile (both global accesses run along a contiguous axis);
 // the back form accumulates dst[a][j * nx + i] += src[a][i * ny + j] for
 // a < nadd and stores it for the others.
-constexpr int NTR = 17;
+constexpr int NTR = 15;
 struct TrArgs {
   const double* src[NTR];
   double* dst[NTR];
   int nadd;
 };
 
-__global__ __launch_bounds__(256) void k_adj_transpose(TrArgs a, int nx, int ny)
+// tile (bx, by) of array bz: one 32 x 32 tile per 256-thread workgroup
+__device__ __forceinline__ void transpose_tile(const TrArgs& a, int nx, int ny, int bx, int by, int bz)
 {
   __shared__ double tile[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-  const double* s = a.src[blockIdx.z];
-  double* d = a.dst[blockIdx.z];
+  const int i0 = bx * 32, j0 = by * 32;
+  const double* s = a.src[bz];
+  double* d = a.dst[bz];
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     if (i < nx && j < ny) tile[r][tx] = s[(long long)j * nx + i];
@@ -652,6 +654,23 @@ __global__ __launch_bounds__(256) void k_adj_transpose(TrArgs a, int nx, int ny)
     const int j = j0 + tx, i = i0 + r;
     if (i < nx && j < ny) d[(long long)i * ny + j] = tile[tx][r];
   }
+}
+
+// One launch: workgroups [0, nbc) run the corner adjoint (one thread per
+// corner), the rest transpose the arrays the y line pass reads (they do not
+// depend on the corner outputs).
+template <int CTW>
+__global__ __launch_bounds__(256) void k_adj_corner_tr(AdjPtrs p, Geo g, TrArgs a, int nbc)
+{
+  const int b = (int)blockIdx.x;
+  if (b < nbc) {
+    corner_at<CTW>(p, g, (long long)b * blockDim.x + threadIdx.x);
+    return;
+  }
+  const int ntx = (g.nx + 31) / 32, nty = (g.ny + 31) / 32;
+  const int k = b - nbc;
+  const int bz = k / (ntx * nty), r = k - bz * (ntx * nty);
+  transpose_tile(a, g.nx, g.ny, r % ntx, r / ntx, bz);
 }
 
 __global__ __launch_bounds__(256) void k_adj_untranspose_add(TrArgs a, int nx, int ny)
@@ -700,6 +719,8 @@ struct qas_adj_work {
   double* tin[NTR] = {};      // transposed inputs of the y pass
   double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
+  double* d_dts = nullptr;    // the recorded step sizes (device)
+  const double* cur_dt = nullptr;  // d_dts + the step being reversed
   int ck_cap = 0;
   int nrec = 0;               // steps recorded
   std::vector<double> dts;
@@ -711,6 +732,7 @@ void adj_release(qas_ctx* c)
   if (!c || !c->adj) return;
   if (c->adj->slab) (void)hipFree(c->adj->slab);
   if (c->adj->ck) (void)hipFree(c->adj->ck);
+  if (c->adj->d_dts) (void)hipFree(c->adj->d_dts);
   delete c->adj;
   c->adj = nullptr;
 }
@@ -798,7 +820,7 @@ static hipError_t fwd_stage(qas_ctx* c, int s, double* const in[8], double* cons
   for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
   for (int k = 0; k < 8; k++) p.emf[k] = w->emf[s][k];
   p.ez = w->ez[s];
-  p.dt = c->d_dt;
+  p.dt = w->cur_dt;
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
   p.b = rb[rk];
@@ -833,29 +855,29 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   p.gez = w->gez;
   p.ctv = w->ctv;
   for (int k = 0; k < 5; k++) p.gcc[k] = w->gcc[k];
-  p.dt = c->d_dt;
+  p.dt = w->cur_dt;
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
   p.b = rb[rk];
   const unsigned nb = nblk(g.nd, 256);
   const bool ctw = c->s.ctw != 0;
-  if (ctw) hipLaunchKernelGGL((k_adj_corner<1>), nb, 256, 0, st, p, g);
-  else hipLaunchKernelGGL((k_adj_corner<0>), nb, 256, 0, st, p, g);
-  hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
   const int csx = (g.nx + SEGA - 1) / SEGA, csy = (g.ny + SEGA - 1) / SEGA;
   {
-    // y pass on transposed copies of what it reads (columns contiguous), its
-    // outputs added back onto gp, gbf[1]
+    // the y pass runs on transposed copies of what it reads (columns
+    // contiguous), made in the corner launch; its outputs are added back
+    // onto gp, gbf[1]
     const double* src[NTR] = {p.in[0], p.in[5], p.in[6], p.vx, p.vy, p.vz, p.cx, p.cy, p.pr,
-                              p.gO[0], p.gO[1], p.gO[2], p.gO[3], p.gO[6], p.gO[7], p.gemf[2], p.gemf[3]};
+                              p.gO[0], p.gO[1], p.gO[2], p.gO[3], p.gO[6], p.gO[7]};
     TrArgs ti;
     for (int k = 0; k < NTR; k++) {
       ti.src[k] = src[k];
       ti.dst[k] = w->tin[k];
     }
     ti.nadd = 0;
-    const dim3 tg((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), NTR);
-    hipLaunchKernelGGL(k_adj_transpose, tg, 256, 0, st, ti, g.nx, g.ny);
+    const unsigned ntr = (unsigned)(((g.nx + 31) / 32) * ((g.ny + 31) / 32) * NTR);
+    if (ctw) hipLaunchKernelGGL((k_adj_corner_tr<1>), nb + ntr, 256, 0, st, p, g, ti, (int)nb);
+    else hipLaunchKernelGGL((k_adj_corner_tr<0>), nb + ntr, 256, 0, st, p, g, ti, (int)nb);
+    hipLaunchKernelGGL(k_adj_faceparts, nb, 256, 0, st, p, g);
     AdjPtrs pt = p;
     pt.in[0] = w->tin[0];
     pt.in[5] = w->tin[1];
@@ -868,8 +890,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     pt.pr = w->tin[8];
     const int gom[6] = {0, 1, 2, 3, 6, 7};
     for (int v = 0; v < 6; v++) pt.gO[gom[v]] = w->tin[9 + v];
-    pt.gemf[2] = w->tin[15];
-    pt.gemf[3] = w->tin[16];
+    // (pt.gemf[2], pt.gemf[3]: in place, see the line pass)
     for (int m = 0; m < 8; m++) pt.gp[m] = w->tout[m];
     pt.gbf[1] = w->tout[8];
     // x pass in place, y pass on the transposed copies, one launch
@@ -912,13 +933,19 @@ extern "C" int qas_adj_forward(qas_ctx* c, int nsteps, const double* dts)
   const size_t per = 8 * w->stride;
   if (nsteps > w->ck_cap) {
     if (w->ck) QAS_ACHECK(c, hipFree(w->ck));
+    if (w->d_dts) QAS_ACHECK(c, hipFree(w->d_dts));
     w->ck = nullptr;
+    w->d_dts = nullptr;
     w->ck_cap = 0;
     hipError_t e = hipMalloc(&w->ck, per * (size_t)nsteps * sizeof(double));
     if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint checkpoints: ") + hipGetErrorName(e));
+    e = hipMalloc(&w->d_dts, (size_t)nsteps * sizeof(double));
+    if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint step sizes: ") + hipGetErrorName(e));
     w->ck_cap = nsteps;
   }
   w->dts.assign(dts, dts + nsteps);
+  QAS_ACHECK(c, hipMemcpyAsync(w->d_dts, w->dts.data(), (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
   w->nrec = 0;
   if (!c->bounded) {
     QAS_ACHECK(c, launch_bound(c, c->U[c->cur], 8));
@@ -957,7 +984,7 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
     QAS_ACHECK(c, hipGetLastError());
   }
   for (int n = w->nrec - 1; n >= 0; n--) {
-    QAS_ACHECK(c, launch_set_dt(c, w->dts[n]));
+    w->cur_dt = w->d_dts + n;  // the step's dt, read by the recompute and the stage adjoints
     double* U[8];
     for (int m = 0; m < 8; m++) U[m] = w->ck + per * n + w->stride * m;
     // forward recompute: stage inputs S1, S2 and every stage's intermediates
