@@ -468,16 +468,16 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_xy(AdjPtrs px, AdjPtrs py,
 }
 
 // ---------------------------------------------------------------- cell
-// Adjoint of the stage input at cell s before the ghost fold (gi[0..7]):
-//   ro, mx, my, mz, bz, en: the primitive recovery (k_prim2d) with the CUCT
-//     cell terms, plus the RK pass-through b * g_out;
-//   bx, by: the face-B terms of the sweeps and the CUCT corners, the adjoint
-//     of fcen (cx, cy of the kept cells, with their ghost copies folded
-//     on), plus b * g_out.
+// Adjoint of the stage input at cell s before the ghost fold (gI):
+//   gin_prim_at: ro, mx, my, mz, bz, en from the primitive recovery
+//     (k_prim2d) with the CUCT cell terms, plus the RK pass-through b * g_out;
+//   gin_face_at: bx, by from the face-B terms of the sweeps and the CUCT
+//     corners, the adjoint of fcen (cx, cy of the kept cells, with their
+//     ghost copies folded on), plus b * g_out.
 // The cell-centred B adjoint totals (sweep terms + the pressure term of
 // the primitive recovery) are formed where fcen's adjoint reads them.
 template <int CTW>
-__device__ __forceinline__ void gin_at(const AdjPtrs& p, const Geo& g, double gam, long long s, double (&gi)[8])
+__device__ __forceinline__ void gin_prim_at(const AdjPtrs& p, const Geo& g, double gam, long long s)
 {
   const int i = (int)(s % g.nx), j = (int)(s / g.nx);
   const long long nx = g.nx;
@@ -520,13 +520,24 @@ __device__ __forceinline__ void gin_at(const AdjPtrs& p, const Geo& g, double ga
     // v = m / ro
     const double giro = mx * gvx + my * gvy + mz * gvz;
     gro += -iro * iro * giro;
-    gi[0] = b * p.gO[0][s] + gro;
-    gi[1] = b * p.gO[1][s] + iro * gvx;
-    gi[2] = b * p.gO[2][s] + iro * gvy;
-    gi[3] = b * p.gO[3][s] + iro * gvz;
-    gi[6] = b * p.gO[6][s] + gbz;
-    gi[7] = b * p.gO[7][s] + gen;
+    const double go0 = p.gO[0][s], go1 = p.gO[1][s], go2 = p.gO[2][s], go3 = p.gO[3][s];
+    const double go6 = p.gO[6][s], go7 = p.gO[7][s];
+    p.gI[0][s] = b * go0 + gro;
+    p.gI[1][s] = b * go1 + iro * gvx;
+    p.gI[2][s] = b * go2 + iro * gvy;
+    p.gI[3][s] = b * go3 + iro * gvz;
+    p.gI[6][s] = b * go6 + gbz;
+    p.gI[7][s] = b * go7 + gen;
   }
+}
+
+template <int CTW>
+__device__ __forceinline__ void gin_face_at(const AdjPtrs& p, const Geo& g, double gam, long long s)
+{
+  const int i = (int)(s % g.nx), j = (int)(s / g.nx);
+  const long long nx = g.nx;
+  const double gm1 = gam - 1.0;
+  const double b = p.b;
   // face B
   double gbx = b * p.gO[4][s] + p.gbf[0][s];
   double gby = b * p.gO[5][s] + p.gbf[1][s];
@@ -564,20 +575,21 @@ __device__ __forceinline__ void gin_at(const AdjPtrs& p, const Geo& g, double ga
     const double v = fold_with(tcy, ik, jqk, g);
     gby += (kept(i, g.nx) && kept(jq, g.ny)) ? fw[kk] * v : 0.0;
   }
-  gi[4] = gbx;
-  gi[5] = gby;
+  p.gI[4][s] = gbx;
+  p.gI[5][s] = gby;
 }
 
-// the stage input adjoint at every cell, ghosts included (gI)
+// the stage input adjoint at every cell, ghosts included (gI): workgroups
+// [0, nbh) the six variables of the primitive recovery, the rest face B
+// (two independent halves: shorter per-thread chains on small domains)
 template <int CTW>
-__global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam)
+__global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam, int nbh)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)blockIdx.x;
+  const long long t = (long long)(b < nbh ? b : b - nbh) * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
-  double v[8];
-  gin_at<CTW>(p, g, gam, t, v);
-#pragma unroll
-  for (int m = 0; m < 8; m++) p.gI[m][t] = v[m];
+  if (b < nbh) gin_prim_at<CTW>(p, g, gam, t);
+  else gin_face_at<CTW>(p, g, gam, t);
 }
 
 // grid (cells, 8 variables): per kept cell, gI folded over the cell's
@@ -911,8 +923,8 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     const dim3 tb((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), 9);
     hipLaunchKernelGGL(k_adj_untranspose_add, tb, 256, 0, st, to, g.nx, g.ny);
   }
-  if (ctw) hipLaunchKernelGGL((k_adj_gin<1>), nb, 256, 0, st, p, g, c->s.gam);
-  else hipLaunchKernelGGL((k_adj_gin<0>), nb, 256, 0, st, p, g, c->s.gam);
+  if (ctw) hipLaunchKernelGGL((k_adj_gin<1>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
+  else hipLaunchKernelGGL((k_adj_gin<0>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
   // stage 3 is the first one processed, stage 1 the last
   hipLaunchKernelGGL(k_adj_foldstage, dim3(nb, 8), 256, 0, st, p, g, rk == 2 ? 1 : 0, last ? 1 : 0);
   return hipGetLastError();
