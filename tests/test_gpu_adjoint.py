@@ -39,9 +39,9 @@ def _with_vz_bz(p, a=0.1):
     return p
 
 
-def _setup(n=32, seed=5, ctw=1):
+def _setup(n=32, seed=5, ctw=1, ny=None):
     from qasmhd_amd import capi, problems
-    p = _with_vz_bz(problems.turbulence2d(n=n, seed=seed, vrms=0.3, brms=0.3))
+    p = _with_vz_bz(problems.turbulence2d(n=n, seed=seed, vrms=0.3, brms=0.3, ny=ny))
     s = capi.solver_for(p, ctw=ctw)
     s.upload(p.state)
     hist, _ = s.run(3, flg=1)          # a realistic dt schedule
@@ -64,10 +64,12 @@ def _dir_deriv(s, u0, dts, W, d, h):
     return (4.0 * D(h / 2) - D(h)) / 3.0
 
 
-@pytest.mark.parametrize("ctw", [1, 0])
-def test_adjoint_vs_fd(ctw):
-    print("case: adjoint vs FD, 32^2, 3 steps, ctw", ctw, flush=True)
-    p, s, dts = _setup(ctw=ctw)
+# (nx, ny): the y line pass runs on transposed copies and reads the EMF-part
+# adjoints in place, so a non-square domain checks both index maps
+@pytest.mark.parametrize("ctw,nx,ny", [(1, 32, 32), (0, 32, 32), (1, 40, 24), (1, 24, 40)])
+def test_adjoint_vs_fd(ctw, nx, ny):
+    print("case: adjoint vs FD, %dx%d, 3 steps, ctw %d" % (nx, ny, ctw), flush=True)
+    p, s, dts = _setup(n=nx, ny=ny, ctw=ctw)
     rng = np.random.default_rng(11)
     u0 = p.state.copy()
     W = rng.standard_normal(u0.shape)
@@ -106,6 +108,23 @@ def test_adjoint_linear_in_seed():
     g12 = s.adj_backward(2.0 * W1 - 3.0 * W2)
     s.close()
     assert np.allclose(g12, 2.0 * g1 - 3.0 * g2, rtol=1e-12, atol=1e-12 * np.abs(g12).max())
+
+
+def test_adjoint_device_entry_matches_host():
+    # qas_adj_backward_device on device-resident buffers gives the host entry's result
+    from qasmhd_amd import capi
+    print("case: device-resident backward vs host entry, 40x24", flush=True)
+    p, s, dts = _setup(n=40, ny=24, seed=9)
+    W = np.random.default_rng(4).standard_normal(p.state.shape)
+    s.upload(p.state)
+    s.adj_forward(dts)
+    g = s.adj_backward(W)
+    with capi.DeviceArray(W.shape) as dg, capi.DeviceArray(W.shape) as di:
+        dg.upload(W)
+        s.adj_backward_device(dg, di)
+        gd = di.download()
+    s.close()
+    assert np.array_equal(g, gd)
 
 
 def test_adjoint_rejects_unsupported():
