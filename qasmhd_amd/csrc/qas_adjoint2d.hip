@@ -732,6 +732,7 @@ struct qas_adj_work {
   double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
   double* d_dts = nullptr;    // the recorded step sizes (device)
+  double* stage = nullptr;    // qas_adj_backward's device copies of g_final, g_init
   const double* cur_dt = nullptr;  // d_dts + the step being reversed
   int ck_cap = 0;
   int nrec = 0;               // steps recorded
@@ -745,6 +746,7 @@ void adj_release(qas_ctx* c)
   if (c->adj->slab) (void)hipFree(c->adj->slab);
   if (c->adj->ck) (void)hipFree(c->adj->ck);
   if (c->adj->d_dts) (void)hipFree(c->adj->d_dts);
+  if (c->adj->stage) (void)hipFree(c->adj->stage);
   delete c->adj;
   c->adj = nullptr;
 }
@@ -1017,21 +1019,21 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
 extern "C" int qas_adj_backward(qas_ctx* c, const double* g_final, double* g_init)
 {
   if (!c || !c->slab || !g_final || !g_init) return afail(c, QAS_E_ARG, "qas_adj_backward: bad arguments");
+  if (!c->adj || c->adj->nrec == 0) return afail(c, QAS_E_STATE, "qas_adj_backward: no steps recorded");
+  qas_adj_work* w = c->adj;
   const size_t bytes = 8 * (size_t)c->geo.nd * sizeof(double);
-  double *dg = nullptr, *di = nullptr;
-  QAS_ACHECK(c, hipMalloc(&dg, bytes));
-  hipError_t e = hipMalloc(&di, bytes);
-  if (e != hipSuccess) {
-    (void)hipFree(dg);
-    return afail(c, QAS_E_NOMEM, "qas_adj_backward: staging");
+  if (!w->stage) {  // device staging of the two gradients, kept with the workspace
+    hipError_t e = hipMalloc(&w->stage, 2 * bytes);
+    if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("qas_adj_backward: staging: ") + hipGetErrorName(e));
   }
-  int r = QAS_OK;
-  e = hipMemcpy(dg, g_final, bytes, hipMemcpyHostToDevice);
-  if (e != hipSuccess) r = afail(c, QAS_E_HIP, "qas_adj_backward: upload");
-  if (!r) r = qas_adj_backward_device(c, dg, di);
-  if (!r && hipMemcpy(g_init, di, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-    r = afail(c, QAS_E_HIP, "qas_adj_backward: download");
-  (void)hipFree(dg);
-  (void)hipFree(di);
-  return r;
+  double* dg = w->stage;
+  double* di = w->stage + 8 * (size_t)c->geo.nd;
+  // copies on the context's stream (a null-stream copy would also wait for
+  // every other context's work)
+  QAS_ACHECK(c, hipMemcpyAsync(dg, g_final, bytes, hipMemcpyHostToDevice, c->stream));
+  int r = qas_adj_backward_device(c, dg, di);
+  if (r) return r;
+  QAS_ACHECK(c, hipMemcpyAsync(g_init, di, bytes, hipMemcpyDeviceToHost, c->stream));
+  QAS_ACHECK(c, hipStreamSynchronize(c->stream));
+  return QAS_OK;
 }

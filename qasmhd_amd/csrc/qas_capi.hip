@@ -225,12 +225,14 @@ extern "C" int qas_download(qas_ctx* c, double* const val[8])
 {
   if (!ready(c) || !val) return fail(c, QAS_E_ARG, "qas_download: null context or arrays");
   if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_download: no state uploaded");
-  QAS_CHECK(c, hipStreamSynchronize(c->stream));
   const size_t bytes = (size_t)c->geo.nd * sizeof(double);
-  for (int m = 0; m < 8; m++) {
+  for (int m = 0; m < 8; m++)
     if (!val[m]) return fail(c, QAS_E_ARG, "qas_download: null array");
-    QAS_CHECK(c, hipMemcpy(val[m], c->U[c->cur][m], bytes, hipMemcpyDeviceToHost));
-  }
+  // on the context's stream (a null-stream copy would also wait for every
+  // other context's work)
+  for (int m = 0; m < 8; m++)
+    QAS_CHECK(c, hipMemcpyAsync(val[m], c->U[c->cur][m], bytes, hipMemcpyDeviceToHost, c->stream));
+  QAS_CHECK(c, hipStreamSynchronize(c->stream));
   return QAS_OK;
 }
 
@@ -432,8 +434,11 @@ extern "C" int qas_run(qas_ctx* c, int nsteps, int flg, double* dt_inout, double
   QAS_CHECK(c, hipMemcpyAsync(c->h_pinned, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   QAS_CHECK(c, hipStreamSynchronize(c->stream));
   *dt_inout = c->h_pinned[0];
-  if (hist && nsteps > 0)
-    QAS_CHECK(c, hipMemcpy(dt_hist, c->d_hist, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost));
+  if (hist && nsteps > 0) {
+    QAS_CHECK(c, hipMemcpyAsync(dt_hist, c->d_hist, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost,
+                                c->stream));
+    QAS_CHECK(c, hipStreamSynchronize(c->stream));
+  }
   return QAS_OK;
 }
 

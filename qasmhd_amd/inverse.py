@@ -141,11 +141,13 @@ class VyModeInversion:
         return sum(self.instance_loss(i, a[i]) for i in range(self.ninst))
 
     # ------------------------------------------------------------ gradient
-    def instance_grad(self, i, a):
-        """(loss_i, dloss_i/da_i) by the discrete adjoint."""
+    def instance_grad(self, i, a, solver=None):
+        """(loss_i, dloss_i/da_i) by the discrete adjoint; `solver` (a
+        context of instance i's shape, kept open) is used when given."""
         tgt = self.targets()[i]
         u0 = self.state(i, a)
-        with self._solver(i) as s:
+        s = solver or self._solver(i)
+        try:
             s.upload(u0)
             s.adj_forward(self.dts[i])
             uf = s.download()
@@ -153,17 +155,38 @@ class VyModeInversion:
             g = np.zeros_like(uf)
             g[0, NG:-NG, NG:-NG] = 2.0 * r[NG:-NG, NG:-NG]
             gi = s.adj_backward(g)
+        finally:
+            if solver is None:
+                s.close()
         w = gi[2] * u0[0] + gi[7] * u0[2]
         grad = np.tensordot(self.psi[i], w, axes=([1, 2], [0, 1]))
         return float(np.sum(r[NG:-NG, NG:-NG] ** 2)), grad
 
-    def loss_and_grad(self, a):
+    def loss_and_grad(self, a, workers=4):
+        """Loss and gradient of all instances.  Each instance has its own
+        context (its own HIP stream); `workers` host threads drive them, so
+        up to that many instances' kernels run on the GPU side by side
+        (small domains alone leave most of it idle).  The contexts stay open
+        until every instance is done: freeing device memory waits for the
+        whole device.  Memory: each instance keeps 8 x nd doubles per step
+        of checkpoints (about 100 GB for the 64-instance workload)."""
         if self.dts is None:
             self.schedule()
+        self.targets()
+        solvers = [self._solver(i) for i in range(self.ninst)]
+        try:
+            if workers and workers > 1:
+                from concurrent.futures import ThreadPoolExecutor
+                with ThreadPoolExecutor(max_workers=workers) as ex:
+                    res = list(ex.map(lambda i: self.instance_grad(i, a[i], solvers[i]), range(self.ninst)))
+            else:
+                res = [self.instance_grad(i, a[i], solvers[i]) for i in range(self.ninst)]
+        finally:
+            for s in solvers:
+                s.close()
         L = 0.0
         G = np.zeros_like(np.asarray(a, dtype=np.float64))
-        for i in range(self.ninst):
-            li, gi = self.instance_grad(i, a[i])
+        for i, (li, gi) in enumerate(res):
             L += li
             G[i] = gi
         return L, G

@@ -57,6 +57,14 @@ def test_gradient_vs_fd(inv):
             assert abs(G[i, m] - fd) <= 1e-5 * scale, (i, m, G[i, m], fd)
 
 
+def test_threaded_gradient_equals_sequential(inv):
+    # instances on their own contexts and streams, driven by host threads
+    _say("case: inverse gradient, 3 threads vs 1")
+    L1, G1 = inv.loss_and_grad(inv.a0, workers=1)
+    L3, G3 = inv.loss_and_grad(inv.a0, workers=3)
+    assert L1 == L3 and np.array_equal(G1, G3)
+
+
 def test_loss_matches_oracle(inv, oracle):
     pyoracle = oracle
     _say("case: inverse loss vs oracle, instance 0")
