@@ -364,8 +364,9 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
             "batched_forward_s": t_fwd, "batched_forward_value": zones * steps / t_fwd,
             "gradient_s": t_grad, "ratio": t_grad / t_fwd, "fd_max_rel_err": max(errs) if errs else None,
             "fd_checked": nfd, "finite": bool(np.isfinite(G).all()),
-            "note": "gradient = per instance: forward with checkpoints + backward (3 recomputed stages and 3 stage "
-                    "adjoints per step) + host copies; forward = one batched run of all instances"}
+            "note": "gradient = per instance (own context and stream, 4 host threads): forward with checkpoints + "
+                    "backward (3 recomputed stages and 3 stage adjoints per step) + host copies; forward = one "
+                    "batched run of all instances"}
 
 
 def gravity_line(nx, ny, steps, cpu=True):
