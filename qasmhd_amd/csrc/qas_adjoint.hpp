@@ -370,6 +370,7 @@ QAS_HD void lrstate_adj(const T (&s)[7][5], T bn, T gam, const T (&gvl)[7], cons
 template <typename T>
 QAS_HD void lr_fb_adj(const T* p, const T* q, T bnm, T bnp, T gvl, T gvr, T* gp, T* gq, T& gbnm, T& gbnp)
 {
+  (void)p;  // the face states are linear in p
   T fl1, fr1;
   lin5<4>(q, fl1, fr1);
   lin4_adj(gvl, gvr, gp);

@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void kb_cfl(const BatchInst* I, const int* off
 // one block per instance
 __global__ __launch_bounds__(256) void kb_cfl_final(const BatchInst* I, int n)
 {
+  (void)n;  // the grid has one block per instance
   const BatchInst& B = I[blockIdx.x];
   cfl_final_block(B.cflpart, B.cflblocks, B.dt, nullptr, B.cfl_dr);
 }

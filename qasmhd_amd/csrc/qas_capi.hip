@@ -166,17 +166,18 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
 extern "C" void qas_destroy(qas_ctx* c)
 {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  // teardown: errors here have nowhere to go
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   adj_release(c);
   dsp_release(c);
   grav_release(c);
-  if (c->slab) hipFree(c->slab);
-  if (c->d_hist) hipFree(c->d_hist);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  if (c->h_pinned) hipHostFree(c->h_pinned);
-  for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->slab) (void)hipFree(c->slab);
+  if (c->d_hist) (void)hipFree(c->d_hist);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -370,7 +371,7 @@ static int build_pair_graph(qas_ctx* c, int flg, bool hist)
   c->cur = cur0;  // capture does not execute; restore the host view
   c->bounded = b0;
   if (r) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return r;
   }
   if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");

@@ -30,12 +30,12 @@ static inline CT operator*(CT a, CT b) { g_ops++; return CT(a.v * b.v); }
 static inline CT operator/(CT a, CT b) { g_ops++; return CT(a.v / b.v); }
 static inline CT operator-(CT a) { return CT(-a.v); }
 static inline CT operator+(CT a) { return a; }
-static inline CT& operator+=(CT& a, CT b) { a = a + b; return a; }
+[[maybe_unused]] static inline CT& operator+=(CT& a, CT b) { a = a + b; return a; }
 static inline CT& operator-=(CT& a, CT b) { a = a - b; return a; }
 static inline CT& operator*=(CT& a, CT b) { a = a * b; return a; }
 static inline CT& operator/=(CT& a, CT b) { a = a / b; return a; }
 static inline bool operator>(CT a, CT b) { return a.v > b.v; }
-static inline bool operator<(CT a, CT b) { return a.v < b.v; }
+[[maybe_unused]] static inline bool operator<(CT a, CT b) { return a.v < b.v; }
 static inline bool operator>=(CT a, CT b) { return a.v >= b.v; }
 static inline bool operator<=(CT a, CT b) { return a.v <= b.v; }
 static inline bool operator==(CT a, CT b) { return a.v == b.v; }
