@@ -222,6 +222,7 @@ int qas_point_flops(int rmn, int odr, double out[8]);
 #define QAS_ADJ_LRFB 3    /* B-flux face states                             */
 #define QAS_ADJ_CUCT 4    /* CUCT upwind weight                             */
 #define QAS_ADJ_HLLD 5    /* HLLD flux                                      */
+#define QAS_ADJ_MLAU 6    /* MLAU flux (incl. the shock-sensor inputs dv)   */
 int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long seed, double *relerr);
 
 /* ---- batched per-point evaluation on the device (tests) -----------------

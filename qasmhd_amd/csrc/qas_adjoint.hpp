@@ -783,4 +783,624 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   gbn += gbn_;
 }
 
+// ---------------------------------------------------------------- MLAU
+// gL, gR (7 each), gbn, gdv0, gdv1 += adjoint of calc_flux_mlau
+// (mhd_flux.c:7-192, flux_mlau of qas_device.hpp): the forward is recomputed
+// with the same expressions, then reversed statement by statement; min/max
+// route the derivative to the operand taken, the subsonic/supersonic
+// polynomial branches, the flux direction and the FL_EPS guards stay as
+// the forward takes them.
+template <typename T>
+QAS_HD void mlau_adj(const T* L, const T* R, T bnc, T gamma, T dv0, T dv1, const T* gF, T* gL, T* gR, T& gbn,
+                     T& gdv0, T& gdv1)
+{
+  const T FL_EPS = T(QAS_FL_EPS);
+  const T rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
+  const T ror = R[0], vnr = R[1], vtr = R[2], vur = R[3], btr = R[4], bur = R[5], prr = R[6];
+  // ---- forward
+  const T gammam1i = T(1.0) / (gamma - T(1.0));
+  const T bnc2 = bnc * bnc;
+  const T sgn = (bnc > T(0)) ? T(1.0) : T(-1.0);
+  const T bnca = sgn * bnc;
+  const T roli = T(1.0) / rol;
+  const T vl2 = vnl * vnl + vtl * vtl + vul * vul;
+  const T pml = T(0.5) * (btl * btl + bul * bul);
+  const T ptl = prl + pml;
+  const T enl = gammam1i * prl + pml + T(0.5) * rol * vl2;
+  const T hnl = (enl + ptl) * roli;
+  const T rori = T(1.0) / ror;
+  const T vr2 = vnr * vnr + vtr * vtr + vur * vur;
+  const T pmr = T(0.5) * (btr * btr + bur * bur);
+  const T ptr = prr + pmr;
+  const T enr = gammam1i * prr + pmr + T(0.5) * ror * vr2;
+  const T hnr = (enr + ptr) * rori;
+  const T cl2 = gamma * prl * roli;
+  const T cr2 = gamma * prr * rori;
+  const T cal2 = bnc2 * roli;
+  const T car2 = bnc2 * rori;
+  const T cbl2a = cl2 + cal2 + T(2.0) * pml * roli;
+  const T cbr2a = cr2 + car2 + T(2.0) * pmr * rori;
+  const T dl = cbl2a * cbl2a - T(4.0) * cl2 * cal2, dr = cbr2a * cbr2a - T(4.0) * cr2 * car2;
+  const T sql = dsqrt(dabs(dl)), sqr = dsqrt(dabs(dr));
+  const T cfl2 = T(0.5) * (cbl2a + sql);
+  const T cfr2 = T(0.5) * (cbr2a + sqr);
+  const T cbl2b = vl2 + cal2 + T(2.0) * pml * roli;
+  const T cbr2b = vr2 + car2 + T(2.0) * pmr * rori;
+  const T el = cbl2b * cbl2b - T(4.0) * vl2 * cal2, er = cbr2b * cbr2b - T(4.0) * vr2 * car2;
+  const T sel = dsqrt(dabs(el)), ser = dsqrt(dabs(er));
+  const T ccl2 = T(0.5) * (cbl2b + sel);
+  const T ccr2 = T(0.5) * (cbr2b + ser);
+  const T cmax = dsqrt(tmax(cfl2, cfr2));
+  const T cfn = cmax;
+  const T cfni = T(1.0) / cfn;
+  const T xsl = tmin(vnl, vnr) - cmax, xsr = tmax(vnl, vnr) + cmax;
+  const T sl = tmin(T(0.0), xsl);
+  const T sr = tmax(T(0.0), xsr);
+  const T mal = vnl * cfni;
+  const T mar = vnr * cfni;
+  const T snl = (mal > T(0)) ? T(1.0) : T(-1.0);
+  const T snr = (mar > T(0)) ? T(1.0) : T(-1.0);
+  const T tA = cfn - tmin(dv0, T(0.0)), tB = cfn - tmin(dv1, T(0.0));
+  const T tr = tA / tB;
+  const T theta0 = tmin(T(1.0), tr);
+  T theta = theta0;
+  theta *= theta;
+  theta *= theta;
+  const T alpha = T(0.1875);
+  const T beta = T(0.125);
+  const T malp1 = mal + T(1.0);
+  const T marm1 = mar - T(1.0);
+  const T mal2m1 = mal * mal - T(1.0);
+  const T mar2m1 = mar * mar - T(1.0);
+  const bool supl = snl * mal > T(1.0), supr = snr * mar > T(1.0);
+  T bfl, ffl, bfr, ffr;
+  if (supl) {
+    bfl = T(0.5) * (T(1.0) + snl);
+    ffl = bfl * mal;
+  } else {
+    bfl = +T(0.25) * malp1 * malp1 * (T(2.0) - mal) + alpha * mal * mal2m1 * mal2m1;
+    ffl = +T(0.25) * malp1 * malp1 + beta * mal2m1 * mal2m1;
+  }
+  if (supr) {
+    bfr = T(0.5) * (T(1.0) - snr);
+    ffr = bfr * mar;
+  } else {
+    bfr = +T(0.25) * marm1 * marm1 * (T(2.0) + mar) - alpha * mar * mar2m1 * mar2m1;
+    ffr = T(-0.25) * marm1 * marm1 - beta * mar2m1 * mar2m1;
+  }
+  const T mnc = ffl + ffr;
+  const T mflux0 = mnc * cfn;
+  const T omx = T(1.0) - dabs(mnc);
+  const T mx = tmax(omx, T(0.0));
+  const T rs = rol + ror;
+  const T dp = ptr - ptl;
+  const T mflux1 = mflux0 - mx * theta * cfni * dp / rs;
+  const T dir = (mflux1 > T(0)) ? T(1.0) : T(-1.0);
+  const T dirl = T(0.5) * (T(1.0) + dir);
+  const T dirr = T(1.0) - dirl;
+  const T rup = dirl * rol + dirr * ror;
+  const T mflux = mflux1 * rup;
+  const T mfl = mflux * dirl;
+  const T mfr = mflux - mfl;
+  const T ptot0 = bfl * ptl + bfr * ptr;
+  const T cfmod = dsqrt(tmax(ccl2, ccr2));
+  const T dvn = vnr - vnl;
+  const T ptot1 = ptot0 - T(0.25) * bfl * bfr * rs * cfmod * dvn;
+  const T k1 = T(1.0) - cfmod * cfni, k2 = bfl + bfr - T(1.0), k3 = ptl + ptr;
+  const T ptot = ptot1 - T(0.5) * k1 * k2 * k3;
+  const T slvl = sl - vnl;
+  const T srvr = sr - vnr;
+  const T snc = dirl * sl + dirr * sr;
+  const T vndnm = mflux + dirl * rol * slvl + dirr * ror * srvr;
+  const bool vdiv = (dabs(vndnm) > FL_EPS) && (snc != T(0));
+  const T vnc = vdiv ? (mflux * snc / vndnm) : (dirl * vnl + dirr * vnr);
+  const T slvc = sl - vnc;
+  const T srvc = sr - vnc;
+  const T slvci = T(1.0) / slvc;
+  const T srvci = T(1.0) / srvc;
+  const T rhdl = rol * slvl * slvc - bnc2;
+  const T rhdr = ror * srvr * srvc - bnc2;
+  const bool okl = dabs(rhdl) > FL_EPS, okr = dabs(rhdr) > FL_EPS;
+  T ql = T(0.0), qr = T(0.0);
+  T dvtl = T(0.0), dvul = T(0.0), dbtl = T(0.0), dbul = T(0.0);
+  T dvtr = T(0.0), dvur = T(0.0), dbtr = T(0.0), dbur = T(0.0);
+  if (okl) {
+    ql = (vnc - vnl) / rhdl;
+    dvtl = -bnc * btl * ql;
+    dvul = -bnc * bul * ql;
+    dbtl = bnc2 * btl * ql * slvci;
+    dbul = bnc2 * bul * ql * slvci;
+  }
+  if (okr) {
+    qr = (vnc - vnr) / rhdr;
+    dvtr = -bnc * btr * qr;
+    dvur = -bnc * bur * qr;
+    dbtr = bnc2 * btr * qr * srvci;
+    dbur = bnc2 * bur * qr * srvci;
+  }
+  const T cmpl = slvl * slvci;
+  const T cmpr = srvr * srvci;
+  const T ro2l = rol * cmpl;
+  const T srol = dsqrt(ro2l);
+  const T vt2l = dvtl + vtl, vu2l = dvul + vul;
+  const T bt2l = dbtl + btl * cmpl, bu2l = dbul + bul * cmpl;
+  const T ro2r = ror * cmpr;
+  const T sror = dsqrt(ro2r);
+  const T vt2r = dvtr + vtr, vu2r = dvur + vur;
+  const T bt2r = dbtr + btr * cmpr, bu2r = dbur + bur * cmpr;
+  const T sroi = T(1.0) / (srol + sror);
+  const T srollri = srol * sroi;
+  const T srorlri = sror * sroi;
+  // clamp(w) = min(max(0, w), bnca) as tmin(tmax(0, w), bnca); which: 0 -> 0, 1 -> w, 2 -> bnca
+  auto clampsel = [&](T w) -> int {
+    const T m = (T(0.0) >= w) ? T(0.0) : w;
+    return (m >= bnca) ? 2 : ((T(0.0) >= w) ? 0 : 1);
+  };
+  const T wflb = srollri * (bnca + sror * vnc);
+  const T wfrb = srorlri * (bnca - srol * vnc);
+  const T wflv = srorlri * (bnca + mflux / sror);
+  const T wfrv = srollri * (bnca - mflux / srol);
+  const int cflb = clampsel(wflb), cfrb = clampsel(wfrb), cflv = clampsel(wflv), cfrv = clampsel(wfrv);
+  auto clampval = [&](T w, int c) -> T { return c == 2 ? bnca : (c == 1 ? w : T(0.0)); };
+  const T bnflb = sgn * clampval(wflb, cflb);
+  const T bnfrb = sgn * clampval(wfrb, cfrb);
+  const T bnflv = sgn * clampval(wflv, cflv);
+  const T bnfrv = sgn * clampval(wfrv, cfrv);
+  const T zsrv = bnca - dir * vnc * (dirl * srol + dirr * sror);
+  const T bnsrv = tmax(zsrv, T(0.0));
+  const T ccb = sroi * bnsrv;
+  const T ccv = srol * sror * ccb;
+  const T bnvt = bnflb * vt2l + bnfrb * vt2r + ccb * (bt2r - bt2l) - vnc * (dirl * dbtl + dirr * dbtr);
+  const T bnvu = bnflb * vu2l + bnfrb * vu2r + ccb * (bu2r - bu2l) - vnc * (dirl * dbul + dirr * dbur);
+  const T bnbt = bnflv * bt2l + bnfrv * bt2r + ccv * (vt2r - vt2l) - mflux * (dirl * dvtl + dirr * dvtr);
+  const T bnbu = bnflv * bu2l + bnfrv * bu2r + ccv * (vu2r - vu2l) - mflux * (dirl * dvul + dirr * dvur);
+  const T fmt = mfl * vtl + mfr * vtr - bnbt;
+  const T fmu = mfl * vul + mfr * vur - bnbu;
+  const T fbt = mfl * btl * roli + mfr * btr * rori - bnvt;
+  const T fbu = mfl * bul * roli + mfr * bur * rori - bnvu;
+  const T vb1 = dirl * (vtl * btl + vul * bul) + dirr * (vtr * btr + vur * bur);
+  const T vb2 = dirl * (vt2l * bt2l + vu2l * bu2l) + dirr * (vt2r * bt2r + vu2r * bu2r);
+  const T rvb = mflux * vnc - bnc2;
+  const T rv2b2 = rvb * rvb;
+  const bool v3 = rv2b2 > FL_EPS * FL_EPS;
+  const T a1 = fmt * vnc + fbt * bnc, b1 = fmt * bnc + fbt * mflux;
+  const T a2 = fmu * vnc + fbu * bnc, b2 = fmu * bnc + fbu * mflux;
+  const T vb3 = v3 ? (a1 * b1 + a2 * b2) / rv2b2 : T(0.0);
+  const T bx = snc * vb2 - vnc * vb1;
+  const T by = dirl * slvci + dirr * srvci;
+  // ---- reverse
+  T grol = 0, gvnl = 0, gvtl = 0, gvul = 0, gbtl = 0, gbul = 0, gprl = 0;
+  T gror = 0, gvnr = 0, gvtr = 0, gvur = 0, gbtr = 0, gbur = 0, gprr = 0;
+  T gbnc = 0, gbnc2 = 0, gbnca = 0;
+  T gmflux = gF[0], gfmn = gF[1], gfmt = gF[2], gfmu = gF[3], gfbt = gF[4], gfbu = gF[5];
+  // F6 = mfl hnl + mfr hnr - bnvb
+  T gmfl = gF[6] * hnl, gmfr = gF[6] * hnr;
+  T ghnl = gF[6] * mfl, ghnr = gF[6] * mfr;
+  const T gbnvb = -gF[6];
+  // bnvb = sgn (bnca bx by + bnsrv (vb3 - vb2))
+  T gvb2 = 0, gvb1 = 0, gvb3 = 0, gsnc = 0, gvnc = 0, gslvci = 0, gsrvci = 0, gbnsrv = 0;
+  {
+    const T t = sgn * gbnvb;
+    gbnca += t * bx * by;
+    const T gbx = t * bnca * by, gby = t * bnca * bx;
+    gbnsrv += t * (vb3 - vb2);
+    gvb3 += t * bnsrv;
+    gvb2 += -t * bnsrv;
+    gsnc += gbx * vb2;
+    gvb2 += gbx * snc;
+    gvnc += -gbx * vb1;
+    gvb1 += -gbx * vnc;
+    gslvci += gby * dirl;
+    gsrvci += gby * dirr;
+  }
+  // vb3
+  if (v3) {
+    const T gN = gvb3 / rv2b2;
+    const T gD = -gvb3 * vb3 / rv2b2;
+    const T ga1 = gN * b1, gb1 = gN * a1, ga2 = gN * b2, gb2 = gN * a2;
+    gfmt += ga1 * vnc + gb1 * bnc;
+    gfbt += ga1 * bnc + gb1 * mflux;
+    gfmu += ga2 * vnc + gb2 * bnc;
+    gfbu += ga2 * bnc + gb2 * mflux;
+    gvnc += ga1 * fmt + ga2 * fmu;
+    gbnc += ga1 * fbt + gb1 * fmt + ga2 * fbu + gb2 * fmu;
+    gmflux += gb1 * fbt + gb2 * fbu;
+    const T grvb = T(2.0) * rvb * gD;
+    gmflux += grvb * vnc;
+    gvnc += grvb * mflux;
+    gbnc2 += -grvb;
+  }
+  // vb2, vb1
+  T gvt2l = gvb2 * dirl * bt2l, gbt2l = gvb2 * dirl * vt2l, gvu2l = gvb2 * dirl * bu2l, gbu2l = gvb2 * dirl * vu2l;
+  T gvt2r = gvb2 * dirr * bt2r, gbt2r = gvb2 * dirr * vt2r, gvu2r = gvb2 * dirr * bu2r, gbu2r = gvb2 * dirr * vu2r;
+  gvtl += gvb1 * dirl * btl;
+  gbtl += gvb1 * dirl * vtl;
+  gvul += gvb1 * dirl * bul;
+  gbul += gvb1 * dirl * vul;
+  gvtr += gvb1 * dirr * btr;
+  gbtr += gvb1 * dirr * vtr;
+  gvur += gvb1 * dirr * bur;
+  gbur += gvb1 * dirr * vur;
+  // fbu, fbt, fmu, fmt, fmn
+  T groli = 0, grori = 0;
+  gmfl += gfbu * bul * roli + gfbt * btl * roli + gfmu * vul + gfmt * vtl + gfmn * vnl;
+  gmfr += gfbu * bur * rori + gfbt * btr * rori + gfmu * vur + gfmt * vtr + gfmn * vnr;
+  gbul += gfbu * mfl * roli;
+  gbur += gfbu * mfr * rori;
+  gbtl += gfbt * mfl * roli;
+  gbtr += gfbt * mfr * rori;
+  groli += gfbu * mfl * bul + gfbt * mfl * btl;
+  grori += gfbu * mfr * bur + gfbt * mfr * btr;
+  gvul += gfmu * mfl;
+  gvur += gfmu * mfr;
+  gvtl += gfmt * mfl;
+  gvtr += gfmt * mfr;
+  gvnl += gfmn * mfl;
+  gvnr += gfmn * mfr;
+  const T gbnvu = -gfbu, gbnvt = -gfbt, gbnbu = -gfmu, gbnbt = -gfmt;
+  T gptot = gfmn;
+  gbnc2 += T(-0.5) * gfmn;
+  // bnbu, bnbt, bnvu, bnvt
+  T gdvul = 0, gdvur = 0, gdvtl = 0, gdvtr = 0, gdbul = 0, gdbur = 0, gdbtl = 0, gdbtr = 0;
+  gmflux += -gbnbu * (dirl * dvul + dirr * dvur) - gbnbt * (dirl * dvtl + dirr * dvtr);
+  gdvul += -gbnbu * mflux * dirl;
+  gdvur += -gbnbu * mflux * dirr;
+  gdvtl += -gbnbt * mflux * dirl;
+  gdvtr += -gbnbt * mflux * dirr;
+  gvnc += -gbnvu * (dirl * dbul + dirr * dbur) - gbnvt * (dirl * dbtl + dirr * dbtr);
+  gdbul += -gbnvu * vnc * dirl;
+  gdbur += -gbnvu * vnc * dirr;
+  gdbtl += -gbnvt * vnc * dirl;
+  gdbtr += -gbnvt * vnc * dirr;
+  T gbnflv = gbnbu * bu2l + gbnbt * bt2l, gbnfrv = gbnbu * bu2r + gbnbt * bt2r;
+  gbu2l += gbnbu * bnflv;
+  gbu2r += gbnbu * bnfrv;
+  gbt2l += gbnbt * bnflv;
+  gbt2r += gbnbt * bnfrv;
+  T gccv = gbnbu * (vu2r - vu2l) + gbnbt * (vt2r - vt2l);
+  gvu2r += gbnbu * ccv;
+  gvu2l -= gbnbu * ccv;
+  gvt2r += gbnbt * ccv;
+  gvt2l -= gbnbt * ccv;
+  T gbnflb = gbnvu * vu2l + gbnvt * vt2l, gbnfrb = gbnvu * vu2r + gbnvt * vt2r;
+  gvu2l += gbnvu * bnflb;
+  gvu2r += gbnvu * bnfrb;
+  gvt2l += gbnvt * bnflb;
+  gvt2r += gbnvt * bnfrb;
+  T gccb = gbnvu * (bu2r - bu2l) + gbnvt * (bt2r - bt2l);
+  gbu2r += gbnvu * ccb;
+  gbu2l -= gbnvu * ccb;
+  gbt2r += gbnvt * ccb;
+  gbt2l -= gbnvt * ccb;
+  // ccv = srol sror ccb ; ccb = sroi bnsrv
+  T gsrol = gccv * sror * ccb, gsror = gccv * srol * ccb;
+  gccb += gccv * srol * sror;
+  T gsroi = gccb * bnsrv;
+  gbnsrv += gccb * sroi;
+  // bnsrv = max(zsrv, 0)
+  if (zsrv >= T(0.0)) {
+    gbnca += gbnsrv;
+    gvnc += -gbnsrv * dir * (dirl * srol + dirr * sror);
+    gsrol += -gbnsrv * dir * vnc * dirl;
+    gsror += -gbnsrv * dir * vnc * dirr;
+  }
+  // the four clamps
+  T gsrollri = 0, gsrorlri = 0;
+  {
+    T g = sgn * gbnfrv;  // wfrv = srollri (bnca - mflux / srol)
+    if (cfrv == 2) gbnca += g;
+    else if (cfrv == 1) {
+      gsrollri += g * (bnca - mflux / srol);
+      gbnca += g * srollri;
+      gmflux += -g * srollri / srol;
+      gsrol += g * srollri * mflux / (srol * srol);
+    }
+    g = sgn * gbnflv;  // wflv = srorlri (bnca + mflux / sror)
+    if (cflv == 2) gbnca += g;
+    else if (cflv == 1) {
+      gsrorlri += g * (bnca + mflux / sror);
+      gbnca += g * srorlri;
+      gmflux += g * srorlri / sror;
+      gsror += -g * srorlri * mflux / (sror * sror);
+    }
+    g = sgn * gbnfrb;  // wfrb = srorlri (bnca - srol vnc)
+    if (cfrb == 2) gbnca += g;
+    else if (cfrb == 1) {
+      gsrorlri += g * (bnca - srol * vnc);
+      gbnca += g * srorlri;
+      gsrol += -g * srorlri * vnc;
+      gvnc += -g * srorlri * srol;
+    }
+    g = sgn * gbnflb;  // wflb = srollri (bnca + sror vnc)
+    if (cflb == 2) gbnca += g;
+    else if (cflb == 1) {
+      gsrollri += g * (bnca + sror * vnc);
+      gbnca += g * srollri;
+      gsror += g * srollri * vnc;
+      gvnc += g * srollri * sror;
+    }
+  }
+  gsrol += gsrollri * sroi;
+  gsroi += gsrollri * srol;
+  gsror += gsrorlri * sroi;
+  gsroi += gsrorlri * sror;
+  {
+    const T gsum = -gsroi * sroi * sroi;
+    gsrol += gsum;
+    gsror += gsum;
+  }
+  // right states
+  T gcmpr = 0, gcmpl = 0;
+  gdbur += gbu2r;
+  gbur += gbu2r * cmpr;
+  gcmpr += gbu2r * bur;
+  gdbtr += gbt2r;
+  gbtr += gbt2r * cmpr;
+  gcmpr += gbt2r * btr;
+  gdvur += gvu2r;
+  gvur += gvu2r;
+  gdvtr += gvt2r;
+  gvtr += gvt2r;
+  {
+    const T gro2r = gsror * (sror > T(0) ? T(0.5) / sror : T(0));
+    gror += gro2r * cmpr;
+    gcmpr += gro2r * ror;
+  }
+  gdbul += gbu2l;
+  gbul += gbu2l * cmpl;
+  gcmpl += gbu2l * bul;
+  gdbtl += gbt2l;
+  gbtl += gbt2l * cmpl;
+  gcmpl += gbt2l * btl;
+  gdvul += gvu2l;
+  gvul += gvu2l;
+  gdvtl += gvt2l;
+  gvtl += gvt2l;
+  {
+    const T gro2l = gsrol * (srol > T(0) ? T(0.5) / srol : T(0));
+    grol += gro2l * cmpl;
+    gcmpl += gro2l * rol;
+  }
+  T gsrvr = gcmpr * srvci, gslvl = gcmpl * slvci;
+  gsrvci += gcmpr * srvr;
+  gslvci += gcmpl * slvl;
+  // transverse corrections
+  T gslvc = 0, gsrvc = 0;
+  if (okr) {
+    const T gq = -bnc * btr * gdvtr - bnc * bur * gdvur + bnc2 * btr * srvci * gdbtr + bnc2 * bur * srvci * gdbur;
+    gbnc += -btr * qr * gdvtr - bur * qr * gdvur;
+    gbtr += -bnc * qr * gdvtr + bnc2 * qr * srvci * gdbtr;
+    gbur += -bnc * qr * gdvur + bnc2 * qr * srvci * gdbur;
+    gbnc2 += btr * qr * srvci * gdbtr + bur * qr * srvci * gdbur;
+    gsrvci += bnc2 * btr * qr * gdbtr + bnc2 * bur * qr * gdbur;
+    gvnc += gq / rhdr;
+    gvnr += -gq / rhdr;
+    const T grhd = -gq * qr / rhdr;
+    gror += grhd * srvr * srvc;
+    gsrvr += grhd * ror * srvc;
+    gsrvc += grhd * ror * srvr;
+    gbnc2 += -grhd;
+  }
+  if (okl) {
+    const T gq = -bnc * btl * gdvtl - bnc * bul * gdvul + bnc2 * btl * slvci * gdbtl + bnc2 * bul * slvci * gdbul;
+    gbnc += -btl * ql * gdvtl - bul * ql * gdvul;
+    gbtl += -bnc * ql * gdvtl + bnc2 * ql * slvci * gdbtl;
+    gbul += -bnc * ql * gdvul + bnc2 * ql * slvci * gdbul;
+    gbnc2 += btl * ql * slvci * gdbtl + bul * ql * slvci * gdbul;
+    gslvci += bnc2 * btl * ql * gdbtl + bnc2 * bul * ql * gdbul;
+    gvnc += gq / rhdl;
+    gvnl += -gq / rhdl;
+    const T grhd = -gq * ql / rhdl;
+    grol += grhd * slvl * slvc;
+    gslvl += grhd * rol * slvc;
+    gslvc += grhd * rol * slvl;
+    gbnc2 += -grhd;
+  }
+  gsrvc += -gsrvci * srvci * srvci;
+  gslvc += -gslvci * slvci * slvci;
+  T gsl = gslvc, gsr = gsrvc;
+  gvnc += -gslvc - gsrvc;
+  // vnc
+  if (vdiv) {
+    gmflux += gvnc * snc / vndnm;
+    gsnc += gvnc * mflux / vndnm;
+    const T gvd = -gvnc * vnc / vndnm;
+    gmflux += gvd;
+    grol += gvd * dirl * slvl;
+    gslvl += gvd * dirl * rol;
+    gror += gvd * dirr * srvr;
+    gsrvr += gvd * dirr * ror;
+  } else {
+    gvnl += gvnc * dirl;
+    gvnr += gvnc * dirr;
+  }
+  gsl += gsnc * dirl + gslvl;
+  gsr += gsnc * dirr + gsrvr;
+  gvnl -= gslvl;
+  gvnr -= gsrvr;
+  // ptot
+  T gbfl = 0, gbfr = 0, gptl = 0, gptr = 0, gcfmod = 0, gcfni = 0, grs = 0;
+  {
+    const T gk = T(-0.5) * gptot;
+    const T gk1 = gk * k2 * k3, gk2 = gk * k1 * k3, gk3 = gk * k1 * k2;
+    gcfmod += -gk1 * cfni;
+    gcfni += -gk1 * cfmod;
+    gbfl += gk2;
+    gbfr += gk2;
+    gptl += gk3;
+    gptr += gk3;
+    const T gP = -gptot;  // ptot1 = ptot0 - P, P = 0.25 bfl bfr rs cfmod dvn
+    gbfl += gP * T(0.25) * bfr * rs * cfmod * dvn;
+    gbfr += gP * T(0.25) * bfl * rs * cfmod * dvn;
+    grs += gP * T(0.25) * bfl * bfr * cfmod * dvn;
+    gcfmod += gP * T(0.25) * bfl * bfr * rs * dvn;
+    const T gdvn = gP * T(0.25) * bfl * bfr * rs * cfmod;
+    gvnr += gdvn;
+    gvnl -= gdvn;
+    gbfl += gptot * ptl;
+    gptl += gptot * bfl;
+    gbfr += gptot * ptr;
+    gptr += gptot * bfr;
+  }
+  T gccl2 = 0, gccr2 = 0;
+  {
+    const T gm = gcfmod * (cfmod > T(0) ? T(0.5) / cfmod : T(0));
+    if (ccl2 >= ccr2) gccl2 += gm;
+    else gccr2 += gm;
+  }
+  // mfl = mflux dirl, mfr = mflux - mfl
+  gmflux += gmfr;
+  gmfl -= gmfr;
+  gmflux += gmfl * dirl;
+  // mflux = mflux1 rup
+  const T gmflux1 = gmflux * rup;
+  {
+    const T grup = gmflux * mflux1;
+    grol += grup * dirl;
+    gror += grup * dirr;
+  }
+  // mflux1 = mflux0 - mx theta cfni dp / rs
+  T gmnc = gmflux1 * cfn, gcfn = gmflux1 * mnc, gtheta = 0;
+  {
+    const T gpd = -gmflux1;
+    const T pd = mx * theta * cfni * dp / rs;
+    const T gmx = gpd * theta * cfni * dp / rs;
+    gtheta += gpd * mx * cfni * dp / rs;
+    gcfni += gpd * mx * theta * dp / rs;
+    const T gdp = gpd * mx * theta * cfni / rs;
+    grs += -gpd * pd / rs;
+    gptr += gdp;
+    gptl -= gdp;
+    if (omx >= T(0.0)) gmnc += -gmx * sgnd(mnc);
+  }
+  grol += grs;
+  gror += grs;
+  // mnc = ffl + ffr ; the polynomial branches
+  T gmal = 0, gmar = 0;
+  {
+    const T gffl = gmnc, gffr = gmnc;
+    if (supl) gmal += gffl * bfl;
+    else {
+      const T dbfl = T(0.25) * (T(2.0) * malp1 * (T(2.0) - mal) - malp1 * malp1) +
+                     alpha * (mal2m1 * mal2m1 + T(4.0) * mal * mal * mal2m1);
+      const T dffl = T(0.5) * malp1 + T(4.0) * beta * mal * mal2m1;
+      gmal += gbfl * dbfl + gffl * dffl;
+    }
+    if (supr) gmar += gffr * bfr;
+    else {
+      const T dbfr = T(0.25) * (T(2.0) * marm1 * (T(2.0) + mar) + marm1 * marm1) -
+                     alpha * (mar2m1 * mar2m1 + T(4.0) * mar * mar * mar2m1);
+      const T dffr = T(-0.5) * marm1 - T(4.0) * beta * mar * mar2m1;
+      gmar += gbfr * dbfr + gffr * dffr;
+    }
+  }
+  // theta = theta0^4, theta0 = min(1, tA / tB)
+  if (T(1.0) >= tr) {
+    const T gtr = gtheta * T(4.0) * theta0 * theta0 * theta0;
+    const T gtA = gtr / tB, gtB = -gtr * tr / tB;
+    gcfn += gtA + gtB;
+    if (!(dv0 >= T(0.0))) gdv0 += -gtA;
+    if (!(dv1 >= T(0.0))) gdv1 += -gtB;
+  }
+  // mal = vnl cfni, mar = vnr cfni
+  gvnl += gmal * cfni;
+  gvnr += gmar * cfni;
+  gcfni += gmal * vnl + gmar * vnr;
+  // sl, sr
+  T gcmax = 0;
+  if (T(0.0) >= xsl) {  // sl = xsl
+    if (vnl >= vnr) gvnr += gsl;
+    else gvnl += gsl;
+    gcmax -= gsl;
+  }
+  if (!(T(0.0) >= xsr)) {  // sr = xsr
+    if (vnl >= vnr) gvnl += gsr;
+    else gvnr += gsr;
+    gcmax += gsr;
+  }
+  gcfn += -gcfni * cfni * cfni;
+  gcmax += gcfn;
+  T gcfl2 = 0, gcfr2 = 0;
+  {
+    const T gm = gcmax * (cmax > T(0) ? T(0.5) / cmax : T(0));
+    if (cfl2 >= cfr2) gcfl2 += gm;
+    else gcfr2 += gm;
+  }
+  T gvl2 = 0, gvr2 = 0, gpml = 0, gpmr = 0, gcal2 = 0, gcar2 = 0, gcl2 = 0, gcr2 = 0;
+  {  // cc*2 = 0.5 (cb*2b + sqrt|e*|), cb*2b = v*2 + ca*2 + 2 pm* ro*i
+    T gcb = T(0.5) * gccl2;
+    const T ge = T(0.5) * gccl2 * (sel > T(0) ? T(0.5) / sel : T(0)) * sgnd(el);
+    gcb += ge * T(2.0) * cbl2b;
+    gvl2 += -T(4.0) * cal2 * ge + gcb;
+    gcal2 += -T(4.0) * vl2 * ge + gcb;
+    gpml += T(2.0) * roli * gcb;
+    groli += T(2.0) * pml * gcb;
+    gcb = T(0.5) * gccr2;
+    const T ge2 = T(0.5) * gccr2 * (ser > T(0) ? T(0.5) / ser : T(0)) * sgnd(er);
+    gcb += ge2 * T(2.0) * cbr2b;
+    gvr2 += -T(4.0) * car2 * ge2 + gcb;
+    gcar2 += -T(4.0) * vr2 * ge2 + gcb;
+    gpmr += T(2.0) * rori * gcb;
+    grori += T(2.0) * pmr * gcb;
+  }
+  {  // cf*2 = 0.5 (cb*2a + sqrt|d*|), cb*2a = c*2 + ca*2 + 2 pm* ro*i
+    T gcb = T(0.5) * gcfl2;
+    const T gd = T(0.5) * gcfl2 * (sql > T(0) ? T(0.5) / sql : T(0)) * sgnd(dl);
+    gcb += gd * T(2.0) * cbl2a;
+    gcl2 += -T(4.0) * cal2 * gd + gcb;
+    gcal2 += -T(4.0) * cl2 * gd + gcb;
+    gpml += T(2.0) * roli * gcb;
+    groli += T(2.0) * pml * gcb;
+    gcb = T(0.5) * gcfr2;
+    const T gd2 = T(0.5) * gcfr2 * (sqr > T(0) ? T(0.5) / sqr : T(0)) * sgnd(dr);
+    gcb += gd2 * T(2.0) * cbr2a;
+    gcr2 += -T(4.0) * car2 * gd2 + gcb;
+    gcar2 += -T(4.0) * cr2 * gd2 + gcb;
+    gpmr += T(2.0) * rori * gcb;
+    grori += T(2.0) * pmr * gcb;
+  }
+  gbnc2 += gcal2 * roli + gcar2 * rori;
+  groli += gcal2 * bnc2 + gcl2 * gamma * prl;
+  grori += gcar2 * bnc2 + gcr2 * gamma * prr;
+  gprl += gcl2 * gamma * roli;
+  gprr += gcr2 * gamma * rori;
+  // h*n = (en* + pt*) ro*i ; en* = gm1i pr* + pm* + ro* v*2 / 2 ; pt* = pr* + pm*
+  {
+    const T genl = ghnl * roli;
+    gptl += ghnl * roli;
+    groli += ghnl * (enl + ptl);
+    gprl += genl * gammam1i;
+    gpml += genl;
+    grol += genl * T(0.5) * vl2;
+    gvl2 += genl * T(0.5) * rol;
+    const T genr = ghnr * rori;
+    gptr += ghnr * rori;
+    grori += ghnr * (enr + ptr);
+    gprr += genr * gammam1i;
+    gpmr += genr;
+    gror += genr * T(0.5) * vr2;
+    gvr2 += genr * T(0.5) * ror;
+  }
+  gprl += gptl;
+  gpml += gptl;
+  gprr += gptr;
+  gpmr += gptr;
+  gbtl += gpml * btl;
+  gbul += gpml * bul;
+  gbtr += gpmr * btr;
+  gbur += gpmr * bur;
+  gvnl += T(2.0) * vnl * gvl2;
+  gvtl += T(2.0) * vtl * gvl2;
+  gvul += T(2.0) * vul * gvl2;
+  gvnr += T(2.0) * vnr * gvr2;
+  gvtr += T(2.0) * vtr * gvr2;
+  gvur += T(2.0) * vur * gvr2;
+  grol += -groli * roli * roli;
+  gror += -grori * rori * rori;
+  gbnc += sgn * gbnca + T(2.0) * bnc * gbnc2;
+  gL[0] += grol; gL[1] += gvnl; gL[2] += gvtl; gL[3] += gvul; gL[4] += gbtl; gL[5] += gbul; gL[6] += gprl;
+  gR[0] += gror; gR[1] += gvnr; gR[2] += gvtr; gR[3] += gvur; gR[4] += gbtr; gR[5] += gbur; gR[6] += gprr;
+  gbn += gbnc;
+}
+
 }  // namespace qas

@@ -143,6 +143,32 @@ struct Hlld {
   }
 };
 
+// x = L[7], R[7], bn, dv0, dv1
+struct Mlau {
+  static const int NI = 17, NO = 7;
+  static constexpr double gam = 5.0 / 3.0;
+  static void sample(Rng& r, double* x)
+  {
+    // Mach up to about 2 so that the sub- and supersonic polynomial branches
+    // both occur; dv of either sign so that the pressure-diffusion weight
+    // theta varies
+    const double mach = r.in(0.0, 2.5);
+    for (int k = 0; k < 7; k++) {
+      const double base = prim_sample(r, k) * ((k >= 1 && k <= 3) ? mach : 1.0);
+      x[k] = base;
+      x[7 + k] = base * (1.0 + 0.5 * r.in(-1.0, 1.0)) + ((k >= 1 && k <= 5) ? 0.3 * r.in(-1.0, 1.0) : 0.0);
+    }
+    x[14] = r.in(-1.0, 1.0);
+    x[15] = r.in(-1.0, 1.0);
+    x[16] = r.in(-1.0, 1.0);
+  }
+  static void fwd(const double* x, double* y) { qas::flux_mlau<double>(x, x + 7, x[14], gam, x[15], x[16], y); }
+  static void adj(const double* x, const double* g, double* gx)
+  {
+    qas::mlau_adj(x, x + 7, x[14], gam, x[15], x[16], g, gx, gx + 7, gx[14], gx[15], gx[16]);
+  }
+};
+
 template <class F>
 void run_check(int ntrials, uint64_t seed, double* relerr)
 {
@@ -189,6 +215,7 @@ extern "C" int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long 
     case QAS_ADJ_LRFB: run_check<LrFb>(ntrials, seed, relerr); break;
     case QAS_ADJ_CUCT: run_check<Cuct>(ntrials, seed, relerr); break;
     case QAS_ADJ_HLLD: run_check<Hlld>(ntrials, seed, relerr); break;
+    case QAS_ADJ_MLAU: run_check<Mlau>(ntrials, seed, relerr); break;
     default: return QAS_E_ARG;
   }
   return QAS_OK;
