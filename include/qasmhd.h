@@ -164,9 +164,9 @@ void qas_batch_destroy(qas_batch *b);
 
 /* ---- discrete adjoint (no counterpart in the reference) ----------------
  * Reverse-mode derivative of `nsteps` steps of ideal(dt) (with the bound()
- * before the first step) at frozen step sizes.  Scope: 2D, FP64, HLLD
- * (rmn 1), odr 4, r_k 3, ctw 0 or 1, periodic on both axes; other schemes
- * are rejected with QAS_E_ARG.
+ * before the first step) at frozen step sizes.  Scope: 2D, FP64, HLLD or
+ * MLAU (rmn 1 or 3), odr 4, r_k 3, ctw 0 or 1, periodic on both axes;
+ * other schemes are rejected with QAS_E_ARG.
  * qas_adj_forward   advances the current state by nsteps steps with the
  *                   given dt values (as qas_step would), keeping each step's
  *                   input state on the device (8 x nd doubles per step).
@@ -177,7 +177,8 @@ void qas_batch_destroy(qas_batch *b);
  *                   entries zero, as the forward overwrites them).  Host
  *                   arrays; the _device variant takes device pointers.
  *                   Non-smooth operations (WCNS weights, wave-speed and
- *                   HLLD region selections, positivity fallback) are
+ *                   HLLD region selections, MLAU polynomial branches and
+ *                   shock sensor, positivity fallback) are
  *                   differentiated at the branch the forward takes. */
 int qas_adj_forward(qas_ctx *c, int nsteps, const double *dts);
 int qas_adj_backward(qas_ctx *c, const double *g_final, double *g_init);

@@ -1,7 +1,7 @@
 // qas_adjoint2d.hip -- hand-written discrete adjoint of the 2D step.
 //
-// Scope: FP64, HLLD (RMN 1), ODR 4 (WCNS-4 / 4th-order linear / fcen /
-// difference), SSP-RK3, CUCT on or off, periodic on both axes, dt frozen
+// Scope: FP64, HLLD or MLAU (RMN 1, 3), ODR 4 (WCNS-4 / 4th-order linear /
+// fcen / difference), SSP-RK3, CUCT on or off, periodic on both axes, dt frozen
 // (the step sizes are inputs, as a recorded dt history is).  The adjoint is
 // that of exactly the forward kernels of qas_kernels2d.hip / qas_sweep2d.hpp,
 // which reproduce MHD2D::ideal (2D/SERIAL/OTvortex/mhd2d_solve.cpp:3-400)
@@ -59,6 +59,7 @@ struct AdjPtrs {
   double* gp[8];         // adjoint of ro, vx, vy, vz, cx, cy, bz, pr as the sweeps read them
   double* gbf[2];        // adjoint of face bx (x sweep), by (y sweep)
   double* gemf[4];       // adjoint of the face parts: x upwind, x central, y upwind, y central
+  double* gdv[2];        // MLAU: adjoint of the shock sensor's transverse part dv[1] per x / y face
   double *gez, *ctv;     // corner adjoint of ez; forward CUCT weight
   double* gcc[5];        // per corner, the CUCT weight's input adjoints: ro, vx, vy (each of the
                          // four cells), bx, by (each of the two faces)
@@ -239,8 +240,23 @@ __device__ __forceinline__ long long line_base(const Geo& g, int line)
   return DIR == 0 ? g.sy * line : (TR ? (long long)g.ny * line : (long long)line);
 }
 
-// Forward of k_sweep2d (RMN 1, ODR 4) up to the face states, for the cell
-// of this lane; shared by the two adjoint sweep kernels.
+// The shock sensor's per-cell part (dv_at of the sweeps) with periodic
+// ghost resolution: cell (a, b) of an na x nb plane resolved to its kept
+// source, then min(v[Q+so] - v[Q], v[Q] - v[Q-so]) at Q = sa ra + sb rb.
+// The same subtractions on the same values as dv_at: bit-identical.
+__device__ __forceinline__ int per_res(int i, int n)
+{
+  return i < NG ? n - 2 * NG + i : (i >= n - NG ? i - (n - 2 * NG) : i);
+}
+__device__ __forceinline__ double dv_per(const double* v, int a, int b, int na, int nb, long long sa, long long sb,
+                                         long long so)
+{
+  const long long Q = sa * per_res(a, na) + sb * per_res(b, nb);
+  return tmin(v[Q + so] - v[Q], v[Q] - v[Q - so]);
+}
+
+// Forward of k_sweep2d (ODR 4) up to the face states, for the cell of this
+// lane.
 template <int DIR>
 struct SweepFwd {
   const double *vn, *vt, *vu, *bt, *bu, *cn, *bf;
@@ -314,9 +330,10 @@ struct SlotAcc {
 };
 
 // blk: this workgroup's index within the pass
-template <int DIR, bool TR>
+template <int RMN, int DIR, bool TR>
 __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, double gam, int nseg, int blk)
 {
+  static_assert(RMN == 1 || RMN == 3, "HLLD or MLAU");
   static_assert(TR == (DIR == 1), "the y pass runs on transposed arrays, the x pass in place");
   __shared__ double gacc[256 / WAVE][7 * (WAVE + 4)];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -349,7 +366,7 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
 #pragma unroll
     for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);
     double gL[7] = {};
-    double gqlf = 0.0;
+    double gqlf = 0.0, gdv1 = 0.0;
 #pragma unroll
     for (int k = 0; k < 7; k++) gvr[k] = 0.0;
     if (lane >= 1 && q >= 3 && q <= n - 3) {
@@ -381,7 +398,32 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
       gqlf = 0.5 * gfc;
       gqr = 0.5 * gfc;
       const double bnf = 0.5 * (f.bf[fq] + f.bf[fq]);
-      hlld_adj(L, vr, bnf, gam, gF, gL, gvr, gbn);  // d(0.5 (b + b))/db = 1
+      if constexpr (RMN == 3) {
+        // shock sensor of the face (k_sweep2d / k_sweep2d_y): dv0 along the
+        // line, dv1 from the transverse differences of the two cells
+        const double* vn = DIR == 0 ? p.vx : p.vy;
+        const double dv0 = vn[fq] - vn[fq - o];
+        double dv1;
+        if (DIR == 0)  // dvy of cells (q-1, line), (q, line)
+          dv1 = tmin(dv_per(p.vy, q - 1, line, g.nx, g.ny, 1, g.nx, g.nx),
+                     dv_per(p.vy, q, line, g.nx, g.ny, 1, g.nx, g.nx));
+        else  // dvx of cells (line, q-1), (line, q), on the transposed vx
+          dv1 = tmin(dv_per(p.vx, line, q - 1, g.nx, g.ny, g.ny, 1, g.ny),
+                     dv_per(p.vx, line, q, g.nx, g.ny, g.ny, 1, g.ny));
+        double gdv0 = 0.0;
+        mlau_adj(L, vr, bnf, gam, dv0, dv1, gF, gL, gvr, gbn, gdv0, gdv1);
+        // dv0 = vn(q) - vn(q-1): slots of cells q (lane + 2) and q-1 (lane + 1)
+        atomicAdd(&row[1 * (WAVE + 4) + lane + 2], gdv0);
+        atomicAdd(&row[1 * (WAVE + 4) + lane + 1], -gdv0);
+      } else {
+        hlld_adj(L, vr, bnf, gam, gF, gL, gvr, gbn);  // d(0.5 (b + b))/db = 1
+      }
+    }
+    if constexpr (RMN == 3) {
+      // dv1's adjoint per face, natural layout, one writer per face (the
+      // owning lanes); gin adds it onto the transverse velocity
+      if (lane >= 3 && lane < 3 + SEGA && q >= 0 && q < n)
+        p.gdv[DIR][DIR == 0 ? base + q : (long long)g.nx * q + line] = gdv1;
     }
     // this cell's left state is the left state of face q+1 (lane l+1)
 #pragma unroll
@@ -459,12 +501,13 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
 // SIMD), so running them side by side saves a partly empty round.  One
 // wave per SIMD: the reverse reconstruction needs about 350 registers (held
 // to 256 for two waves it spills and runs slower, measured).
+template <int RMN>
 __global__ __launch_bounds__(256, 1) void k_adj_sweep_xy(AdjPtrs px, AdjPtrs py, Geo g, double gam, int nsx,
                                                          int nsy, int nbx)
 {
   const int b = (int)blockIdx.x;
-  if (b < nbx) adj_sweep_body<0, false>(px, g, gam, nsx, b);
-  else adj_sweep_body<1, true>(py, g, gam, nsy, b - nbx);
+  if (b < nbx) adj_sweep_body<RMN, 0, false>(px, g, gam, nsx, b);
+  else adj_sweep_body<RMN, 1, true>(py, g, gam, nsy, b - nbx);
 }
 
 // ---------------------------------------------------------------- cell
@@ -476,7 +519,80 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_xy(AdjPtrs px, AdjPtrs py,
 //     ghost copies folded on), plus b * g_out.
 // The cell-centred B adjoint totals (sweep terms + the pressure term of
 // the primitive recovery) are formed where fcen's adjoint reads them.
-template <int CTW>
+// MLAU: the adjoint of the faces' dv1 = min(dvt(left cell), dvt(right
+// cell)) onto the transverse velocity of cell (i, j), dvt(c) = min(fwd,
+// bwd) of that velocity at c's resolved kept cell.  Gather form: for each
+// kept centre Q within one cell of (i, j) along the difference axis whose
+// selected difference reads (i, j), every face on the periodic images of Q
+// that selected Q adds its stored adjoint.
+__device__ __forceinline__ double dv1_adj_vy(const AdjPtrs& p, const Geo& g, int i, int j)
+{
+  const long long nx = g.nx;
+  const double* vy = p.vy;
+  double acc = 0.0;
+#pragma unroll
+  for (int d = -1; d <= 1; d++) {  // (i, j) = Q + d rows
+    const int jc = j - d;
+    if (!kept(i, g.nx) || !kept(jc, g.ny)) continue;
+    const long long Q = nx * jc + i;
+    const double fwd = vy[Q + nx] - vy[Q], bwd = vy[Q] - vy[Q - nx];
+    const bool useb = fwd >= bwd;
+    const double coef = d == 1 ? (useb ? 0.0 : 1.0) : (d == 0 ? (useb ? 1.0 : -1.0) : (useb ? -1.0 : 0.0));
+    if (coef == 0.0) continue;
+    const double dq = useb ? bwd : fwd;
+    const Img ir = images(jc, g.ny), ic = images(i, g.nx);
+    for (int b = 0; b < ir.n; b++)
+      for (int a = 0; a < ic.n; a++) {
+        const int r = ir.i[b], col = ic.i[a];
+        int f = col + 1;  // this cell on the face's left
+        if (f >= 3 && f <= g.nx - 3) {
+          const double other = dv_per(vy, col + 1, r, g.nx, g.ny, 1, nx, nx);
+          if (!(dq >= other)) acc += coef * p.gdv[0][nx * r + f];
+        }
+        f = col;  // this cell on the face's right
+        if (f >= 3 && f <= g.nx - 3) {
+          const double other = dv_per(vy, col - 1, r, g.nx, g.ny, 1, nx, nx);
+          if (other >= dq) acc += coef * p.gdv[0][nx * r + f];
+        }
+      }
+  }
+  return acc;
+}
+__device__ __forceinline__ double dv1_adj_vx(const AdjPtrs& p, const Geo& g, int i, int j)
+{
+  const long long nx = g.nx;
+  const double* vx = p.vx;
+  double acc = 0.0;
+#pragma unroll
+  for (int d = -1; d <= 1; d++) {  // (i, j) = Q + d columns
+    const int icq = i - d;
+    if (!kept(icq, g.nx) || !kept(j, g.ny)) continue;
+    const long long Q = nx * j + icq;
+    const double fwd = vx[Q + 1] - vx[Q], bwd = vx[Q] - vx[Q - 1];
+    const bool useb = fwd >= bwd;
+    const double coef = d == 1 ? (useb ? 0.0 : 1.0) : (d == 0 ? (useb ? 1.0 : -1.0) : (useb ? -1.0 : 0.0));
+    if (coef == 0.0) continue;
+    const double dq = useb ? bwd : fwd;
+    const Img ic = images(icq, g.nx), ir = images(j, g.ny);
+    for (int a = 0; a < ic.n; a++)
+      for (int b = 0; b < ir.n; b++) {
+        const int c = ic.i[a], rr = ir.i[b];
+        int f = rr + 1;  // this cell below the face
+        if (f >= 3 && f <= g.ny - 3) {
+          const double other = dv_per(vx, c, rr + 1, g.nx, g.ny, 1, nx, 1);
+          if (!(dq >= other)) acc += coef * p.gdv[1][nx * f + c];
+        }
+        f = rr;  // this cell above the face
+        if (f >= 3 && f <= g.ny - 3) {
+          const double other = dv_per(vx, c, rr - 1, g.nx, g.ny, 1, nx, 1);
+          if (other >= dq) acc += coef * p.gdv[1][nx * f + c];
+        }
+      }
+  }
+  return acc;
+}
+
+template <int CTW, bool DV>
 __device__ __forceinline__ void gin_prim_at(const AdjPtrs& p, const Geo& g, double gam, long long s)
 {
   const int i = (int)(s % g.nx), j = (int)(s / g.nx);
@@ -501,6 +617,10 @@ __device__ __forceinline__ void gin_prim_at(const AdjPtrs& p, const Geo& g, doub
     }
     double gro = p.gp[0][s] + cro;
     double gvx = p.gp[1][s] + cvx, gvy = p.gp[2][s] + cvy, gvz = p.gp[3][s];
+    if (DV) {  // MLAU shock sensor, transverse part
+      gvx += dv1_adj_vx(p, g, i, j);
+      gvy += dv1_adj_vy(p, g, i, j);
+    }
     double gbz = p.gp[6][s];
     const double gpr = p.gp[7][s];
     // forward of k_prim2d
@@ -582,13 +702,13 @@ __device__ __forceinline__ void gin_face_at(const AdjPtrs& p, const Geo& g, doub
 // the stage input adjoint at every cell, ghosts included (gI): workgroups
 // [0, nbh) the six variables of the primitive recovery, the rest face B
 // (two independent halves: shorter per-thread chains on small domains)
-template <int CTW>
+template <int CTW, bool DV>
 __global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam, int nbh)
 {
   const int b = (int)blockIdx.x;
   const long long t = (long long)(b < nbh ? b : b - nbh) * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
-  if (b < nbh) gin_prim_at<CTW>(p, g, gam, t);
+  if (b < nbh) gin_prim_at<CTW, DV>(p, g, gam, t);
   else gin_face_at<CTW>(p, g, gam, t);
 }
 
@@ -726,6 +846,7 @@ struct qas_adj_work {
   double* gp[8] = {};
   double* gbf[2] = {};
   double* gemf[4] = {};
+  double* gdv[2] = {};
   double *gez = nullptr, *ctv = nullptr;
   double* gcc[5] = {};
   double* tin[NTR] = {};      // transposed inputs of the y pass
@@ -768,7 +889,7 @@ static std::string adj_supported(const qas_ctx* c)
 {
   if (c->g.ndim != 2) return "the adjoint covers the 2D path only";
   if (c->s.precision != 0) return "the adjoint covers the FP64 path only";
-  if (c->s.rmn != 1) return "the adjoint covers the HLLD solver (rmn 1) only";
+  if (c->s.rmn != 1 && c->s.rmn != 3) return "the adjoint covers the HLLD and MLAU solvers (rmn 1, 3) only";
   if (c->s.odr != 4) return "the adjoint covers odr 4 only";
   if (c->s.r_k != 3) return "the adjoint covers r_k 3 only";
   if (c->d_phi || c->d_nu) return "the adjoint covers the ideal step without gravity or dissipation";
@@ -785,7 +906,7 @@ static int adj_workspace(qas_ctx* c)
   c->adj = w;
   const size_t stride = (size_t)((c->geo.nd + 63) / 64 * 64);
   w->stride = stride;
-  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 2 + 5 + NTR + 9;
+  const size_t narr = 24 + 18 + 24 + 3 + 24 + 8 + 2 + 4 + 2 + 2 + 5 + NTR + 9;
   hipError_t e = hipMalloc(&w->slab, narr * stride * sizeof(double));
   if (e != hipSuccess) {
     adj_release(c);
@@ -806,6 +927,7 @@ static int adj_workspace(qas_ctx* c)
   for (int m = 0; m < 8; m++) w->gp[m] = take();
   for (int k = 0; k < 2; k++) w->gbf[k] = take();
   for (int k = 0; k < 4; k++) w->gemf[k] = take();
+  for (int k = 0; k < 2; k++) w->gdv[k] = take();
   w->gez = take();
   w->ctv = take();
   for (int k = 0; k < 5; k++) w->gcc[k] = take();
@@ -866,6 +988,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   p.cx = w->prim[rk][4]; p.cy = w->prim[rk][5];
   for (int k = 0; k < 2; k++) p.gbf[k] = w->gbf[k];
   for (int k = 0; k < 4; k++) p.gemf[k] = w->gemf[k];
+  for (int k = 0; k < 2; k++) p.gdv[k] = w->gdv[k];
   p.gez = w->gez;
   p.ctv = w->ctv;
   for (int k = 0; k < 5; k++) p.gcc[k] = w->gcc[k];
@@ -909,7 +1032,10 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     pt.gbf[1] = w->tout[8];
     // x pass in place, y pass on the transposed copies, one launch
     const unsigned nbx = nblk((long long)csx * g.ny * WAVE, 256), nby = nblk((long long)csy * g.nx * WAVE, 256);
-    hipLaunchKernelGGL(k_adj_sweep_xy, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
+    if (c->s.rmn == 3)
+      hipLaunchKernelGGL(k_adj_sweep_xy<3>, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
+    else
+      hipLaunchKernelGGL(k_adj_sweep_xy<1>, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
     TrArgs to;
     for (int k = 0; k < NTR; k++) {
       to.src[k] = nullptr;
@@ -925,8 +1051,11 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     const dim3 tb((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), 9);
     hipLaunchKernelGGL(k_adj_untranspose_add, tb, 256, 0, st, to, g.nx, g.ny);
   }
-  if (ctw) hipLaunchKernelGGL((k_adj_gin<1>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
-  else hipLaunchKernelGGL((k_adj_gin<0>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
+  const bool dv = c->s.rmn == 3;
+  if (ctw && dv) hipLaunchKernelGGL((k_adj_gin<1, true>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
+  else if (ctw) hipLaunchKernelGGL((k_adj_gin<1, false>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
+  else if (dv) hipLaunchKernelGGL((k_adj_gin<0, true>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
+  else hipLaunchKernelGGL((k_adj_gin<0, false>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
   // stage 3 is the first one processed, stage 1 the last
   hipLaunchKernelGGL(k_adj_foldstage, dim3(nb, 8), 256, 0, st, p, g, rk == 2 ? 1 : 0, last ? 1 : 0);
   return hipGetLastError();

@@ -39,10 +39,10 @@ def _with_vz_bz(p, a=0.1):
     return p
 
 
-def _setup(n=32, seed=5, ctw=1, ny=None):
+def _setup(n=32, seed=5, ctw=1, ny=None, rmn=1):
     from qasmhd_amd import capi, problems
     p = _with_vz_bz(problems.turbulence2d(n=n, seed=seed, vrms=0.3, brms=0.3, ny=ny))
-    s = capi.solver_for(p, ctw=ctw)
+    s = capi.solver_for(p, ctw=ctw, rmn=rmn)
     s.upload(p.state)
     hist, _ = s.run(3, flg=1)          # a realistic dt schedule
     return p, s, hist
@@ -65,11 +65,14 @@ def _dir_deriv(s, u0, dts, W, d, h):
 
 
 # (nx, ny): the y line pass runs on transposed copies and reads the EMF-part
-# adjoints in place, so a non-square domain checks both index maps
-@pytest.mark.parametrize("ctw,nx,ny", [(1, 32, 32), (0, 32, 32), (1, 40, 24), (1, 24, 40)])
-def test_adjoint_vs_fd(ctw, nx, ny):
-    print("case: adjoint vs FD, %dx%d, 3 steps, ctw %d" % (nx, ny, ctw), flush=True)
-    p, s, dts = _setup(n=nx, ny=ny, ctw=ctw)
+# adjoints in place, so a non-square domain checks both index maps.  rmn 3
+# (MLAU) adds the shock sensor: dv0 along the line, dv1 from the transverse
+# velocity differences (gathered onto cells of the neighbouring lines).
+@pytest.mark.parametrize("rmn,ctw,nx,ny", [(1, 1, 32, 32), (1, 0, 32, 32), (1, 1, 40, 24), (1, 1, 24, 40),
+                                           (3, 1, 32, 32), (3, 0, 32, 32), (3, 1, 40, 24), (3, 1, 24, 40)])
+def test_adjoint_vs_fd(rmn, ctw, nx, ny):
+    print("case: adjoint vs FD, rmn %d, %dx%d, 3 steps, ctw %d" % (rmn, nx, ny, ctw), flush=True)
+    p, s, dts = _setup(n=nx, ny=ny, ctw=ctw, rmn=rmn)
     rng = np.random.default_rng(11)
     u0 = p.state.copy()
     W = rng.standard_normal(u0.shape)
@@ -130,7 +133,7 @@ def test_adjoint_device_entry_matches_host():
 def test_adjoint_rejects_unsupported():
     from qasmhd_amd import capi, problems
     p = problems.turbulence2d(n=16, seed=1)
-    s = capi.solver_for(p, rmn=3)
+    s = capi.solver_for(p, rmn=0)
     s.upload(p.state)
     with pytest.raises(capi.QasError):
         s.adj_forward([1e-3])
