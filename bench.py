@@ -220,17 +220,17 @@ def batch_line(ninst, steps, seed=5):
             "speedup": t_seq / t_batch, "sizes": [int(v) for v in sizes]}
 
 
-def adjoint_line(n, steps):
+def adjoint_line(n, steps, rmn=1):
     """Discrete adjoint (qas_adj_forward / qas_adj_backward) on the 2D
-    periodic workload at rms Mach 0.3 (HLLD, WCNS-4, CUCT, RK3): time of the
-    backward pass over `steps` steps against the plain forward of the same
-    steps at the same dt values."""
+    periodic workload at rms Mach 0.3 (HLLD, or MLAU with rmn 3; WCNS-4,
+    CUCT, RK3): time of the backward pass over `steps` steps against the
+    plain forward of the same steps at the same dt values."""
     import numpy as np
     from qasmhd_amd import capi, problems
     p = problems.turbulence2d(n=n, seed=5, vrms=0.3, brms=0.3)
     rng = np.random.default_rng(5)
     W = rng.standard_normal(p.state.shape)
-    with capi.solver_for(p) as s:
+    with capi.solver_for(p, rmn=rmn) as s:
         s.upload(p.state)
         dts, _ = s.run(steps, flg=1)
         # plain forward at the recorded step sizes
@@ -257,7 +257,8 @@ def adjoint_line(n, steps):
             s.adj_backward_device(dg, di)
             t_bwd = time.perf_counter() - t0
             gd = di.download()
-    return {"workload": "2d_decaying_mhd_%dx%d_mach0.3_hlld_wcns4_ct_rk3" % (n, n), "steps": steps,
+    return {"workload": "2d_decaying_mhd_%dx%d_mach0.3_%s_wcns4_ct_rk3" % (n, n, "mlau" if rmn == 3 else "hlld"),
+            "steps": steps,
             "forward_s": t_fwd, "forward_with_checkpoints_s": t_rec, "backward_s": t_bwd,
             "backward_with_host_copies_s": t_bwd_host, "ratio": t_bwd / t_fwd,
             "gradient_ratio": (t_rec + t_bwd) / t_fwd,
@@ -551,6 +552,7 @@ def main():
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--adj-steps", type=int, default=20)
+    ap.add_argument("--adj-rmn", type=int, default=1, help="Riemann solver of the adjoint line (1 HLLD, 3 MLAU)")
     ap.add_argument("--inverse-n", type=int, default=64, help="instances of the inversion line (0: skip it)")
     ap.add_argument("--dsp-nx", type=int, default=4096, help="dissipative line size along x (0: skip it)")
     ap.add_argument("--dsp-ny", type=int, default=512)
@@ -669,7 +671,7 @@ def main():
         if args.adj_n > 0:
             try:
                 _log("bench: adjoint, %d^2, %d steps" % (args.adj_n, args.adj_steps))
-                extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps)
+                extra["adjoint"] = adjoint_line(args.adj_n, args.adj_steps, rmn=args.adj_rmn)
             except Exception as e:
                 _log("bench: adjoint line failed: %s" % e)
 
