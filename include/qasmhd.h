@@ -164,8 +164,8 @@ void qas_batch_destroy(qas_batch *b);
 
 /* ---- discrete adjoint (no counterpart in the reference) ----------------
  * Reverse-mode derivative of `nsteps` steps of ideal(dt) (with the bound()
- * before the first step) at frozen step sizes.  Scope: 2D, FP64, HLLD or
- * MLAU (rmn 1 or 3), odr 4, r_k 3, ctw 0 or 1, periodic on both axes;
+ * before the first step) at frozen step sizes.  Scope: 2D, FP64, HLLD,
+ * LHLLD or MLAU (rmn 1-3), odr 4, r_k 3, ctw 0 or 1, periodic on both axes;
  * other schemes are rejected with QAS_E_ARG.
  * qas_adj_forward   advances the current state by nsteps steps with the
  *                   given dt values (as qas_step would), keeping each step's
@@ -224,6 +224,7 @@ int qas_point_flops(int rmn, int odr, double out[8]);
 #define QAS_ADJ_CUCT 4    /* CUCT upwind weight                             */
 #define QAS_ADJ_HLLD 5    /* HLLD flux                                      */
 #define QAS_ADJ_MLAU 6    /* MLAU flux (incl. the shock-sensor inputs dv)   */
+#define QAS_ADJ_LHLLD 7   /* LHLLD flux (incl. the shock-sensor inputs dv)  */
 int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long seed, double *relerr);
 
 /* ---- batched per-point evaluation on the device (tests) -----------------

@@ -103,7 +103,7 @@ def load(path: str | None = None):
     return l
 
 
-ADJ_POINT = {"wcns4": 0, "lin4": 1, "lrstate": 2, "lrfb": 3, "cuct": 4, "hlld": 5, "mlau": 6}
+ADJ_POINT = {"wcns4": 0, "lin4": 1, "lrstate": 2, "lrfb": 3, "cuct": 4, "hlld": 5, "mlau": 6, "lhlld": 7}
 
 
 def _p(a):

@@ -418,8 +418,13 @@ QAS_HD void cuct_weight_adj(const T* x, T gw, T* g)
 
 // ---------------------------------------------------------------- HLLD
 // gL, gR (7 each) and gbn += adjoint of calc_flux_hlld; gm1i = 1/(gam-1)
-template <typename T>
-QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T* gR, T& gbn)
+// LOW: the low-dissipation LHLLD of the same family (calc_flux_lhlld,
+// mhd_flux.c:563-698): the pressure jump in v* scaled by the shock-sensor
+// weight theta(dv0, dv1), the velocity jump in p* by mcc (2 - mcc); gdv0,
+// gdv1 get the adjoints of the sensor inputs.
+template <bool LOW, typename T>
+QAS_HD void hlld_adj_g(const T* L, const T* R, T bn, T gam, T dv0, T dv1, const T* gF, T* gL, T* gR, T& gbn,
+                       T& gdv0, T& gdv1)
 {
   const T FL_EPS = T(QAS_FL_EPS);
   const T rol = L[0], vnl = L[1], vtl = L[2], vul = L[3], btl = L[4], bul = L[5], prl = L[6];
@@ -441,8 +446,21 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   const T sql = dsqrt(dabs(dl)), sqr = dsqrt(dabs(dr));
   const T cfl2 = T(0.5) * (cbl2 + sql), cfr2 = T(0.5) * (cbr2 + sqr);
   const T cfl = dsqrt(cfl2), cfr = dsqrt(cfr2);
-  const bool selL = cfl >= cfr;
-  const T cmax = selL ? cfl : cfr;
+  // HLLD: max(sqrt(cfl2), sqrt(cfr2)); LHLLD: sqrt(max(cfl2, cfr2)) and the
+  // second speed pair cc*2 (as in MLAU)
+  const bool selL = LOW ? (cfl2 >= cfr2) : (cfl >= cfr);
+  const T cmax = LOW ? dsqrt(selL ? cfl2 : cfr2) : (selL ? cfl : cfr);
+  T cbl2c = T(0), cbr2c = T(0), elc = T(0), erc = T(0), selc = T(0), serc = T(0), ccl2 = T(0), ccr2 = T(0);
+  if (LOW) {
+    cbl2c = vl2 + cal2 + T(2.0) * pml * roli;
+    cbr2c = vr2 + car2 + T(2.0) * pmr * rori;
+    elc = cbl2c * cbl2c - T(4.0) * vl2 * cal2;
+    erc = cbr2c * cbr2c - T(4.0) * vr2 * car2;
+    selc = dsqrt(dabs(elc));
+    serc = dsqrt(dabs(erc));
+    ccl2 = T(0.5) * (cbl2c + selc);
+    ccr2 = T(0.5) * (cbr2c + serc);
+  }
   const bool nr_min = vnl >= vnr;  // min(vnl,vnr) = vnr, max = vnl
   const T mnv = nr_min ? vnr : vnl, mxv = nr_min ? vnl : vnr;
   const T xs = mnv - cmax, ys = mxv + cmax;
@@ -451,9 +469,24 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   const T slvl = sl - vnl, srvr = sr - vnr;
   const T rslvl = rol * slvl, rsrvr = ror * srvr;
   const T drsvi = T(1.0) / (rsrvr - rslvl);
-  const T Nvnc = rsrvr * vnr - rslvl * vnl - ptr + ptl;
+  T tA = T(0), tB = T(1), tr = T(0), theta0 = T(1), theta = T(1), mccu = T(0), mcc = T(1), mccx = T(0);
+  if (LOW) {
+    tA = cmax - tmin(dv0, T(0.0));
+    tB = cmax - tmin(dv1, T(0.0));
+    tr = tA / tB;
+    theta0 = tmin(T(1.0), tr);
+    theta = theta0;
+    theta *= theta;
+    theta *= theta;
+    mccx = tmax(ccl2, ccr2);
+    mccu = dsqrt(mccx) / cmax;
+    mcc = tmin(T(1.0), mccu);
+  }
+  const T Nvnc = LOW ? (rsrvr * vnr - rslvl * vnl - theta * (ptr - ptl)) : (rsrvr * vnr - rslvl * vnl - ptr + ptl);
   const T vnc = Nvnc * drsvi;
-  const T Nptc = rsrvr * ptl - rslvl * ptr + rsrvr * rslvl * (vnr - vnl);
+  const T kp = LOW ? (mcc * (T(2.0) - mcc)) : T(1.0);
+  const T Nptc = LOW ? (rsrvr * ptl - rslvl * ptr + kp * rsrvr * rslvl * (vnr - vnl))
+                     : (rsrvr * ptl - rslvl * ptr + rsrvr * rslvl * (vnr - vnl));
   const T ptc = Nptc * drsvi;
   const T slvc = sl - vnc, srvc = sr - vnc;
   const T ro2l = rslvl / slvc, ro2r = rsrvr / srvc;
@@ -675,21 +708,42 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   gvnc -= gslvc + gsrvc;
   // vnc, ptc
   T gdrsvi = Nvnc * gvnc + Nptc * gptc;
+  T gcmax_low = T(0), gccl2 = T(0), gccr2 = T(0);
   {
     const T gN = drsvi * gvnc;
     grsrvr += vnr * gN;
     gvnr += rsrvr * gN;
     grslvl -= vnl * gN;
     gvnl -= rslvl * gN;
-    gptr -= gN;
-    gptl += gN;
+    gptr -= theta * gN;
+    gptl += theta * gN;
     const T gM = drsvi * gptc;
-    grsrvr += ptl * gM + rslvl * (vnr - vnl) * gM;
+    grsrvr += ptl * gM + kp * rslvl * (vnr - vnl) * gM;
     gptl += rsrvr * gM;
-    grslvl += -ptr * gM + rsrvr * (vnr - vnl) * gM;
+    grslvl += -ptr * gM + kp * rsrvr * (vnr - vnl) * gM;
     gptr -= rslvl * gM;
-    gvnr += rsrvr * rslvl * gM;
-    gvnl -= rsrvr * rslvl * gM;
+    gvnr += kp * rsrvr * rslvl * gM;
+    gvnl -= kp * rsrvr * rslvl * gM;
+    if (LOW) {
+      // theta = min(1, tA / tB)^4, tA = cmax - min(dv0, 0), tB = cmax - min(dv1, 0)
+      const T gtheta = -(ptr - ptl) * gN;
+      if (T(1.0) >= tr) {
+        const T gtr = gtheta * T(4.0) * theta0 * theta0 * theta0;
+        const T gtA = gtr / tB, gtB = -gtr * tr / tB;
+        gcmax_low += gtA + gtB;
+        if (!(dv0 >= T(0.0))) gdv0 += -gtA;
+        if (!(dv1 >= T(0.0))) gdv1 += -gtB;
+      }
+      // kp = mcc (2 - mcc), mcc = min(1, sqrt(max(ccl2, ccr2)) / cmax)
+      const T gmcc = rsrvr * rslvl * (vnr - vnl) * gM * (T(2.0) - T(2.0) * mcc);
+      if (T(1.0) >= mccu) {
+        const T sq = dsqrt(mccx);
+        gcmax_low += -gmcc * mccu / cmax;
+        const T gx = gmcc / cmax * (sq > T(0) ? T(0.5) / sq : T(0));
+        if (ccl2 >= ccr2) gccl2 += gx;
+        else gccr2 += gx;
+      }
+    }
   }
   {
     const T gd = -drsvi * drsvi * gdrsvi;
@@ -704,7 +758,7 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   gsr += gsrvr;
   gvnr -= gsrvr;
   // sl, sr, cmax
-  T gcmax = T(0), gmnv = T(0), gmxv = T(0);
+  T gcmax = gcmax_low, gmnv = T(0), gmxv = T(0);
   if (slx) {
     gmnv += gsl;
     gcmax -= gsl;
@@ -721,9 +775,28 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
     gvnr += gmxv;
   }
   T gcfl2 = T(0), gcfr2 = T(0);
-  if (selL) gcfl2 += (cfl > T(0) ? T(0.5) / cfl : T(0)) * gcmax;
+  if (selL) gcfl2 += (cfl > T(0) ? T(0.5) / cfl : T(0)) * gcmax;  // (LHLLD: cmax = cfl when selL)
   else gcfr2 += (cfr > T(0) ? T(0.5) / cfr : T(0)) * gcmax;
   T groli = T(0), grori = T(0), gpml = T(0), gpmr = T(0), gprl = T(0), gprr = T(0);
+  T gvl2x = T(0), gvr2x = T(0);
+  if (LOW) {  // cc*2 = 0.5 (cb*2c + sqrt|e*c|), cb*2c = v*2 + ca*2 + 2 pm* ro*i
+    T gcb = T(0.5) * gccl2;
+    const T ge = T(0.5) * gccl2 * (selc > T(0) ? T(0.5) / selc : T(0)) * sgnd(elc);
+    gcb += ge * T(2.0) * cbl2c;
+    gvl2x += -T(4.0) * cal2 * ge + gcb;
+    const T gcal2 = -T(4.0) * vl2 * ge + gcb;
+    gpml += T(2.0) * roli * gcb;
+    groli += T(2.0) * pml * gcb + bnc2 * gcal2;
+    gbnc2 += roli * gcal2;
+    gcb = T(0.5) * gccr2;
+    const T ge2 = T(0.5) * gccr2 * (serc > T(0) ? T(0.5) / serc : T(0)) * sgnd(erc);
+    gcb += ge2 * T(2.0) * cbr2c;
+    gvr2x += -T(4.0) * car2 * ge2 + gcb;
+    const T gcar2 = -T(4.0) * vr2 * ge2 + gcb;
+    gpmr += T(2.0) * rori * gcb;
+    grori += T(2.0) * pmr * gcb + bnc2 * gcar2;
+    gbnc2 += rori * gcar2;
+  }
   {  // left fast speed
     T gcb = T(0.5) * gcfl2;
     const T gsq = T(0.5) * gcfl2;
@@ -760,11 +833,11 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   gprl += gm1i * genl + gptl;
   gpml += genl + gptl;
   grol += T(0.5) * vl2 * genl;
-  const T gvl2 = T(0.5) * rol * genl;
+  const T gvl2 = T(0.5) * rol * genl + gvl2x;
   gprr += gm1i * genr + gptr;
   gpmr += genr + gptr;
   gror += T(0.5) * vr2 * genr;
-  const T gvr2 = T(0.5) * ror * genr;
+  const T gvr2 = T(0.5) * ror * genr + gvr2x;
   gvnl += T(2.0) * vnl * gvl2;
   gvtl += T(2.0) * vtl * gvl2;
   gvul += T(2.0) * vul * gvl2;
@@ -781,6 +854,14 @@ QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T*
   gL[0] += grol; gL[1] += gvnl; gL[2] += gvtl; gL[3] += gvul; gL[4] += gbtl; gL[5] += gbul; gL[6] += gprl;
   gR[0] += gror; gR[1] += gvnr; gR[2] += gvtr; gR[3] += gvur; gR[4] += gbtr; gR[5] += gbur; gR[6] += gprr;
   gbn += gbn_;
+}
+
+// HLLD (calc_flux_hlld): no shock-sensor inputs
+template <typename T>
+QAS_HD void hlld_adj(const T* L, const T* R, T bn, T gam, const T* gF, T* gL, T* gR, T& gbn)
+{
+  T gd0 = T(0), gd1 = T(0);
+  hlld_adj_g<false>(L, R, bn, gam, T(0), T(0), gF, gL, gR, gbn, gd0, gd1);
 }
 
 // ---------------------------------------------------------------- MLAU

@@ -1,6 +1,6 @@
 // qas_adjoint2d.hip -- hand-written discrete adjoint of the 2D step.
 //
-// Scope: FP64, HLLD or MLAU (RMN 1, 3), ODR 4 (WCNS-4 / 4th-order linear /
+// Scope: FP64, HLLD, LHLLD or MLAU (RMN 1-3), ODR 4 (WCNS-4 / 4th-order linear /
 // fcen / difference), SSP-RK3, CUCT on or off, periodic on both axes, dt frozen
 // (the step sizes are inputs, as a recorded dt history is).  The adjoint is
 // that of exactly the forward kernels of qas_kernels2d.hip / qas_sweep2d.hpp,
@@ -333,7 +333,7 @@ struct SlotAcc {
 template <int RMN, int DIR, bool TR>
 __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, double gam, int nseg, int blk)
 {
-  static_assert(RMN == 1 || RMN == 3, "HLLD or MLAU");
+  static_assert(RMN >= 1 && RMN <= 3, "HLLD, LHLLD or MLAU");
   static_assert(TR == (DIR == 1), "the y pass runs on transposed arrays, the x pass in place");
   __shared__ double gacc[256 / WAVE][7 * (WAVE + 4)];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -398,7 +398,7 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
       gqlf = 0.5 * gfc;
       gqr = 0.5 * gfc;
       const double bnf = 0.5 * (f.bf[fq] + f.bf[fq]);
-      if constexpr (RMN == 3) {
+      if constexpr (RMN >= 2) {
         // shock sensor of the face (k_sweep2d / k_sweep2d_y): dv0 along the
         // line, dv1 from the transverse differences of the two cells
         const double* vn = DIR == 0 ? p.vx : p.vy;
@@ -411,7 +411,8 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
           dv1 = tmin(dv_per(p.vx, line, q - 1, g.nx, g.ny, g.ny, 1, g.ny),
                      dv_per(p.vx, line, q, g.nx, g.ny, g.ny, 1, g.ny));
         double gdv0 = 0.0;
-        mlau_adj(L, vr, bnf, gam, dv0, dv1, gF, gL, gvr, gbn, gdv0, gdv1);
+        if constexpr (RMN == 3) mlau_adj(L, vr, bnf, gam, dv0, dv1, gF, gL, gvr, gbn, gdv0, gdv1);
+        else hlld_adj_g<true>(L, vr, bnf, gam, dv0, dv1, gF, gL, gvr, gbn, gdv0, gdv1);
         // dv0 = vn(q) - vn(q-1): slots of cells q (lane + 2) and q-1 (lane + 1)
         atomicAdd(&row[1 * (WAVE + 4) + lane + 2], gdv0);
         atomicAdd(&row[1 * (WAVE + 4) + lane + 1], -gdv0);
@@ -419,7 +420,7 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
         hlld_adj(L, vr, bnf, gam, gF, gL, gvr, gbn);  // d(0.5 (b + b))/db = 1
       }
     }
-    if constexpr (RMN == 3) {
+    if constexpr (RMN >= 2) {
       // dv1's adjoint per face, natural layout, one writer per face (the
       // owning lanes); gin adds it onto the transverse velocity
       if (lane >= 3 && lane < 3 + SEGA && q >= 0 && q < n)
@@ -889,7 +890,7 @@ static std::string adj_supported(const qas_ctx* c)
 {
   if (c->g.ndim != 2) return "the adjoint covers the 2D path only";
   if (c->s.precision != 0) return "the adjoint covers the FP64 path only";
-  if (c->s.rmn != 1 && c->s.rmn != 3) return "the adjoint covers the HLLD and MLAU solvers (rmn 1, 3) only";
+  if (c->s.rmn < 1 || c->s.rmn > 3) return "the adjoint covers the HLLD, LHLLD and MLAU solvers (rmn 1-3) only";
   if (c->s.odr != 4) return "the adjoint covers odr 4 only";
   if (c->s.r_k != 3) return "the adjoint covers r_k 3 only";
   if (c->d_phi || c->d_nu) return "the adjoint covers the ideal step without gravity or dissipation";
@@ -1034,6 +1035,8 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     const unsigned nbx = nblk((long long)csx * g.ny * WAVE, 256), nby = nblk((long long)csy * g.nx * WAVE, 256);
     if (c->s.rmn == 3)
       hipLaunchKernelGGL(k_adj_sweep_xy<3>, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
+    else if (c->s.rmn == 2)
+      hipLaunchKernelGGL(k_adj_sweep_xy<2>, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
     else
       hipLaunchKernelGGL(k_adj_sweep_xy<1>, nbx + nby, 256, 0, st, p, pt, g, c->s.gam, csx, csy, (int)nbx);
     TrArgs to;
@@ -1051,7 +1054,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     const dim3 tb((unsigned)((g.nx + 31) / 32), (unsigned)((g.ny + 31) / 32), 9);
     hipLaunchKernelGGL(k_adj_untranspose_add, tb, 256, 0, st, to, g.nx, g.ny);
   }
-  const bool dv = c->s.rmn == 3;
+  const bool dv = c->s.rmn == 2 || c->s.rmn == 3;  // the shock sensor feeds LHLLD and MLAU
   if (ctw && dv) hipLaunchKernelGGL((k_adj_gin<1, true>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
   else if (ctw) hipLaunchKernelGGL((k_adj_gin<1, false>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);
   else if (dv) hipLaunchKernelGGL((k_adj_gin<0, true>), 2 * nb, 256, 0, st, p, g, c->s.gam, (int)nb);

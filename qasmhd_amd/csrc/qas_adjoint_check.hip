@@ -169,6 +169,21 @@ struct Mlau {
   }
 };
 
+// x = L[7], R[7], bn, dv0, dv1 (sampled as for MLAU)
+struct Lhlld {
+  static const int NI = 17, NO = 7;
+  static constexpr double gam = 5.0 / 3.0;
+  static void sample(Rng& r, double* x) { Mlau::sample(r, x); }
+  static void fwd(const double* x, double* y)
+  {
+    qas::flux_hlld_family<double>(true, x, x + 7, x[14], gam, x[15], x[16], y);
+  }
+  static void adj(const double* x, const double* g, double* gx)
+  {
+    qas::hlld_adj_g<true>(x, x + 7, x[14], gam, x[15], x[16], g, gx, gx + 7, gx[14], gx[15], gx[16]);
+  }
+};
+
 template <class F>
 void run_check(int ntrials, uint64_t seed, double* relerr)
 {
@@ -216,6 +231,7 @@ extern "C" int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long 
     case QAS_ADJ_CUCT: run_check<Cuct>(ntrials, seed, relerr); break;
     case QAS_ADJ_HLLD: run_check<Hlld>(ntrials, seed, relerr); break;
     case QAS_ADJ_MLAU: run_check<Mlau>(ntrials, seed, relerr); break;
+    case QAS_ADJ_LHLLD: run_check<Lhlld>(ntrials, seed, relerr); break;
     default: return QAS_E_ARG;
   }
   return QAS_OK;

@@ -21,7 +21,7 @@ from qasmhd_amd import capi
 pytestmark = pytest.mark.skipif(not os.path.exists(capi.LIB_PATH), reason="library not built")
 
 
-@pytest.mark.parametrize("name", ["wcns4", "lin4", "lrfb", "cuct", "hlld", "mlau"])
+@pytest.mark.parametrize("name", ["wcns4", "lin4", "lrfb", "cuct", "hlld", "mlau", "lhlld"])
 def test_point_adjoint_smooth(name):
     e = capi.adjoint_selfcheck(name, 2000, seed=11)
     assert np.median(e) < 1e-9, np.median(e)

@@ -66,10 +66,12 @@ def _dir_deriv(s, u0, dts, W, d, h):
 
 # (nx, ny): the y line pass runs on transposed copies and reads the EMF-part
 # adjoints in place, so a non-square domain checks both index maps.  rmn 3
-# (MLAU) adds the shock sensor: dv0 along the line, dv1 from the transverse
-# velocity differences (gathered onto cells of the neighbouring lines).
+# (MLAU) and 2 (LHLLD) add the shock sensor: dv0 along the line, dv1 from
+# the transverse velocity differences (gathered onto cells of the
+# neighbouring lines).
 @pytest.mark.parametrize("rmn,ctw,nx,ny", [(1, 1, 32, 32), (1, 0, 32, 32), (1, 1, 40, 24), (1, 1, 24, 40),
-                                           (3, 1, 32, 32), (3, 0, 32, 32), (3, 1, 40, 24), (3, 1, 24, 40)])
+                                           (3, 1, 32, 32), (3, 0, 32, 32), (3, 1, 40, 24), (3, 1, 24, 40),
+                                           (2, 1, 32, 32), (2, 1, 40, 24)])
 def test_adjoint_vs_fd(rmn, ctw, nx, ny):
     print("case: adjoint vs FD, rmn %d, %dx%d, 3 steps, ctw %d" % (rmn, nx, ny, ctw), flush=True)
     p, s, dts = _setup(n=nx, ny=ny, ctw=ctw, rmn=rmn)
