@@ -180,7 +180,8 @@ def batch_line(ninst, steps, seed=5):
     square sizes drawn (numpy PCG64(seed)) from {128, 192, 256, 384, 512},
     rms Mach 0.3, HLLD / WCNS-4 / CUCT / RK3, `steps` steps in one batch
     (one launch per stage kernel for all instances, two steps per graph
-    replay); against the same instances run one after another with qas_run."""
+    replay; the launch shape picked by the batch's autotuner on the first
+    run); against the same instances run one after another with qas_run."""
     import numpy as np
     from qasmhd_amd import capi, problems
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -191,7 +192,8 @@ def batch_line(ninst, steps, seed=5):
     for s, p in zip(solvers, ps):
         s.upload(p.state)
     with capi.Batch(solvers) as b:
-        b.run(2, flg=1, history=False)  # graph capture, warm-up
+        b.run(2, flg=1, history=False)  # launch-shape tuning, graph capture, warm-up
+        shape = b.shape()
         for s, p in zip(solvers, ps):
             s.upload(p.state)
             s.reset_counter()
@@ -217,7 +219,10 @@ def batch_line(ninst, steps, seed=5):
         s.close()
     return {"workload": "%d_instances_2d_periodic_sizes_128-512_mach0.3_hlld_wcns4_ct_rk3" % ninst, "steps": steps,
             "zones": zones, "batched_value": zones * steps / t_batch, "sequential_value": zones * steps / t_seq,
-            "speedup": t_seq / t_batch, "sizes": [int(v) for v in sizes]}
+            "speedup": t_seq / t_batch,
+            "shape": {"order": ["as_given", "largest_first"][shape[0]], "x_waves_per_workgroup": shape[1],
+                      "elementwise_workgroup": shape[2]},
+            "sizes": [int(v) for v in sizes]}
 
 
 def adjoint_line(n, steps, rmn=1):

@@ -154,10 +154,20 @@ int qas_set_gravity(qas_ctx *c, const double *phi);
  * qas_batch_run(b, n, flg, dt, hist) gives every domain i the result of
  * qas_run(ctx_i, n, flg, &dt[i], hist + i*n) (dt[i] <= 0: from the CFL
  * condition; hist may be NULL).  The contexts stay owned by the caller and
- * must outlive the batch; calls on one batch are not thread-safe. */
+ * must outlive the batch; calls on one batch are not thread-safe.
+ * Launch shape (autotuned): qas_batch_tune times RK stage 1 of the batch
+ * in each candidate shape -- instance block order (0 as given, 1 largest
+ * domain first), x-sweep wavefronts per workgroup (xwpb 2 or 4), element-
+ * wise workgroup size (ewb 128 or 256) -- and keeps the fastest; the first
+ * qas_batch_run tunes unless a shape was set.  Results are bit-identical in
+ * every shape.  qas_batch_set_shape fixes one (xwpb 1/2/4, ewb 64/128/256);
+ * qas_batch_shape returns it (1 when tuned or set, 0 for the default). */
 typedef struct qas_batch qas_batch;
 int qas_batch_create(qas_batch **b, qas_ctx *const *ctxs, int n);
 int qas_batch_run(qas_batch *b, int nsteps, int flg, double *dt_inout, double *dt_hist);
+int qas_batch_tune(qas_batch *b);
+int qas_batch_set_shape(qas_batch *b, int order, int xwpb, int ewb);
+int qas_batch_shape(const qas_batch *b, int *order, int *xwpb, int *ewb);
 const char *qas_batch_last_error(const qas_batch *b);
 void qas_batch_destroy(qas_batch *b);
 

@@ -38,6 +38,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_step", "qas_cfl_dt", "qas_run", "qas_reset_counter", "qas_use_graphs", "qas_sync", "qas_device_bytes",
            "qas_prof_enable", "qas_prof_query", "qas_kernel_model", "qas_point_flops", "qas_adjoint_selfcheck",
            "qas_batch_create", "qas_batch_run", "qas_batch_last_error", "qas_batch_destroy",
+           "qas_batch_tune", "qas_batch_set_shape", "qas_batch_shape",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct",
            "qas_set_dissipation", "qas_dsptv", "qas_set_gravity"]
@@ -84,6 +85,9 @@ def load(path: str | None = None):
     l.qas_adjoint_selfcheck.argtypes = [C.c_int, C.c_int, C.c_ulonglong, dp]
     l.qas_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
     l.qas_batch_run.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    l.qas_batch_tune.argtypes = [vp]
+    l.qas_batch_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    l.qas_batch_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     l.qas_batch_last_error.argtypes = [vp]
     l.qas_batch_last_error.restype = C.c_char_p
     l.qas_batch_destroy.argtypes = [vp]
@@ -422,6 +426,24 @@ class Batch:
         if r != QAS_OK:
             raise QasError("qas_batch_run: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
         return hist[:, :nsteps], d
+
+    def tune(self):
+        """Time the candidate launch shapes and keep the fastest (qas_batch_tune)."""
+        r = self.lib.qas_batch_tune(self.b)
+        if r != QAS_OK:
+            raise QasError("qas_batch_tune: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
+        return self.shape()
+
+    def set_shape(self, order, xwpb, ewb):
+        r = self.lib.qas_batch_set_shape(self.b, int(order), int(xwpb), int(ewb))
+        if r != QAS_OK:
+            raise QasError("qas_batch_set_shape: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
+
+    def shape(self):
+        """(order, x-sweep wavefronts per workgroup, element-wise workgroup size)."""
+        o, x, e = C.c_int(), C.c_int(), C.c_int()
+        self.lib.qas_batch_shape(self.b, C.byref(o), C.byref(x), C.byref(e))
+        return o.value, x.value, e.value
 
     def close(self):
         if self.b:

@@ -132,6 +132,12 @@ struct qas_batch {
   hipGraphExec_t gexec = nullptr;
   int graph_flg = -1, graph_hist = -1;
   long long nstep = 0;
+  // launch shape (qas_batch_tune): block order of the instances (0 as given,
+  // 1 largest first), wavefronts per x-sweep workgroup, threads per
+  // workgroup of the element-wise stage kernels
+  int order = 0, xwpb = 4, ewb = 256;
+  bool tuned = false;
+  std::vector<int> perm;  // table position -> context index
   std::string err;
 };
 
@@ -163,9 +169,14 @@ static int fill_table(qas_batch* b)
   const int n = b->n;
   const int R = b->s.r_k;
   b->inst.assign(n, BatchInst());
+  b->perm.resize(n);
+  for (int i = 0; i < n; i++) b->perm[i] = i;
+  if (b->order == 1)  // largest domains first: the last workgroups of a launch are the small ones
+    std::stable_sort(b->perm.begin(), b->perm.end(),
+                     [b](int x, int y) { return b->c[x]->geo.nd > b->c[y]->geo.nd; });
   std::vector<int> off((size_t)BK_COUNT * (n + 1), 0);
   for (int i = 0; i < n; i++) {
-    qas_ctx* c = b->c[i];
+    qas_ctx* c = b->c[b->perm[i]];
     BatchInst& B = b->inst[i];
     std::memset(&B, 0, sizeof(B));
     const Geo& g = c->geo;
@@ -201,11 +212,11 @@ static int fill_table(qas_batch* b)
     B.nsy = (g.ny - 5 + SEG - 1) / SEG;
     const long long ninter = (long long)(g.nx - 2 * NG) * (g.ny - 2 * NG);
     B.cflblocks = std::min(cdiv(ninter, 256), CFL_MAX_BLOCKS);
-    const int cnt[BK_COUNT] = {cdiv(g.nd, 256),
-                               cdiv((long long)B.nsx * g.ny * WAVE, 256),
+    const int cnt[BK_COUNT] = {cdiv(g.nd, b->ewb),
+                               cdiv((long long)B.nsx * g.ny * WAVE, (long long)WAVE * b->xwpb),
                                ((g.nx + YC - 1) / YC) * B.nsy,
-                               cdiv((long long)(g.nx - 5) * (g.ny - 5), 256),
-                               cdiv((long long)(g.nx - 7) * (g.ny - 7), 256),
+                               cdiv((long long)(g.nx - 5) * (g.ny - 5), b->ewb),
+                               cdiv((long long)(g.nx - 7) * (g.ny - 7), b->ewb),
                                cdiv(B.nghost * 8, 256),
                                B.cflblocks};
     for (int k = 0; k < BK_COUNT; k++) off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
@@ -234,6 +245,22 @@ static hipError_t launch_cfl_all(qas_batch* b)
   return hipGetLastError();
 }
 
+// the five kernels of RK stage rk for every instance, in the batch's shape
+static hipError_t launch_stage_all(qas_batch* b, int rk)
+{
+  hipStream_t st = b->stream;
+  const int n = b->n;
+  const unsigned eb = (unsigned)b->ewb, xb = (unsigned)(WAVE * b->xwpb);
+  hipLaunchKernelGGL(b->prim, (unsigned)b->nblk[BK_PRIM], eb, 0, st, b->d_inst, boff(b, BK_PRIM), n, rk, b->s.gam);
+  hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
+                     (const BatchInst*)b->d_inst, boff(b, BK_SY), n, rk);
+  hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], xb, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
+                     (const BatchInst*)b->d_inst, boff(b, BK_SX), n, rk);
+  hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], eb, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
+  hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], eb, 0, st, b->d_inst, boff(b, BK_CT), n, rk);
+  return hipGetLastError();
+}
+
 // one step of exec_'s loop body for every instance
 static hipError_t batch_step(qas_batch* b, int flg, bool hist, bool even)
 {
@@ -241,14 +268,8 @@ static hipError_t batch_step(qas_batch* b, int flg, bool hist, bool even)
   const int n = b->n;
   if (hist) hipLaunchKernelGGL(kb_record, (unsigned)cdiv(n, 64), 64, 0, st, b->d_inst, n);
   for (int rk = 0; rk < b->s.r_k; rk++) {
-    hipLaunchKernelGGL(b->prim, (unsigned)b->nblk[BK_PRIM], 256, 0, st, b->d_inst, boff(b, BK_PRIM), n, rk, b->s.gam);
-    hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
-                       (const BatchInst*)b->d_inst, boff(b, BK_SY), n, rk);
-    hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
-                       (const BatchInst*)b->d_inst, boff(b, BK_SX), n, rk);
-    hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], 256, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
-    hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], 256, 0, st, b->d_inst, boff(b, BK_CT), n, rk);
-    hipError_t e = launch_bound_all(b, rk);
+    hipError_t e = launch_stage_all(b, rk);
+    if (e == hipSuccess) e = launch_bound_all(b, rk);
     if (e != hipSuccess) return e;
   }
   if (flg && even) return launch_cfl_all(b);
@@ -373,6 +394,10 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
     }
   }
   (void)table_changed;
+  if (!b->tuned) {
+    const int rt = qas_batch_tune(b);
+    if (rt) return rt;
+  }
   int r = fill_table(b);
   if (r) return r;
   if (hist) hipLaunchKernelGGL(kb_zero_counts, (unsigned)cdiv(n, 64), 64, 0, b->stream, b->d_inst, n);
@@ -380,7 +405,10 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
   bool any_cfl = false;
   for (int i = 0; i < n; i++) any_cfl = any_cfl || !(dt_inout[i] > 0.0);
   if (any_cfl) QAS_BCHECK(b, launch_cfl_all(b));
-  QAS_BCHECK(b, hipMemcpyAsync(b->d_dts, dt_inout, sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
+  // the device table is in the batch's block order (perm)
+  std::vector<double> dts(n);
+  for (int k = 0; k < n; k++) dts[k] = dt_inout[b->perm[k]];
+  QAS_BCHECK(b, hipMemcpyAsync(b->d_dts, dts.data(), sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(kb_set_dt, (unsigned)cdiv(n, 64), 64, 0, b->stream, b->d_inst, n, b->d_dts);
   bool bounded = true;
   for (int i = 0; i < n; i++) bounded = bounded && b->c[i]->bounded;
@@ -407,8 +435,9 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
     QAS_BCHECK(b, batch_step(b, flg, hist, ns % 2 == 0));
   }
   hipLaunchKernelGGL(kb_get_dt, (unsigned)cdiv(n, 64), 64, 0, b->stream, b->d_inst, n, b->d_dts);
-  QAS_BCHECK(b, hipMemcpyAsync(dt_inout, b->d_dts, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+  QAS_BCHECK(b, hipMemcpyAsync(dts.data(), b->d_dts, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
   QAS_BCHECK(b, hipStreamSynchronize(b->stream));
+  for (int k = 0; k < n; k++) dt_inout[b->perm[k]] = dts[k];
   for (int i = 0; i < n; i++) {
     qas_ctx* c = b->c[i];
     c->nstep = ns;
@@ -418,4 +447,79 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
                               hipMemcpyDeviceToHost));
   }
   return QAS_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Launch shapes of the batched stage: block order of the instances, x-sweep
+// wavefronts per workgroup, element-wise workgroup size.  Every shape gives
+// the same arithmetic per cell (bit-identical results); they differ in how
+// the workgroups of uneven domains fill the GPU.  The tuner times RK stage 1
+// (prim, both sweeps, EMF, CT) in each shape, three times each after one
+// warm-up, and keeps the fastest.  Stage 1 reads the current states and
+// writes only the next-stage buffers and scratch, which the run overwrites.
+extern "C" int qas_batch_tune(qas_batch* b)
+{
+  if (!b || !b->stream) return bfail(b, QAS_E_ARG, "qas_batch_tune: invalid batch");
+  const int cand[8][3] = {{0, 4, 256}, {1, 4, 256}, {0, 2, 256}, {1, 2, 256},
+                          {0, 4, 128}, {1, 4, 128}, {0, 2, 128}, {1, 2, 128}};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  QAS_BCHECK(b, hipEventCreate(&e0));
+  QAS_BCHECK(b, hipEventCreate(&e1));
+  float best = -1.0f;
+  int bi = 0;
+  int r = QAS_OK;
+  for (int k = 0; k < 8 && !r; k++) {
+    b->order = cand[k][0];
+    b->xwpb = cand[k][1];
+    b->ewb = cand[k][2];
+    r = fill_table(b);
+    if (r) break;
+    if (launch_stage_all(b, 0) != hipSuccess) { r = bfail(b, QAS_E_HIP, "qas_batch_tune: launch"); break; }
+    float tmin = -1.0f;
+    for (int rep = 0; rep < 3; rep++) {
+      hipError_t e = hipEventRecord(e0, b->stream);
+      if (e == hipSuccess) e = launch_stage_all(b, 0);
+      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) { r = bfail(b, QAS_E_HIP, std::string("qas_batch_tune: ") + hipGetErrorName(e)); break; }
+      if (tmin < 0.0f || ms < tmin) tmin = ms;
+    }
+    if (!r && (best < 0.0f || tmin < best)) {
+      best = tmin;
+      bi = k;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (r) return r;
+  b->order = cand[bi][0];
+  b->xwpb = cand[bi][1];
+  b->ewb = cand[bi][2];
+  b->tuned = true;
+  b->graph_flg = -1;  // the captured graph has the old grid sizes
+  return fill_table(b);
+}
+
+extern "C" int qas_batch_set_shape(qas_batch* b, int order, int xwpb, int ewb)
+{
+  if (!b) return QAS_E_ARG;
+  if ((order != 0 && order != 1) || (xwpb != 1 && xwpb != 2 && xwpb != 4) || (ewb != 64 && ewb != 128 && ewb != 256))
+    return bfail(b, QAS_E_ARG, "qas_batch_set_shape: order 0/1, xwpb 1/2/4, ewb 64/128/256");
+  b->order = order;
+  b->xwpb = xwpb;
+  b->ewb = ewb;
+  b->tuned = true;
+  b->graph_flg = -1;
+  return QAS_OK;
+}
+
+extern "C" int qas_batch_shape(const qas_batch* b, int* order, int* xwpb, int* ewb)
+{
+  if (!b || !order || !xwpb || !ewb) return QAS_E_ARG;
+  *order = b->order;
+  *xwpb = b->xwpb;
+  *ewb = b->ewb;
+  return b->tuned ? 1 : 0;
 }

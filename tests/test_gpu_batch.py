@@ -49,6 +49,49 @@ def test_batch_matches_single_runs(precision, nsteps):
         s.close()
 
 
+@pytest.mark.parametrize("shape", [(0, 4, 256), (1, 4, 256), (0, 2, 128), (1, 1, 64), (1, 2, 256)])
+def test_batch_shapes_are_bit_identical(shape):
+    # every launch shape the tuner may pick (block order, x-sweep wavefronts
+    # per workgroup, element-wise workgroup size) gives the single-run result
+    from qasmhd_amd import capi
+    print("case: batch shape", shape, flush=True)
+    ps = _problems()
+    ref = []
+    for p in ps:
+        with capi.solver_for(p) as s:
+            s.upload(p.state)
+            h, dt = s.run(6, flg=1)
+            ref.append((h, dt, s.download()))
+    solvers = [capi.solver_for(p) for p in ps]
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+    with capi.Batch(solvers) as b:
+        b.set_shape(*shape)
+        hist, dts = b.run(6, flg=1)
+        assert b.shape() == shape
+    for i, s in enumerate(solvers):
+        assert np.array_equal(hist[i], ref[i][0]), i
+        assert dts[i] == ref[i][1], i
+        assert np.array_equal(s.download(), ref[i][2]), i
+        s.close()
+
+
+def test_batch_autotune_picks_a_candidate():
+    from qasmhd_amd import capi
+    print("case: batch autotune", flush=True)
+    ps = _problems()
+    solvers = [capi.solver_for(p) for p in ps]
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+    with capi.Batch(solvers) as b:
+        order, xwpb, ewb = b.tune()
+        assert order in (0, 1) and xwpb in (2, 4) and ewb in (128, 256)
+        hist, _ = b.run(2, flg=1)
+        assert b.shape() == (order, xwpb, ewb)
+    for s in solvers:
+        s.close()
+
+
 def test_batch_rejects_mixed_schemes():
     from qasmhd_amd import capi, problems
     p = problems.turbulence2d(n=16, seed=1)
