@@ -15,7 +15,13 @@
 // Output and restart files follow the reference: dout_ writes params.dat,
 // offsets.dat, t.dat, x/y/z.dat and outdat_%05d.dat (FP64, 8 arrays of nd
 // doubles, ghosts included) into fildir; bkup_save/bkup_load the
-// bkup_stamp/time/data files of common_func.c:281-344.
+// bkup_stamp/time/data files of common_func.c:281-344.  With mpi_files set
+// (the default of the 3D driver, whose reference is MPI only) they follow
+// the MPI drivers run as one rank instead (2D/MPI/OTvortex/mhd2d_class.cpp,
+// 3D/MPI/OTvortex/mhd3d_class.cpp): exec_ resumes from the backup in fildir
+// when there is one and saves a backup at every output, and dout_ adds
+// mpinum.dat, names the coordinate files x_00000.dat.. and writes
+// outdat_%05d_00000.dat in FP32.
 //
 // Header-only; link with libqasmhd_amd.so.  Errors throw qasmhd::Error
 // (the C ABI itself never throws).
@@ -149,6 +155,8 @@ class MHDDevice {
 
   // Output directory of dout_ and the backup files (empty: no files).
   std::string fildir;
+  // the file set and backup cycle of the MPI drivers, as rank 0 of 1
+  bool mpi_files = false;
 
   // Backup files of common_func.c:318-344 (bkup_save, rank 0):
   // bkup_stamp.dat "n cnt", bkup_time.dat {tim, dt, trec} FP64,
@@ -156,12 +164,7 @@ class MHDDevice {
   void bkup_save()
   {
     download();
-    const double vald[3] = {tim, dt, trec};
-    write_text(path("bkup_stamp.dat"), "w", std::to_string(n) + " " + std::to_string(cnt) + "\n");
-    FILE* f = open_or_throw(path("bkup_time.dat"), "wb");
-    std::fwrite(vald, sizeof(double), 3, f);
-    std::fclose(f);
-    write_state(path("bkup_data_00000.dat"));
+    write_backup();
   }
   // bkup_load (common_func.c:281-316): restores n, cnt, tim, dt, trec and
   // the state if bkup_stamp.dat exists; returns whether it did.
@@ -215,6 +218,40 @@ class MHDDevice {
     for (int m = 0; m < nm; m++) std::fwrite(val[m], sizeof(double), store_.size() / nm, f);
     std::fclose(f);
   }
+  // the same in FP32 (conv_d2f, common_func.c:22-26: a plain cast per value)
+  void write_state_f32(const std::string& p)
+  {
+    FILE* f = open_or_throw(p, "wb");
+    const size_t nd = store_.size() / nm;
+    std::vector<float> buf(nd);
+    for (int m = 0; m < nm; m++) {
+      for (size_t i = 0; i < nd; i++) buf[i] = (float)val[m][i];
+      std::fwrite(buf.data(), sizeof(float), nd, f);
+    }
+    std::fclose(f);
+  }
+  // bkup_save's files from the host arrays as they are
+  void write_backup()
+  {
+    const double vald[3] = {tim, dt, trec};
+    write_text(path("bkup_stamp.dat"), "w", std::to_string(n) + " " + std::to_string(cnt) + "\n");
+    FILE* f = open_or_throw(path("bkup_time.dat"), "wb");
+    std::fwrite(vald, sizeof(double), 3, f);
+    std::fclose(f);
+    write_state(path("bkup_data_00000.dat"));
+  }
+  // exec_ of the MPI drivers starts with bkup_(0) (mhd2d/mhd3d_class.cpp
+  // of 2D/MPI and 3D/MPI): resume from the backup in fildir if one is there
+  void resume_if_mpi()
+  {
+    if (mpi_files && !fildir.empty()) bkup_load();
+  }
+  // the output step of exec_ (state already downloaded): the MPI drivers
+  // save a backup (bkup_(1)) before dout_
+  void backup_if_mpi()
+  {
+    if (mpi_files && !fildir.empty()) write_backup();
+  }
   // dout_ of the 2D/1D/3D serial drivers (e.g. 2D/SERIAL/OTvortex/
   // mhd2d_class.cpp:62-112): params.dat, offsets.dat, t.dat and the
   // coordinate files at n == 0, t.dat appended afterwards, and
@@ -224,6 +261,7 @@ class MHDDevice {
   void dout_files(int msg, const std::vector<std::vector<double>>& coords, const std::vector<int>& offs)
   {
     if (fildir.empty()) return;
+    if (mpi_files) return dout_files_mpi(msg, coords, offs);
     const bool one = offs.size() == 1;
     const char* tf = one ? "%.12f\n" : "%f\n";
     char buf[64];
@@ -250,6 +288,43 @@ class MHDDevice {
     }
     std::snprintf(buf, sizeof(buf), "outdat_%05d.dat", cnt);
     write_state(path(buf));
+    if (msg) std::printf("Output data at t=%.4f (%d iterations).\n", tim, n);
+  }
+  // dout_ of the MPI drivers as rank 0 of a 1-rank decomposition
+  // (2D/MPI/OTvortex/mhd2d_class.cpp dout_, 3D/MPI/OTvortex/
+  // mhd3d_class.cpp:78-145): params.dat and t.dat with "%f", offsets.dat,
+  // mpinum.dat "1 1[ 1]", x_00000.dat.. with "%.12f", and
+  // outdat_%05d_00000.dat, the 8 arrays in FP32 (ghosts included).
+  void dout_files_mpi(int msg, const std::vector<std::vector<double>>& coords, const std::vector<int>& offs)
+  {
+    char buf[64];
+    if (n == 0) {
+      std::snprintf(buf, sizeof(buf), "%f\n", gam);
+      write_text(path("params.dat"), "w", buf);
+      std::string o, r;
+      for (size_t a = 0; a < offs.size(); a++) {
+        o += (a ? " " : "") + std::to_string(offs[a]);
+        r += a ? " 1" : "1";
+      }
+      write_text(path("offsets.dat"), "w", o + "\n");
+      write_text(path("mpinum.dat"), "w", r + "\n");
+      std::snprintf(buf, sizeof(buf), "%f\n", tim);
+      write_text(path("t.dat"), "w", buf);
+      const char* names[3] = {"x_00000.dat", "y_00000.dat", "z_00000.dat"};
+      for (size_t a = 0; a < coords.size(); a++) {
+        std::string t;
+        for (double v : coords[a]) {
+          std::snprintf(buf, sizeof(buf), "%.12f\n", v);
+          t += buf;
+        }
+        write_text(path(names[a]), "w", t);
+      }
+    } else {
+      std::snprintf(buf, sizeof(buf), "%f\n", tim);
+      write_text(path("t.dat"), "a", buf);
+    }
+    std::snprintf(buf, sizeof(buf), "outdat_%05d_00000.dat", cnt);
+    write_state_f32(path(buf));
     if (msg) std::printf("Output data at t=%.4f (%d iterations).\n", tim, n);
   }
 
@@ -287,9 +362,11 @@ class MHD2DDevice : public MHDDevice {
       nflg_ = 1;
     }
   }
-  // exec_ (mhd2d_class.cpp:35-61): the step loop; output via dout_
+  // exec_ (mhd2d_class.cpp:35-61): the step loop; output via dout_ (with
+  // mpi_files, the MPI driver's exec_: backup resume and a backup per output)
   void exec_(int flg)
   {
+    resume_if_mpi();
     if (n == 0) dout_(0);
     while (n++ < nmax && tim < tmax) {
       tim += dt;
@@ -300,6 +377,7 @@ class MHD2DDevice : public MHDDevice {
         cnt++;
         trec += dtrec;
         download();
+        backup_if_mpi();
         dout_(1);
       }
     }
@@ -396,6 +474,7 @@ class MHD3DDevice : public MHDDevice {
     alloc(nd);
     trec = dtrec;
     nmax = nout;
+    mpi_files = true;  // the 3D reference driver is MPI only
   }
   void setdt(int flg)
   {
@@ -407,6 +486,7 @@ class MHD3DDevice : public MHDDevice {
   }
   void exec_(int flg)
   {
+    resume_if_mpi();
     if (n == 0) dout_(0);
     while (n++ < nmax && tim < tmax) {
       tim += dt;
@@ -417,6 +497,7 @@ class MHD3DDevice : public MHDDevice {
         cnt++;
         trec += dtrec;
         download();
+        backup_if_mpi();
         dout_(1);
       }
     }

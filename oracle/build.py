@@ -81,6 +81,17 @@ def build_ref_unit(force: bool = False) -> str:
     return out
 
 
+def build_ref_merge(force: bool = False) -> str:
+    """The reference's merge tool of the 2D MPI drivers (2D/MPI/common/merge.c:
+    per-rank outdat_%05d_%05d.dat -> merge_outdat_%05d.dat without ghosts),
+    compiled from its one source file."""
+    os.makedirs(REFOUT, exist_ok=True)
+    out = os.path.join(REFOUT, "merge2d")
+    if force or not os.path.exists(out):
+        _run(["gcc", "-O2", os.path.join(REF, "2D", "MPI", "common", "merge.c"), "-o", out])
+    return out
+
+
 def _cfl_tag(cfl: float) -> str:
     return ("%g" % cfl).replace(".", "p").replace("-", "m")
 

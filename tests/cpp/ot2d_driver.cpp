@@ -5,10 +5,12 @@
 // exec_(1).  It dumps the initial and final FP64 states so a test can rerun
 // the oracle on the same initial data.
 //
-// usage: ot2d_driver <xmesh> <ymesh> <nsteps> <rmn> <init.bin> <final.bin> [outdir [restart]]
+// usage: ot2d_driver <xmesh> <ymesh> <nsteps> <rmn> <init.bin> <final.bin> [outdir [restart|mpi]]
 // With outdir, the initial output (dout_) and a backup at the end go there;
 // with "restart", the run instead resumes from that backup (bkup_load) and
-// advances nsteps more steps.
+// advances nsteps more steps.  With "mpi" it runs as the MPI driver does
+// (mpi_files): an output and a backup after every step, and a run that
+// finds a backup in outdir resumes from it up to step nsteps.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -18,7 +20,10 @@
 
 class OTVortex : public qasmhd::MHD2DDevice {
  public:
-  OTVortex(int xm, int ym, int nsteps) : MHD2DDevice(xm, ym, 1, 1e6, 0.4), steps_(nsteps) {}
+  // every_step: dtrec = 1e-9 (an output after every step) and n_out large
+  // enough that tmax (= dtrec * n_out) never ends the run
+  OTVortex(int xm, int ym, int nsteps, bool every_step = false)
+      : MHD2DDevice(xm, ym, every_step ? 1000000000 : 1, every_step ? 1e-9 : 1e6, 0.4), steps_(nsteps) {}
   void paras()
   {
     xmin = 0.0;
@@ -89,11 +94,19 @@ int main(int argc, char** argv)
     return 1;
   }
   try {
-    OTVortex ot(std::atoi(argv[1]), std::atoi(argv[2]), std::atoi(argv[3]));
+    const std::string mode = argc > 8 ? argv[8] : "";
+    OTVortex ot(std::atoi(argv[1]), std::atoi(argv[2]), std::atoi(argv[3]), mode == "mpi");
     ot.set_scheme(std::atoi(argv[4]));
     ot.paras();
     if (argc > 7) ot.fildir = argv[7];
-    if (argc > 8 && std::string(argv[8]) == "restart") {
+    if (mode == "mpi") {
+      ot.mpi_files = true;
+      ot.init_();
+      ot.save(argv[5]);
+      ot.setdt(1);
+      ot.run_steps(1);
+      ot.save(argv[6]);
+    } else if (mode == "restart") {
       ot.restart_steps(1);
       ot.save(argv[6]);
     } else {

@@ -154,3 +154,57 @@ def test_torch_autograd_function():
     r = subprocess.run([sys.executable, "-u", script], capture_output=True, text=True, timeout=240)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
+
+
+def _ref_binary(rmn, n):
+    """The reference's own 2D driver (2D/SERIAL/OTvortex/mhd2d_{solve,class}.cpp,
+    built from source by oracle/build.py where the sources lie); None where
+    it was not built (the reference sources are absent on the GPU host, so
+    the binary travels prebuilt)."""
+    import os
+    from oracle import build as obuild
+    path = os.path.join(obuild.REFOUT, obuild.ref_binary_name(2, n, n, rmn, 4, 3, 1, 0.4))
+    return path if os.access(path, os.X_OK) else None
+
+
+@pytest.mark.parametrize("rmn", [1, 3])
+def test_adjoint_vs_reference_fd(rmn):
+    """The adjoint gradient against central finite differences of the
+    reference's own CPU code: its 2D driver run at a fixed dt (exec_ with
+    flg = 0: bound, then MHD2D::ideal(dt), every step) on U0 +- h d."""
+    from oracle import build as obuild
+    n = 32
+    binary = _ref_binary(rmn, n)
+    if binary is None:
+        pytest.skip("reference 2D driver %dx%d rmn %d not built" % (n, n, rmn))
+    print("case: adjoint vs FD of the reference driver, rmn %d, %dx%d, 3 steps" % (rmn, n, n), flush=True)
+    p, s, hist = _setup(n=n, rmn=rmn)
+    dts = [float(hist[0])] * 3
+    rng = np.random.default_rng(17)
+    u0 = p.state.copy()
+    W = rng.standard_normal(u0.shape)
+    s.upload(u0)
+    s.adj_forward(dts)
+    g = s.adj_backward(W)
+    s.close()
+    inner = np.zeros(u0.shape[1:], bool)
+    inner[NG:-NG, NG:-NG] = True
+
+    def ref_final(u):
+        out, _ = obuild.run_ref(binary, u, p.dx, p.gam, len(dts), 0, dt=dts[0], dy=p.dy, threads=1)
+        return out
+
+    def D(d, hh):
+        return float(np.sum(W * (ref_final(u0 + hh * d) - ref_final(u0 - hh * d)))) / (2.0 * hh)
+
+    errs = []
+    for m in range(8):
+        d = np.zeros_like(u0)
+        d[m][inner] = rng.standard_normal(inner.sum())
+        h = 1e-7 * (np.abs(u0[m][inner]).max() + 1e-3)
+        fd = (4.0 * D(d, h / 2) - D(d, h)) / 3.0
+        ad = float(np.sum(g * d))
+        err = abs(ad - fd) / max(abs(ad), abs(fd), 1e-300)
+        print("  var %d: adjoint %.12e  reference fd %.12e  rel %.2e" % (m, ad, fd, err), flush=True)
+        errs.append(err)
+    assert max(errs) < 1e-5, errs

@@ -231,6 +231,69 @@ def test_cpp_driver_output_and_restart(oracle, tmp_path):
     assert np.array_equal(np.fromfile(fin).reshape(shape), ref)
 
 
+def test_cpp_driver_mpi_files_and_resume(oracle, tmp_path):
+    """The MPI drivers' file set and backup cycle as one rank (mpi_files;
+    2D/MPI/OTvortex/mhd2d_class.cpp exec_ and dout_): an output and a
+    backup after every step, outdat_%05d_00000.dat in FP32 (a plain cast of
+    each value, common/common_func.c:22-26), mpinum.dat "1 1"; a second run
+    over the same directory resumes from the backup and ends where 12
+    straight steps of the oracle do, bit for bit."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLDEN), "cpp", "_build", "ot2d_driver")
+    if not os.path.exists(exe):
+        pytest.skip("C++ driver not built (run __graft_entry__.build())")
+    out = tmp_path / "dat"
+    out.mkdir()
+    init, mid, fin = str(tmp_path / "init.bin"), str(tmp_path / "mid.bin"), str(tmp_path / "fin.bin")
+    _say("cpp driver, MPI file set")
+    r = subprocess.run([exe, "32", "24", "6", "1", init, mid, str(out), "mpi"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    shape = (8, 32, 40)
+    st0 = np.fromfile(init).reshape(shape)
+    st6 = np.fromfile(mid).reshape(shape)
+    assert (out / "mpinum.dat").read_text() == "1 1\n"
+    assert (out / "offsets.dat").read_text() == "4 4\n"
+    assert np.loadtxt(str(out / "x_00000.dat")).size == 40
+    assert np.loadtxt(str(out / "y_00000.dat")).size == 32
+    assert np.array_equal(np.fromfile(str(out / "outdat_00000_00000.dat"), dtype=np.float32).reshape(shape),
+                          st0.astype(np.float32))
+    assert np.array_equal(np.fromfile(str(out / "outdat_00006_00000.dat"), dtype=np.float32).reshape(shape),
+                          st6.astype(np.float32))
+    assert not (out / "outdat_00007_00000.dat").exists()
+    assert (out / "bkup_stamp.dat").read_text().split() == ["6", "6"]
+    assert np.array_equal(np.fromfile(str(out / "bkup_data_00000.dat")).reshape(shape), st6)
+    assert len((out / "t.dat").read_text().split()) == 7
+    # resume: the same command with 12 steps continues from step 6
+    r = subprocess.run([exe, "32", "24", "12", "1", init, fin, str(out), "mpi"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = 2 * np.pi / 32, 2 * np.pi / 24
+    cfg = oracle.make_cfg(shape[1:], d[0], d[1])
+    ref = st0.copy()
+    oracle.run(cfg, ref, 12, 1)
+    st12 = np.fromfile(fin).reshape(shape)
+    assert np.array_equal(st12, ref)
+    assert (out / "bkup_stamp.dat").read_text().split() == ["12", "12"]
+    assert np.array_equal(np.fromfile(str(out / "outdat_00012_00000.dat"), dtype=np.float32).reshape(shape),
+                          ref.astype(np.float32))
+    assert len((out / "t.dat").read_text().split()) == 13
+    # the reference's own merge tool (2D/MPI/common/merge.c, built from its
+    # source) reads the file set: merged outputs are the FP32 interiors
+    from oracle import build as obuild
+    merge = os.path.join(obuild.REFOUT, "merge2d")
+    if os.access(merge, os.X_OK):
+        (tmp_path / "merged").mkdir()
+        r = subprocess.run([merge, "dat", "merged"], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+        assert r.returncode == 0, r.stderr
+        mdir = tmp_path / "merged"
+        assert np.loadtxt(str(mdir / "merge_x.dat")).size == 32
+        assert np.loadtxt(str(mdir / "merge_y.dat")).size == 24
+        for c, st in ((0, st0), (6, st6), (12, ref)):
+            m = np.fromfile(str(mdir / ("merge_outdat_%05d.dat" % c)), dtype=np.float32).reshape(8, 24, 32)
+            assert np.array_equal(m, st[:, 4:-4, 4:-4].astype(np.float32)), c
+
+
 @pytest.mark.parametrize("nsteps", [7, 10])
 def test_graph_replay_matches_direct_launches(qas, oracle, nsteps):
     """qas_run with captured 2-step graphs == host-launched steps == oracle,
