@@ -253,13 +253,18 @@ int qas_adjoint_selfcheck(int which, int ntrials, unsigned long long seed, doubl
  *                     dv = n x 2 shock sensor
  *   qas_test_cuct     mhd_cuct_weight, common/mhd_func.c:56-76; cells = n x
  *                     (ro[4], vx[4], vy[4]) of cells (i-1,j-1),(i,j-1),(i-1,j),
- *                     (i,j); faces = n x (bx(i,j-1), bx(i,j), by(i-1,j), by(i,j)) */
+ *                     (i,j); faces = n x (bx(i,j-1), bx(i,j), by(i-1,j), by(i,j))
+ *   qas_test_rcp      no reference counterpart: the in-range reciprocal the
+ *                     WCNS weights use (fast) and the IEEE division 1.0 / d
+ *                     (div), to show they agree bit for bit where the
+ *                     kernels take the former (2^-767 <= |d| <= 2^1022) */
 int qas_test_interp(int kind, int n, const double *f5, double *fl, double *fr);
 int qas_test_f4(int op, int odr, int n, const double *f4, double *out);
 int qas_test_lrstate(int odr, int n, const double *st, const double *bn, double gam, double *vl, double *vr);
 int qas_test_flux(int rmn, int n, const double *ql, const double *qr, const double *bn, double gam,
                   const double *dv, double *f);
 int qas_test_cuct(int n, const double *cells, const double *faces, double *out);
+int qas_test_rcp(int n, const double *d, double *fast, double *div);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_batch_create", "qas_batch_run", "qas_batch_last_error", "qas_batch_destroy",
            "qas_batch_tune", "qas_batch_set_shape", "qas_batch_shape",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
-           "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct",
+           "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct", "qas_test_rcp",
            "qas_set_dissipation", "qas_dsptv", "qas_set_gravity"]
 
 _lib = None
@@ -100,6 +100,7 @@ def load(path: str | None = None):
     l.qas_test_lrstate.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
     l.qas_test_flux.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, dp, dp]
     l.qas_test_cuct.argtypes = [C.c_int, dp, dp, dp]
+    l.qas_test_rcp.argtypes = [C.c_int, dp, dp, dp]
     l.qas_set_dissipation.argtypes = [vp, dp, dp, C.c_double]
     l.qas_dsptv.argtypes = [vp, C.c_double]
     l.qas_set_gravity.argtypes = [vp, dp]
@@ -160,6 +161,13 @@ class points:
         out = np.zeros(cells.shape[0])
         _chk(load().qas_test_cuct(cells.shape[0], _p(cells), _p(faces), _p(out)), "qas_test_cuct")
         return out
+
+    @staticmethod
+    def rcp(d):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        fast, div = np.zeros(d.shape[0]), np.zeros(d.shape[0])
+        _chk(load().qas_test_rcp(d.shape[0], _p(d), _p(fast), _p(div)), "qas_test_rcp")
+        return fast, div
 
 
 def adjoint_selfcheck(name: str, ntrials: int = 1000, seed: int = 1):

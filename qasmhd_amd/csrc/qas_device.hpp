@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 // host+device: the host instantiation with an op-counting scalar gives the
 // algorithmic flop model (qas_flops.hip)
 #define QAS_HD __host__ __device__ __forceinline__
@@ -36,6 +38,29 @@ template <typename T> QAS_HD T dsqrt(T x) { return sqrt(x); }
 template <typename T> QAS_HD T dabs(T x) { return fabs(x); }
 template <> QAS_HD float dsqrt<float>(float x) { return sqrtf(x); }
 template <> QAS_HD float dabs<float>(float x) { return fabsf(x); }
+
+// 1.0 / d for 2^-767 <= d <= 2^1022, bit-identical to the IEEE division.
+// On gfx950 an FP64 division is: operand scaling (v_div_scale, twice), a
+// reciprocal estimate, two Newton steps, the quotient and its residual,
+// the corrected quotient (v_div_fmas) and the special-case fix-up
+// (v_div_fixup).  Inside that range neither operand of 1.0 / d is scaled,
+// v_div_fmas is a plain FMA and the fix-up returns its input, so the same
+// estimate, Newton steps and correction give the same bits in 7
+// instructions instead of 11.  Callers check the range.
+QAS_HD double rcp_inrange(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(e, r, r);
+#else
+  return 1.0 / d;
+#endif
+}
 
 // ------------------------------------------------------------------ 1-D interpolation
 // f[0..4] = stencil i-2..i+2; fl = left state at i+1/2, fr = right state at i-1/2.
@@ -68,6 +93,44 @@ template <typename T>
 QAS_HD T minmod(T a, T b)
 {
   return (+tmin(tmax(a, b), T(0.0)) + tmax(tmin(a, b), T(0.0)));
+}
+
+// The weights and face values of WCNS-4 (common_interp.c:102-150) from
+// the three denominators d_k = IS_k^2 + eps; IN_RANGE takes the reciprocals
+// with rcp_inrange (FP64, the caller checked the range), else 1.0 / x.
+template <bool IN_RANGE, typename T>
+QAS_HD void wcns4_weights(const T* f, T d0, T d1, T d2, T& fl, T& fr)
+{
+  auto inv = [](T x) -> T {
+    if constexpr (IN_RANGE) return rcp_inrange(x);
+    else return T(1.0) / x;
+  };
+  const T s0 = inv(d0);
+  const T s1 = inv(d1);
+  const T s2 = inv(d2);
+  // sub-stencil values (shared by both sides)
+  const T qa = T(3.0) * f[0] - T(10.0) * f[1] + T(15.0) * f[2];
+  const T qb = -f[1] + T(6.0) * f[2] + T(3.0) * f[3];
+  const T qc = +T(3.0) * f[2] + T(6.0) * f[3] - f[4];
+  const T ra = T(3.0) * f[4] - T(10.0) * f[3] + T(15.0) * f[2];
+  const T rb = -f[3] + T(6.0) * f[2] + T(3.0) * f[1];
+  const T rc = +T(3.0) * f[2] + T(6.0) * f[1] - f[0];
+  T w0 = T(0.075) * s0;
+  T w1 = T(0.650) * s1;
+  T w2 = T(0.275) * s2;
+  T den = inv(w0 + w1 + w2);
+  w0 *= den;
+  w1 *= den;
+  w2 = T(1.0) - w0 - w1;
+  fl = T(0.125) * (+w0 * qa + w1 * qb + w2 * qc);
+  w0 = T(0.275) * s0;
+  w1 = T(0.650) * s1;
+  w2 = T(0.075) * s2;
+  den = inv(w0 + w1 + w2);
+  w0 *= den;
+  w1 *= den;
+  w2 = T(1.0) - w0 - w1;
+  fr = T(0.125) * (+w2 * ra + w1 * rb + w0 * rc);
 }
 
 template <int ODR, typename T>
@@ -105,39 +168,25 @@ QAS_HD void interp5(const T* f, T& fl, T& fr)
     const T c13 = T(13. / 12.);
     T a = f[0] - T(4.0) * f[1] + T(3.0) * f[2];
     T b = f[0] - T(2.0) * f[1] + f[2];
-    T s0 = c13 * b * b + T(0.25) * a * a;
+    const T s0 = c13 * b * b + T(0.25) * a * a;
     a = f[1] - f[3];
     b = f[1] - T(2.0) * f[2] + f[3];
-    T s1 = c13 * b * b + T(0.25) * a * a;
+    const T s1 = c13 * b * b + T(0.25) * a * a;
     a = f[4] - T(4.0) * f[3] + T(3.0) * f[2];
     b = f[4] - T(2.0) * f[3] + f[2];
-    T s2 = c13 * b * b + T(0.25) * a * a;
-    s0 = T(1.0) / (s0 * s0 + eps);
-    s1 = T(1.0) / (s1 * s1 + eps);
-    s2 = T(1.0) / (s2 * s2 + eps);
-    // sub-stencil values (shared by both sides)
-    const T qa = T(3.0) * f[0] - T(10.0) * f[1] + T(15.0) * f[2];
-    const T qb = -f[1] + T(6.0) * f[2] + T(3.0) * f[3];
-    const T qc = +T(3.0) * f[2] + T(6.0) * f[3] - f[4];
-    const T ra = T(3.0) * f[4] - T(10.0) * f[3] + T(15.0) * f[2];
-    const T rb = -f[3] + T(6.0) * f[2] + T(3.0) * f[1];
-    const T rc = +T(3.0) * f[2] + T(6.0) * f[1] - f[0];
-    T w0 = T(0.075) * s0;
-    T w1 = T(0.650) * s1;
-    T w2 = T(0.275) * s2;
-    T den = T(1.0) / (w0 + w1 + w2);
-    w0 *= den;
-    w1 *= den;
-    w2 = T(1.0) - w0 - w1;
-    fl = T(0.125) * (+w0 * qa + w1 * qb + w2 * qc);
-    w0 = T(0.275) * s0;
-    w1 = T(0.650) * s1;
-    w2 = T(0.075) * s2;
-    den = T(1.0) / (w0 + w1 + w2);
-    w0 *= den;
-    w1 *= den;
-    w2 = T(1.0) - w0 - w1;
-    fr = T(0.125) * (+w2 * ra + w1 * rb + w0 * rc);
+    const T s2 = c13 * b * b + T(0.25) * a * a;
+    const T d0 = s0 * s0 + eps, d1 = s1 * s1 + eps, d2 = s2 * s2 + eps;
+    if constexpr (std::is_same<T, double>::value) {
+      // d >= eps = 1e-40 always; d <= 2^700 keeps every 1/d in
+      // [2^-700, 1e40] and both weight sums in [0.65 * 2^-700, 1e40], so
+      // all five reciprocals are inside rcp_inrange's range (NaN fails
+      // the test and takes the division)
+      if ((d0 <= 0x1p700) & (d1 <= 0x1p700) & (d2 <= 0x1p700)) {
+        wcns4_weights<true>(f, d0, d1, d2, fl, fr);
+        return;
+      }
+    }
+    wcns4_weights<false>(f, d0, d1, d2, fl, fr);
   }
 }
 

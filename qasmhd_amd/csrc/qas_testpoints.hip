@@ -76,6 +76,16 @@ __global__ void k_t_cuct(int n, const double* c, const double* b, double* out)
                        y[3]);
 }
 
+// rcp_inrange(d) and the IEEE division 1.0 / d side by side
+__global__ void k_t_rcp(int n, const double* d, double* fast, double* div)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double x = d[q];
+  fast[q] = rcp_inrange(x);
+  div[q] = 1.0 / x;
+}
+
 // device copies of host arrays for one call
 struct Batch {
   std::vector<void*> bufs;
@@ -215,5 +225,20 @@ extern "C" int qas_test_cuct(int n, const double* cells, const double* faces, do
     b.err = hipGetLastError();
   }
   b.get(out, d, n);
+  return b.finish();
+}
+
+extern "C" int qas_test_rcp(int n, const double* d, double* fast, double* div)
+{
+  if (n <= 0 || !d || !fast || !div) return QAS_E_ARG;
+  Batch b;
+  const double* dd = b.in(d, n);
+  double *df = b.out(n), *dv = b.out(n);
+  if (b.err == hipSuccess) {
+    hipLaunchKernelGGL(k_t_rcp, nb(n), 128, 0, 0, n, dd, df, dv);
+    b.err = hipGetLastError();
+  }
+  b.get(fast, df, n);
+  b.get(div, dv, n);
   return b.finish();
 }
