@@ -67,3 +67,26 @@ def test_mixed_3d_runs(oracle):
     e = l1rel(b[0], a[0])
     _say("mixed vs fp64 3d: L1(rho) %.3e" % e)
     assert np.isfinite(b).all() and e < 1e-5
+
+
+def test_mixed_full_size_khi():
+    """The low-Mach Kelvin-Helmholtz workload at its full size and length
+    (2048^2, M = 0.01, MLAU / WCNS-4 / CT / RK3, 2000 steps at one fixed dt):
+    mean |rho_mixed - rho_fp64| and mean |p_mixed - p_fp64| over the
+    interior at most 1e-6 (rho = 1 and p = 1/gam, so absolute and relative
+    L1 coincide up to the factor gam for p)."""
+    from qasmhd_amd import capi, problems
+    p = problems.kelvin_helmholtz(n=2048)
+    with capi.solver_for(p) as s:
+        s.upload(p.state)
+        dt = s.cfl_dt()
+    _say("case: mixed vs fp64, KH 2048^2, 2000 steps, fp64 run")
+    a = run(p, 2000, 0, dt)
+    _say("  mixed run")
+    b = run(p, 2000, 1, dt)
+    assert np.isfinite(b).all()
+    I = (slice(4, -4), slice(4, -4))
+    e_ro = float(np.abs(b[0][I] - a[0][I]).mean())
+    e_pr = float(np.abs(pressure(b, p.gam)[I] - pressure(a, p.gam)[I]).mean())
+    _say("  L1(rho) %.3e  L1(p) %.3e" % (e_ro, e_pr))
+    assert e_ro <= 1e-6 and e_pr <= 1e-6
