@@ -150,6 +150,19 @@ def test_2d_full_size_invariants(qas):
     assert divb(out) < 1e-11 * bmax / p.dx + 10 * d0
 
 
+def test_3d_full_size_invariants():
+    """The 3D turbulence workload at its full size and length (512^3, 200
+    steps) in a fresh process (tests/gpu_turb3d_full_check.py: the state is
+    built and checked with torch on the device): periodic totals of rho, m
+    and E at round-off, div B at round-off, finite."""
+    import subprocess
+    _say("3d 512^3 invariants, 200 steps (subprocess)")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_turb3d_full_check.py")
+    r = subprocess.run([sys.executable, "-u", script], capture_output=True, text=True, timeout=280)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+
+
 @pytest.mark.parametrize("rmn,odr,bc", [(1, 4, "periodic"), (3, 4, "mixed"), (2, 3, "periodic"), (0, 4, "mixed")])
 def test_3d_vs_oracle(qas, oracle, rmn, odr, bc):
     from qasmhd_amd import problems
