@@ -265,35 +265,64 @@ __global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
 }
 
 // ------------------------------------------------------------------ CT
+// Staggered-B update (mhd_updt3d_ctb, common/mhd_func.c:302-325), marching
+// along z: one thread per (i, j) column of a 64 x 4 tile over CT_KC planes.
+// The z-direction E values (Ey for bx, Ex for by) sit in 4-plane register
+// windows, so each of those planes is read once per chunk instead of four
+// times; the in-plane neighbours come through the caches.  Per face the
+// operations and their order are those of the reference.
+constexpr int CT_KC = 32;
+
 template <int ODR>
 __global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
 {
-  const int ni = g.nx - 7, nj = g.ny - 7, nk = g.nz - 7;  // [4, n-4]
-  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= (long long)ni * nj * nk) return;
-  const int i = 4 + (int)(t % ni);
-  const int j = 4 + (int)((t / ni) % nj);
-  const int k = 4 + (int)(t / ((long long)ni * nj));
+  const int ni = g.nx - 7, nj = g.ny - 7, nk = g.nz - 7;  // cells [4, n-4]
+  const int ntx = (ni + EMF_TX - 1) / EMF_TX, nty = (nj + EMF_TY - 1) / EMF_TY;
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int tile = L % (ntx * nty), chunk = L / (ntx * nty);
+  const int i = 4 + (tile % ntx) * EMF_TX + (int)(threadIdx.x % EMF_TX);
+  const int j = 4 + (tile / ntx) * EMF_TY + (int)(threadIdx.x / EMF_TX);
+  const int k0 = 4 + chunk * CT_KC;
+  if (i > g.nx - 4 || j > g.ny - 4 || k0 > g.nz - 4) return;  // no barriers below
+  const int k1 = (k0 + CT_KC < 4 + nk) ? k0 + CT_KC : 4 + nk;
   const long long sy = g.sy, sz = g.sz;
-  const long long ss = sz * k + sy * j + i;
+  const long long c0 = sy * j + i;
   const double dtv = *p.dt;
   const double dtdx = dtv / g.dx, dtdy = dtv / g.dy, dtdz = dtv / g.dz;
   const double *ex = p.ex, *ey = p.ey, *ez = p.ez;
-  const bool ji = j <= g.ny - 5, ki = k <= g.nz - 5, ii = i <= g.nx - 5;
-  if (ji && ki) {  // bx: one extra face in x
-    const double du0 = +dfc<ODR>(ez[ss - sy], ez[ss], ez[ss + sy], ez[ss + 2 * sy]);
-    const double du1 = -dfc<ODR>(ey[ss - sz], ey[ss], ey[ss + sz], ey[ss + 2 * sz]);
-    p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du0 - dtdz * du1, p.a, p.b);
+  const bool ji = j <= g.ny - 5, ii = i <= g.nx - 5;
+  // windows: planes k-1 .. k+2 of ey and ex at (i, j); planes [3, nz-2]
+  double eyw[4], exw[4];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    eyw[m] = ey[c0 + sz * (k0 - 1 + m)];
+    exw[m] = ex[c0 + sz * (k0 - 1 + m)];
   }
-  if (ii && ki) {  // by: one extra face in y
-    const double du0 = +dfc<ODR>(ex[ss - sz], ex[ss], ex[ss + sz], ex[ss + 2 * sz]);
-    const double du1 = -dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
-    p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -dtdz * du0 - dtdx * du1, p.a, p.b);
-  }
-  if (ii && ji) {  // bz: one extra face in z
-    const double du0 = +dfc<ODR>(ey[ss - 1], ey[ss], ey[ss + 1], ey[ss + 2]);
-    const double du1 = -dfc<ODR>(ex[ss - sy], ex[ss], ex[ss + sy], ex[ss + 2 * sy]);
-    p.out[6][ss] = rk_upd(p.in[6][ss], p.u0[6][ss], -dtdx * du0 - dtdy * du1, p.a, p.b);
+  for (int k = k0; k < k1; k++) {
+    const long long ss = sz * k + c0;
+    eyw[3] = ey[ss + 2 * sz];
+    exw[3] = ex[ss + 2 * sz];
+    const bool ki = k <= g.nz - 5;
+    if (ji && ki) {  // bx: one extra face in x
+      const double du0 = +dfc<ODR>(ez[ss - sy], ez[ss], ez[ss + sy], ez[ss + 2 * sy]);
+      const double du1 = -dfc<ODR>(eyw[0], eyw[1], eyw[2], eyw[3]);
+      p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du0 - dtdz * du1, p.a, p.b);
+    }
+    if (ii && ki) {  // by: one extra face in y
+      const double du0 = +dfc<ODR>(exw[0], exw[1], exw[2], exw[3]);
+      const double du1 = -dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
+      p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -dtdz * du0 - dtdx * du1, p.a, p.b);
+    }
+    if (ii && ji) {  // bz: one extra face in z
+      const double du0 = +dfc<ODR>(ey[ss - 1], eyw[1], ey[ss + 1], ey[ss + 2]);
+      const double du1 = -dfc<ODR>(ex[ss - sy], exw[1], ex[ss + sy], ex[ss + 2 * sy]);
+      p.out[6][ss] = rk_upd(p.in[6][ss], p.u0[6][ss], -dtdx * du0 - dtdy * du1, p.a, p.b);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      eyw[m] = eyw[m + 1];
+      exw[m] = exw[m + 1];
+    }
   }
 }
 
@@ -353,10 +382,13 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   const long long nemf = (long long)((g.nx - 5 + EMF_TX - 1) / EMF_TX) * ((g.ny - 5 + EMF_TY - 1) / EMF_TY) *
                          ((g.nz - 5 + EMF_KC - 1) / EMF_KC);
   (void)ne;
+  (void)nc;
+  const long long nct = (long long)((g.nx - 7 + EMF_TX - 1) / EMF_TX) * ((g.ny - 7 + EMF_TY - 1) / EMF_TY) *
+                        ((g.nz - 7 + CT_KC - 1) / CT_KC);
 #define QAS_EMF_CT(O)                                                                                      \
   if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d_z<O, float>), (unsigned)nemf, 256, 0, st, p, g, c->s.ctw);   \
   else QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d_z<O, double>), (unsigned)nemf, 256, 0, st, p, g, c->s.ctw);       \
-  QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O>), nblk3(nc, 256), 256, 0, st, p, g);
+  QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O>), (unsigned)nct, 256, 0, st, p, g);
   switch (odr) {
     case 1: QAS_EMF_CT(1) break;
     case 2: QAS_EMF_CT(2) break;
