@@ -176,7 +176,7 @@ QAS_HD void lrstate_adj_g(const Ld& ld, T bn, T gam, const T (&gvl_in)[7], const
       T wk[5];
 #pragma unroll
       for (int m = 0; m < 5; m++) wk[m] = proj(k, m);
-      interp5<4>(wk, wl[k], wr[k]);
+      interp5<4, T, false>(wk, wl[k], wr[k]);
     }
     ld.fence();
     for (int side = 0; side < 2; side++) {

@@ -361,7 +361,7 @@ __device__ __forceinline__ void adj_sweep_body(const AdjPtrs& p, const Geo& g, d
     SweepFwd<DIR> f;
     f.load(p, fq, o);
     double vl[7], vr[7];
-    lrstate<4>(f.s, f.bnc, gam, vl, vr, &fbc);
+    lrstate<4, double, 0, false>(f.s, f.bnc, gam, vl, vr, &fbc);
     double L[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) L[k] = __shfl_up(vl[k], 1, WAVE);

@@ -133,7 +133,9 @@ QAS_HD void wcns4_weights(const T* f, T d0, T d1, T d2, T& fl, T& fr)
   fr = T(0.125) * (+w2 * ra + w1 * rb + w0 * rc);
 }
 
-template <int ODR, typename T>
+// INRANGE: the WCNS-4 reciprocals take rcp_inrange when in range (FP64);
+// false keeps the plain division (a smaller body, for the adjoint kernels)
+template <int ODR, typename T, bool INRANGE = true>
 QAS_HD void interp5(const T* f, T& fl, T& fr)
 {
   if constexpr (ODR == 1) {
@@ -176,7 +178,7 @@ QAS_HD void interp5(const T* f, T& fl, T& fr)
     b = f[4] - T(2.0) * f[3] + f[2];
     const T s2 = c13 * b * b + T(0.25) * a * a;
     const T d0 = s0 * s0 + eps, d1 = s1 * s1 + eps, d2 = s2 * s2 + eps;
-    if constexpr (std::is_same<T, double>::value) {
+    if constexpr (INRANGE && std::is_same<T, double>::value) {
       // d >= eps = 1e-40 always; d <= 2^700 keeps every 1/d in
       // [2^-700, 1e40] and both weight sums in [0.65 * 2^-700, 1e40], so
       // all five reciprocals are inside rcp_inrange's range (NaN fails
@@ -211,7 +213,7 @@ QAS_HD T dfc(T fm1, T f0, T f1, T f2)
 // MODE 1 / 2: only the fast/Alfven resp. Alfven/entropy projection is
 // evaluated (used by the flop model to count one branch).  fbc, when given,
 // receives the positivity-fallback flags (1: left state, 2: right state).
-template <int ODR, typename T, int MODE = 0>
+template <int ODR, typename T, int MODE = 0, bool INRANGE = true>
 QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int* fbc = nullptr)
 {
   const T ro0 = s[0][2], bt0 = s[4][2], bu0 = s[5][2], pr0 = s[6][2];
@@ -256,11 +258,11 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
       w[m] = fast ? wf : s[1][m];
     }
   }
-  interp5<ODR>(w, wl[0], wr[0]);
+  interp5<ODR, T, INRANGE>(w, wl[0], wr[0]);
   // k = 1: cf2 ro - bt0 bt - bu0 bu - pr (both)
 #pragma unroll
   for (int m = 0; m < 5; m++) w[m] = cf2 * s[0][m] - bt0 * s[4][m] - bu0 * s[5][m] - s[6][m];
-  interp5<ODR>(w, wl[1], wr[1]);
+  interp5<ODR, T, INRANGE>(w, wl[1], wr[1]);
   // k = 2: fast -> rcf vn + bt0 bt + bu0 bu + pr ; A/E -> bt0 bt + bu0 bu + pr
 #pragma unroll
   for (int m = 0; m < 5; m++) {
@@ -272,20 +274,20 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
       w[m] = fast ? wf : wa;
     }
   }
-  interp5<ODR>(w, wl[2], wr[2]);
+  interp5<ODR, T, INRANGE>(w, wl[2], wr[2]);
   // k = 3..6: Alfven pairs (both)
 #pragma unroll
   for (int m = 0; m < 5; m++) w[m] = sro * s[2][m] + s[4][m];
-  interp5<ODR>(w, wl[3], wr[3]);
+  interp5<ODR, T, INRANGE>(w, wl[3], wr[3]);
 #pragma unroll
   for (int m = 0; m < 5; m++) w[m] = sro * s[2][m] - s[4][m];
-  interp5<ODR>(w, wl[4], wr[4]);
+  interp5<ODR, T, INRANGE>(w, wl[4], wr[4]);
 #pragma unroll
   for (int m = 0; m < 5; m++) w[m] = sro * s[3][m] + s[5][m];
-  interp5<ODR>(w, wl[5], wr[5]);
+  interp5<ODR, T, INRANGE>(w, wl[5], wr[5]);
 #pragma unroll
   for (int m = 0; m < 5; m++) w[m] = sro * s[3][m] - s[5][m];
-  interp5<ODR>(w, wl[6], wr[6]);
+  interp5<ODR, T, INRANGE>(w, wl[6], wr[6]);
 
   // back projection
 #pragma unroll
