@@ -81,19 +81,53 @@ def workload(name):
 def recorded_traffic(workload_name, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary of
     this workload (profiles/*_traffic.md, written from two rocprofv3 --pmc
-    passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration);
-    None when there is none."""
+    passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration)
+    and where they come from (file, and the code commit the counters were
+    taken at, as the file states it); (None, None) when there is none.  The
+    counters cannot be read from inside this process, so the figure is the
+    committed measurement, not one of this run."""
     import glob
     import re
     tag = {"2d_decaying_mhd_1024x1024_hlld_wcns4_ct_rk3": "stage2d_turb1024_fp64"}.get(workload_name)
     if tag is None:
-        return None
+        return None, None
     for path in glob.glob(os.path.join(ROOT, "profiles", tag + "_traffic.md")):
         with open(path) as f:
-            m = re.search(r"%s=(\d+)" % kernel, f.read())
+            text = f.read()
+        m = re.search(r"%s=(\d+)" % kernel, text)
+        c = re.search(r"code commit: ([0-9a-f]+)", text)
         if m:
-            return float(m.group(1))
-    return None
+            return float(m.group(1)), {"file": os.path.relpath(path, ROOT),
+                                       "code_commit": c.group(1) if c else None,
+                                       "how": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, per dispatch, scaled by "
+                                              "a device copy of known size; read from the committed summary"}
+    return None, None
+
+
+def host_info():
+    """The host CPU the baselines ran on (lscpu) and the OpenMP placement."""
+    import subprocess
+    info = {"visible_logical_cpus": len(os.sched_getaffinity(0)),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v.strip()
+    except Exception as e:  # reported, not required
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_threads():
+    """OpenMP threads of the CPU baselines: OMP_NUM_THREADS when set (the
+    GPU host sets it to its CPU share), else the visible CPUs up to 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return min(16, len(os.sched_getaffinity(0)))
 
 
 def cpu_baseline(p, nsteps):
@@ -101,7 +135,7 @@ def cpu_baseline(p, nsteps):
     oracle (kind 'port'); same initial state, first `nsteps` steps."""
     import numpy as np
     from oracle import build as obuild
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = cpu_threads()
     zones = int(np.prod(p.interior))
     if p.ndim == 2:
         nx, ny = p.interior[1], p.interior[0]
@@ -110,8 +144,10 @@ def cpu_baseline(p, nsteps):
             _, _, el = obuild.run_ref(binary, p.state, p.dx, p.gam, nsteps, 1, dnx=list(p.dn[0]),
                                       dny=list(p.dn[1]), dy=p.dy, threads=threads, timing=True)
             return {"value": zones * nsteps / el, "unit": "zone-cycles/s", "cores": threads, "kind": "reference",
-                    "sample": "first %d steps of the same %dx%d run (reference MHD2D::ideal/bound/setdt, g++ -O2 "
-                              "-fopenmp, OMP_NUM_THREADS=%d), %.2f s" % (nsteps, nx, ny, threads, el)}
+                    "sample": "first %d steps of the same %dx%d run (reference MHD2D::ideal/bound/setdt with its "
+                              "setdt schedule, g++ -O2 -fopenmp, OMP_NUM_THREADS=%d, OMP_PROC_BIND=%s, OMP_PLACES=%s), "
+                              "%.2f s wall for the step loop" % (nsteps, nx, ny, threads, os.environ.get("OMP_PROC_BIND"),
+                                                                 os.environ.get("OMP_PLACES"), el)}
     from oracle import pyoracle
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     cfg = pyoracle.make_cfg(p.state.shape[1:], p.dx, p.dy, p.dz, gam=p.gam, cfl=p.cfl, rmn=p.rmn, odr=p.odr,
@@ -122,6 +158,101 @@ def cpu_baseline(p, nsteps):
     el = time.perf_counter() - t0
     return {"value": zones * nsteps / el, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
             "sample": "first %d steps of the same run (C oracle, OpenMP %d threads), %.2f s" % (nsteps, threads, el)}
+
+
+def brio_wu_line(cpu=True):
+    """The 1D Brio-Wu shock tube (1024 cells, gam 2, HLLD, WCNS-4, RK3, CFL
+    0.4, open ends) to t = 0.1 at the fixed dt of the initial setdt, as the
+    reference's 1D driver runs it (exec_(0), 1D/SERIAL/shock/main.cpp:9):
+    GPU zone-cycles/s over the whole run, and the reference's own serial 1D
+    driver (oracle/_ref) over the same steps on one host thread."""
+    from qasmhd_amd import capi, problems
+    p = problems.brio_wu(1024)
+    zones = int(np.prod(p.interior))
+    with capi.solver_for(p) as s:
+        s.upload(p.state)
+        dt0 = s.cfl_dt()
+        nmax = int(0.1 / dt0 + 0.5)  # mhd1d_class.cpp: nmax = (int)(tmax/dt + 0.5)
+        s.run(4, flg=0, dt=dt0, history=False)  # warm-up (graph capture)
+        s.upload(p.state)
+        s.reset_counter()
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(nmax, flg=0, dt=dt0, history=False)
+        s.sync()
+        el = time.perf_counter() - t0
+        out = s.download()
+    line = {"workload": "1d_brio_wu_1024_hlld_wcns4_rk3", "steps": nmax, "dt": dt0, "value": zones * nmax / el,
+            "unit": "zone-cycles/s", "ms_per_step": el / nmax * 1e3, "finite": bool(np.isfinite(out).all())}
+    if cpu:
+        try:
+            from oracle import build as obuild
+            binary = os.path.join(obuild.REFOUT, obuild.ref_binary_name(1, 1024, 1, p.rmn, p.odr, p.r_k, 1, p.cfl))
+            if os.path.exists(binary):
+                ref, _, el_ref = obuild.run_ref(binary, p.state, p.dx, p.gam, nmax, 0, dt=dt0, dnx=list(p.dn[0]),
+                                                threads=1, timing=True)
+                line["cpu_baseline"] = {"value": zones * nmax / el_ref, "unit": "zone-cycles/s", "cores": 1,
+                                        "kind": "reference",
+                                        "sample": "the whole run, %d steps (reference MHD1D::ideal/bound, g++ -O2 "
+                                                  "-fopenmp, 1 thread: the serial 1D path), %.3f s" % (nmax, el_ref)}
+                line["equals_reference"] = bool(np.array_equal(ref, out))
+        except Exception as e:  # reported, not required
+            _log("bench: reference 1D timing failed: %s" % e)
+    return line
+
+
+def batch_cpu_reference(sizes, steps=4):
+    """The reference's 2D driver on this host for the batched workload: its
+    grid size is a compile-time macro and its step keeps static state
+    (mhd2d_solve.cpp:8), so each instance is one process; every distinct
+    size is timed over `steps` steps of its first instance and the batch
+    rate is total zones over the summed per-instance step times."""
+    from qasmhd_amd import problems
+    from oracle import build as obuild
+    threads = cpu_threads()
+    per = {}
+    for n in sorted(set(int(v) for v in sizes)):
+        binary = os.path.join(obuild.REFOUT, obuild.ref_binary_name(2, n, n, 1, 4, 3, 1, 0.4))
+        if not os.path.exists(binary):
+            return None
+        p = problems.turbulence2d(n=n, seed=5, vrms=0.3, brms=0.3)
+        _, _, el = obuild.run_ref(binary, p.state, p.dx, p.gam, steps, 1, dnx=list(p.dn[0]), dny=list(p.dn[1]),
+                                  dy=p.dy, threads=threads, timing=True)
+        per[n] = el / steps
+    zones = sum(int(n) * int(n) for n in sizes)
+    t_step = sum(per[int(n)] for n in sizes)
+    return {"value": zones / t_step, "unit": "zone-cycles/s", "cores": threads, "kind": "reference",
+            "sample": "each distinct instance size (%s) for %d steps of the reference MHD2D (g++ -O2 -fopenmp, "
+                      "OMP_NUM_THREADS=%d), one process per instance, instances one after another"
+                      % (",".join(str(n) for n in sorted(per)), steps, threads),
+            "seconds_per_step": {str(k): v for k, v in per.items()}}
+
+
+def cpu_port_3d(n, steps, seed=4):
+    """The oracle (this build's C restatement of the reference's 3D step,
+    OpenMP) on the 3D turbulence set-up at n^3 for `steps` steps of the
+    setdt schedule: the 3D reference driver needs MPI, which this host
+    lacks.  The initial state is built on the device with torch and
+    copied to the host."""
+    import torch
+    from oracle import pyoracle
+    from qasmhd_amd import problems
+    threads = cpu_threads()
+    p = problems.turbulence3d(n=n, seed=seed, device="cuda:0")
+    st = p.state.cpu().numpy().copy()
+    del p.state
+    torch.cuda.empty_cache()
+    cfg = pyoracle.make_cfg(st.shape[1:], p.dx, p.dy, p.dz, gam=p.gam, cfl=p.cfl, rmn=p.rmn, odr=p.odr, r_k=p.r_k,
+                            ctw=p.ctw)
+    t0 = time.perf_counter()
+    pyoracle.run(cfg, st, steps, 1)
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / el, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": "%d steps of the same set-up at %d^3 (C oracle: the reference's 3D step restated, gcc -O2 "
+                      "-fopenmp, OMP_NUM_THREADS=%d, OMP_PROC_BIND=%s, OMP_PLACES=%s; the reference's 3D driver "
+                      "needs MPI), %.2f s" % (steps, n, threads, os.environ.get("OMP_PROC_BIND"),
+                                             os.environ.get("OMP_PLACES"), el),
+            "finite": bool(np.isfinite(st).all())}
 
 
 def _interior(a, ndim):
@@ -302,7 +433,8 @@ def kernel_roofline(s, lib, sweeps, wname):
     else:
         roof = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": tflops / FP64_PEAK_TFLOPS}
-    roof.update({"traffic": recorded_traffic(wname, names[best["kid"]]), "kernel": names[best["kid"]],
+    traffic, tsrc = recorded_traffic(wname, names[best["kid"]])
+    roof.update({"traffic": traffic, "traffic_source": tsrc, "kernel": names[best["kid"]],
                  "avg_launch_us": best["avg_s"] * 1e6, "launches": best["launches"],
                  "alg_flops_per_launch": best["flops"], "alg_bytes_per_launch": best["bytes"],
                  "fp64_tflops": tflops, "hbm_gbs": gbs, "fp64_frac": tflops / FP64_PEAK_TFLOPS,
@@ -404,7 +536,7 @@ def gravity_line(nx, ny, steps, cpu=True):
             binary = os.path.join(obuild.REFOUT, obuild.ref_binary_name(2, nx, ny, p.rmn, p.odr, p.r_k, p.ctw, p.cfl,
                                                                         gravity=True))
             if os.path.exists(binary):
-                threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+                threads = cpu_threads()
                 k = 4
                 _, _, el_ref = obuild.run_ref(binary, p.state, p.dx, p.gam, k, 1, dnx=list(p.dn[0]),
                                               dny=list(p.dn[1]), dy=p.dy, threads=threads, timing=True, phi=phi)
@@ -458,7 +590,7 @@ def dissipative_line(nx, ny, steps, cpu=True):
             from oracle import build as obuild
             binary = os.path.join(obuild.REFOUT, obuild.ref_dsptv_name(nx, ny, p.rmn, p.odr, p.r_k, p.ctw))
             if os.path.exists(binary):
-                threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+                threads = cpu_threads()
                 k = 4
                 _, el_ref = obuild.run_ref_dsptv(binary, p.state, nu, eta, p.dx, p.dy, p.gam, dt_sub, nsub=k,
                                                  dnx=list(p.dn[0]), dny=list(p.dn[1]), threads=threads,
@@ -518,19 +650,21 @@ def mixed3d_line(p, steps):
             "l1_rho": l1r, "l1_p": l1p, "finite": finite}
 
 
-def turb3d_child(n, steps, warmup=2):
+def turb3d_child(n, steps, warmup=2, cpu_n=0):
     """3D periodic MHD turbulence at n^3 (HLLD, WCNS-4, 3D CT, RK3, CFL 0.4;
     initial state built on the device with torch): this script's own main
     line for that workload, run as a child process so that this process
-    keeps its HIP runtime free of torch's."""
+    keeps its HIP runtime free of torch's; with cpu_n, also the C oracle's
+    rate on the same set-up at cpu_n^3."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--workload", "turb3d%d" % n, "--steps", str(steps),
-           "--warmup", str(warmup), "--no-cpu", "--no-extra", "--mixed3d-steps", str(steps)]
+           "--warmup", str(warmup), "--no-cpu", "--no-extra", "--mixed3d-steps", str(steps),
+           "--cpu3d-n", str(cpu_n)]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
     if r.returncode != 0:
         raise RuntimeError("3D child exited with %d" % r.returncode)
     line = json.loads(r.stdout.strip().splitlines()[-1])
-    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "roofline", "mixed")
+    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "roofline", "mixed", "cpu_port")
     out = {"workload": line["config"]["workload"]}
     out.update({k: line[k] for k in keep if k in line})
     out.update({k: line["config"][k] for k in ("dt_final", "finite")})
@@ -568,6 +702,11 @@ def main():
     ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--turb3d-n", type=int, default=512, help="3D turbulence line size (0: skip it)")
     ap.add_argument("--turb3d-steps", type=int, default=20)
+    ap.add_argument("--cpu3d-n", type=int, default=0,
+                    help="with a 3D workload: the C oracle as CPU baseline on the same set-up at this size")
+    ap.add_argument("--cpu3d-steps", type=int, default=1)
+    ap.add_argument("--khi-cpu-steps", type=int, default=4)
+    ap.add_argument("--no-brio-wu", dest="brio_wu", action="store_false", help="skip the 1D line")
     ap.add_argument("--mixed3d-steps", type=int, default=0,
                     help="with a 3D workload: also FP64 vs mixed over this many fixed-dt steps")
     args = ap.parse_args()
@@ -576,6 +715,8 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import ctypes as C
     if args.workload.startswith("turb3d"):
         import torch  # before the library: both then bind torch's HIP runtime
@@ -595,14 +736,31 @@ def main():
     upload_problem(s, p)
     s.run(args.warmup, flg=1, history=False)
     s.sync()
-    # timing hooks on the sweep kernels (the dominant ones)
-    sweeps = enable_sweep_prof(s, lib, p.ndim, p.r_k * args.steps)
-    s.sync()
+    # the timed region: K steps as a user runs them (graph replay on, no
+    # timing hooks), the state resident in HBM
     t0 = time.perf_counter()
     hist, dt_now = s.run(args.steps, flg=1, history=False)
     s.sync()
     el = time.perf_counter() - t0
     finite = state_finite(s, p)
+    value = zones * args.steps / el
+    _log("bench: %.3f s for %d steps -> %.4g zone-cycles/s (finite=%s)" % (el, args.steps, value, finite))
+    # a second pass of K steps, continuing the run, with the sweep kernels
+    # launched through hipExtLaunchKernel with start/stop events (each event
+    # stamped by the kernel's own dispatch): per-launch kernel times for the
+    # roofline.  Graph replay is off in this pass.
+    sweeps = enable_sweep_prof(s, lib, p.ndim, p.r_k * args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    s.run(args.steps, flg=1, history=False)
+    s.sync()
+    el_prof = time.perf_counter() - t1
+    roof = kernel_roofline(s, lib, sweeps, wname)
+    if roof is not None:
+        roof["timing"] = ("HIP events of hipExtLaunchKernel (the kernel's own start and end) on the library's stream "
+                          "over a second pass of %d steps right after the timed region (%.4f ms per step in that pass, "
+                          "host-launched)" % (args.steps, el_prof / args.steps * 1e3))
+    s._check(lib.qas_prof_enable(s.ctx, 0, 0), "qas_prof_enable")
     if args.calibrate_copy:
         out = s.download()
         hip = C.CDLL("libamdhip64.so.7")
@@ -613,10 +771,6 @@ def main():
         s.sync()
         hip.hipFree(buf)
         _log("bench: copied 8 x %d bytes device to device (counter calibration)" % (out.nbytes // 8))
-    value = zones * args.steps / el
-    _log("bench: %.3f s for %d steps -> %.4g zone-cycles/s (finite=%s)" % (el, args.steps, value, finite))
-
-    roof = kernel_roofline(s, lib, sweeps, wname)
     s.close()
 
     cpu = None
@@ -634,10 +788,17 @@ def main():
             extra["mixed"] = mixed3d_line(p, args.mixed3d_steps)
         except Exception as e:
             _log("bench: 3D mixed line failed: %s" % e)
+    if p.ndim == 3 and args.cpu3d_n > 0 and not isinstance(p.state, np.ndarray):
+        try:
+            _log("bench: 3D CPU port baseline, %d^3, %d steps" % (args.cpu3d_n, args.cpu3d_steps))
+            extra["cpu_port"] = cpu_port_3d(args.cpu3d_n, args.cpu3d_steps)
+        except Exception as e:
+            _log("bench: 3D CPU port baseline failed: %s" % e)
     if not args.no_extra and p.ndim == 2 and args.workload.startswith("turb2d") and args.turb3d_n > 0:
         try:
             _log("bench: 3D turbulence, %d^3, %d steps" % (args.turb3d_n, args.turb3d_steps))
-            extra["turb3d"] = turb3d_child(args.turb3d_n, args.turb3d_steps)
+            extra["turb3d"] = turb3d_child(args.turb3d_n, args.turb3d_steps,
+                                           cpu_n=0 if args.no_cpu else args.cpu3d_n or 256)
         except Exception as e:
             _log("bench: 3D line failed: %s" % e)
     if not args.no_extra and p.ndim == 2 and args.workload.startswith("turb2d"):
@@ -647,12 +808,32 @@ def main():
                 extra["mixed"] = mixed_line(args.mixed_n, args.mixed_steps)
             except Exception as e:
                 _log("bench: mixed line failed: %s" % e)
+            if not args.no_cpu and "mixed" in extra:
+                try:
+                    from qasmhd_amd import problems
+                    _log("bench: cpu baseline of the KH workload, %d steps" % args.khi_cpu_steps)
+                    extra["mixed"]["cpu_baseline"] = cpu_baseline(problems.kelvin_helmholtz(n=args.mixed_n),
+                                                                  args.khi_cpu_steps)
+                except Exception as e:
+                    _log("bench: KH cpu baseline failed: %s" % e)
         if args.batch_n > 0:
             try:
                 _log("bench: batched instances, %d x %d steps" % (args.batch_n, args.batch_steps))
                 extra["batch"] = batch_line(args.batch_n, args.batch_steps)
             except Exception as e:
                 _log("bench: batch line failed: %s" % e)
+            if not args.no_cpu and "batch" in extra:
+                try:
+                    _log("bench: cpu baseline of the batched workload")
+                    extra["batch"]["cpu_baseline"] = batch_cpu_reference(extra["batch"]["sizes"])
+                except Exception as e:
+                    _log("bench: batch cpu baseline failed: %s" % e)
+        if args.brio_wu:
+            try:
+                _log("bench: 1D Brio-Wu, 1024 cells to t = 0.1")
+                extra["brio_wu"] = brio_wu_line(cpu=not args.no_cpu)
+            except Exception as e:
+                _log("bench: 1D line failed: %s" % e)
         if args.dsp_nx > 0:
             try:
                 _log("bench: dissipative reconnection, %dx%d, %d steps" % (args.dsp_nx, args.dsp_ny, args.dsp_steps))
@@ -691,7 +872,36 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)
+    if not args.no_extra and p.ndim == 2:
+        line["host"] = host_info()
+        line["configs"] = config_table(line)
     print(json.dumps(line), flush=True)
+
+
+def config_table(line):
+    """GPU rate beside the CPU baseline for each of the five workloads, from
+    the lines above (None where a line did not run)."""
+    def g(d, *keys):
+        for k in keys:
+            if not isinstance(d, dict) or k not in d:
+                return None
+            d = d[k]
+        return d
+    cfg = {
+        "1_brio_wu_1d_1024": {"gpu_fp64": g(line, "brio_wu", "value"), "cpu": g(line, "brio_wu", "cpu_baseline")},
+        "2_turb2d_1024": {"gpu_fp64": line.get("value"), "cpu": line.get("cpu_baseline")},
+        "3_khi_2048_mlau": {"gpu_fp64": g(line, "mixed", "fp64_value"), "gpu_mixed": g(line, "mixed", "mixed_value"),
+                            "mixed_l1_rho": g(line, "mixed", "l1_rho"), "cpu": g(line, "mixed", "cpu_baseline")},
+        "4_turb3d_512": {"gpu_fp64": g(line, "turb3d", "value"), "gpu_mixed": g(line, "turb3d", "mixed", "mixed_value"),
+                         "mixed_l1_rho": g(line, "turb3d", "mixed", "l1_rho"), "cpu": g(line, "turb3d", "cpu_port")},
+        "5_batch_64": {"gpu_fp64": g(line, "batch", "batched_value"), "cpu": g(line, "batch", "cpu_baseline"),
+                       "adjoint_gradient_over_forward": g(line, "inverse", "ratio")},
+    }
+    for v in cfg.values():
+        c = v.get("cpu")
+        if isinstance(c, dict) and v.get("gpu_fp64") and c.get("value"):
+            v["gpu_over_cpu"] = v["gpu_fp64"] / c["value"]
+    return cfg
 
 
 if __name__ == "__main__":

@@ -22,10 +22,13 @@
  *   CFL .................. mhd1d_class.cpp:11-30, mhd2d_class.cpp:11-33,
  *                          mhd3d_class.cpp:15-46
  * Pinning: tests/test_oracle_golden.py compares every function here against
- * vectors written by the reference's own code (oracle/build_ref.py), and the
+ * vectors written by the reference's own code (oracle/build.py), and the
  * 1D/2D steps against N-step reference runs, bit for bit.  The 3D step has no
  * buildable reference (its driver needs mpi.h); it is pinned by reduction to
- * the 2D reference and by invariants (tests/test_oracle_3d.py).
+ * the 2D reference and by invariants (tests/test_oracle_props.py); its CFL
+ * (setdt of mhd3d_class.cpp:15-46) by closed forms on uniform states moving
+ * along z with a z-staggered Bz, by rotation invariance and by an
+ * independent numpy evaluation (same file).
  */
 #include "qas_oracle.h"
 #include <math.h>

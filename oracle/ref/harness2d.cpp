@@ -3,7 +3,7 @@
  * (2D/SERIAL/OTvortex/mhd2d_{solve,class}.cpp, built from source) for an
  * exact number of steps on a state read from disk (TEST INFRASTRUCTURE).
  * Grid size and solver switches are the reference's compile-time macros
- * (XMESH, YMESH, RMN, ODR, R_K, CTW, CFL), supplied by oracle/build_ref.py.
+ * (XMESH, YMESH, RMN, ODR, R_K, CTW, CFL), supplied by oracle/build.py.
  *
  * usage: harness2d <param-file>
  *   in/out      raw state: 8 SoA arrays of nx*ny doubles (ghosts included)

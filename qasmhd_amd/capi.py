@@ -328,8 +328,12 @@ class Solver:
 
     def adj_forward(self, dts):
         """Advance len(dts) steps at the given dt values, keeping checkpoints
-        for adj_backward (qas_adj_forward)."""
+        for adj_backward (qas_adj_forward).  The context holds the
+        checkpoints of its latest adj_forward only; adj_generation counts
+        the calls, so a caller that keeps a trajectory for later can tell
+        that another forward has replaced it."""
         d = np.ascontiguousarray(dts, dtype=np.float64)
+        self.adj_generation = getattr(self, "adj_generation", 0) + 1
         self._check(self.lib.qas_adj_forward(self.ctx, int(d.size), _p(d)), "qas_adj_forward")
 
     def adj_backward(self, g_final):

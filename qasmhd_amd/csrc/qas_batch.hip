@@ -348,6 +348,8 @@ extern "C" const char* qas_batch_last_error(const qas_batch* b) { return b ? b->
 
 static int build_batch_graph(qas_batch* b, int flg, bool hist)
 {
+  // wait for replays of the old graph before its executable is destroyed
+  if (b->gexec || b->graph) QAS_BCHECK(b, hipStreamSynchronize(b->stream));
   if (b->gexec) { QAS_BCHECK(b, hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
   if (b->graph) { QAS_BCHECK(b, hipGraphDestroy(b->graph)); b->graph = nullptr; }
   QAS_BCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -380,7 +382,6 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
     QAS_BCHECK(b, hipStreamSynchronize(c->stream));
   }
   const bool hist = dt_hist != nullptr;
-  bool table_changed = false;
   if (hist) {
     for (int i = 0; i < n; i++) {
       qas_ctx* c = b->c[i];
@@ -389,11 +390,14 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
         c->d_hist = nullptr;
         QAS_BCHECK(b, hipMalloc(&c->d_hist, sizeof(double) * (size_t)nsteps));
         c->hist_cap = nsteps;
-        table_changed = true;
+        // the context's own step-pair graph (qas_run) holds the old history
+        // pointer and capacity in its record node: rebuild it on next use
+        // (the batch graph reads the pointers from the device table, which
+        // fill_table below rewrites)
+        c->graph_hist = -1;
       }
     }
   }
-  (void)table_changed;
   if (!b->tuned) {
     const int rt = qas_batch_tune(b);
     if (rt) return rt;

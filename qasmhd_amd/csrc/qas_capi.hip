@@ -92,17 +92,12 @@ static std::string validate(const qas_grid* g, const qas_scheme* s)
   return "";
 }
 
-extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
+namespace qas {
+std::string validate_config(const qas_grid* g, const qas_scheme* s) { return validate(g, s); }
+
+// Geometry and boundary flags of a context from a validated grid / scheme.
+void ctx_init(qas_ctx* c, const qas_grid* g, const qas_scheme* s)
 {
-  if (!out) return QAS_E_ARG;
-  *out = nullptr;
-  qas_ctx* c = new qas_ctx();
-  std::string v = validate(g, s);
-  if (!v.empty()) {
-    c->err = v;
-    *out = c;  // return the context so qas_last_error can report
-    return QAS_E_ARG;
-  }
   c->g = *g;
   c->s = *s;
   Geo& geo = c->geo;
@@ -121,24 +116,34 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
       c->bc.dn[a][m] = (signed char)s->dn[a][m];
       c->bc.st[a][m] = (signed char)s->st[a][m];
     }
-  *out = c;
-  hipError_t e = hipGetDevice(&c->device);
-  if (e != hipSuccess) return hipfail(c, e, "hipGetDevice");
-  QAS_CHECK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  // one slab: 24 state arrays + scratch, each padded to 512 bytes
-  const long long nd = geo.nd;
+}
+
+// Device buffers of a context carved from one block: the three state
+// buffers of the RK stages (the first is `state`, 8 x nd doubles, when
+// given: the caller's own array), primitives, sweep accumulators, face
+// parts of the B fluxes, edge EMFs, and a tail of small words (dt,
+// dt-history counter, CFL partial maxima).  Arrays are padded to 512
+// bytes.  Returns the bytes the block needs; assigns the pointers when
+// base is not null.
+size_t ctx_layout(qas_ctx* c, char* base, double* state)
+{
+  const int ndim = c->geo.ndim;
+  const long long nd = c->geo.nd;
   const size_t stride = (size_t)((nd + 63) / 64 * 64);
-  const int nprim = g->ndim == 3 ? 7 : (g->ndim == 2 ? 6 : 4);
-  const int nacc = g->ndim == 1 ? 0 : (g->ndim == 2 ? 6 : 10);
-  const int nemf = g->ndim == 3 ? 12 : (g->ndim == 2 ? 8 : 0);
-  const int nedge = g->ndim == 3 ? 3 : (g->ndim == 2 ? 1 : 0);
-  const size_t narr = 24 + nprim + nacc + nemf + nedge;
-  c->slab_bytes = narr * stride * sizeof(double) + 4096 + sizeof(double) * CFL_MAX_BLOCKS;
-  e = hipMalloc(&c->slab, c->slab_bytes);
-  if (e != hipSuccess) return fail(c, QAS_E_NOMEM, std::string("hipMalloc of the domain failed: ") + hipGetErrorName(e));
-  double* p = (double*)c->slab;
+  const int nprim = ndim == 3 ? 7 : (ndim == 2 ? 6 : 4);
+  const int nacc = ndim == 1 ? 0 : (ndim == 2 ? 6 : 10);
+  const int nemf = ndim == 3 ? 12 : (ndim == 2 ? 8 : 0);
+  const int nedge = ndim == 3 ? 3 : (ndim == 2 ? 1 : 0);
+  const size_t narr = (state ? 16 : 24) + nprim + nacc + nemf + nedge;
+  const size_t bytes = narr * stride * sizeof(double) + 4096 + sizeof(double) * CFL_MAX_BLOCKS;
+  if (!base) return bytes;
+  double* p = (double*)base;
   for (int b = 0; b < 3; b++)
     for (int m = 0; m < 8; m++) {
+      if (b == 0 && state) {
+        c->U[0][m] = state + (size_t)m * nd;
+        continue;
+      }
       c->U[b][m] = p;
       p += stride;
     }
@@ -146,17 +151,40 @@ extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
   c->vy = p; p += stride;
   c->vz = p; p += stride;
   c->pr = p; p += stride;
-  if (g->ndim >= 2) { c->cx = p; p += stride; c->cy = p; p += stride; }
-  if (g->ndim == 3) { c->cz = p; p += stride; }
+  if (ndim >= 2) { c->cx = p; p += stride; c->cy = p; p += stride; }
+  if (ndim == 3) { c->cz = p; p += stride; }
   for (int k = 0; k < nacc; k++) { c->acc[k] = p; p += stride; }
   for (int k = 0; k < nemf; k++) { c->emf[k] = p; p += stride; }
-  if (g->ndim == 2) { c->ez = p; p += stride; }
-  if (g->ndim == 3) { c->ex = p; p += stride; c->ey = p; p += stride; c->ez = p; p += stride; }
-  // small device words: dt, history counter; CFL partial maxima
+  if (ndim == 2) { c->ez = p; p += stride; }
+  if (ndim == 3) { c->ex = p; p += stride; c->ey = p; p += stride; c->ez = p; p += stride; }
   char* tail = (char*)p;
   c->d_dt = (double*)tail;
-  c->d_cflpart = (double*)(tail + 4096);  // per-block maxima of the CFL reduction
   c->d_hcount = (long long*)(tail + 128);
+  c->d_cflpart = (double*)(tail + 4096);  // per-block maxima of the CFL reduction
+  return bytes;
+}
+}  // namespace qas
+
+extern "C" int qas_create(qas_ctx** out, const qas_grid* g, const qas_scheme* s)
+{
+  if (!out) return QAS_E_ARG;
+  *out = nullptr;
+  qas_ctx* c = new qas_ctx();
+  std::string v = validate(g, s);
+  if (!v.empty()) {
+    c->err = v;
+    *out = c;  // return the context so qas_last_error can report
+    return QAS_E_ARG;
+  }
+  ctx_init(c, g, s);
+  *out = c;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return hipfail(c, e, "hipGetDevice");
+  QAS_CHECK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->slab_bytes = ctx_layout(c, nullptr, nullptr);
+  e = hipMalloc(&c->slab, c->slab_bytes);
+  if (e != hipSuccess) return fail(c, QAS_E_NOMEM, std::string("hipMalloc of the domain failed: ") + hipGetErrorName(e));
+  ctx_layout(c, (char*)c->slab, nullptr);
   QAS_CHECK(c, hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
   QAS_CHECK(c, hipHostMalloc((void**)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
   QAS_CHECK(c, hipStreamSynchronize(c->stream));
@@ -274,7 +302,7 @@ static void rk_coef(int r_k, int rk, double* a, double* b)
 }
 
 // ideal(dt) with dt already in device memory
-static int do_step(qas_ctx* c)
+int qas::ctx_step(qas_ctx* c)
 {
   const int R = c->s.r_k;
   const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
@@ -318,7 +346,7 @@ extern "C" int qas_step(qas_ctx* c, double dt)
   if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_step: no state uploaded");
   if (!(dt > 0.0)) return fail(c, QAS_E_ARG, "qas_step: dt must be positive");
   QAS_CHECK(c, launch_set_dt(c, dt));
-  return do_step(c);
+  return ctx_step(c);
 }
 
 extern "C" int qas_cfl_dt(qas_ctx* c, double* dt_out)
@@ -342,7 +370,7 @@ static int loop_step(qas_ctx* c, int flg, bool hist, bool even)
   // state since the last one -- not with gravity, whose setdt rewrites the
   // interior energy (so a captured step pair always carries it)
   if (!c->bounded || c->d_phi) QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
-  int r = do_step(c);
+  int r = ctx_step(c);
   if (r) return r;
   // DMHD2D::exec_: the sub-cycled dissipative step after ideal(dt)
   r = dsp_after_ideal(c);
@@ -359,6 +387,9 @@ static int loop_step(qas_ctx* c, int flg, bool hist, bool even)
 // buffer after R_K = 2 or 3 stages, so one graph serves every pair.
 static int build_pair_graph(qas_ctx* c, int flg, bool hist)
 {
+  // an earlier replay of the old graph may still run on the stream: wait
+  // for it before its executable is destroyed
+  if (c->gexec || c->graph) QAS_CHECK(c, hipStreamSynchronize(c->stream));
   if (c->gexec) { QAS_CHECK(c, hipGraphExecDestroy(c->gexec)); c->gexec = nullptr; }
   if (c->graph) { QAS_CHECK(c, hipGraphDestroy(c->graph)); c->graph = nullptr; }
   const int cur0 = c->cur;

@@ -1,6 +1,7 @@
 // qas_internal.hpp -- device context and kernel-launch interface shared by
 // the HIP translation units of libqasmhd_amd.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
@@ -184,27 +185,30 @@ struct qas_ctx {
 };
 
 namespace qas {
-inline void prof_begin(qas_ctx* c, int kid)
+// The event pair of the next timed launch of kernel id kid, or false when
+// that kernel is not timed (qas_prof_enable).  The events are passed to
+// hipExtLaunchKernel, which stamps them with the start and end of the
+// kernel's own dispatch (not with markers enqueued around it).
+inline bool prof_slot(qas_ctx* c, int kid, hipEvent_t* e0, hipEvent_t* e1)
 {
-  if ((c->prof_mask >> kid) & 1u)
-    if (c->prof_n < c->prof_kid.size()) (void)hipEventRecord(c->prof_ev[2 * c->prof_n], c->stream);
-}
-inline void prof_end(qas_ctx* c, int kid)
-{
-  if ((c->prof_mask >> kid) & 1u)
-    if (c->prof_n < c->prof_kid.size()) {
-      (void)hipEventRecord(c->prof_ev[2 * c->prof_n + 1], c->stream);
-      c->prof_kid[c->prof_n++] = kid;
-    }
+  if (!((c->prof_mask >> kid) & 1u) || c->prof_n >= c->prof_kid.size()) return false;
+  *e0 = c->prof_ev[2 * c->prof_n];
+  *e1 = c->prof_ev[2 * c->prof_n + 1];
+  c->prof_kid[c->prof_n++] = kid;
+  return true;
 }
 }  // namespace qas
 
-// launch a kernel between the timing hooks of kernel id KID
-#define QAS_LAUNCH(c, KID, ...)  \
-  do {                            \
-    qas::prof_begin((c), (KID));  \
-    hipLaunchKernelGGL(__VA_ARGS__); \
-    qas::prof_end((c), (KID));    \
+// launch KERNEL; a kernel id the timing hooks select goes through
+// hipExtLaunchKernelGGL with the next event pair
+#define QAS_LAUNCH(c, KID, KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)                                      \
+  do {                                                                                                   \
+    hipEvent_t qas_e0_, qas_e1_;                                                                         \
+    if (qas::prof_slot((c), (KID), &qas_e0_, &qas_e1_))                                                 \
+      hipExtLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), (SHMEM), (STREAM), qas_e0_, qas_e1_, 0u,     \
+                            __VA_ARGS__);                                                                \
+    else                                                                                                 \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHMEM, STREAM, __VA_ARGS__);                               \
   } while (0)
 
 // ---- launchers (defined in the per-dimension translation units) ----
@@ -221,6 +225,13 @@ hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
 hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only = false);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
 void adj_release(qas_ctx* c);  // frees the adjoint workspace (qas_adjoint2d.hip)
+// context set-up shared by qas_create and the allocation-free step
+// (qas_capi.hip): configuration check ("" when valid), geometry, buffer
+// layout over one device block, and ideal(dt) with dt in device memory
+std::string validate_config(const qas_grid* g, const qas_scheme* s);
+void ctx_init(qas_ctx* c, const qas_grid* g, const qas_scheme* s);
+size_t ctx_layout(qas_ctx* c, char* base, double* state);
+int ctx_step(qas_ctx* c);
 // dissipative step (qas_dissipative.hip)
 void dsp_release(qas_ctx* c);
 // gravity (qas_gravity.hip)

@@ -44,11 +44,17 @@ class IdealSteps(torch.autograd.Function):
         solver.download_device(out.data_ptr())
         solver.sync()
         ctx.solver = solver
+        ctx.generation = solver.adj_generation
         return out
 
     @staticmethod
     def backward(ctx, g_out):
         s = ctx.solver
+        if getattr(s, "adj_generation", None) != ctx.generation:
+            # the solver keeps the checkpoints of its latest forward only:
+            # a later ideal_steps (or adj_forward) on it replaced this node's
+            raise capi.QasError("ideal_steps backward: the solver ran another forward since this node's; its "
+                                "checkpoints are gone (use one solver per live ideal_steps node)")
         g = g_out.detach().contiguous()
         g_in = torch.empty_like(g)
         torch.cuda.synchronize(g.device)
@@ -59,5 +65,7 @@ class IdealSteps(torch.autograd.Function):
 
 def ideal_steps(u0: torch.Tensor, solver, dts) -> torch.Tensor:
     """len(dts) steps of ideal(dt) from u0 (ghosts included; bound() is
-    applied first), differentiable with respect to u0."""
+    applied first), differentiable with respect to u0.  The solver holds
+    the checkpoints of one forward: a second ideal_steps on the same solver
+    before the first node's backward makes that backward raise."""
     return IdealSteps.apply(u0, solver, dts)

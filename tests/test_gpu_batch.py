@@ -103,3 +103,33 @@ def test_batch_rejects_mixed_schemes():
         capi.Batch([a, b])
     a.close()
     b.close()
+
+
+def test_batch_history_growth_then_single_run():
+    """A batch run with history over more steps than a context's history
+    buffer reallocates that buffer; the context's own step-pair graph
+    (captured by an earlier qas_run with history) must not keep writing the
+    freed one.  s.run(4) -> batch.run(8) -> s.run(4): the last history and
+    state equal those of a fresh context run over the same 16 steps."""
+    from qasmhd_amd import capi
+    print("case: batch history growth", flush=True)
+    ps = _problems()[:2]
+    solvers = [capi.solver_for(p) for p in ps]
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+        s.run(4, flg=1)
+    with capi.Batch(solvers) as b:
+        b.run(8, flg=1)
+    last = [s.run(4, flg=1) for s in solvers]
+    outs = [s.download() for s in solvers]
+    for s in solvers:
+        s.close()
+    for i, p in enumerate(ps):
+        with capi.solver_for(p) as s:
+            s.upload(p.state)
+            s.run(12, flg=1)
+            h, dt = s.run(4, flg=1)
+            ref = s.download()
+        assert np.array_equal(last[i][0], h), i
+        assert last[i][1] == dt, i
+        assert np.array_equal(outs[i], ref), i

@@ -205,3 +205,106 @@ def test_2d_static_zero_fields_stay_exactly_zero(oracle):
     cfg = oracle.make_cfg(st.shape[1:], p.dx, p.dy, gam=p.gam)
     oracle.run(cfg, st, 3, 1)
     assert np.all(st[3] == 0.0) and np.all(st[4:7] == 0.0)
+
+
+# ---------------------------------------------------------------- 3D setdt
+def _uniform3d(n, ro, v, b_faces_z, pr, gam, bxy=(0.0, 0.0)):
+    """(8, nz, ny, nx) state: uniform ro, v, pressure pr and face Bx, By;
+    face Bz given per z plane (b_faces_z[k]); the energy uses the
+    cell-centred Bz = (bz_k + bz_{k+1}) / 2 of the staggered field."""
+    nd = n + 8
+    st = np.zeros((8, nd, nd, nd))
+    st[0] = ro
+    st[1], st[2], st[3] = ro * v[0], ro * v[1], ro * v[2]
+    st[4], st[5] = bxy
+    st[6] = np.asarray(b_faces_z, dtype=float)[:, None, None]
+    czc = 0.5 * (st[6] + np.roll(st[6], -1, axis=0))
+    st[7] = pr / (gam - 1) + 0.5 * ro * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2) + 0.5 * (bxy[0] ** 2 + bxy[1] ** 2 + czc ** 2)
+    return st
+
+
+def test_3d_cfl_closed_form_vz_only(oracle):
+    """setdt of 3D (mhd3d_class.cpp:15-46) on a uniform state moving along z
+    only: dt = cfl dr / (|vz| + sqrt((gam p + B^2) / ro)).  A 2D speed
+    sqrt(vx^2 + vy^2) would give cfl dr / vfast instead."""
+    gam, ro, pr, w, b = 5.0 / 3.0, 1.7, 0.9, 2.5, 0.6
+    n = 8
+    st = _uniform3d(n, ro, (0.0, 0.0, w), np.full(n + 8, b), pr, gam)
+    d = 0.05
+    cfg = oracle.make_cfg(st.shape[1:], d, d, d, gam=gam, cfl=0.4)
+    exact = 0.4 * d / (w + np.sqrt((gam * pr + b * b) / ro))
+    got = oracle.cfl_dt(cfg, st.copy())
+    assert abs(got - exact) < 4e-16 * exact
+    no_vz = 0.4 * d / np.sqrt((gam * pr + b * b) / ro)
+    assert abs(got - no_vz) > 0.5 * exact
+
+
+def test_3d_cfl_cell_centred_bz(oracle):
+    """z-staggered Bz alternating b0 +- db between faces: the cell-centred
+    value setdt uses is (bz_k + bz_{k+1}) / 2 = b0 exactly, so dt is the
+    closed form with B = b0.  Taking the face value bz_k instead changes
+    both the recovered pressure and the fast speed."""
+    gam, ro, pr, b0, db = 5.0 / 3.0, 1.2, 0.5, 0.75, 0.5
+    n = 8
+    faces = b0 + db * (-1.0) ** np.arange(n + 8)
+    st = _uniform3d(n, ro, (0.0, 0.0, -0.3), faces, pr, gam)
+    d = 0.1
+    cfg = oracle.make_cfg(st.shape[1:], d, d, d, gam=gam, cfl=0.4)
+    exact = 0.4 * d / (0.3 + np.sqrt((gam * pr + b0 * b0) / ro))
+    got = oracle.cfl_dt(cfg, st.copy())
+    assert abs(got - exact) < 4e-16 * exact
+    # with the face value the pressure of the cell at an even face would be
+    # pr - (gam - 1)/2 ((b0+db)^2 - b0^2): dt then differs at the 1e-2 level
+    bf = b0 + db
+    pr_face = pr - (gam - 1) * 0.5 * (bf * bf - b0 * b0)
+    wrong = 0.4 * d / (0.3 + np.sqrt((gam * pr_face + bf * bf) / ro))
+    assert abs(got - wrong) > 1e-3 * exact
+
+
+def test_3d_cfl_dr_is_min_spacing(oracle):
+    """dr = min(dx, dy, dz) (mhd3d_class.cpp:29); floor vmax >= 1."""
+    gam, ro, pr = 5.0 / 3.0, 1.0, 0.01
+    n = 8
+    st = _uniform3d(n, ro, (0.0, 0.0, 0.0), np.zeros(n + 8), pr, gam)
+    cfg = oracle.make_cfg(st.shape[1:], 0.3, 0.2, 0.07, gam=gam, cfl=0.4)
+    # vfast = sqrt(gam p / ro) ~ 0.13 < 1: the floor applies
+    assert oracle.cfl_dt(cfg, st.copy()) == 0.4 * 0.07 / 1.0
+
+
+@pytest.mark.parametrize("seed", [3, 11])
+def test_3d_cfl_rotation_invariant(oracle, seed):
+    """setdt is a max over cells of |v| + vfast with the cell-centred B: it
+    is invariant under the cyclic relabelling x->y->z on a cubic grid.  A
+    speed missing vz, or a Bz not averaged over its two faces, breaks it."""
+    from qasmhd_amd import problems
+    p = problems.turbulence3d(n=12, seed=seed, nmodes=10)
+    a = p.state.copy()
+    cfg = _cfg3(oracle, a, p.dx, gam=p.gam)
+    base = oracle.cfl_dt(cfg, a.copy())
+    r1 = _rotate(a)
+    r2 = _rotate(r1)
+    for r in (r1, r2):
+        assert abs(oracle.cfl_dt(cfg, r.copy()) - base) <= 1e-15 * base
+
+
+def test_3d_cfl_against_numpy(oracle):
+    """setdt on a random smooth 3D state against an independent numpy
+    evaluation of max(1, max_cells |v| + vfast) with the 2nd-order
+    cell-centred B of the three staggered components."""
+    from qasmhd_amd import problems
+    p = problems.turbulence3d(n=12, seed=4, nmodes=10)
+    st = p.state
+    g = p.gam
+    I = (slice(4, -4),) * 3
+    ro = st[0]
+    v = st[1:4] / ro
+    cx = 0.5 * (st[4] + np.roll(st[4], -1, axis=2))
+    cy = 0.5 * (st[5] + np.roll(st[5], -1, axis=1))
+    cz = 0.5 * (st[6] + np.roll(st[6], -1, axis=0))
+    b2 = cx ** 2 + cy ** 2 + cz ** 2
+    pr = (g - 1) * (st[7] - 0.5 * (ro * (v ** 2).sum(axis=0) + b2))
+    speed = np.sqrt((v ** 2).sum(axis=0)) + np.sqrt((g * pr + b2) / ro)
+    vmax = max(1.0, float(speed[I].max()))
+    cfg = _cfg3(oracle, st, p.dx, gam=g)
+    exact = 0.4 * p.dx / vmax
+    assert abs(oracle.cfl_dt(cfg, st.copy()) - exact) < 1e-14 * exact
