@@ -119,6 +119,29 @@ int qas_sync(qas_ctx *c);
 /* Device memory held by the context (bytes). */
 long long qas_device_bytes(const qas_ctx *c);
 
+/* ---- allocation-free run (no counterpart in the reference) -------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Runs what
+ * qas_run runs with graphs off -- the bound() before the first step, then
+ * nsteps x {ideal(dt); if flg and n even: dt = setdt(1)} with n = nstep0 + 1
+ * .. nstep0 + nsteps -- on a state the caller keeps in device memory:
+ * d_state = 8 * nx*ny*nz doubles (variable-major val[] order, ghosts
+ * included), read and updated in place.  The RK stage buffers and every
+ * scratch array come from one static device array of
+ * QAS_STATIC_SCRATCH_BYTES per GPU; a domain needing more is rejected with
+ * QAS_E_ARG (roughly 2D up to 660^2 and 3D up to 60^3 interior cells).
+ * dt > 0 is the first step's dt, dt <= 0 takes setdt(1) of the state.
+ * d_dt_out (device pointer, may be NULL) receives the dt after the last
+ * step.  stream: a hipStream_t, NULL for the null stream; the call only
+ * enqueues (no synchronisation, no allocation, no hipMemset).  Messages go
+ * to err (errlen bytes, may be NULL).  FP64 or mixed; no gravity, no
+ * dissipation, no dt history.  All calls on one GPU share its array, whose
+ * used part each call clears first: calls that overlap in time on one GPU
+ * race, so order them (one stream). */
+#define QAS_STATIC_SCRATCH_BYTES (128ull << 20)
+int qas_run_static(const qas_grid *g, const qas_scheme *s, double *d_state, int nsteps, long long nstep0, int flg,
+                   double dt, double *d_dt_out, void *stream, char *err, int errlen);
+
 /* ---- dissipative step (2D, FP64) ---------------------------------------
  * qas_set_dissipation replaces the arrays DMHD2D keeps and DMHD2D::setdc
  * fills (common/mhd_class.hpp:38 nu, eta; 2D/SERIAL/MRX/dmhd2d_init_.cpp:86-103):

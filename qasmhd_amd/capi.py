@@ -41,7 +41,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_batch_tune", "qas_batch_set_shape", "qas_batch_shape",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct", "qas_test_rcp",
-           "qas_set_dissipation", "qas_dsptv", "qas_set_gravity"]
+           "qas_set_dissipation", "qas_dsptv", "qas_set_gravity", "qas_run_static"]
 
 _lib = None
 
@@ -104,6 +104,8 @@ def load(path: str | None = None):
     l.qas_set_dissipation.argtypes = [vp, dp, dp, C.c_double]
     l.qas_dsptv.argtypes = [vp, C.c_double]
     l.qas_set_gravity.argtypes = [vp, dp]
+    l.qas_run_static.argtypes = [C.POINTER(QasGrid), C.POINTER(QasScheme), vp, C.c_int, C.c_longlong, C.c_int,
+                                 C.c_double, vp, vp, C.c_char_p, C.c_int]
     _lib = l
     return l
 
