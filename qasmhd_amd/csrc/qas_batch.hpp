@@ -12,7 +12,7 @@
 namespace qas {
 
 // kernel types with their own block-offset table
-enum BatchKernel { BK_PRIM = 0, BK_SX, BK_SY, BK_EMF, BK_CT, BK_BOUND, BK_CFL, BK_COUNT };
+enum BatchKernel { BK_SX = 0, BK_SY, BK_EMFCT, BK_BOUND, BK_CFL, BK_COUNT };
 
 struct BatchInst {
   StagePtrs st[3];   // stage descriptors of one step (buffer rotation, rk_fac, dt)

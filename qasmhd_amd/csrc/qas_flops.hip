@@ -186,14 +186,33 @@ extern "C" int qas_kernel_model(const qas_ctx* c, int kid, double* flops, double
     // 2*ntan face EMF parts (write), accumulators / state
     double b = cells * (9.0 + 2.0 * ntan);
     if (nd3 == 1) b = cells * 8.0;
+    if (nd3 == 2 && d == 1) {
+      // the 2D y sweep computes the primitives of every cell itself from the
+      // 8 state arrays and stores 6 of them for the x sweep
+      *flops += cells * (1 + 3 + 5 + 5 + 4 + 2 * pc.fc);
+      b = cells * (8.0 + 6.0 + 2.0 * ntan);
+    }
     if (d == 0) b += inter * nupd * ((nd3 - 1) + 3.0);  // stored divergences + in, u0 read; out write
     else b += inter * nupd * 1.0;                        // this sweep's divergence written
     *bytes = b * 8.0;
     return QAS_OK;
   }
   if (kid == QAS_K_PRIM) {
+    if (nd3 == 2) return QAS_OK;  // folded into the y sweep
     *flops = cells * (1 + 3 + 5 + 5 + 4 + (nd3 >= 2 ? nd3 * (pc.fc + 0) : 0));
     *bytes = cells * 8.0 * (8 + (nd3 >= 2 ? nd3 : 0) + 4);
+    return QAS_OK;
+  }
+  if (kid == QAS_K_EMF && nd3 == 2) return QAS_OK;  // folded into the CT kernel
+  if (kid == QAS_K_CT && nd3 == 2) {
+    // corner EMFs and the bx, by update in one tiled kernel: face parts (4)
+    // and, with CUCT, ro mx my read; bx by, their step base read, written
+    const double ne = (n[0] - 5) * (n[1] - 5);
+    const double nf = (n[0] - 7) * (n[1] - 7);
+    const double part = 4 * pc.lin + 2 + 1 + 2 + 1;
+    *flops = ne * (2 * part + 2 + (c->s.ctw ? count_cuct() : 0)) + (c->s.ctw ? cells * 3.0 : 0.0) +
+             nf * 2 * ((pc.df + 1) + 1 + 4);
+    *bytes = cells * 8.0 * (4 + (c->s.ctw ? 3 : 0) + 2 + 2 + 2);
     return QAS_OK;
   }
   if (kid == QAS_K_EMF) {

@@ -1,18 +1,19 @@
 // qas_kernels2d.hip -- one SSP-RK stage of the 2D ideal-MHD step on gfx950.
 //
 // Replaces the stage body of MHD2D::ideal (2D/SERIAL/OTvortex/
-// mhd2d_solve.cpp:44-386).  A stage is five launches:
-//   prim2d   cell-centred B (fcen, ghosts by the bound() index map) and
-//            primitives at every cell                       (:51-89)
-//   sweep<Y> characteristic WCNS + Riemann flux per y face, for every
-//            column (LDS-staged); writes dt/dy times the y flux divergence
-//            of the interior and the upwind/central parts of the Bx flux
+// mhd2d_solve.cpp:44-386).  A stage is three launches:
+//   sweep<Y> cell-centred B and primitives of its LDS tile from the state
+//            (stored for the x sweep)                        (:51-89)
+//            then characteristic WCNS + Riemann flux per y face, for every
+//            column; writes dt/dy times the y flux divergence of the
+//            interior and the upwind/central parts of the Bx flux
 //                                                           (:234-306)
 //   sweep<X> same along x, for every row; applies the RK stage sum
 //            a u0 + b (u + ((-dt/dx Dx) - dt/dy Dy)) to ro, mx, my, mz, bz,
 //            en (the reference's expression)               (:127-199, 341-358)
-//   emf2d    CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
-//   ct2d     staggered bx, by update                        (:359-380)
+//   emfct2d  CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
+//            and the staggered bx, by update from it, per 64 x 16 face
+//            tile with the EMF in LDS                       (:359-380)
 // followed by the bound() kernel (:384-385).
 // A sweep assigns one 64-lane wavefront to 64 consecutive cells of a line;
 // lane l reconstructs cell f0-2+l, neighbour states and fluxes move between
@@ -30,22 +31,10 @@ namespace qas {
 // bodies in qas_stage2d.hpp (shared with the batched kernels).  Blocks are
 // renumbered XCD-aware (xcd_remap): the stencils of neighbouring rows then
 // meet in the same L2.
-template <int ODR, typename PT>
-__global__ __launch_bounds__(256) void k_prim2d(StagePtrs p, Geo g, BcFlags bc, double gam)
-{
-  prim2d_at<ODR, PT>(p, g, bc, gam, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
-}
-
 template <int ODR, typename FT>
-__global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
+__global__ __launch_bounds__(256) void k_emfct2d(StagePtrs p, Geo g, int ctw)
 {
-  emf2d_at<ODR, FT>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
-}
-
-template <int ODR>
-__global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
-{
-  ct2d_at<ODR>(p, g, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
+  emfct2d_tile<ODR, FT>(p, g, ctw, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // ------------------------------------------------------------------ launch
@@ -57,22 +46,15 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
   const int odr = c->s.odr;
   const double gam = c->s.gam;
   hipStream_t st = c->stream;
-  // 1) cell-centred B and primitives
   const bool f32 = c->s.precision == 1;
-  if (odr <= 2) {
-    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<2, float>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<2, double>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-  } else {
-    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4, float>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim2d<4, double>), nblk(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-  }
-  // 2) y sweep, 3) x sweep
+  // 1) y sweep (with the primitives), 2) x sweep
   const int nsx = (g.nx - 5 + SEG - 1) / SEG;
   const int nsy = (g.ny - 5 + SEG - 1) / SEG;
   const bool mixed = c->s.precision == 1;
   xsweep2d_fn fx = mixed ? pick_xsweep2d_f32(c->s.rmn, odr, false) : pick_xsweep2d(c->s.rmn, odr, false);
   xsweep2d_fn fy = mixed ? pick_ysweep2d_f32(c->s.rmn, odr, false) : pick_ysweep2d(c->s.rmn, odr, false);
-  // with gravity the sweeps take the round-tripped stage energy
+  // with gravity the x sweep takes the round-tripped stage energy, which
+  // the y sweep computes from the state's own energy and stores
   StagePtrs ps = p;
   if (p.phi) {
     fx = pick_grav_xsweep(c->s.rmn, odr);
@@ -81,27 +63,25 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
   }
   if (!fx || !fy) return hipErrorInvalidValue;
   // y sweep first (one 256-thread workgroup per (4-column group, segment)):
-  // it leaves dt/dy * Dy in the accumulators; the x sweep applies the stage sum
-  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, ps, g, c->bc, gam, nsy,
+  // it stores the primitives and leaves dt/dy * Dy in the accumulators; the
+  // x sweep applies the stage sum
+  QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, p, g, c->bc, gam, nsy,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   if (faces_only) return hipGetLastError();
-  // 4) corner EMF, 5) CT
-  const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
-  const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
-  // (the face parts are stored in the sweep precision)
-#define QAS_EMF_CT(O)                                                                                      \
-  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);      \
-  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);          \
-  QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<O>), nblk(nc, 256), 256, 0, st, p, g);
+  // 3) corner EMF + CT, tiled (the face parts are stored in the sweep precision)
+  const unsigned nt = (unsigned)emfct2d_tiles(g.nx, g.ny);
+#define QAS_EMFCT(O)                                                                                   \
+  if (f32) QAS_LAUNCH(c, QAS_K_CT, (k_emfct2d<O, float>), nt, 256, 0, st, p, g, c->s.ctw);          \
+  else QAS_LAUNCH(c, QAS_K_CT, (k_emfct2d<O, double>), nt, 256, 0, st, p, g, c->s.ctw);
   switch (odr) {
-    case 1: QAS_EMF_CT(1) break;
-    case 2: QAS_EMF_CT(2) break;
-    case 3: QAS_EMF_CT(3) break;
-    default: QAS_EMF_CT(4) break;
+    case 1: QAS_EMFCT(1) break;
+    case 2: QAS_EMFCT(2) break;
+    case 3: QAS_EMFCT(3) break;
+    default: QAS_EMFCT(4) break;
   }
-#undef QAS_EMF_CT
+#undef QAS_EMFCT
   return hipGetLastError();
 }
 

@@ -10,14 +10,26 @@
 #include "qas_batch.hpp"
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
+#include "qas_stage2d.hpp"
 
 namespace qas {
 
+// velocity component m (1/ro) of a cell from the state, rounded to T as
+// the stored primitive is
+template <typename T>
+struct VelOf {
+  const double* m;
+  const double* ro;
+  __device__ __forceinline__ T operator[](long long q) const { return (T)(m[q] * (1.0 / ro[q])); }
+};
+
 // shock sensor dv of mhd2d_solve.cpp:105-125 at (i, j) after its bound():
-// DIR 0 -> dvx (flags -dnxs[1], dnys[1]); DIR 1 -> dvy (dnxs[2], -dnys[2])
-template <int DIR, typename VT>
-__device__ __forceinline__ double dv_at(const VT* v, int i, int j, const Geo& g, const BcFlags& bc)
+// DIR 0 -> dvx (flags -dnxs[1], dnys[1]); DIR 1 -> dvy (dnxs[2], -dnys[2]).
+// v: the velocity array, or a VelOf<VT> reading it from the state.
+template <int DIR, typename V>
+__device__ __forceinline__ double dv_at(const V& v, int i, int j, const Geo& g, const BcFlags& bc)
 {
+  using VT = typename std::remove_cv<typename std::remove_reference<decltype(v[0])>::type>::type;
   const int dnx = DIR == 0 ? -bc.dn[0][1] : bc.dn[0][2];
   const int dny = DIR == 0 ? bc.dn[1][1] : -bc.dn[1][2];
   const Src sx = resolve(i, g.nx, 0, dnx);
@@ -78,24 +90,45 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : (RMN != 1 ? 2 : 1)) void 
   const int f0 = 3 + seg * SEG;
   const int n = g.ny;
   const long long sy = g.sy;
-  const PrimView<T> pv = prim_view<T>(p);
   {
+    // Staging computes the primitives of the tile's cells from the stage
+    // input state (prim2d_cell, FP64) and rounds them once to the sweep
+    // precision T.  The tile's own cells -- its columns, and the rows of
+    // its segment (the first segment from row 0, the last to row n-1) --
+    // are also stored for the x sweep, so every cell of the domain is
+    // written by exactly one workgroup.
+    const int lo = seg == 0 ? 0 : f0;
+    const int hi = seg == nseg - 1 ? n : f0 + SEG;
+    // a tile of interior cells only skips the ghost-index map (uniform)
+    const bool inner = (i0 >= NG) && (i0 + YC - 1 <= g.nx - NG - 1) && (f0 - 4 >= NG) && (f0 + YR - 5 <= n - NG - 1);
     for (int e = threadIdx.x; e < YC * YR; e += blockDim.x) {
       const int c = e % YC, r = e / YC;
-      int row = f0 - 4 + r;
-      row = row < 0 ? 0 : (row > n - 1 ? n - 1 : row);
-      int col = i0 + c;
-      col = col > g.nx - 1 ? g.nx - 1 : col;
+      const int rowu = f0 - 4 + r;
+      const int row = rowu < 0 ? 0 : (rowu > n - 1 ? n - 1 : rowu);
+      const int colu = i0 + c;
+      const int col = colu > g.nx - 1 ? g.nx - 1 : colu;
       const long long a = sy * row + col;
+      const Prim2d q = inner ? prim2d_cell<ODR, GRAV, true>(p, g, bc, gam, col, row, a)
+                             : prim2d_cell<ODR, GRAV, false>(p, g, bc, gam, col, row, a);
+      const T vx = (T)q.vx, vy = (T)q.vy, vz = (T)q.vz, pr = (T)q.pr, cx = (T)q.cx, cy = (T)q.cy;
       sh[0][c][r] = p.in[0][a];
-      sh[1][c][r] = (double)pv.vy[a];
-      sh[2][c][r] = (double)pv.vz[a];
-      sh[3][c][r] = (double)pv.vx[a];
+      sh[1][c][r] = (double)vy;
+      sh[2][c][r] = (double)vz;
+      sh[3][c][r] = (double)vx;
       sh[4][c][r] = p.in[6][a];
-      sh[5][c][r] = (double)pv.cx[a];
-      sh[6][c][r] = (double)pv.pr[a];
-      sh[7][c][r] = (double)pv.cy[a];
+      sh[5][c][r] = (double)cx;
+      sh[6][c][r] = (double)pr;
+      sh[7][c][r] = (double)cy;
       sh[8][c][r] = p.in[5][a];
+      if (colu < g.nx && rowu >= lo && rowu < hi) {
+        reinterpret_cast<T*>(p.vx)[a] = vx;
+        reinterpret_cast<T*>(p.vy)[a] = vy;
+        reinterpret_cast<T*>(p.vz)[a] = vz;
+        reinterpret_cast<T*>(p.pr)[a] = pr;
+        reinterpret_cast<T*>(p.cx)[a] = cx;
+        reinterpret_cast<T*>(p.cy)[a] = cy;
+        if constexpr (GRAV) p.enrt[a] = q.enrt;
+      }
     }
   }
   __syncthreads();
@@ -143,9 +176,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : (RMN != 1 ? 2 : 1)) void 
   const T bnf = (T)(0.5 * (sh[8][w][rf] + sh[8][w][rf]));
   T dv0 = T(0.0), dv1 = T(0.0);
   if constexpr (RMN >= 2) {
+    // the transverse sensor needs vx of the neighbouring columns, which
+    // other workgroups of this launch store: taken from the state instead,
+    // with the primitive's own expression and rounding
     const int lc = line > g.nx - 1 ? g.nx - 1 : line;
     dv0 = (T)(sh[1][w][rf] - sh[1][w][rf - 1]);
-    dv1 = (T)tmin(dv_at<0>(pv.vx, lc, sf - 1, g, bc), dv_at<0>(pv.vx, lc, sf, g, bc));
+    const VelOf<T> vxs{p.in[1], p.in[0]};
+    dv1 = (T)tmin(dv_at<0>(vxs, lc, sf - 1, g, bc), dv_at<0>(vxs, lc, sf, g, bc));
   }
   riemann<RMN>(Lf, Rf, bnf, (T)gam, dv0, dv1, F);
   const T fc = T(0.5) * (qlf + qr);
