@@ -306,6 +306,45 @@ def mixed_line(n, steps):
             "l1_p": l1p, "finite": bool(np.isfinite(b).all())}
 
 
+def fma64_line(p, wname, warmup, steps):
+    """FP64 with fused multiply-adds in the sweeps (precision 2) against the
+    bit-exact FP64 path on the same workload, both timed as the headline is
+    (graph replay, setdt schedule, `steps` steps after `warmup`), best of two
+    alternating trials; relative L1 distance of every variable over the
+    interior after warmup + steps steps and the final dt."""
+    import numpy as np
+    from qasmhd_amd import capi
+    zones = int(np.prod(p.interior))
+    rate, out, dts = {0: 0.0, 2: 0.0}, {}, {}
+    solvers = {prec: capi.solver_for(p, precision=prec) for prec in (0, 2)}
+    for trial in range(2):
+        for prec in (0, 2):
+            s = solvers[prec]
+            upload_problem(s, p)
+            s.reset_counter()
+            s.run(warmup, flg=1, history=False)
+            s.sync()
+            t0 = time.perf_counter()
+            _, dts[prec] = s.run(steps, flg=1, history=False)
+            s.sync()
+            rate[prec] = max(rate[prec], zones * steps / (time.perf_counter() - t0))
+            out[prec] = s.download()
+    for s in solvers.values():
+        s.close()
+    a, b = out[0], out[2]
+    # a vector component against the mean L1 norm of its vector's components
+    groups = ((0,), (1, 2, 3), (1, 2, 3), (1, 2, 3), (4, 5, 6), (4, 5, 6), (4, 5, 6), (7,))
+    norm = [float(np.abs(_interior(a[m], p.ndim)).sum()) for m in range(8)]
+    e = [float(np.abs(_interior(b[m] - a[m], p.ndim)).sum()) / (sum(norm[k] for k in groups[m]) / len(groups[m]))
+         for m in range(8)]
+    return {"workload": wname, "steps": steps, "warmup": warmup, "fp64_exact_value": rate[0],
+            "fp64_fma_value": rate[2], "speedup": rate[2] / rate[0],
+            "rel_l1_max_over_variables": max(e), "rel_l1": e,
+            "dt_rel_diff": abs(dts[2] / dts[0] - 1.0), "finite": bool(np.isfinite(b).all()),
+            "note": "precision 2: the same FP64 sweeps built with FMA contraction; distance to the bit-exact "
+                    "(reference-identical) path after warmup + steps steps, bar 1e-12"}
+
+
 def batch_line(ninst, steps, seed=5):
     """Batched domains (qas_batch_run): `ninst` periodic 2D instances of
     square sizes drawn (numpy PCG64(seed)) from {128, 192, 256, 384, 512},
@@ -681,7 +720,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and adjoint lines")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host (for counters)")
-    ap.add_argument("--precision", type=int, default=0, help="0 FP64 (default), 1 mixed, for the main line")
+    ap.add_argument("--precision", type=int, default=0,
+                    help="0 FP64 (default), 1 mixed, 2 FP64 with FMA contraction, for the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy the 8 state arrays to a device buffer (8 copy "
                          "dispatches of nd doubles: the calibration of the memory counters)")
@@ -707,6 +747,7 @@ def main():
     ap.add_argument("--cpu3d-steps", type=int, default=1)
     ap.add_argument("--khi-cpu-steps", type=int, default=4)
     ap.add_argument("--no-brio-wu", dest="brio_wu", action="store_false", help="skip the 1D line")
+    ap.add_argument("--no-fma64", action="store_true", help="skip the FMA-contracted FP64 line")
     ap.add_argument("--mixed3d-steps", type=int, default=0,
                     help="with a 3D workload: also FP64 vs mixed over this many fixed-dt steps")
     args = ap.parse_args()
@@ -782,6 +823,12 @@ def main():
             _log("bench: cpu baseline failed: %s" % e)
 
     extra = {}
+    if not args.no_fma64 and p.ndim == 2 and args.precision == 0:
+        try:
+            _log("bench: FP64 with FMA contraction vs bit-exact, %d steps" % args.steps)
+            extra["fma64"] = fma64_line(p, wname, args.warmup, args.steps)
+        except Exception as e:
+            _log("bench: fma64 line failed: %s" % e)
     if p.ndim == 3 and args.mixed3d_steps > 0 and not isinstance(p.state, np.ndarray):
         try:
             _log("bench: 3D FP64 vs mixed, %d steps" % args.mixed3d_steps)
@@ -864,10 +911,10 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if args.precision == 0 else "f32+f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32+f64" if args.precision == 1 else "f64", "data": "synthetic",
         "config": {"workload": wname, "cells": list(p.interior), "solver": ["roe", "hlld", "lhlld", "mlau"][p.rmn],
                    "odr": p.odr, "r_k": p.r_k, "ctw": p.ctw, "cfl": p.cfl,
-                   "precision": "fp64" if args.precision == 0 else "mixed",
+                   "precision": {0: "fp64", 1: "mixed", 2: "fp64_fma"}[args.precision],
                    "dt_final": dt_now, "finite": finite},
         "roofline": roof, "cpu_baseline": cpu,
     }

@@ -59,7 +59,10 @@ typedef struct {
   int st[3][8];     /* staggering flags stxs/stys/stzs (bx:x, by:y, bz:z = 1) */
   int precision;    /* 0 = FP64 (bit-identical to the reference),
                        1 = mixed (FP32 reconstruction and fluxes; FP64 state,
-                           RK stage sum and pressure recovery)               */
+                           RK stage sum and pressure recovery),
+                       2 = FP64 with fused multiply-adds in the sweeps (same
+                           arithmetic, one rounding per multiply-add; not
+                           bit-identical; 2D and 3D)                        */
 } qas_scheme;
 
 /* Fill a scheme with the reference defaults: gam 5/3, cfl 0.4, HLLD, ODR 4,
@@ -180,8 +183,8 @@ int qas_set_gravity(qas_ctx *c, const double *phi);
  * must outlive the batch; calls on one batch are not thread-safe.
  * Launch shape (autotuned): qas_batch_tune times RK stage 1 of the batch
  * in each candidate shape -- instance block order (0 as given, 1 largest
- * domain first), x-sweep wavefronts per workgroup (xwpb 2 or 4), ghost-
- * fill workgroup size (ewb 128 or 256) -- and keeps the fastest; the first
+ * domain first), x-sweep wavefronts per workgroup (xwpb 2 or 4), element-
+ * wise workgroup size (ewb 128 or 256) -- and keeps the fastest; the first
  * qas_batch_run tunes unless a shape was set.  Results are bit-identical in
  * every shape.  qas_batch_set_shape fixes one (xwpb 1/2/4, ewb 64/128/256);
  * qas_batch_shape returns it (1 when tuned or set, 0 for the default). */
@@ -224,8 +227,8 @@ int qas_adj_backward_device(qas_ctx *c, const double *g_final, double *g_init);
 #define QAS_K_SWEEP_X 1   /* reconstruction + Riemann + divergence along x  */
 #define QAS_K_SWEEP_Y 2
 #define QAS_K_SWEEP_Z 3
-#define QAS_K_EMF 4       /* CUCT edge EMFs (2D: in CT)                     */
-#define QAS_K_CT 5        /* staggered-B update (2D: with the corner EMFs)  */
+#define QAS_K_EMF 4       /* CUCT edge EMFs                                 */
+#define QAS_K_CT 5        /* staggered-B update                             */
 #define QAS_K_BOUND 6     /* ghost fill                                     */
 #define QAS_K_CFL 7       /* CFL reduction                                  */
 #define QAS_K_COUNT 8

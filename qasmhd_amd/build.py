@@ -5,7 +5,9 @@
 Every .hip file under csrc/ is compiled with hipcc for gfx950 and linked
 into one shared library exporting the C ABI of include/qasmhd.h.  Flags:
   -ffp-contract=off      no FMA contraction: the FP64 path keeps the
-                         reference's rounding sequence (bit-identical results)
+                         reference's rounding sequence (bit-identical results);
+                         qas_mixed.hip and qas_fma64.hip (precision 1, 2)
+                         are built with contraction
   -Xclang -target-feature -Xclang -packed-fp32-ops
                          no packed FP32 instructions are generated
 """
@@ -27,8 +29,10 @@ COMMON_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-Wno-unused-command-line-argument", "-Wno-unused-result"]
 # per-source flags: every unit is built without FMA contraction except the
-# FP32 sweeps of the mixed path, which also use the fast FP32 divide/sqrt
-EXTRA = {"qas_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+# FP32 sweeps of the mixed path, which also use the fast FP32 divide/sqrt,
+# and the contracted FP64 sweeps (IEEE FP64 divide and sqrt)
+EXTRA = {"qas_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+         "qas_fma64.hip": ["-ffp-contract=fast"]}
 
 
 def lib_path() -> str:

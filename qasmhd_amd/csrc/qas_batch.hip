@@ -27,10 +27,17 @@ __device__ __forceinline__ long long batch_thread(const int* off, int i, int b)
 }
 
 template <int ODR, typename FT>
-__global__ __launch_bounds__(256) void kb_emfct2d(const BatchInst* I, const int* off, int n, int stage, int ctw)
+__global__ __launch_bounds__(256) void kb_emf2d(const BatchInst* I, const int* off, int n, int stage, int ctw)
 {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
-  emfct2d_tile<ODR, FT>(I[i].st[stage], I[i].g, ctw, b - off[i]);
+  emf2d_at<ODR, FT>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
+}
+
+template <int ODR>
+__global__ __launch_bounds__(256) void kb_ct2d(const BatchInst* I, const int* off, int n, int stage)
+{
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
+  ct2d_at<ODR>(I[i].st[stage], I[i].g, batch_thread(off, i, b));
 }
 
 // ghost fill of stage `stage`'s output, or of the step's state (stage < 0)
@@ -91,7 +98,8 @@ __global__ void kb_get_dt(const BatchInst* I, int n, double* out)
   if (i < n) out[i] = *I[i].dt;
 }
 
-typedef void (*emfct_fn)(const BatchInst*, const int*, int, int, int);
+typedef void (*emf_fn)(const BatchInst*, const int*, int, int, int);
+typedef void (*ct_fn)(const BatchInst*, const int*, int, int);
 
 }  // namespace qas
 
@@ -109,14 +117,15 @@ struct qas_batch {
   std::vector<BatchInst> inst;
   xsweep2d_fn sx = nullptr;
   xsweep2d_fn sy = nullptr;
-  emfct_fn emfct = nullptr;
+  emf_fn emf = nullptr;
+  ct_fn ct = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_flg = -1, graph_hist = -1;
   long long nstep = 0;
   // launch shape (qas_batch_tune): block order of the instances (0 as given,
   // 1 largest first), wavefronts per x-sweep workgroup, threads per
-  // workgroup of the ghost fill
+  // workgroup of the element-wise stage kernels
   int order = 0, xwpb = 4, ewb = 256;
   bool tuned = false;
   std::vector<int> perm;  // table position -> context index
@@ -196,8 +205,9 @@ static int fill_table(qas_batch* b)
     B.cflblocks = std::min(cdiv(ninter, 256), CFL_MAX_BLOCKS);
     const int cnt[BK_COUNT] = {cdiv((long long)B.nsx * g.ny * WAVE, (long long)WAVE * b->xwpb),
                                ((g.nx + YC - 1) / YC) * B.nsy,
-                               emfct2d_tiles(g.nx, g.ny),
-                               cdiv(B.nghost * 8, b->ewb),
+                               cdiv((long long)(g.nx - 5) * (g.ny - 5), b->ewb),
+                               cdiv((long long)(g.nx - 7) * (g.ny - 7), b->ewb),
+                               cdiv(B.nghost * 8, 256),
                                B.cflblocks};
     for (int k = 0; k < BK_COUNT; k++) off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
   }
@@ -212,8 +222,8 @@ static const int* boff(const qas_batch* b, int k) { return b->d_off + (size_t)k 
 
 static hipError_t launch_bound_all(qas_batch* b, int stage)
 {
-  hipLaunchKernelGGL(kb_bound, (unsigned)b->nblk[BK_BOUND], (unsigned)b->ewb, 0, b->stream, b->d_inst,
-                     boff(b, BK_BOUND), b->n, stage);
+  hipLaunchKernelGGL(kb_bound, (unsigned)b->nblk[BK_BOUND], 256, 0, b->stream, b->d_inst, boff(b, BK_BOUND), b->n,
+                     stage);
   return hipGetLastError();
 }
 
@@ -225,18 +235,18 @@ static hipError_t launch_cfl_all(qas_batch* b)
   return hipGetLastError();
 }
 
-// the three kernels of RK stage rk for every instance, in the batch's shape
+// the four kernels of RK stage rk for every instance, in the batch's shape
 static hipError_t launch_stage_all(qas_batch* b, int rk)
 {
   hipStream_t st = b->stream;
   const int n = b->n;
-  const unsigned xb = (unsigned)(WAVE * b->xwpb);
+  const unsigned eb = (unsigned)b->ewb, xb = (unsigned)(WAVE * b->xwpb);
   hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
                      (const BatchInst*)b->d_inst, boff(b, BK_SY), n, rk);
   hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], xb, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
                      (const BatchInst*)b->d_inst, boff(b, BK_SX), n, rk);
-  hipLaunchKernelGGL(b->emfct, (unsigned)b->nblk[BK_EMFCT], 256, 0, st, b->d_inst, boff(b, BK_EMFCT), n, rk,
-                     b->s.ctw);
+  hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], eb, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
+  hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], eb, 0, st, b->d_inst, boff(b, BK_CT), n, rk);
   return hipGetLastError();
 }
 
@@ -258,7 +268,8 @@ static hipError_t batch_step(qas_batch* b, int flg, bool hist, bool even)
 template <int ODR>
 static void pick_stage(qas_batch* b, bool f32)
 {
-  b->emfct = f32 ? (emfct_fn)kb_emfct2d<ODR, float> : (emfct_fn)kb_emfct2d<ODR, double>;
+  b->emf = f32 ? (emf_fn)kb_emf2d<ODR, float> : (emf_fn)kb_emf2d<ODR, double>;
+  b->ct = kb_ct2d<ODR>;
 }
 
 extern "C" int qas_batch_create(qas_batch** out, qas_ctx* const* ctxs, int n)
@@ -289,8 +300,11 @@ extern "C" int qas_batch_create(qas_batch** out, qas_ctx* const* ctxs, int n)
   }
   if (b->s.r_k < 2) return bfail(b, QAS_E_ARG, "qas_batch_create: r_k must be 2 or 3");
   const bool f32 = b->s.precision == 1;
-  b->sx = f32 ? pick_xsweep2d_f32(b->s.rmn, b->s.odr, true) : pick_xsweep2d(b->s.rmn, b->s.odr, true);
-  b->sy = f32 ? pick_ysweep2d_f32(b->s.rmn, b->s.odr, true) : pick_ysweep2d(b->s.rmn, b->s.odr, true);
+  const bool fma = b->s.precision == 2;
+  b->sx = f32 ? pick_xsweep2d_f32(b->s.rmn, b->s.odr, true)
+              : (fma ? pick_xsweep2d_fma(b->s.rmn, b->s.odr, true) : pick_xsweep2d(b->s.rmn, b->s.odr, true));
+  b->sy = f32 ? pick_ysweep2d_f32(b->s.rmn, b->s.odr, true)
+              : (fma ? pick_ysweep2d_fma(b->s.rmn, b->s.odr, true) : pick_ysweep2d(b->s.rmn, b->s.odr, true));
   if (!b->sx || !b->sy) return bfail(b, QAS_E_ARG, "qas_batch_create: no kernel for this scheme");
   switch (b->s.odr) {
     case 1: pick_stage<1>(b, f32); break;
@@ -429,13 +443,13 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
 
 // ---------------------------------------------------------------- autotuner
 // Launch shapes of the batched stage: block order of the instances, x-sweep
-// wavefronts per workgroup, workgroup size of the ghost fill.  Every shape
-// gives the same arithmetic per cell (bit-identical results); they differ
-// in how the workgroups of uneven domains fill the GPU.  The tuner times RK
-// stage 1 (both sweeps, EMF + CT, ghost fill of its output) in each shape,
-// three times each after one warm-up, and keeps the fastest.  Stage 1 reads
-// the current states and writes only the next-stage buffers and scratch,
-// which the run overwrites.
+// wavefronts per workgroup, element-wise (EMF, CT) workgroup size.  Every
+// shape gives the same arithmetic per cell (bit-identical results); they
+// differ in how the workgroups of uneven domains fill the GPU.  The tuner
+// times RK stage 1 (both sweeps, EMF, CT, ghost fill of its output) in each
+// shape, three times each after one warm-up, and keeps the fastest.  Stage
+// 1 reads the current states and writes only the next-stage buffers and
+// scratch, which the run overwrites.
 extern "C" int qas_batch_tune(qas_batch* b)
 {
   if (!b || !b->stream) return bfail(b, QAS_E_ARG, "qas_batch_tune: invalid batch");

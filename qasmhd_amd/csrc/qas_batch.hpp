@@ -12,7 +12,7 @@
 namespace qas {
 
 // kernel types with their own block-offset table
-enum BatchKernel { BK_SX = 0, BK_SY, BK_EMFCT, BK_BOUND, BK_CFL, BK_COUNT };
+enum BatchKernel { BK_SX = 0, BK_SY, BK_EMF, BK_CT, BK_BOUND, BK_CFL, BK_COUNT };
 
 struct BatchInst {
   StagePtrs st[3];   // stage descriptors of one step (buffer rotation, rk_fac, dt)
@@ -48,12 +48,15 @@ typedef void (*batch_fn)(const BatchInst*, const int*, int, int, double);
 // for one domain, (instance table, block offsets, instances, stage) for a batch
 typedef void (*xsweep2d_fn)(StagePtrs, Geo, BcFlags, double, int, const BatchInst*, const int*, int, int);
 
-// kernel tables (instantiated in qas_kernels2d.hip for FP64 and in
-// qas_mixed.hip for the mixed path)
+// kernel tables (instantiated in qas_kernels2d.hip for FP64, in
+// qas_mixed.hip for the mixed path and in qas_fma64.hip for contracted FP64)
 xsweep2d_fn pick_ysweep2d(int rmn, int odr, bool batch);
 xsweep2d_fn pick_ysweep2d_f32(int rmn, int odr, bool batch);
 xsweep2d_fn pick_xsweep2d(int rmn, int odr, bool batch);
 xsweep2d_fn pick_xsweep2d_f32(int rmn, int odr, bool batch);
+// FP64 with FMA contraction (qas_fma64.hip)
+xsweep2d_fn pick_ysweep2d_fma(int rmn, int odr, bool batch);
+xsweep2d_fn pick_xsweep2d_fma(int rmn, int odr, bool batch);
 // gravity instantiations (qas_gravity.hip)
 xsweep2d_fn pick_grav_xsweep(int rmn, int odr);
 xsweep2d_fn pick_grav_ysweep(int rmn, int odr);

@@ -72,7 +72,8 @@ static std::string validate(const qas_grid* g, const qas_scheme* s)
   if (s->odr < 1 || s->odr > 4) return "odr must be 1..4";
   if (s->r_k < 1 || s->r_k > 3) return "r_k must be 1..3";
   if (s->ctw != 0 && s->ctw != 1) return "ctw must be 0 or 1";
-  if (s->precision != 0 && s->precision != 1) return "precision must be 0 (FP64) or 1 (mixed)";
+  if (s->precision < 0 || s->precision > 2)
+    return "precision must be 0 (FP64), 1 (mixed) or 2 (FP64 with FMA contraction)";
   if (!(s->gam > 1.0)) return "gam must exceed 1";
   for (int a = 0; a < 3; a++)
     for (int m = 0; m < 8; m++) {

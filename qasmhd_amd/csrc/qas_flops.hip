@@ -203,18 +203,6 @@ extern "C" int qas_kernel_model(const qas_ctx* c, int kid, double* flops, double
     *bytes = cells * 8.0 * (8 + (nd3 >= 2 ? nd3 : 0) + 4);
     return QAS_OK;
   }
-  if (kid == QAS_K_EMF && nd3 == 2) return QAS_OK;  // folded into the CT kernel
-  if (kid == QAS_K_CT && nd3 == 2) {
-    // corner EMFs and the bx, by update in one tiled kernel: face parts (4)
-    // and, with CUCT, ro mx my read; bx by, their step base read, written
-    const double ne = (n[0] - 5) * (n[1] - 5);
-    const double nf = (n[0] - 7) * (n[1] - 7);
-    const double part = 4 * pc.lin + 2 + 1 + 2 + 1;
-    *flops = ne * (2 * part + 2 + (c->s.ctw ? count_cuct() : 0)) + (c->s.ctw ? cells * 3.0 : 0.0) +
-             nf * 2 * ((pc.df + 1) + 1 + 4);
-    *bytes = cells * 8.0 * (4 + (c->s.ctw ? 3 : 0) + 2 + 2 + 2);
-    return QAS_OK;
-  }
   if (kid == QAS_K_EMF) {
     const double ne = (n[0] - 5) * (nd3 >= 2 ? n[1] - 5 : 1) * (nd3 >= 3 ? n[2] - 5 : 1);
     const double ncomp = nd3 == 3 ? 3 : 1;

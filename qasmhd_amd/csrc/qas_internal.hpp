@@ -238,6 +238,8 @@ void dsp_release(qas_ctx* c);
 void grav_release(qas_ctx* c);
 int dsp_step(qas_ctx* c, double dt);
 int dsp_after_ideal(qas_ctx* c);
-// FP32 face sweeps of the 3D mixed path (qas_mixed.hip)
+// FP32 face sweeps of the 3D mixed path (qas_mixed.hip) and the contracted
+// FP64 ones (qas_fma64.hip)
 void (*pick_sweep_f32(int ndim, int dir, int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
+void (*pick_sweep_fma(int ndim, int dir, int rmn, int odr))(StagePtrs, Geo, BcFlags, double, int);
 }  // namespace qas

@@ -1,7 +1,7 @@
 // qas_kernels2d.hip -- one SSP-RK stage of the 2D ideal-MHD step on gfx950.
 //
 // Replaces the stage body of MHD2D::ideal (2D/SERIAL/OTvortex/
-// mhd2d_solve.cpp:44-386).  A stage is three launches:
+// mhd2d_solve.cpp:44-386).  A stage is four launches:
 //   sweep<Y> cell-centred B and primitives of its LDS tile from the state
 //            (stored for the x sweep)                        (:51-89)
 //            then characteristic WCNS + Riemann flux per y face, for every
@@ -11,9 +11,8 @@
 //   sweep<X> same along x, for every row; applies the RK stage sum
 //            a u0 + b (u + ((-dt/dx Dx) - dt/dy Dy)) to ro, mx, my, mz, bz,
 //            en (the reference's expression)               (:127-199, 341-358)
-//   emfct2d  CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
-//            and the staggered bx, by update from it, per 64 x 16 face
-//            tile with the EMF in LDS                       (:359-380)
+//   emf2d    CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
+//   ct2d     staggered bx, by update                        (:359-380)
 // followed by the bound() kernel (:384-385).
 // A sweep assigns one 64-lane wavefront to 64 consecutive cells of a line;
 // lane l reconstructs cell f0-2+l, neighbour states and fluxes move between
@@ -32,9 +31,15 @@ namespace qas {
 // renumbered XCD-aware (xcd_remap): the stencils of neighbouring rows then
 // meet in the same L2.
 template <int ODR, typename FT>
-__global__ __launch_bounds__(256) void k_emfct2d(StagePtrs p, Geo g, int ctw)
+__global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
 {
-  emfct2d_tile<ODR, FT>(p, g, ctw, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  emf2d_at<ODR, FT>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
+}
+
+template <int ODR>
+__global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
+{
+  ct2d_at<ODR>(p, g, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ launch
@@ -50,9 +55,11 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
   // 1) y sweep (with the primitives), 2) x sweep
   const int nsx = (g.nx - 5 + SEG - 1) / SEG;
   const int nsy = (g.ny - 5 + SEG - 1) / SEG;
-  const bool mixed = c->s.precision == 1;
-  xsweep2d_fn fx = mixed ? pick_xsweep2d_f32(c->s.rmn, odr, false) : pick_xsweep2d(c->s.rmn, odr, false);
-  xsweep2d_fn fy = mixed ? pick_ysweep2d_f32(c->s.rmn, odr, false) : pick_ysweep2d(c->s.rmn, odr, false);
+  const int prec = c->s.precision;
+  xsweep2d_fn fx = prec == 1 ? pick_xsweep2d_f32(c->s.rmn, odr, false)
+                             : (prec == 2 ? pick_xsweep2d_fma(c->s.rmn, odr, false) : pick_xsweep2d(c->s.rmn, odr, false));
+  xsweep2d_fn fy = prec == 1 ? pick_ysweep2d_f32(c->s.rmn, odr, false)
+                             : (prec == 2 ? pick_ysweep2d_fma(c->s.rmn, odr, false) : pick_ysweep2d(c->s.rmn, odr, false));
   // with gravity the x sweep takes the round-tripped stage energy, which
   // the y sweep computes from the state's own energy and stores
   StagePtrs ps = p;
@@ -70,18 +77,21 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
   QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   if (faces_only) return hipGetLastError();
-  // 3) corner EMF + CT, tiled (the face parts are stored in the sweep precision)
-  const unsigned nt = (unsigned)emfct2d_tiles(g.nx, g.ny);
-#define QAS_EMFCT(O)                                                                                   \
-  if (f32) QAS_LAUNCH(c, QAS_K_CT, (k_emfct2d<O, float>), nt, 256, 0, st, p, g, c->s.ctw);          \
-  else QAS_LAUNCH(c, QAS_K_CT, (k_emfct2d<O, double>), nt, 256, 0, st, p, g, c->s.ctw);
+  // 3) corner EMF, 4) CT
+  const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
+  const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
+  // (the face parts are stored in the sweep precision)
+#define QAS_EMF_CT(O)                                                                                      \
+  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);      \
+  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);          \
+  QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<O>), nblk(nc, 256), 256, 0, st, p, g);
   switch (odr) {
-    case 1: QAS_EMFCT(1) break;
-    case 2: QAS_EMFCT(2) break;
-    case 3: QAS_EMFCT(3) break;
-    default: QAS_EMFCT(4) break;
+    case 1: QAS_EMF_CT(1) break;
+    case 2: QAS_EMF_CT(2) break;
+    case 3: QAS_EMF_CT(3) break;
+    default: QAS_EMF_CT(4) break;
   }
-#undef QAS_EMFCT
+#undef QAS_EMF_CT
   return hipGetLastError();
 }
 

@@ -362,6 +362,8 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   sweep3_fn f[3] = {pick3<0>(c->s.rmn, odr), pick3<1>(c->s.rmn, odr), pick3<2>(c->s.rmn, odr)};
   if (c->s.precision == 1)
     for (int d = 0; d < 3; d++) f[d] = pick_sweep_f32(3, d, c->s.rmn, odr);
+  if (c->s.precision == 2)
+    for (int d = 0; d < 3; d++) f[d] = pick_sweep_fma(3, d, c->s.rmn, odr);
   // z, then y, then x: the transverse sweeps leave dt/dz Dz and dt/dy Dy in
   // the accumulators, the x sweep applies the stage sum
   for (int d = 2; d >= 0; d--) {

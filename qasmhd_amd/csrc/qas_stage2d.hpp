@@ -1,7 +1,8 @@
 // qas_stage2d.hpp -- 2D stage bodies other than the sweeps' own: the
-// primitives of one cell (used by the y sweep's staging) and the tiled
-// corner-EMF + CT kernel body, shared by the single-domain kernels of
-// qas_kernels2d.hip and the batched kernels of qas_batch.hip.
+// primitives of one cell (used by the y sweep's staging), and the corner
+// EMF and CT bodies (per thread, flat index t of the domain), shared by the
+// single-domain kernels of qas_kernels2d.hip and the batched kernels of
+// qas_batch.hip.
 #pragma once
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
@@ -72,13 +73,32 @@ __device__ __forceinline__ Prim2d prim2d_cell(const StagePtrs& p, const Geo& g, 
   return r;
 }
 
-// ------------------------------------------------------------------ EMF + CT
-// Corner EMF (mhd2d_solve.cpp:91-103, 201-232, 308-339) at corner (i-1/2,
-// j-1/2) stored at ss = nx j + i, from both sweeps' face parts, with the
-// CUCT weight ct.  FT: element type of the face parts (the sweep precision).
-template <int ODR, typename FT>
-__device__ __forceinline__ double emf2d_corner(const StagePtrs& p, long long nx, long long ss, double ct)
+// velocity component of cell c from the state: the same expression as the
+// prim kernel (m * (1/ro)), so the FP64 value is bit-identical to it
+__device__ __forceinline__ double vel(const StagePtrs& p, int comp, long long c)
 {
+  return p.in[1 + comp][c] * (1.0 / p.in[0][c]);
+}
+
+// ------------------------------------------------------------------ EMF
+// FT: element type of the face parts (the sweep precision)
+template <int ODR, typename FT>
+__device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int ctw, long long t)
+{
+  const int ni = g.nx - 5, nj = g.ny - 5;   // corners [3, n-3]
+  if (t >= (long long)ni * nj) return;
+  const int i = 3 + (int)(t % ni);
+  const int j = 3 + (int)(t / ni);
+  const long long nx = g.nx;
+  const long long ss = nx * j + i;
+  double ct = 0.5;
+  if (ctw) {
+    const double* ro = p.in[0];
+    const long long c0 = ss - 1 - nx, c1 = ss - nx, c2 = ss - 1, c3 = ss;
+    ct = cuct_weight(ro[c0], ro[c1], ro[c2], ro[c3], vel(p, 0, c0), vel(p, 0, c1), vel(p, 0, c2), vel(p, 0, c3),
+                     vel(p, 1, c0), vel(p, 1, c1), vel(p, 1, c2), vel(p, 1, c3), p.in[4][ss - nx], p.in[4][ss],
+                     p.in[5][ss - 1], p.in[5][ss]);
+  }
   double ez = 0.0;
   {  // from the x sweep: By flux interpolated along y to (i-1/2, j-1/2)
     const double* fu = p.emf[0];
@@ -108,107 +128,31 @@ __device__ __forceinline__ double emf2d_corner(const StagePtrs& p, long long nx,
     lin5<ODR>(b, dum, ur1);
     ez += +0.5 * ((ul0 + ur0) + ct * (ul1 + ur1));
   }
-  return ez;
+  p.ez[ss] = ez;
 }
 
-// One tile of the corner-EMF + CT stage (mhd_updt2d_ctb, mhd_func.c:286-300,
-// mhd2d_solve.cpp:359-380): the staggered bx, by of the ET_X x ET_Y
-// interior faces [I0, I0+ET_X) x [J0, J0+ET_Y) (the last tile along x also
-// takes bx of face column nx-4, the last along y by of face row ny-4).
-//   1. cells [I0-2, I0+ET_X+1] x [J0-2, J0+ET_Y+1]: ro and v = m (1/ro) into
-//      LDS (CUCT on), once per cell rather than once per corner using it;
-//   2. corners [I0-1, I0+ET_X+1] x [J0-1, J0+ET_Y+1]: CUCT weight and Ez
-//      into LDS (the edge EMF never goes to memory);
-//   3. faces: the 4th-order curl of Ez and the RK stage sum.
-// Same expressions and operands as one thread per corner and per face:
-// bit-identical.  256 threads.
-constexpr int ET_X = 64, ET_Y = 16;
-constexpr int ET_CX = ET_X + 4, ET_CY = ET_Y + 4;  // cell region
-constexpr int ET_KX = ET_X + 3, ET_KY = ET_Y + 3;  // corner region
-
-__host__ __device__ inline int emfct2d_tiles(int nx, int ny)
+// ------------------------------------------------------------------ CT
+template <int ODR>
+__device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long long t)
 {
-  return ((nx - 2 * NG + ET_X - 1) / ET_X) * ((ny - 2 * NG + ET_Y - 1) / ET_Y);
-}
-
-template <int ODR, typename FT>
-__device__ __forceinline__ void emfct2d_tile(const StagePtrs& p, const Geo& g, int ctw, int tile)
-{
-  __shared__ double sro[ET_CY][ET_CX], svx[ET_CY][ET_CX], svy[ET_CY][ET_CX];
-  __shared__ double sez[ET_KY][ET_KX + 1];
-  const int ntx = (g.nx - 2 * NG + ET_X - 1) / ET_X;
-  const int nty = (g.ny - 2 * NG + ET_Y - 1) / ET_Y;
-  if (tile >= ntx * nty) return;  // whole workgroup
-  const int tx = tile % ntx, ty = tile / ntx;
-  const int I0 = NG + tx * ET_X, J0 = NG + ty * ET_Y;
+  // faces i, j in [4, n-4]: bx needs j <= ny-5, by needs i <= nx-5
+  const int ni = g.nx - 7, nj = g.ny - 7;
+  if (t >= (long long)ni * nj) return;
+  const int i = 4 + (int)(t % ni);
+  const int j = 4 + (int)(t / ni);
   const long long nx = g.nx;
-  const int t = (int)threadIdx.x;
-  if (ctw) {
-    for (int e = t; e < ET_CX * ET_CY; e += blockDim.x) {
-      const int cx = e % ET_CX, cy = e / ET_CX;
-      int i = I0 - 2 + cx, j = J0 - 2 + cy;  // cells outside the domain are never used
-      i = i > g.nx - 1 ? g.nx - 1 : i;
-      j = j > g.ny - 1 ? g.ny - 1 : j;
-      const long long c = nx * j + i;
-      const double ro = p.in[0][c];
-      const double iro = 1.0 / ro;  // m * (1/ro), the primitives' expression
-      sro[cy][cx] = ro;
-      svx[cy][cx] = p.in[1][c] * iro;
-      svy[cy][cx] = p.in[2][c] * iro;
-    }
-    __syncthreads();
-  }
-  for (int e = t; e < ET_KX * ET_KY; e += blockDim.x) {
-    const int kx = e % ET_KX, ky = e / ET_KX;
-    const int i = I0 - 1 + kx, j = J0 - 1 + ky;
-    double ez = 0.0;
-    if (i <= g.nx - 3 && j <= g.ny - 3) {  // corners [3, n-3]
-      const long long ss = nx * j + i;
-      double ct = 0.5;
-      if (ctw) {
-        // cells (i-1, j-1), (i, j-1), (i-1, j), (i, j) at region (kx, ky) + (0|1, 0|1)
-        ct = cuct_weight(sro[ky][kx], sro[ky][kx + 1], sro[ky + 1][kx], sro[ky + 1][kx + 1], svx[ky][kx],
-                         svx[ky][kx + 1], svx[ky + 1][kx], svx[ky + 1][kx + 1], svy[ky][kx], svy[ky][kx + 1],
-                         svy[ky + 1][kx], svy[ky + 1][kx + 1], p.in[4][ss - nx], p.in[4][ss], p.in[5][ss - 1],
-                         p.in[5][ss]);
-      }
-      ez = emf2d_corner<ODR, FT>(p, nx, ss, ct);
-    }
-    sez[ky][kx] = ez;
-  }
-  __syncthreads();
+  const long long ss = nx * j + i;
   const double dtv = *p.dt;
-  const double dtdy = dtv / g.dy;
-  const double mdtdx = -(dtv / g.dx);  // mhd_updt2d_ctb is called with -dtdx
-  // bx at (i, j): Ez at (i, j-1 .. j+2); by at (i, j): Ez at (i-1 .. i+2, j)
-  auto upd_bx = [&](int i, int j) {
-    const int kx = i - (I0 - 1), ky = j - (J0 - 1);
-    const long long ss = nx * j + i;
-    const double du = dfc<ODR>(sez[ky - 1][kx], sez[ky][kx], sez[ky + 1][kx], sez[ky + 2][kx]);
+  const double* ez = p.ez;
+  if (j <= g.ny - 5) {
+    const double dtdy = dtv / g.dy;
+    const double du = dfc<ODR>(ez[ss - nx], ez[ss], ez[ss + nx], ez[ss + 2 * nx]);
     p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du, p.a, p.b);
-  };
-  auto upd_by = [&](int i, int j) {
-    const int kx = i - (I0 - 1), ky = j - (J0 - 1);
-    const long long ss = nx * j + i;
-    const double du = dfc<ODR>(sez[ky][kx - 1], sez[ky][kx], sez[ky][kx + 1], sez[ky][kx + 2]);
+  }
+  if (i <= g.nx - 5) {
+    const double mdtdx = -(dtv / g.dx);  // mhd_updt2d_ctb is called with -dtdx
+    const double du = dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
     p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -mdtdx * du, p.a, p.b);
-  };
-  const int i = I0 + t % ET_X;
-#pragma unroll
-  for (int k = 0; k < ET_Y / 4; k++) {
-    const int j = J0 + t / ET_X + 4 * k;
-    if (i <= g.nx - NG - 1 && j <= g.ny - NG - 1) {
-      upd_bx(i, j);
-      upd_by(i, j);
-    }
-  }
-  if (tx == ntx - 1 && t < ET_Y) {  // bx of the face column nx-4
-    const int j = J0 + t;
-    if (j <= g.ny - NG - 1) upd_bx(g.nx - NG, j);
-  }
-  if (ty == nty - 1 && t >= ET_X && t < 2 * ET_X) {  // by of the face row ny-4
-    const int ii = I0 + t - ET_X;
-    if (ii <= g.nx - NG - 1) upd_by(ii, g.ny - NG);
   }
 }
 

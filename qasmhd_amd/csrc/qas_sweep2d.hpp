@@ -57,7 +57,10 @@ constexpr int YRP = YR + 1;    // padded LDS column length
 // (BATCH true), as the x sweep below; with the body as a device function
 // called from thin kernels the FP64 HLLD / ODR 4 instance compiled to 250
 // VGPRs (2 waves per SIMD) instead of 163 (3 waves).
-template <int RMN, int ODR, typename T, bool BATCH, bool GRAV = false>
+// FMA: a tag (0 or 1) naming the build of the instantiation: the FP64
+// sweeps built with FMA contraction (qas_fma64.hip) are the same template
+// under FMA = 1, so their kernel symbols differ from the bit-exact ones.
+template <int RMN, int ODR, typename T, bool BATCH, bool GRAV = false, int FMA = 0>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : (RMN != 1 ? 2 : 1)) void k_sweep2d_y(
     StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam, int nseg_arg, const BatchInst* I, const int* off,
     int ninst, int stage)
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : (RMN != 1 ? 2 : 1)) void 
 // from thin kernels it compiled to 246 VGPRs (2 waves per SIMD) instead of
 // 158 (3 waves) for FP64 HLLD / ODR 4.  The FP64 Roe and MLAU instances ask
 // for 2 waves per SIMD, which they otherwise miss by a few registers.
-template <int DIR, int RMN, int ODR, typename T, bool BATCH, bool GRAV = false>
+template <int DIR, int RMN, int ODR, typename T, bool BATCH, bool GRAV = false, int FMA = 0>
 __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_sweep2d(StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam,
                                                  int nseg_arg, const BatchInst* I, const int* off, int ninst,
                                                  int stage)
@@ -390,11 +393,11 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
 }
 
 // kernel tables: y sweep and x sweep (single domain or batch)
-template <typename T>
+template <typename T, int FMA = 0>
 xsweep2d_fn pick_ysweep2d_t(int rmn, int odr, bool batch)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return batch ? k_sweep2d_y<R, O, T, true> : k_sweep2d_y<R, O, T, false>;
+  if (rmn == R && odr == O) return batch ? k_sweep2d_y<R, O, T, true, false, FMA> : k_sweep2d_y<R, O, T, false, false, FMA>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)
@@ -403,11 +406,11 @@ xsweep2d_fn pick_ysweep2d_t(int rmn, int odr, bool batch)
   return nullptr;
 }
 
-template <typename T>
+template <typename T, int FMA = 0>
 xsweep2d_fn pick_xsweep2d_t(int rmn, int odr, bool batch)
 {
 #define QAS_S(R, O) \
-  if (rmn == R && odr == O) return batch ? k_sweep2d<0, R, O, T, true> : k_sweep2d<0, R, O, T, false>;
+  if (rmn == R && odr == O) return batch ? k_sweep2d<0, R, O, T, true, false, FMA> : k_sweep2d<0, R, O, T, false, false, FMA>;
   QAS_S(0, 1) QAS_S(0, 2) QAS_S(0, 3) QAS_S(0, 4)
   QAS_S(1, 1) QAS_S(1, 2) QAS_S(1, 3) QAS_S(1, 4)
   QAS_S(2, 1) QAS_S(2, 2) QAS_S(2, 3) QAS_S(2, 4)

@@ -184,7 +184,8 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
   }
 }
 
-template <int DIR, int RMN, int ODR, typename T>
+// FMA: build tag of the instantiation (see k_sweep2d_y)
+template <int DIR, int RMN, int ODR, typename T, int FMA = 0>
 __global__ __launch_bounds__(256, 2) void k_sweep3d_t(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
 {
   sweep3d_t<DIR, RMN, ODR, T>(p, g, bc, gam, nseg);
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(256, 2) void k_sweep3d_t(StagePtrs p, Geo g, BcFlag
 // ------------------------------------------------------------------ x sweep
 // One wavefront per 60-face segment of an x line; runs after the z and y
 // sweeps and applies the RK stage sum.
-template <int DIR, int RMN, int ODR, typename T>
+template <int DIR, int RMN, int ODR, typename T, int FMA = 0>
 __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc, double gam, int nseg)
 {
   constexpr int TA = (DIR + 1) % 3;  // first transverse axis (t)
