@@ -2,13 +2,14 @@
 //
 // Replaces the stage body of MHD3D::ideal (3D/MPI/OTvortex/mhd3d_solve.cpp:
 // 42-715) on one device-resident domain:
-//   prim3d    cell-centred B (fcen along x, y, z; ghosts by the bound()
-//             index map) and primitives at every cell        (:49-98)
 //   sweep<d>  for d = z, y, x: characteristic WCNS + Riemann flux on every
 //             line; z and y store dt/dz Dz and dt/dy Dy of ro, mx, my, mz,
 //             en, the x sweep applies the RK stage sum with the reference's
 //             ((-dtdx Dx) - dtdy Dy) - dtdz Dz; upwind/central parts of both
-//             tangential B fluxes stored per face          (:166-258, 330-422, 494-586, 658-676)
+//             tangential B fluxes stored per face          (:166-258, 330-422, 494-586, 658-676);
+//             the z sweep first computes the cell-centred B (fcen along x,
+//             y, z; ghosts by the bound() index map) and primitives of its
+//             LDS tile and stores them for the y and x sweeps (:49-98)
 //   emf3d     Ex, Ey, Ez at cell edges from the face parts (CUCT)
 //                                                            (:100-136, 259-328, 423-492, 587-656)
 //   ct3d      staggered bx, by, bz update                    (:677-709)
@@ -19,46 +20,8 @@
 
 namespace qas {
 
-// PT: storage type of the primitives (double, or float for the mixed path);
-// the arithmetic is FP64 either way.
-template <int ODR, typename PT>
-__global__ __launch_bounds__(256) void k_prim3d(StagePtrs p, Geo g, BcFlags bc, double gam)
-{
-  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= g.nd) return;
-  const int i = (int)(t % g.nx);
-  const int j = (int)((t / g.nx) % g.ny);
-  const int k = (int)(t / g.sz);
-  double c3[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    const double* b = p.in[4 + a];
-    const Src sx = resolve(i, g.nx, 0, bc.dn[0][4 + a]);
-    const Src sy = resolve(j, g.ny, 0, bc.dn[1][4 + a]);
-    const Src sz = resolve(k, g.nz, 0, bc.dn[2][4 + a]);
-    const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
-    const long long o = a == 0 ? 1 : (a == 1 ? g.sy : g.sz);
-    const double v = fcen<ODR>(b[q - o], b[q], b[q + o], b[q + 2 * o]);
-    c3[a] = apply(sz, apply(sy, apply(sx, v)));
-  }
-  const double ro = p.in[0][t];
-  const double iro = 1.0 / ro;
-  const double vx = p.in[1][t] * iro;
-  const double vy = p.in[2][t] * iro;
-  const double vz = p.in[3][t] * iro;
-  const double v2 = vx * vx + vy * vy + vz * vz;
-  const double b2 = c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2];
-  store_prim<PT>(p.pr, t, (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2)));
-  store_prim<PT>(p.vx, t, vx);
-  store_prim<PT>(p.vy, t, vy);
-  store_prim<PT>(p.vz, t, vz);
-  store_prim<PT>(p.cx, t, c3[0]);
-  store_prim<PT>(p.cy, t, c3[1]);
-  store_prim<PT>(p.cz, t, c3[2]);
-}
-
-// velocity of cell c from the state: the prim kernel's expression
-// (m * (1/ro)), so the FP64 value is bit-identical to it
+// velocity of cell c from the state: the primitives' expression
+// (m * (1/ro)), so the FP64 value is bit-identical to them
 struct Vel3 {
   const double *ro, *m;
   __device__ __forceinline__ double operator[](long long c) const { return m[c] * (1.0 / ro[c]); }
@@ -351,13 +314,6 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
   const double gam = c->s.gam;
   hipStream_t st = c->stream;
   const bool f32 = c->s.precision == 1;
-  if (odr <= 2) {
-    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<2, float>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<2, double>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-  } else {
-    if (f32) QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<4, float>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-    else QAS_LAUNCH(c, QAS_K_PRIM, (k_prim3d<4, double>), nblk3(g.nd, 256), 256, 0, st, p, g, c->bc, gam);
-  }
   const int dims[3] = {g.nx, g.ny, g.nz};
   sweep3_fn f[3] = {pick3<0>(c->s.rmn, odr), pick3<1>(c->s.rmn, odr), pick3<2>(c->s.rmn, odr)};
   if (c->s.precision == 1)

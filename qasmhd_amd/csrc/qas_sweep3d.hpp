@@ -34,6 +34,66 @@ __device__ __forceinline__ double dv3_face(const PrimView<VT>& pv, int i, int j,
   return dv3<A>(v, i, j, k, g, bc);
 }
 
+// the same from the state (velocity m (1/ro) rounded to VT, the stored
+// primitive's value), for the sweep that stores the primitives
+template <int A, typename VT>
+__device__ __forceinline__ double dv3_state(const StagePtrs& p, int i, int j, int k, const Geo& g, const BcFlags& bc)
+{
+  const VelOf<VT> v{p.in[1 + A], p.in[0]};
+  const int dnx = A == 0 ? -bc.dn[0][1] : bc.dn[0][1 + A];
+  const int dny = A == 1 ? -bc.dn[1][2] : bc.dn[1][1 + A];
+  const int dnz = A == 2 ? -bc.dn[2][3] : bc.dn[2][1 + A];
+  const Src sx = resolve(i, g.nx, 0, dnx);
+  const Src sy = resolve(j, g.ny, 0, dny);
+  const Src sz = resolve(k, g.nz, 0, dnz);
+  const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
+  const long long o = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
+  const VT r = tmin((VT)(v[q + o] - v[q]), (VT)(v[q] - v[q - o]));
+  return apply(sz, apply(sy, apply(sx, (double)r)));
+}
+
+// Cell-centred B (fcen along x, y, z on the staggered components, then the
+// bound() index map at ghost cells) and primitives of cell (i, j, k) = flat
+// index t from the stage input state (mhd_class.hpp:80-91, mhd_func.c:6-16;
+// mhd3d_solve.cpp:49-98), FP64 with the reference's operation order in every
+// precision (no FMA contraction here).  INNER: the caller knows the cell is
+// no ghost cell, where the index map is the identity.
+struct Prim3d {
+  double vx, vy, vz, pr, c[3];
+};
+
+template <int ODR, bool INNER>
+__device__ __forceinline__ Prim3d prim3d_cell(const StagePtrs& p, const Geo& g, const BcFlags& bc, double gam, int i,
+                                              int j, int k, long long t)
+{
+#pragma clang fp contract(off)
+  Prim3d r;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double* b = p.in[4 + a];
+    const long long o = a == 0 ? 1 : (a == 1 ? g.sy : g.sz);
+    if constexpr (INNER) {
+      r.c[a] = fcen<ODR>(b[t - o], b[t], b[t + o], b[t + 2 * o]);
+    } else {
+      const Src sx = resolve(i, g.nx, 0, bc.dn[0][4 + a]);
+      const Src sy = resolve(j, g.ny, 0, bc.dn[1][4 + a]);
+      const Src sz = resolve(k, g.nz, 0, bc.dn[2][4 + a]);
+      const long long q = g.sz * sz.i + g.sy * sy.i + sx.i;
+      const double v = fcen<ODR>(b[q - o], b[q], b[q + o], b[q + 2 * o]);
+      r.c[a] = apply(sz, apply(sy, apply(sx, v)));
+    }
+  }
+  const double ro = p.in[0][t];
+  const double iro = 1.0 / ro;
+  r.vx = p.in[1][t] * iro;
+  r.vy = p.in[2][t] * iro;
+  r.vz = p.in[3][t] * iro;
+  const double v2 = r.vx * r.vx + r.vy * r.vy + r.vz * r.vz;
+  const double b2 = r.c[0] * r.c[0] + r.c[1] * r.c[1] + r.c[2] * r.c[2];
+  r.pr = (gam - 1) * (p.in[7][t] - 0.5 * (ro * v2 + b2));
+  return r;
+}
+
 // ------------------------------------------------------------------ y, z sweeps
 // LDS-staged as the 2D y sweep (qas_sweep2d.hpp): one workgroup = 4
 // wavefronts = 4 adjacent x-columns of one line segment along DIR at one
@@ -69,7 +129,48 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
   const PrimView<T> pv = prim_view<T>(p);
   const T* vax[3] = {pv.vx, pv.vy, pv.vz};
   const T* cax[3] = {pv.cx, pv.cy, pv.cz};
-  {
+  if constexpr (DIR == 2) {
+    // The z sweep runs first: its staging computes the primitives of the
+    // tile's cells from the stage input state and rounds them once to T;
+    // the tile's own cells (its x-columns at this y, the z-rows of its
+    // segment, the first segment from row 0 and the last to row n-1) are
+    // stored for the y and x sweeps, so every cell is written once.
+    const int lo = seg == 0 ? 0 : f0;
+    const int hi = seg == nseg - 1 ? n : f0 + SEG;
+    const bool inner = (i0 >= NG) && (i0 + YC - 1 <= g.nx - NG - 1) && (f0 - 4 >= NG) &&
+                       (f0 + YR - 5 <= n - NG - 1) && (lb >= NG) && (lb <= nlb - NG - 1);
+    for (int e = threadIdx.x; e < YC * YR; e += blockDim.x) {
+      const int c = e % YC, r = e / YC;
+      const int rowu = f0 - 4 + r;
+      const int row = rowu < 0 ? 0 : (rowu > n - 1 ? n - 1 : rowu);
+      const int colu = i0 + c;
+      const int col = colu > g.nx - 1 ? g.nx - 1 : colu;
+      const long long a = blb + o * row + col;
+      const Prim3d q = inner ? prim3d_cell<ODR, true>(p, g, bc, gam, col, lb, row, a)
+                             : prim3d_cell<ODR, false>(p, g, bc, gam, col, lb, row, a);
+      const T v3[3] = {(T)q.vx, (T)q.vy, (T)q.vz};
+      const T c3[3] = {(T)q.c[0], (T)q.c[1], (T)q.c[2]};
+      const T pr = (T)q.pr;
+      sh[0][c][r] = p.in[0][a];
+      sh[1][c][r] = (double)v3[DIR];
+      sh[2][c][r] = (double)v3[TA];
+      sh[3][c][r] = (double)v3[TB];
+      sh[4][c][r] = (double)c3[TA];
+      sh[5][c][r] = (double)c3[TB];
+      sh[6][c][r] = (double)pr;
+      sh[7][c][r] = (double)c3[DIR];
+      sh[8][c][r] = p.in[4 + DIR][a];
+      if (colu < g.nx && rowu >= lo && rowu < hi) {
+        reinterpret_cast<T*>(p.vx)[a] = v3[0];
+        reinterpret_cast<T*>(p.vy)[a] = v3[1];
+        reinterpret_cast<T*>(p.vz)[a] = v3[2];
+        reinterpret_cast<T*>(p.pr)[a] = pr;
+        reinterpret_cast<T*>(p.cx)[a] = c3[0];
+        reinterpret_cast<T*>(p.cy)[a] = c3[1];
+        reinterpret_cast<T*>(p.cz)[a] = c3[2];
+      }
+    }
+  } else {
     const T* src[6] = {vax[DIR], vax[TA], vax[TB], cax[TA], cax[TB], pv.pr};
     for (int e = threadIdx.x; e < YC * YR; e += blockDim.x) {
       const int c = e % YC, r = e / YC;
@@ -134,10 +235,20 @@ __device__ __forceinline__ void sweep3d_t(const StagePtrs& p, const Geo& g, cons
     int c0[3], c1[3];
     c0[0] = lac; c0[LB] = lb; c0[DIR] = qf - 1;
     c1[0] = lac; c1[LB] = lb; c1[DIR] = qf;
-    const double a0 = dv3_face<TA>(pv, c0[0], c0[1], c0[2], g, bc);
-    const double a1 = dv3_face<TA>(pv, c1[0], c1[1], c1[2], g, bc);
-    const double b0 = dv3_face<TB>(pv, c0[0], c0[1], c0[2], g, bc);
-    const double b1 = dv3_face<TB>(pv, c1[0], c1[1], c1[2], g, bc);
+    double a0, a1, b0, b1;
+    if constexpr (DIR == 2) {
+      // the velocities of the neighbouring columns are being stored by other
+      // workgroups of this launch: from the state instead
+      a0 = dv3_state<TA, T>(p, c0[0], c0[1], c0[2], g, bc);
+      a1 = dv3_state<TA, T>(p, c1[0], c1[1], c1[2], g, bc);
+      b0 = dv3_state<TB, T>(p, c0[0], c0[1], c0[2], g, bc);
+      b1 = dv3_state<TB, T>(p, c1[0], c1[1], c1[2], g, bc);
+    } else {
+      a0 = dv3_face<TA>(pv, c0[0], c0[1], c0[2], g, bc);
+      a1 = dv3_face<TA>(pv, c1[0], c1[1], c1[2], g, bc);
+      b0 = dv3_face<TB>(pv, c0[0], c0[1], c0[2], g, bc);
+      b1 = dv3_face<TB>(pv, c1[0], c1[1], c1[2], g, bc);
+    }
     dv1 = (T)tmin(tmin(a0, a1), tmin(b0, b1));
   }
   riemann<RMN>(Lf, vr, bnf, (T)gam, dv0, dv1, F);
@@ -279,10 +390,20 @@ __global__ __launch_bounds__(256) void k_sweep3d(StagePtrs p, Geo g, BcFlags bc,
     int c0[3], c1[3];
     c0[LA] = la; c0[LB] = lb; c0[DIR] = qf - 1;
     c1[LA] = la; c1[LB] = lb; c1[DIR] = qf;
-    const double a0 = dv3_face<TA>(pv, c0[0], c0[1], c0[2], g, bc);
-    const double a1 = dv3_face<TA>(pv, c1[0], c1[1], c1[2], g, bc);
-    const double b0 = dv3_face<TB>(pv, c0[0], c0[1], c0[2], g, bc);
-    const double b1 = dv3_face<TB>(pv, c1[0], c1[1], c1[2], g, bc);
+    double a0, a1, b0, b1;
+    if constexpr (DIR == 2) {
+      // the velocities of the neighbouring columns are being stored by other
+      // workgroups of this launch: from the state instead
+      a0 = dv3_state<TA, T>(p, c0[0], c0[1], c0[2], g, bc);
+      a1 = dv3_state<TA, T>(p, c1[0], c1[1], c1[2], g, bc);
+      b0 = dv3_state<TB, T>(p, c0[0], c0[1], c0[2], g, bc);
+      b1 = dv3_state<TB, T>(p, c1[0], c1[1], c1[2], g, bc);
+    } else {
+      a0 = dv3_face<TA>(pv, c0[0], c0[1], c0[2], g, bc);
+      a1 = dv3_face<TA>(pv, c1[0], c1[1], c1[2], g, bc);
+      b0 = dv3_face<TB>(pv, c0[0], c0[1], c0[2], g, bc);
+      b1 = dv3_face<TB>(pv, c1[0], c1[1], c1[2], g, bc);
+    }
     dv1 = (T)tmin(tmin(a0, a1), tmin(b0, b1));
   }
   riemann<RMN>(L, vr, bnf, (T)gam, dv0, dv1, F);
