@@ -20,13 +20,6 @@
 
 namespace qas {
 
-// velocity of cell c from the state: the primitives' expression
-// (m * (1/ro)), so the FP64 value is bit-identical to them
-struct Vel3 {
-  const double *ro, *m;
-  __device__ __forceinline__ double operator[](long long c) const { return m[c] * (1.0 / ro[c]); }
-};
-
 // the face sweeps live in qas_sweep3d.hpp (shared with the mixed path)
 
 // ------------------------------------------------------------------ EMF
@@ -50,61 +43,20 @@ __device__ __forceinline__ double emf_part(const double* fu, const double* fc, l
   return sgn * ((ul0 + ur0) + w * (ul1 + ur1));
 }
 
-template <int ODR, typename FT>
-__global__ __launch_bounds__(256) void k_emf3d(StagePtrs p, Geo g, int ctw)
-{
-  // edges on [3, n-3]^3
-  const int ni = g.nx - 5, nj = g.ny - 5, nk = g.nz - 5;
-  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= (long long)ni * nj * nk) return;
-  const int i = 3 + (int)(t % ni);
-  const int j = 3 + (int)((t / ni) % nj);
-  const int k = 3 + (int)(t / ((long long)ni * nj));
-  const long long sy = g.sy, sz = g.sz;
-  const long long ss = sz * k + sy * j + i;
-  const double* ro = p.in[0];
-  const double *bx = p.in[4], *by = p.in[5], *bz = p.in[6];
-  const Vel3 vx{p.in[0], p.in[1]}, vy{p.in[0], p.in[2]}, vz{p.in[0], p.in[3]};
-  double ctz = 0.5, ctx = 0.5, cty = 0.5;
-  if (ctw) {
-    // x-y plane for Ez: strides (1, ny)
-    ctz = cuct_weight(ro[ss - 1 - sy], ro[ss - sy], ro[ss - 1], ro[ss], vx[ss - 1 - sy], vx[ss - sy], vx[ss - 1],
-                      vx[ss], vy[ss - 1 - sy], vy[ss - sy], vy[ss - 1], vy[ss], bx[ss - sy], bx[ss], by[ss - 1],
-                      by[ss]);
-    // y-z plane for Ex: strides (sy, sz)
-    ctx = cuct_weight(ro[ss - sy - sz], ro[ss - sz], ro[ss - sy], ro[ss], vy[ss - sy - sz], vy[ss - sz], vy[ss - sy],
-                      vy[ss], vz[ss - sy - sz], vz[ss - sz], vz[ss - sy], vz[ss], by[ss - sz], by[ss], bz[ss - sy],
-                      bz[ss]);
-    // z-x plane for Ey: strides (sz, 1)
-    cty = cuct_weight(ro[ss - sz - 1], ro[ss - 1], ro[ss - sz], ro[ss], vz[ss - sz - 1], vz[ss - 1], vz[ss - sz],
-                      vz[ss], vx[ss - sz - 1], vx[ss - 1], vx[ss - sz], vx[ss], bz[ss - 1], bz[ss], bx[ss - sz],
-                      bx[ss]);
-  }
-  // Ez: x sweep B_y (t) along y, then y sweep B_x (u) along x
-  double ez = 0.0;
-  ez += emf_part<ODR, FT>(p.emf[0], p.emf[1], ss, sy, 1.0 - ctz, -0.5);
-  ez += emf_part<ODR, FT>(p.emf[6], p.emf[7], ss, 1, ctz, +0.5);
-  // Ey: x sweep B_z (u) along z, then z sweep B_x (t) along x
-  double ey = 0.0;
-  ey += emf_part<ODR, FT>(p.emf[2], p.emf[3], ss, sz, cty, +0.5);
-  ey += emf_part<ODR, FT>(p.emf[8], p.emf[9], ss, 1, 1.0 - cty, -0.5);
-  // Ex: y sweep B_z (t) along z, then z sweep B_y (u) along y
-  double ex = 0.0;
-  ex += emf_part<ODR, FT>(p.emf[4], p.emf[5], ss, sz, 1.0 - ctx, -0.5);
-  ex += emf_part<ODR, FT>(p.emf[10], p.emf[11], ss, sy, ctx, +0.5);
-  p.ex[ss] = ex;
-  p.ey[ss] = ey;
-  p.ez[ss] = ez;
-}
-
-// The same, marching along z: one thread per (i, j) edge column of a chunk
+// Edge EMFs Ex, Ey, Ez (CUCT, mhd3d_solve.cpp:100-136, 259-328, 423-492,
+// 587-656) marching along z: one thread per (i, j) edge column of a chunk
 // of EMF_KC planes.  The four z-interpolated face-part arrays (Ey's and
 // Ex's contributions from the x and y sweeps' partner sweeps along z) are
 // kept as 6-point register windows shifted one plane per step, and the cell
 // values of plane k-1 the CUCT weights of Ex and Ey need are carried from
 // the previous step, so each is read from memory once per point instead of
-// 6 (resp. 2) times from planes far apart.  Same values, same expressions:
-// bit-identical to k_emf3d.
+// 6 (resp. 2) times from planes far apart.  Per edge the operations and
+// their order are the reference's.
+// FT: the sweep precision.  FP64: the CUCT velocities are m (1/ro) of the
+// state (the primitives' expression) and the edge EMFs are stored in FP64.
+// Mixed (FT = float): the velocities are the FP32 primitives the z sweep
+// stored, and the edge EMFs -- built from FP32 face parts -- are stored in
+// FP32 for the CT kernel, which halves both kernels' traffic in them.
 constexpr int EMF_KC = 32;     // planes per thread
 constexpr int EMF_TX = 64;     // tile: 64 x 4 columns per 256-thread workgroup
 constexpr int EMF_TY = 4;
@@ -140,7 +92,13 @@ __global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
   const double* ro = p.in[0];
   const double *mx = p.in[1], *my = p.in[2], *mz = p.in[3];
   const double *bx = p.in[4], *by = p.in[5], *bz = p.in[6];
-  auto vel = [&](const double* m, long long c) { return m[c] * (1.0 / ro[c]); };
+  constexpr bool F32 = sizeof(FT) == 4;
+  const float *fvx = reinterpret_cast<const float*>(p.vx), *fvy = reinterpret_cast<const float*>(p.vy),
+              *fvz = reinterpret_cast<const float*>(p.vz);
+  auto vel = [&](const double* m, const float* fv, long long c) -> double {
+    if constexpr (F32) return (double)fv[c];
+    else return m[c] * (1.0 / ro[c]);
+  };
   // z windows at planes k-3 .. k+2
   double u2[6], c2[6], u4[6], c4[6];
   {
@@ -160,9 +118,9 @@ __global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
   if (ctw) {
     const long long s = sz * (k0 - 1) + sy * j + i;
     ro_p = ro[s]; ro_pi = ro[s - 1]; ro_pj = ro[s - sy];
-    vx_p = vel(mx, s); vx_pi = vel(mx, s - 1);
-    vy_p = vel(my, s); vy_pj = vel(my, s - sy);
-    vz_p = vel(mz, s); vz_pi = vel(mz, s - 1); vz_pj = vel(mz, s - sy);
+    vx_p = vel(mx, fvx, s); vx_pi = vel(mx, fvx, s - 1);
+    vy_p = vel(my, fvy, s); vy_pj = vel(my, fvy, s - sy);
+    vz_p = vel(mz, fvz, s); vz_pi = vel(mz, fvz, s - 1); vz_pj = vel(mz, fvz, s - sy);
     bx_p = bx[s]; by_p = by[s];
   }
   for (int k = k0; k < k1; k++) {
@@ -180,11 +138,11 @@ __global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
     if (ctw) {
       ro0 = ro[ss]; roi = ro[ss - 1]; roj = ro[ss - sy];
       const double roij = ro[ss - 1 - sy];
-      vx0 = vel(mx, ss); vxi = vel(mx, ss - 1);
-      const double vxj = vel(mx, ss - sy), vxij = vel(mx, ss - 1 - sy);
-      vy0 = vel(my, ss); vyj = vel(my, ss - sy);
-      const double vyi = vel(my, ss - 1), vyij = vel(my, ss - 1 - sy);
-      vz0 = vel(mz, ss); vzi = vel(mz, ss - 1); vzj = vel(mz, ss - sy);
+      vx0 = vel(mx, fvx, ss); vxi = vel(mx, fvx, ss - 1);
+      const double vxj = vel(mx, fvx, ss - sy), vxij = vel(mx, fvx, ss - 1 - sy);
+      vy0 = vel(my, fvy, ss); vyj = vel(my, fvy, ss - sy);
+      const double vyi = vel(my, fvy, ss - 1), vyij = vel(my, fvy, ss - 1 - sy);
+      vz0 = vel(mz, fvz, ss); vzi = vel(mz, fvz, ss - 1); vzj = vel(mz, fvz, ss - sy);
       bx0 = bx[ss]; by0 = by[ss];
       // x-y plane for Ez: strides (1, ny)
       ctz = cuct_weight(roij, roj, roi, ro0, vxij, vxj, vxi, vx0, vyij, vyj, vyi, vy0, bx[ss - sy], bx0, by[ss - 1],
@@ -208,9 +166,9 @@ __global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
     double ex = 0.0;
     ex += emf_part_w<ODR>(u4, c4, 1.0 - ctx, -0.5);
     ex += emf_part<ODR, FT>(p.emf[10], p.emf[11], ss, sy, ctx, +0.5);
-    p.ex[ss] = ex;
-    p.ey[ss] = ey;
-    p.ez[ss] = ez;
+    store_as<FT>(p.ex, ss, ex);
+    store_as<FT>(p.ey, ss, ey);
+    store_as<FT>(p.ez, ss, ez);
     // next plane: shift the windows, carry this plane's values
 #pragma unroll
     for (int m = 0; m < 5; m++) {
@@ -236,7 +194,8 @@ __global__ __launch_bounds__(256) void k_emf3d_z(StagePtrs p, Geo g, int ctw)
 // operations and their order are those of the reference.
 constexpr int CT_KC = 32;
 
-template <int ODR>
+// FT: element type of the stored edge EMFs (the sweep precision)
+template <int ODR, typename FT>
 __global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
 {
   const int ni = g.nx - 7, nj = g.ny - 7, nk = g.nz - 7;  // cells [4, n-4]
@@ -252,7 +211,8 @@ __global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
   const long long c0 = sy * j + i;
   const double dtv = *p.dt;
   const double dtdx = dtv / g.dx, dtdy = dtv / g.dy, dtdz = dtv / g.dz;
-  const double *ex = p.ex, *ey = p.ey, *ez = p.ez;
+  const FT *ex = reinterpret_cast<const FT*>(p.ex), *ey = reinterpret_cast<const FT*>(p.ey),
+           *ez = reinterpret_cast<const FT*>(p.ez);
   const bool ji = j <= g.ny - 5, ii = i <= g.nx - 5;
   // windows: planes k-1 .. k+2 of ey and ex at (i, j); planes [3, nz-2]
   double eyw[4], exw[4];
@@ -267,18 +227,18 @@ __global__ __launch_bounds__(256) void k_ct3d(StagePtrs p, Geo g)
     exw[3] = ex[ss + 2 * sz];
     const bool ki = k <= g.nz - 5;
     if (ji && ki) {  // bx: one extra face in x
-      const double du0 = +dfc<ODR>(ez[ss - sy], ez[ss], ez[ss + sy], ez[ss + 2 * sy]);
+      const double du0 = +dfc<ODR>((double)ez[ss - sy], (double)ez[ss], (double)ez[ss + sy], (double)ez[ss + 2 * sy]);
       const double du1 = -dfc<ODR>(eyw[0], eyw[1], eyw[2], eyw[3]);
       p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du0 - dtdz * du1, p.a, p.b);
     }
     if (ii && ki) {  // by: one extra face in y
       const double du0 = +dfc<ODR>(exw[0], exw[1], exw[2], exw[3]);
-      const double du1 = -dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
+      const double du1 = -dfc<ODR>((double)ez[ss - 1], (double)ez[ss], (double)ez[ss + 1], (double)ez[ss + 2]);
       p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -dtdz * du0 - dtdx * du1, p.a, p.b);
     }
     if (ii && ji) {  // bz: one extra face in z
-      const double du0 = +dfc<ODR>(ey[ss - 1], eyw[1], ey[ss + 1], ey[ss + 2]);
-      const double du1 = -dfc<ODR>(ex[ss - sy], exw[1], ex[ss + sy], ex[ss + 2 * sy]);
+      const double du0 = +dfc<ODR>((double)ey[ss - 1], eyw[1], (double)ey[ss + 1], (double)ey[ss + 2]);
+      const double du1 = -dfc<ODR>((double)ex[ss - sy], exw[1], (double)ex[ss + sy], (double)ex[ss + 2 * sy]);
       p.out[6][ss] = rk_upd(p.in[6][ss], p.u0[6][ss], -dtdx * du0 - dtdy * du1, p.a, p.b);
     }
 #pragma unroll
@@ -346,7 +306,8 @@ hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p)
 #define QAS_EMF_CT(O)                                                                                      \
   if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d_z<O, float>), (unsigned)nemf, 256, 0, st, p, g, c->s.ctw);   \
   else QAS_LAUNCH(c, QAS_K_EMF, (k_emf3d_z<O, double>), (unsigned)nemf, 256, 0, st, p, g, c->s.ctw);       \
-  QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O>), (unsigned)nct, 256, 0, st, p, g);
+  if (f32) QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O, float>), (unsigned)nct, 256, 0, st, p, g);                    \
+  else QAS_LAUNCH(c, QAS_K_CT, (k_ct3d<O, double>), (unsigned)nct, 256, 0, st, p, g);
   switch (odr) {
     case 1: QAS_EMF_CT(1) break;
     case 2: QAS_EMF_CT(2) break;

@@ -90,3 +90,19 @@ def test_mixed_full_size_khi():
     e_pr = float(np.abs(pressure(b, p.gam)[I] - pressure(a, p.gam)[I]).mean())
     _say("  L1(rho) %.3e  L1(p) %.3e" % (e_ro, e_pr))
     assert e_ro <= 1e-6 and e_pr <= 1e-6
+
+
+def test_mixed_full_size_3d():
+    """The 3D turbulence workload at its full size and length (512^3, 200
+    steps at one fixed dt) in mixed precision against FP64: L1(rho) and
+    L1(p) at most 1e-6 (tests/gpu_mixed3d_full_check.py, fresh process)."""
+    import os
+    import subprocess
+    _say("case: mixed vs fp64 3D 512^3, 200 steps (subprocess)")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_mixed3d_full_check.py")
+    r = subprocess.run([sys.executable, "-u", script], capture_output=True, text=True, timeout=280)
+    _say(r.stdout[-3000:])
+    if r.returncode != 0:
+        _say(r.stderr[-3000:])
+    assert r.returncode == 0
+    assert "OK" in r.stdout
