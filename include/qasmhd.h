@@ -196,6 +196,17 @@ int qas_batch_set_shape(qas_batch *b, int order, int xwpb, int ewb);
 int qas_batch_shape(const qas_batch *b, int *order, int *xwpb, int *ewb);
 const char *qas_batch_last_error(const qas_batch *b);
 void qas_batch_destroy(qas_batch *b);
+/* Batched discrete adjoint (no counterpart in the reference): the adjoint
+ * entry points below for every domain of a batch at once, each kernel
+ * launched once for all domains (same scope as qas_adj_*, r_k 3).
+ * qas_batch_adj_forward(b, n, dts): every domain i advances n steps with
+ *   step sizes dts[i*n .. i*n+n-1], keeping its checkpoints (as
+ *   qas_adj_forward on its context).
+ * qas_batch_adj_backward(b, g_final, g_init): g_final[i], g_init[i] are host
+ *   arrays (packed 8 x nd_i) of domain i, as for qas_adj_backward; every
+ *   domain must hold the same number of recorded steps. */
+int qas_batch_adj_forward(qas_batch *b, int nsteps, const double *dts);
+int qas_batch_adj_backward(qas_batch *b, const double *const *g_final, double *const *g_init);
 
 
 /* ---- discrete adjoint (no counterpart in the reference) ----------------

@@ -41,7 +41,8 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_batch_tune", "qas_batch_set_shape", "qas_batch_shape",
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct", "qas_test_rcp",
-           "qas_set_dissipation", "qas_dsptv", "qas_set_gravity", "qas_run_static"]
+           "qas_set_dissipation", "qas_dsptv", "qas_set_gravity", "qas_run_static",
+           "qas_batch_adj_forward", "qas_batch_adj_backward"]
 
 _lib = None
 
@@ -104,6 +105,8 @@ def load(path: str | None = None):
     l.qas_set_dissipation.argtypes = [vp, dp, dp, C.c_double]
     l.qas_dsptv.argtypes = [vp, C.c_double]
     l.qas_set_gravity.argtypes = [vp, dp]
+    l.qas_batch_adj_forward.argtypes = [vp, C.c_int, dp]
+    l.qas_batch_adj_backward.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
     l.qas_run_static.argtypes = [C.POINTER(QasGrid), C.POINTER(QasScheme), vp, C.c_int, C.c_longlong, C.c_int,
                                  C.c_double, vp, vp, C.c_char_p, C.c_int]
     _lib = l
@@ -440,6 +443,34 @@ class Batch:
         if r != QAS_OK:
             raise QasError("qas_batch_run: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
         return hist[:, :nsteps], d
+
+    def adj_forward(self, dts):
+        """qas_adj_forward on every domain in one batched run: dts[i] are
+        domain i's step sizes (all the same length); each domain keeps its
+        checkpoints (qas_batch_adj_forward)."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        assert d.ndim == 2 and d.shape[0] == len(self.solvers)
+        for s in self.solvers:
+            s.adj_generation = getattr(s, "adj_generation", 0) + 1
+        r = self.lib.qas_batch_adj_forward(self.b, int(d.shape[1]), _p(d))
+        if r != QAS_OK:
+            raise QasError("qas_batch_adj_forward: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
+
+    def adj_backward(self, g_final):
+        """qas_adj_backward on every domain in one batched run: g_final[i]
+        the (8, ny, nx) adjoint of domain i's final state; returns the list
+        of initial-state adjoints (qas_batch_adj_backward)."""
+        n = len(self.solvers)
+        gf = [np.ascontiguousarray(g, dtype=np.float64) for g in g_final]
+        out = [np.empty_like(g) for g in gf]
+        for g, s in zip(gf, self.solvers):
+            assert g.shape == (8,) + s.shape
+        pf = (C.POINTER(C.c_double) * n)(*[_p(g) for g in gf])
+        po = (C.POINTER(C.c_double) * n)(*[_p(g) for g in out])
+        r = self.lib.qas_batch_adj_backward(self.b, pf, po)
+        if r != QAS_OK:
+            raise QasError("qas_batch_adj_backward: %s (%d)" % (self.lib.qas_batch_last_error(self.b).decode(), r))
+        return out
 
     def tune(self):
         """Time the candidate launch shapes and keep the fastest (qas_batch_tune)."""

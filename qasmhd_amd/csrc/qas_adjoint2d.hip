@@ -40,7 +40,9 @@
 #include <string>
 #include <vector>
 
+#include "qas_adjbatch.hpp"
 #include "qas_adjoint.hpp"
+#include "qas_batch.hpp"
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
 
@@ -196,9 +198,8 @@ __device__ __forceinline__ void corner_at(const AdjPtrs& p, const Geo& g, long l
 // ---------------------------------------------------------------- face parts
 // ez(c) = -1/2 [(uL+uR) + (1-w)(cL+cR)]_x + 1/2 [(uL+uR) + w (cL+cR)]_y with
 // the one-sided linear interpolations of k_emf2d
-__global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
+__device__ __forceinline__ void faceparts_at(const AdjPtrs& p, const Geo& g, long long t)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
   const long long nx = g.nx;
@@ -222,6 +223,11 @@ __global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
   p.gemf[1][t] = xc;
   p.gemf[2][t] = yu;
   p.gemf[3][t] = yc;
+}
+
+__global__ __launch_bounds__(256) void k_adj_faceparts(AdjPtrs p, Geo g)
+{
+  faceparts_at(p, g, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- sweeps
@@ -704,13 +710,18 @@ __device__ __forceinline__ void gin_face_at(const AdjPtrs& p, const Geo& g, doub
 // [0, nbh) the six variables of the primitive recovery, the rest face B
 // (two independent halves: shorter per-thread chains on small domains)
 template <int CTW, bool DV>
-__global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam, int nbh)
+__device__ __forceinline__ void gin_block(const AdjPtrs& p, const Geo& g, double gam, int nbh, int b)
 {
-  const int b = (int)blockIdx.x;
   const long long t = (long long)(b < nbh ? b : b - nbh) * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
   if (b < nbh) gin_prim_at<CTW, DV>(p, g, gam, t);
   else gin_face_at<CTW>(p, g, gam, t);
+}
+
+template <int CTW, bool DV>
+__global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam, int nbh)
+{
+  gin_block<CTW, DV>(p, g, gam, nbh, (int)blockIdx.x);
 }
 
 // grid (cells, 8 variables): per kept cell, gI folded over the cell's
@@ -719,11 +730,9 @@ __global__ __launch_bounds__(256) void k_adj_gin(AdjPtrs p, Geo g, double gam, i
 // own entry first; ghost entries stay zero).  The step-base adjoint takes
 // a * g_out (set by the first stage processed, added by the others); the
 // last one processed adds it to its result.
-__global__ __launch_bounds__(256) void k_adj_foldstage(AdjPtrs p, Geo g, int first, int last)
+__device__ __forceinline__ void foldstage_at(const AdjPtrs& p, const Geo& g, int first, int last, long long t, int m)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
-  const int m = blockIdx.y;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
   if (!kept(i, g.nx) || !kept(j, g.ny)) return;
   double v = fold_at(p.gI[m], i, j, g);
@@ -736,6 +745,11 @@ __global__ __launch_bounds__(256) void k_adj_foldstage(AdjPtrs p, Geo g, int fir
   p.gO[m][t] = v;
 }
 
+__global__ __launch_bounds__(256) void k_adj_foldstage(AdjPtrs p, Geo g, int first, int last)
+{
+  foldstage_at(p, g, first, last, (long long)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y);
+}
+
 // ---------------------------------------------------------------- fold
 struct FoldArgs {
   const double* src[8];
@@ -744,11 +758,9 @@ struct FoldArgs {
 };
 
 // grid: (cells, 8 variables)
-__global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
+__device__ __forceinline__ void fold_cell(const FoldArgs& f, const Geo& g, long long t, int m)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nd) return;
-  const int m = blockIdx.y;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
   double v = 0.0;
   if (kept(i, g.nx) && kept(j, g.ny)) {
@@ -756,6 +768,11 @@ __global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
     if (f.extra[m]) v += f.extra[m][t];
   }
   f.dst[m][t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_adj_fold(FoldArgs f, Geo g)
+{
+  fold_cell(f, g, (long long)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y);
 }
 
 // ---------------------------------------------------------------- transposes
@@ -793,9 +810,8 @@ __device__ __forceinline__ void transpose_tile(const TrArgs& a, int nx, int ny, 
 // corner), the rest transpose the arrays the y line pass reads (they do not
 // depend on the corner outputs).
 template <int CTW>
-__global__ __launch_bounds__(256) void k_adj_corner_tr(AdjPtrs p, Geo g, TrArgs a, int nbc)
+__device__ __forceinline__ void corner_tr_block(const AdjPtrs& p, const Geo& g, const TrArgs& a, int nbc, int b)
 {
-  const int b = (int)blockIdx.x;
   if (b < nbc) {
     corner_at<CTW>(p, g, (long long)b * blockDim.x + threadIdx.x);
     return;
@@ -806,13 +822,19 @@ __global__ __launch_bounds__(256) void k_adj_corner_tr(AdjPtrs p, Geo g, TrArgs 
   transpose_tile(a, g.nx, g.ny, r % ntx, r / ntx, bz);
 }
 
-__global__ __launch_bounds__(256) void k_adj_untranspose_add(TrArgs a, int nx, int ny)
+template <int CTW>
+__global__ __launch_bounds__(256) void k_adj_corner_tr(AdjPtrs p, Geo g, TrArgs a, int nbc)
+{
+  corner_tr_block<CTW>(p, g, a, nbc, (int)blockIdx.x);
+}
+
+__device__ __forceinline__ void untranspose_tile(const TrArgs& a, int nx, int ny, int bx, int by, int bz)
 {
   __shared__ double tile[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-  const double* s = a.src[blockIdx.z];
-  double* d = a.dst[blockIdx.z];
+  const int i0 = bx * 32, j0 = by * 32;
+  const double* s = a.src[bz];
+  double* d = a.dst[bz];
   for (int r = ty; r < 32; r += 8) {
     const int j = j0 + tx, i = i0 + r;
     if (i < nx && j < ny) tile[r][tx] = s[(long long)i * ny + j];
@@ -821,11 +843,104 @@ __global__ __launch_bounds__(256) void k_adj_untranspose_add(TrArgs a, int nx, i
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     if (i < nx && j < ny) {
-      if ((int)blockIdx.z < a.nadd) d[(long long)j * nx + i] += tile[tx][r];
+      if (bz < a.nadd) d[(long long)j * nx + i] += tile[tx][r];
       else d[(long long)j * nx + i] = tile[tx][r];
     }
   }
 }
+
+__global__ __launch_bounds__(256) void k_adj_untranspose_add(TrArgs a, int nx, int ny)
+{
+  untranspose_tile(a, nx, ny, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
+// ---------------------------------------------------------------- batches
+// Batched stage adjoints (qas_batch_adj_backward): one launch per kernel for
+// every instance of a batch.  Instance i owns the workgroups [off[i],
+// off[i+1]) of a launch (per-kernel prefix sums built on the host), finds
+// itself by binary search on its uniform workgroup index and runs the
+// single-domain body with its own descriptor and local workgroup index.
+struct AdjInst {
+  AdjPtrs st[3];   // stage rk: the x pass and the element-wise kernels
+  AdjPtrs tst[3];  // stage rk: the y pass on the transposed copies
+  TrArgs tin[3];   // stage rk: the arrays the y pass reads, to transpose
+  TrArgs tout;     // the y pass's outputs, added back
+  FoldArgs fin;    // the staged final-state adjoint, folded into gO
+  Geo g;
+  int nb;          // cell workgroups (256 cells)
+  int nbx, csx, csy;  // x line-pass workgroups; segments per x and y line
+  int ntt;         // 32 x 32 tiles of one array
+};
+enum AdjBatchKernel { KA_FOLD = 0, KA_CORNER, KA_FACE, KA_SWEEP, KA_UNTR, KA_GIN, KA_FSTAGE, KA_COUNT };
+
+#define QAS_ABI                                                                                         \
+  const int b_ = (int)blockIdx.x, i_ = batch_find(off, n, b_), l = b_ - off[i_];                          \
+  const AdjInst& I_ = I[i_];
+
+// (the descriptors are used in place: a local copy indexed by a variable
+// or array number would live in scratch memory)
+__global__ __launch_bounds__(256) void kba_fold(const AdjInst* I, const int* off, int n)
+{
+  QAS_ABI
+  const Geo g = I_.g;
+  fold_cell(I_.fin, g, (long long)(l % I_.nb) * blockDim.x + threadIdx.x, l / I_.nb);
+}
+
+template <int CTW>
+__global__ __launch_bounds__(256) void kba_corner_tr(const AdjInst* I, const int* off, int n, int rk)
+{
+  QAS_ABI
+  const Geo g = I_.g;
+  corner_tr_block<CTW>(I_.st[rk], g, I_.tin[rk], I_.nb, l);
+}
+
+__global__ __launch_bounds__(256) void kba_faceparts(const AdjInst* I, const int* off, int n, int rk)
+{
+  QAS_ABI
+  const AdjPtrs p = I_.st[rk];
+  const Geo g = I_.g;
+  faceparts_at(p, g, (long long)l * blockDim.x + threadIdx.x);
+}
+
+template <int RMN>
+__global__ __launch_bounds__(256, 1) void kba_sweep_xy(const AdjInst* I, const int* off, int n, int rk, double gam)
+{
+  QAS_ABI
+  const Geo g = I_.g;
+  if (l < I_.nbx) {
+    const AdjPtrs p = I_.st[rk];
+    adj_sweep_body<RMN, 0, false>(p, g, gam, I_.csx, l);
+  } else {
+    const AdjPtrs p = I_.tst[rk];
+    adj_sweep_body<RMN, 1, true>(p, g, gam, I_.csy, l - I_.nbx);
+  }
+}
+
+__global__ __launch_bounds__(256) void kba_untranspose(const AdjInst* I, const int* off, int n)
+{
+  QAS_ABI
+  const int ntx = (I_.g.nx + 31) / 32;
+  const int bz = l / I_.ntt, r = l - bz * I_.ntt;
+  untranspose_tile(I_.tout, I_.g.nx, I_.g.ny, r % ntx, r / ntx, bz);
+}
+
+template <int CTW, bool DV>
+__global__ __launch_bounds__(256) void kba_gin(const AdjInst* I, const int* off, int n, int rk, double gam)
+{
+  QAS_ABI
+  const AdjPtrs p = I_.st[rk];
+  const Geo g = I_.g;
+  gin_block<CTW, DV>(p, g, gam, I_.nb, l);
+}
+
+__global__ __launch_bounds__(256) void kba_foldstage(const AdjInst* I, const int* off, int n, int rk, int first,
+                                                     int last)
+{
+  QAS_ABI
+  const Geo g = I_.g;
+  foldstage_at(I_.st[rk], g, first, last, (long long)(l % I_.nb) * blockDim.x + threadIdx.x, l / I_.nb);
+}
+#undef QAS_ABI
 
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -853,7 +968,8 @@ struct qas_adj_work {
   double* tin[NTR] = {};      // transposed inputs of the y pass
   double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
-  double* d_dts = nullptr;    // the recorded step sizes (device)
+  double* d_dts = nullptr;    // the recorded step sizes (device), ck_cap + 1 entries
+  double* dslot = nullptr;    // d_dts[ck_cap]: the step size a batched backward reads
   double* stage = nullptr;    // qas_adj_backward's device copies of g_final, g_init
   const double* cur_dt = nullptr;  // d_dts + the step being reversed
   int ck_cap = 0;
@@ -1064,7 +1180,13 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
   return hipGetLastError();
 }
 
-extern "C" int qas_adj_forward(qas_ctx* c, int nsteps, const double* dts)
+namespace qas {
+std::string adj_scope(const qas_ctx* c) { return adj_supported(c); }
+
+// Set-up of a recorded forward of nsteps steps with the given step sizes:
+// scope check, workspace, checkpoint room, step sizes on the device,
+// gradient staging; no step recorded yet.
+int adj_prepare(qas_ctx* c, int nsteps, const double* dts)
 {
   if (!c || !c->slab) return afail(c, QAS_E_ARG, "qas_adj_forward: null context");
   if (!c->uploaded) return afail(c, QAS_E_STATE, "qas_adj_forward: no state uploaded");
@@ -1085,14 +1207,201 @@ extern "C" int qas_adj_forward(qas_ctx* c, int nsteps, const double* dts)
     w->ck_cap = 0;
     hipError_t e = hipMalloc(&w->ck, per * (size_t)nsteps * sizeof(double));
     if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint checkpoints: ") + hipGetErrorName(e));
-    e = hipMalloc(&w->d_dts, (size_t)nsteps * sizeof(double));
+    e = hipMalloc(&w->d_dts, (size_t)(nsteps + 1) * sizeof(double));
     if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint step sizes: ") + hipGetErrorName(e));
     w->ck_cap = nsteps;
+  }
+  w->dslot = w->d_dts + w->ck_cap;
+  if (!w->stage) {  // device staging of the two gradients, kept with the workspace
+    hipError_t e = hipMalloc(&w->stage, 2 * 8 * (size_t)c->geo.nd * sizeof(double));
+    if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint staging: ") + hipGetErrorName(e));
   }
   w->dts.assign(dts, dts + nsteps);
   QAS_ACHECK(c, hipMemcpyAsync(w->d_dts, w->dts.data(), (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice,
                                c->stream));
   w->nrec = 0;
+  return QAS_OK;
+}
+
+void adj_bufs(const qas_ctx* c, AdjBufs* b)
+{
+  const qas_adj_work* w = c->adj;
+  b->ck = w->ck;
+  b->stride = w->stride;
+  b->d_dts = w->d_dts;
+  b->dslot = w->dslot;
+  for (int s = 0; s < 3; s++) {
+    for (int m = 0; m < 8; m++) b->S[s][m] = w->S[s][m];
+    for (int k = 0; k < 6; k++) b->prim[s][k] = w->prim[s][k];
+    for (int k = 0; k < 8; k++) b->emf[s][k] = w->emf[s][k];
+    b->ez[s] = w->ez[s];
+  }
+  for (int m = 0; m < 8; m++) b->gO[m] = w->gO[m];
+  b->stage = w->stage;
+}
+
+void adj_set_recorded(qas_ctx* c, int nsteps) { c->adj->nrec = nsteps; }
+int adj_recorded(const qas_ctx* c) { return c->adj ? c->adj->nrec : 0; }
+
+// ---- batched stage adjoints
+struct AdjBatchDev {
+  int n = 0;
+  AdjInst* d_inst = nullptr;
+  int* d_off = nullptr;  // KA_COUNT tables of n + 1 offsets
+  int nblk[KA_COUNT] = {};
+  int rmn = 1, ctw = 1;
+  double gam = 0.0;
+};
+
+void adjb_destroy(AdjBatchDev* d)
+{
+  if (!d) return;
+  if (d->d_inst) (void)hipFree(d->d_inst);
+  if (d->d_off) (void)hipFree(d->d_off);
+  delete d;
+}
+
+int adjb_create(AdjBatchDev** out, qas_ctx* const* ctxs, double* const* ub, int n, hipStream_t st, std::string* err)
+{
+  *out = nullptr;
+  AdjBatchDev* d = new AdjBatchDev();
+  d->n = n;
+  d->rmn = ctxs[0]->s.rmn;
+  d->ctw = ctxs[0]->s.ctw;
+  d->gam = ctxs[0]->s.gam;
+  std::vector<AdjInst> inst(n);
+  std::vector<int> off((size_t)KA_COUNT * (n + 1), 0);
+  const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
+  for (int i = 0; i < n; i++) {
+    qas_ctx* c = ctxs[i];
+    qas_adj_work* w = c->adj;
+    AdjInst& A = inst[i];
+    std::memset(&A, 0, sizeof(A));
+    const Geo& g = c->geo;
+    A.g = g;
+    for (int rk = 0; rk < 3; rk++) {
+      AdjPtrs& p = A.st[rk];
+      for (int m = 0; m < 8; m++) {
+        p.in[m] = rk == 2 ? w->S[1][m] : (rk == 1 ? w->S[0][m] : ub[8 * (size_t)i + m]);
+        p.gO[m] = w->gO[m];
+        p.gU0[m] = w->gU0[m];
+        p.gI[m] = w->gI[m];
+        p.gp[m] = w->gp[m];
+        p.emf[m] = w->emf[rk][m];
+      }
+      p.vx = w->prim[rk][0]; p.vy = w->prim[rk][1]; p.vz = w->prim[rk][2]; p.pr = w->prim[rk][3];
+      p.cx = w->prim[rk][4]; p.cy = w->prim[rk][5];
+      for (int k = 0; k < 2; k++) p.gbf[k] = w->gbf[k];
+      for (int k = 0; k < 4; k++) p.gemf[k] = w->gemf[k];
+      for (int k = 0; k < 2; k++) p.gdv[k] = w->gdv[k];
+      p.gez = w->gez;
+      p.ctv = w->ctv;
+      for (int k = 0; k < 5; k++) p.gcc[k] = w->gcc[k];
+      p.dt = w->dslot;
+      p.a = ra[rk];
+      p.b = rb[rk];
+      // the y pass's view on the transposed copies (as adj_stage)
+      AdjPtrs pt = p;
+      pt.in[0] = w->tin[0];
+      pt.in[5] = w->tin[1];
+      pt.in[6] = w->tin[2];
+      pt.vx = w->tin[3];
+      pt.vy = w->tin[4];
+      pt.vz = w->tin[5];
+      pt.cx = w->tin[6];
+      pt.cy = w->tin[7];
+      pt.pr = w->tin[8];
+      const int gom[6] = {0, 1, 2, 3, 6, 7};
+      for (int v = 0; v < 6; v++) pt.gO[gom[v]] = w->tin[9 + v];
+      for (int m = 0; m < 8; m++) pt.gp[m] = w->tout[m];
+      pt.gbf[1] = w->tout[8];
+      A.tst[rk] = pt;
+      const double* src[NTR] = {p.in[0], p.in[5], p.in[6], p.vx, p.vy, p.vz, p.cx, p.cy, p.pr,
+                                p.gO[0], p.gO[1], p.gO[2], p.gO[3], p.gO[6], p.gO[7]};
+      for (int k = 0; k < NTR; k++) {
+        A.tin[rk].src[k] = src[k];
+        A.tin[rk].dst[k] = w->tin[k];
+      }
+      A.tin[rk].nadd = 0;
+    }
+    for (int m = 0; m < 8; m++) {
+      A.tout.src[m] = w->tout[m];
+      A.tout.dst[m] = w->gp[m];
+    }
+    A.tout.src[8] = w->tout[8];
+    A.tout.dst[8] = w->gbf[1];
+    A.tout.nadd = 8;
+    for (int m = 0; m < 8; m++) {
+      A.fin.src[m] = w->stage + (size_t)m * g.nd;
+      A.fin.dst[m] = w->gO[m];
+      A.fin.extra[m] = nullptr;
+    }
+    A.nb = (int)nblk(g.nd, 256);
+    A.csx = (g.nx + SEGA - 1) / SEGA;
+    A.csy = (g.ny + SEGA - 1) / SEGA;
+    A.nbx = (int)nblk((long long)A.csx * g.ny * WAVE, 256);
+    const int nby = (int)nblk((long long)A.csy * g.nx * WAVE, 256);
+    A.ntt = ((g.nx + 31) / 32) * ((g.ny + 31) / 32);
+    const int cnt[KA_COUNT] = {A.nb * 8, A.nb + A.ntt * NTR, A.nb, A.nbx + nby, A.ntt * 9, 2 * A.nb, A.nb * 8};
+    for (int k = 0; k < KA_COUNT; k++) off[(size_t)k * (n + 1) + i + 1] = off[(size_t)k * (n + 1) + i] + cnt[k];
+  }
+  for (int k = 0; k < KA_COUNT; k++) d->nblk[k] = off[(size_t)k * (n + 1) + n];
+  hipError_t e = hipMalloc(&d->d_inst, sizeof(AdjInst) * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc(&d->d_off, sizeof(int) * off.size());
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d->d_inst, inst.data(), sizeof(AdjInst) * (size_t)n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d->d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    if (err) *err = std::string("batched adjoint tables: ") + hipGetErrorName(e);
+    adjb_destroy(d);
+    return e == hipErrorOutOfMemory ? QAS_E_NOMEM : QAS_E_HIP;
+  }
+  *out = d;
+  return QAS_OK;
+}
+
+static const int* aboff(const AdjBatchDev* d, int k) { return d->d_off + (size_t)k * (d->n + 1); }
+
+hipError_t adjb_fold_final(AdjBatchDev* d, hipStream_t st)
+{
+  hipLaunchKernelGGL(kba_fold, (unsigned)d->nblk[KA_FOLD], 256, 0, st, d->d_inst, aboff(d, KA_FOLD), d->n);
+  return hipGetLastError();
+}
+
+hipError_t adjb_stage(AdjBatchDev* d, int rk, bool last, hipStream_t st)
+{
+  const AdjInst* I = d->d_inst;
+  const int n = d->n;
+  if (d->ctw) hipLaunchKernelGGL(kba_corner_tr<1>, (unsigned)d->nblk[KA_CORNER], 256, 0, st, I, aboff(d, KA_CORNER), n, rk);
+  else hipLaunchKernelGGL(kba_corner_tr<0>, (unsigned)d->nblk[KA_CORNER], 256, 0, st, I, aboff(d, KA_CORNER), n, rk);
+  hipLaunchKernelGGL(kba_faceparts, (unsigned)d->nblk[KA_FACE], 256, 0, st, I, aboff(d, KA_FACE), n, rk);
+  if (d->rmn == 3)
+    hipLaunchKernelGGL(kba_sweep_xy<3>, (unsigned)d->nblk[KA_SWEEP], 256, 0, st, I, aboff(d, KA_SWEEP), n, rk, d->gam);
+  else if (d->rmn == 2)
+    hipLaunchKernelGGL(kba_sweep_xy<2>, (unsigned)d->nblk[KA_SWEEP], 256, 0, st, I, aboff(d, KA_SWEEP), n, rk, d->gam);
+  else
+    hipLaunchKernelGGL(kba_sweep_xy<1>, (unsigned)d->nblk[KA_SWEEP], 256, 0, st, I, aboff(d, KA_SWEEP), n, rk, d->gam);
+  hipLaunchKernelGGL(kba_untranspose, (unsigned)d->nblk[KA_UNTR], 256, 0, st, I, aboff(d, KA_UNTR), n);
+  const bool dv = d->rmn == 2 || d->rmn == 3;
+  const unsigned ng = (unsigned)d->nblk[KA_GIN];
+  const int* og = aboff(d, KA_GIN);
+  if (d->ctw && dv) hipLaunchKernelGGL((kba_gin<1, true>), ng, 256, 0, st, I, og, n, rk, d->gam);
+  else if (d->ctw) hipLaunchKernelGGL((kba_gin<1, false>), ng, 256, 0, st, I, og, n, rk, d->gam);
+  else if (dv) hipLaunchKernelGGL((kba_gin<0, true>), ng, 256, 0, st, I, og, n, rk, d->gam);
+  else hipLaunchKernelGGL((kba_gin<0, false>), ng, 256, 0, st, I, og, n, rk, d->gam);
+  hipLaunchKernelGGL(kba_foldstage, (unsigned)d->nblk[KA_FSTAGE], 256, 0, st, I, aboff(d, KA_FSTAGE), n, rk,
+                     rk == 2 ? 1 : 0, last ? 1 : 0);
+  return hipGetLastError();
+}
+}  // namespace qas
+
+extern "C" int qas_adj_forward(qas_ctx* c, int nsteps, const double* dts)
+{
+  int r = adj_prepare(c, nsteps, dts);
+  if (r) return r;
+  qas_adj_work* w = c->adj;
+  const size_t per = 8 * w->stride;
   if (!c->bounded) {
     QAS_ACHECK(c, launch_bound(c, c->U[c->cur], 8));
     c->bounded = true;

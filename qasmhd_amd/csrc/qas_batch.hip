@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "qas_adjbatch.hpp"
 #include "qas_batch.hpp"
 #include "qas_common.hpp"
 #include "qas_device.hpp"
@@ -96,6 +97,43 @@ __global__ void kb_get_dt(const BatchInst* I, int n, double* out)
 {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < n) out[i] = *I[i].dt;
+}
+
+// ---- batched adjoint: per-instance buffers of a recorded forward
+struct AdjStepInst {
+  double* ck;           // checkpoints: step n, variable m at ck + (8 n + m) * stride
+  long long stride, nd;
+  const double* dts;    // recorded step sizes
+  double* dslot;        // the step size the backward's kernels read
+  double* dt;           // the context's dt (read by the forward stages)
+  double* cur[8];       // the context's state (forward)
+  double* ub[8];        // the backward's step input buffer
+};
+
+// forward of step `step`: checkpoint of every instance's state and its dt
+__global__ __launch_bounds__(256) void kb_adj_ckpt(const AdjStepInst* A, const int* off, int n, int step)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const AdjStepInst& a = A[i];
+  const long long e = (long long)(b - off[i]) * blockDim.x + threadIdx.x;
+  if (e == 0) *a.dt = a.dts[step];
+  if (e >= 8 * a.nd) return;
+  const int m = (int)(e / a.nd);
+  const long long q = e - (long long)m * a.nd;
+  a.ck[(8LL * step + m) * a.stride + q] = a.cur[m][q];
+}
+
+// backward of step `step`: its input state and dt into the fixed buffers
+__global__ __launch_bounds__(256) void kb_adj_load(const AdjStepInst* A, const int* off, int n, int step)
+{
+  const int b = (int)blockIdx.x, i = batch_find(off, n, b);
+  const AdjStepInst& a = A[i];
+  const long long e = (long long)(b - off[i]) * blockDim.x + threadIdx.x;
+  if (e == 0) *a.dslot = a.dts[step];
+  if (e >= 8 * a.nd) return;
+  const int m = (int)(e / a.nd);
+  const long long q = e - (long long)m * a.nd;
+  a.ub[m][q] = a.ck[(8LL * step + m) * a.stride + q];
 }
 
 typedef void (*emf_fn)(const BatchInst*, const int*, int, int, int);
@@ -220,10 +258,10 @@ static int fill_table(qas_batch* b)
 
 static const int* boff(const qas_batch* b, int k) { return b->d_off + (size_t)k * (b->n + 1); }
 
-static hipError_t launch_bound_all(qas_batch* b, int stage)
+static hipError_t launch_bound_all(qas_batch* b, int stage, const BatchInst* tbl = nullptr)
 {
-  hipLaunchKernelGGL(kb_bound, (unsigned)b->nblk[BK_BOUND], 256, 0, b->stream, b->d_inst, boff(b, BK_BOUND), b->n,
-                     stage);
+  hipLaunchKernelGGL(kb_bound, (unsigned)b->nblk[BK_BOUND], 256, 0, b->stream, tbl ? tbl : b->d_inst,
+                     boff(b, BK_BOUND), b->n, stage);
   return hipGetLastError();
 }
 
@@ -235,18 +273,22 @@ static hipError_t launch_cfl_all(qas_batch* b)
   return hipGetLastError();
 }
 
-// the four kernels of RK stage rk for every instance, in the batch's shape
-static hipError_t launch_stage_all(qas_batch* b, int rk)
+// the four kernels of RK stage rk for every instance, in the batch's shape,
+// on the instance table `tbl` (the batch's own by default; faces_only: the
+// two sweeps without corner EMF and CT)
+static hipError_t launch_stage_all(qas_batch* b, int rk, const BatchInst* tbl = nullptr, bool faces_only = false)
 {
   hipStream_t st = b->stream;
   const int n = b->n;
+  const BatchInst* I = tbl ? tbl : b->d_inst;
   const unsigned eb = (unsigned)b->ewb, xb = (unsigned)(WAVE * b->xwpb);
-  hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
-                     (const BatchInst*)b->d_inst, boff(b, BK_SY), n, rk);
-  hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], xb, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0,
-                     (const BatchInst*)b->d_inst, boff(b, BK_SX), n, rk);
-  hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], eb, 0, st, b->d_inst, boff(b, BK_EMF), n, rk, b->s.ctw);
-  hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], eb, 0, st, b->d_inst, boff(b, BK_CT), n, rk);
+  hipLaunchKernelGGL(b->sy, (unsigned)b->nblk[BK_SY], 256, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0, I,
+                     boff(b, BK_SY), n, rk);
+  hipLaunchKernelGGL(b->sx, (unsigned)b->nblk[BK_SX], xb, 0, st, StagePtrs{}, Geo{}, BcFlags{}, b->s.gam, 0, I,
+                     boff(b, BK_SX), n, rk);
+  if (faces_only) return hipGetLastError();
+  hipLaunchKernelGGL(b->emf, (unsigned)b->nblk[BK_EMF], eb, 0, st, I, boff(b, BK_EMF), n, rk, b->s.ctw);
+  hipLaunchKernelGGL(b->ct, (unsigned)b->nblk[BK_CT], eb, 0, st, I, boff(b, BK_CT), n, rk);
   return hipGetLastError();
 }
 
@@ -519,4 +561,193 @@ extern "C" int qas_batch_shape(const qas_batch* b, int* order, int* xwpb, int* e
   *xwpb = b->xwpb;
   *ewb = b->ewb;
   return b->tuned ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- batched adjoint
+// qas_adj_forward / qas_adj_backward for every domain of the batch, with one
+// launch per kernel for all of them (the adjoint kernels of
+// qas_adjoint2d.hip in their batched form, the forward stage kernels of the
+// batch for the recorded forward and the recompute).  Per domain the
+// operations are those of the single-domain entry points.
+static int adj_step_table(qas_batch* b, AdjStepInst** d_tbl, int** d_off, int* nblocks)
+{
+  const int n = b->n;
+  std::vector<AdjStepInst> t(n);
+  std::vector<int> off(n + 1, 0);
+  for (int k = 0; k < n; k++) {
+    qas_ctx* c = b->c[b->perm[k]];
+    AdjBufs a;
+    adj_bufs(c, &a);
+    AdjStepInst& s = t[k];
+    s.ck = a.ck;
+    s.stride = (long long)a.stride;
+    s.nd = c->geo.nd;
+    s.dts = a.d_dts;
+    s.dslot = a.dslot;
+    s.dt = c->d_dt;
+    // the backward's step input buffer: a state buffer the context does not
+    // hold its state in (its layout has the workspace stride)
+    const int ubi = (c->cur + 1) % 3;
+    for (int m = 0; m < 8; m++) {
+      s.cur[m] = c->U[c->cur][m];
+      s.ub[m] = c->U[ubi][m];
+    }
+    off[k + 1] = off[k] + cdiv(8 * c->geo.nd, 256);
+  }
+  *nblocks = off[n];
+  QAS_BCHECK(b, hipMalloc(d_tbl, sizeof(AdjStepInst) * n));
+  QAS_BCHECK(b, hipMalloc(d_off, sizeof(int) * (n + 1)));
+  QAS_BCHECK(b, hipMemcpyAsync(*d_tbl, t.data(), sizeof(AdjStepInst) * n, hipMemcpyHostToDevice, b->stream));
+  QAS_BCHECK(b, hipMemcpyAsync(*d_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+  QAS_BCHECK(b, hipStreamSynchronize(b->stream));
+  return QAS_OK;
+}
+
+static int adj_batch_scope(qas_batch* b, const char* who)
+{
+  if (!b || !b->stream) return bfail(b, QAS_E_ARG, std::string(who) + ": invalid batch");
+  for (int i = 0; i < b->n; i++) {
+    const std::string why = adj_scope(b->c[i]);
+    if (!why.empty()) return bfail(b, QAS_E_ARG, std::string(who) + ": " + why);
+  }
+  return QAS_OK;
+}
+
+extern "C" int qas_batch_adj_forward(qas_batch* b, int nsteps, const double* dts)
+{
+  int r = adj_batch_scope(b, "qas_batch_adj_forward");
+  if (r) return r;
+  if (nsteps <= 0 || !dts) return bfail(b, QAS_E_ARG, "qas_batch_adj_forward: bad arguments");
+  const int n = b->n;
+  for (int i = 0; i < n; i++) {
+    qas_ctx* c = b->c[i];
+    if (!c->uploaded) return bfail(b, QAS_E_STATE, "qas_batch_adj_forward: a domain has no state uploaded");
+    r = adj_prepare(c, nsteps, dts + (size_t)i * nsteps);
+    if (r) return bfail(b, r, std::string("qas_batch_adj_forward: ") + c->err);
+    QAS_BCHECK(b, hipStreamSynchronize(c->stream));
+  }
+  if (!b->tuned) {
+    r = qas_batch_tune(b);
+    if (r) return r;
+  }
+  r = fill_table(b);
+  if (r) return r;
+  AdjStepInst* d_tbl = nullptr;
+  int* d_toff = nullptr;
+  int nbk = 0;
+  r = adj_step_table(b, &d_tbl, &d_toff, &nbk);
+  bool bounded = true;
+  for (int i = 0; i < n; i++) bounded = bounded && b->c[i]->bounded;
+  hipError_t e = hipSuccess;
+  if (!r && !bounded) e = launch_bound_all(b, -1);
+  for (int k = 0; !r && e == hipSuccess && k < nsteps; k++) {
+    hipLaunchKernelGGL(kb_adj_ckpt, (unsigned)nbk, 256, 0, b->stream, d_tbl, d_toff, n, k);
+    for (int rk = 0; rk < 3 && e == hipSuccess; rk++) {
+      e = launch_stage_all(b, rk);
+      if (e == hipSuccess) e = launch_bound_all(b, rk);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (d_tbl) (void)hipFree(d_tbl);
+  if (d_toff) (void)hipFree(d_toff);
+  if (r) return r;
+  if (e != hipSuccess) return bfail(b, QAS_E_HIP, std::string("qas_batch_adj_forward: ") + hipGetErrorName(e));
+  for (int i = 0; i < n; i++) {
+    adj_set_recorded(b->c[i], nsteps);
+    b->c[i]->bounded = true;
+  }
+  return QAS_OK;
+}
+
+extern "C" int qas_batch_adj_backward(qas_batch* b, const double* const* g_final, double* const* g_init)
+{
+  int r = adj_batch_scope(b, "qas_batch_adj_backward");
+  if (r) return r;
+  if (!g_final || !g_init) return bfail(b, QAS_E_ARG, "qas_batch_adj_backward: bad arguments");
+  const int n = b->n;
+  const int nrec = adj_recorded(b->c[0]);
+  for (int i = 0; i < n; i++) {
+    if (!g_final[i] || !g_init[i]) return bfail(b, QAS_E_ARG, "qas_batch_adj_backward: null gradient array");
+    if (adj_recorded(b->c[i]) == 0 || adj_recorded(b->c[i]) != nrec)
+      return bfail(b, QAS_E_STATE, "qas_batch_adj_backward: every domain needs the same recorded forward");
+  }
+  r = fill_table(b);  // block order (perm) and shape of the batch
+  if (r) return r;
+  // the final-state adjoints into each context's staging
+  for (int i = 0; i < n; i++) {
+    qas_ctx* c = b->c[i];
+    AdjBufs a;
+    adj_bufs(c, &a);
+    QAS_BCHECK(b, hipMemcpyAsync(a.stage, g_final[i], 8 * (size_t)c->geo.nd * sizeof(double),
+                                 hipMemcpyHostToDevice, b->stream));
+  }
+  // recompute table: the batch's descriptors with the three stages reading
+  // the step input buffer (ub) and writing the adjoint workspace
+  std::vector<BatchInst> rec = b->inst;
+  std::vector<qas_ctx*> ord(n);
+  std::vector<double*> ub(8 * (size_t)n);
+  const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rbc[3] = {1.0, 0.25, 2.0 / 3.0};
+  for (int k = 0; k < n; k++) {
+    qas_ctx* c = b->c[b->perm[k]];
+    ord[k] = c;
+    AdjBufs a;
+    adj_bufs(c, &a);
+    double* const* U = c->U[(c->cur + 1) % 3];
+    for (int m = 0; m < 8; m++) ub[8 * (size_t)k + m] = U[m];
+    for (int rk = 0; rk < 3; rk++) {
+      StagePtrs& p = rec[k].st[rk];
+      std::memset(&p, 0, sizeof(p));
+      for (int m = 0; m < 8; m++) {
+        p.in[m] = rk == 0 ? U[m] : a.S[rk - 1][m];
+        p.out[m] = a.S[rk][m];
+        p.u0[m] = U[m];
+      }
+      p.vx = a.prim[rk][0]; p.vy = a.prim[rk][1]; p.vz = a.prim[rk][2]; p.pr = a.prim[rk][3];
+      p.cx = a.prim[rk][4]; p.cy = a.prim[rk][5];
+      for (int q = 0; q < 6; q++) p.acc[q] = c->acc[q];
+      for (int q = 0; q < 8; q++) p.emf[q] = a.emf[rk][q];
+      p.ez = a.ez[rk];
+      p.dt = a.dslot;
+      p.a = ra[rk];
+      p.b = rbc[rk];
+    }
+  }
+  BatchInst* d_rec = nullptr;
+  AdjStepInst* d_tbl = nullptr;
+  int* d_toff = nullptr;
+  int nbk = 0;
+  AdjBatchDev* ad = nullptr;
+  std::string aerr;
+  hipError_t e = hipMalloc(&d_rec, sizeof(BatchInst) * n);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_rec, rec.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) r = adj_step_table(b, &d_tbl, &d_toff, &nbk);
+  if (e == hipSuccess && !r) r = adjb_create(&ad, ord.data(), ub.data(), n, b->stream, &aerr);
+  if (e == hipSuccess && !r) e = adjb_fold_final(ad, b->stream);
+  for (int step = nrec - 1; e == hipSuccess && !r && step >= 0; step--) {
+    hipLaunchKernelGGL(kb_adj_load, (unsigned)nbk, 256, 0, b->stream, d_tbl, d_toff, n, step);
+    // forward recompute of the step: stage inputs and intermediates
+    for (int rk = 0; rk < 3 && e == hipSuccess; rk++) {
+      e = launch_stage_all(b, rk, d_rec, rk == 2);
+      if (e == hipSuccess && rk < 2) e = launch_bound_all(b, rk, d_rec);
+    }
+    for (int rk = 2; rk >= 0 && e == hipSuccess; rk--) e = adjb_stage(ad, rk, rk == 0, b->stream);
+  }
+  // the initial-state adjoints back to the host
+  for (int i = 0; e == hipSuccess && !r && i < n; i++) {
+    qas_ctx* c = b->c[i];
+    AdjBufs a;
+    adj_bufs(c, &a);
+    for (int m = 0; m < 8 && e == hipSuccess; m++)
+      e = hipMemcpyAsync(g_init[i] + (size_t)m * c->geo.nd, a.gO[m], (size_t)c->geo.nd * sizeof(double),
+                         hipMemcpyDeviceToHost, b->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  adjb_destroy(ad);
+  if (d_rec) (void)hipFree(d_rec);
+  if (d_tbl) (void)hipFree(d_tbl);
+  if (d_toff) (void)hipFree(d_toff);
+  if (r) return aerr.empty() ? r : bfail(b, r, "qas_batch_adj_backward: " + aerr);
+  if (e != hipSuccess) return bfail(b, QAS_E_HIP, std::string("qas_batch_adj_backward: ") + hipGetErrorName(e));
+  return QAS_OK;
 }

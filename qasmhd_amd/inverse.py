@@ -25,7 +25,9 @@ the initial-condition map:
 
 The starting-amplitude forward that fixes the schedule runs all instances in
 one batch (qas_batch_run: one launch per stage kernel for all domains, two
-steps per graph replay).  Losses at other amplitudes replay the frozen
+steps per graph replay).  The gradient runs batched too
+(loss_and_grad_batched: qas_batch_adj_forward / qas_batch_adj_backward), or
+per instance on host threads (loss_and_grad).  Losses at other amplitudes replay the frozen
 schedule step by step (qas_step), so a central finite difference compares
 two runs with identical step sizes.
 """
@@ -190,6 +192,79 @@ class VyModeInversion:
             L += li
             G[i] = gi
         return L, G
+
+    def loss_and_grad_batched(self, a, groups=4, workers=4):
+        """Loss and gradient of all instances with the batched adjoint:
+        the instances split into `groups` batches of about equal zone count,
+        driven by `workers` host threads, each batch on its own stream, so
+        that the batches' kernels run side by side (the adjoint line pass holds one
+        wave per SIMD; other kernels fill the rest of each SIMD).  Per batch:
+        the recorded forward of its instances in one batched run
+        (qas_batch_adj_forward) and the backward with one launch per kernel
+        for all of them (qas_batch_adj_backward).  Same per-instance
+        arithmetic as instance_grad.  The contexts, their checkpoint room
+        and the batches are kept for the next call (an iterative inversion
+        evaluates many gradients): close() frees them."""
+        if self.dts is None:
+            self.schedule()
+        tgt = self.targets()
+        groups = max(1, min(int(groups), self.ninst))
+        pool = getattr(self, "_gpool", None)
+        if pool is None or len(pool) != groups:
+            self.close()
+            order = sorted(range(self.ninst), key=lambda i: -self.sizes[i] ** 2)
+            members, load = [[] for _ in range(groups)], [0] * groups
+            for i in order:  # largest first onto the lightest group
+                k = load.index(min(load))
+                members[k].append(i)
+                load[k] += self.sizes[i] ** 2
+            pool = []
+            for mem in members:
+                mem = sorted(mem)
+                solvers = [self._solver(i) for i in mem]
+                pool.append((mem, solvers, capi.Batch(solvers)))
+            self._gpool = pool
+        u0 = [self.state(i, a[i]) for i in range(self.ninst)]
+        gi = [None] * self.ninst
+        Ls = [0.0] * self.ninst
+
+        def run(group):
+            mem, solvers, b = group
+            for i, s in zip(mem, solvers):
+                s.upload(u0[i])
+                s.reset_counter()
+            b.adj_forward(np.array([self.dts[i] for i in mem]))
+            gfin = []
+            for i, s in zip(mem, solvers):
+                r = s.download()[0] - tgt[i]
+                g = np.zeros((8,) + s.shape)
+                g[0, NG:-NG, NG:-NG] = 2.0 * r[NG:-NG, NG:-NG]
+                gfin.append(g)
+                Ls[i] = float(np.sum(r[NG:-NG, NG:-NG] ** 2))
+            for i, g in zip(mem, b.adj_backward(gfin)):
+                gi[i] = g
+
+        if len(pool) > 1 and workers > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(max_workers=min(len(pool), workers)) as ex:
+                list(ex.map(run, pool))
+        else:
+            run(pool[0])
+        L = sum(Ls)
+        G = np.zeros_like(np.asarray(a, dtype=np.float64))
+        for i in range(self.ninst):
+            w = gi[i][2] * u0[i][0] + gi[i][7] * u0[i][2]
+            G[i] = np.tensordot(self.psi[i], w, axes=([1, 2], [0, 1]))
+        return L, G
+
+    def close(self):
+        """Free the contexts loss_and_grad_batched keeps."""
+        pool = getattr(self, "_gpool", None)
+        self._gpool = None
+        for _, solvers, b in pool or []:
+            b.close()
+            for s in solvers:
+                s.close()
 
     def fd_grad(self, i, a, m, h=1e-6, richardson=True):
         """Central difference of instance i's loss in a_m over the frozen

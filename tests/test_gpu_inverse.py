@@ -84,3 +84,40 @@ def test_loss_matches_oracle(inv, oracle):
     r = ofinal(a)[0] - tgt[0]
     lo = float(np.sum(r[NG:-NG, NG:-NG] ** 2))
     assert inv.instance_loss(i, a) == lo
+
+
+def test_batched_gradient_equals_per_instance(inv):
+    """The batched adjoint (one launch per kernel for all instances) gives
+    each instance's gradient bit for bit as its own context does."""
+    _say("case: inverse gradient, batched adjoint vs per instance")
+    L1, G1 = inv.loss_and_grad(inv.a0, workers=1)
+    Lb, Gb = inv.loss_and_grad_batched(inv.a0, groups=1)    # one batch of all instances
+    Lb2, Gb2 = inv.loss_and_grad_batched(inv.a0, groups=1)  # kept contexts, second call
+    Lg, Gg = inv.loss_and_grad_batched(inv.a0, groups=2)    # two batches on two host threads
+    inv.close()
+    assert L1 == Lb == Lb2 == Lg
+    assert np.array_equal(G1, Gb) and np.array_equal(G1, Gb2) and np.array_equal(G1, Gg)
+
+
+def test_config_sizes_batched_gradient_vs_fd():
+    """8 instances drawn from the workload's size set {128, ..., 512}^2 at 24
+    steps: batched gradient equal to the per-instance one, and against
+    central differences of the GPU forward for two amplitudes of the
+    largest and the smallest instance."""
+    from qasmhd_amd import capi, inverse
+    capi.load()
+    sizes = inverse.config_sizes(8, seed=11)
+    _say("case: inverse, 8 instances of sizes", sizes, "24 steps")
+    m = inverse.VyModeInversion(sizes, steps=24, seed=11)
+    m.schedule()
+    Lb, Gb = m.loss_and_grad_batched(m.a0)
+    m.close()
+    L1, G1 = m.loss_and_grad(m.a0, workers=4)
+    assert Lb == L1 and np.array_equal(Gb, G1)
+    big, small = int(np.argmax(sizes)), int(np.argmin(sizes))
+    for i in (big, small):
+        scale = np.abs(Gb[i]).max()
+        for k in (0, m.nmodes - 1):
+            fd = m.fd_grad(i, m.a0[i], k, h=1e-6)
+            _say("  inst %d (n %d) mode %d adjoint %.12e fd %.12e" % (i, sizes[i], k, Gb[i, k], fd))
+            assert abs(Gb[i, k] - fd) <= 1e-5 * scale
