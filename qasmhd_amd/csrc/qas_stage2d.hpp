@@ -95,9 +95,15 @@ __device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int c
   if (ctw) {
     const double* ro = p.in[0];
     const long long c0 = ss - 1 - nx, c1 = ss - nx, c2 = ss - 1, c3 = ss;
-    ct = cuct_weight(ro[c0], ro[c1], ro[c2], ro[c3], vel(p, 0, c0), vel(p, 0, c1), vel(p, 0, c2), vel(p, 0, c3),
-                     vel(p, 1, c0), vel(p, 1, c1), vel(p, 1, c2), vel(p, 1, c3), p.in[4][ss - nx], p.in[4][ss],
-                     p.in[5][ss - 1], p.in[5][ss]);
+    // the cells' velocities: in FP64 the primitives the y sweep stored
+    // (m * (1/ro) of the same state, so the same values); in the mixed path
+    // those are FP32, and the weight takes m * (1/ro) of the state instead
+    auto v = [&](int comp, long long c) -> double {
+      if constexpr (sizeof(FT) == 8) return (comp == 0 ? p.vx : p.vy)[c];
+      else return vel(p, comp, c);
+    };
+    ct = cuct_weight(ro[c0], ro[c1], ro[c2], ro[c3], v(0, c0), v(0, c1), v(0, c2), v(0, c3), v(1, c0), v(1, c1),
+                     v(1, c2), v(1, c3), p.in[4][ss - nx], p.in[4][ss], p.in[5][ss - 1], p.in[5][ss]);
   }
   double ez = 0.0;
   {  // from the x sweep: By flux interpolated along y to (i-1/2, j-1/2)
