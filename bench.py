@@ -173,7 +173,7 @@ def brio_wu_line(cpu=True):
         s.upload(p.state)
         dt0 = s.cfl_dt()
         nmax = int(0.1 / dt0 + 0.5)  # mhd1d_class.cpp: nmax = (int)(tmax/dt + 0.5)
-        s.run(4, flg=0, dt=dt0, history=False)  # warm-up (graph capture)
+        s.run(4, flg=0, dt=dt0, history=False)  # warm-up
         s.upload(p.state)
         s.reset_counter()
         s.sync()
@@ -275,7 +275,7 @@ def mixed_line(n, steps):
     solvers = {prec: capi.solver_for(p, precision=prec) for prec in (0, 1)}
     for s in solvers.values():
         s.upload(p.state)
-        s.run(4, flg=0, dt=dt, history=False)  # builds the step graph
+        s.run(4, flg=0, dt=dt, history=False)  # warm-up
     for trial in range(2):  # alternate the two precisions; best of two each
         for prec in (0, 1):
             s = solvers[prec]
@@ -309,7 +309,7 @@ def mixed_line(n, steps):
 def fma64_line(p, wname, warmup, steps):
     """FP64 with fused multiply-adds in the sweeps (precision 2) against the
     bit-exact FP64 path on the same workload, both timed as the headline is
-    (graph replay, setdt schedule, `steps` steps after `warmup`), best of two
+    (setdt schedule, `steps` steps after `warmup`), best of two
     alternating trials; relative L1 distance of every variable over the
     interior after warmup + steps steps and the final dt."""
     import numpy as np
@@ -578,7 +578,7 @@ def gravity_line(nx, ny, steps, cpu=True):
     with capi.solver_for(p) as s:
         s.upload(p.state)
         s.set_gravity(phi)
-        s.run(2, flg=1, history=False)  # warm-up (graph capture)
+        s.run(2, flg=1, history=False)  # warm-up
         s.upload(p.state)
         s.reset_counter()
         s.sync()
@@ -678,7 +678,7 @@ def mixed3d_line(p, steps):
             upload_problem(s, p)
             if dt is None:
                 dt = s.cfl_dt()
-            s.run(2, flg=0, dt=dt, history=False)  # builds the step graph
+            s.run(2, flg=0, dt=dt, history=False)  # warm-up
             upload_problem(s, p)
             s.reset_counter()
             s.sync()
@@ -740,7 +740,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and adjoint lines")
-    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host (for counters)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="qas_run launches every kernel from the host in every line (the library default; the "
+                         "batch line keeps its graph)")
+    ap.add_argument("--graphs", action="store_true", help="qas_run replays captured step-pair graphs in every line")
     ap.add_argument("--precision", type=int, default=0,
                     help="0 FP64 (default), 1 mixed, 2 FP64 with FMA contraction, for the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -784,6 +787,8 @@ def main():
         import torch  # before the library: both then bind torch's HIP runtime
         torch.cuda.init()
     from qasmhd_amd import capi
+    if args.no_graphs or args.graphs:
+        capi.GRAPHS = bool(args.graphs) and not args.no_graphs
 
     p, wname = workload(args.workload)
     zones = int(np.prod(p.interior))
@@ -793,13 +798,11 @@ def main():
     lib.qas_prof_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     lib.qas_kernel_model.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     s = capi.solver_for(p, precision=args.precision)
-    if args.no_graphs:
-        s.lib.qas_use_graphs(s.ctx, 0)
     upload_problem(s, p)
     s.run(args.warmup, flg=1, history=False)
     s.sync()
-    # the timed region: K steps as a user runs them (graph replay on, no
-    # timing hooks), the state resident in HBM
+    # the timed region: K steps as a user runs them (qas_run's default launch
+    # mode, no timing hooks), the state resident in HBM
     t0 = time.perf_counter()
     hist, dt_now = s.run(args.steps, flg=1, history=False)
     s.sync()
@@ -936,7 +939,8 @@ def main():
         "config": {"workload": wname, "cells": list(p.interior), "solver": ["roe", "hlld", "lhlld", "mlau"][p.rmn],
                    "odr": p.odr, "r_k": p.r_k, "ctw": p.ctw, "cfl": p.cfl,
                    "precision": {0: "fp64", 1: "mixed", 2: "fp64_fma"}[args.precision],
-                   "dt_final": dt_now, "finite": finite},
+                   "dt_final": dt_now, "finite": finite,
+                   "launch": {None: "host (library default)", False: "host", True: "graph replay"}[capi.GRAPHS]},
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)

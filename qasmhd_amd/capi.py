@@ -223,6 +223,13 @@ def make_grid(shape, dx, dy=1.0, dz=1.0):
     return g
 
 
+# qas_run's launch mode for the solvers created from here on: None keeps
+# the library's default (every kernel launched from the host); True replays
+# pairs of steps as captured HIP graphs, False sets host launches
+# (qas_use_graphs)
+GRAPHS = None
+
+
 class Solver:
     """One device-resident domain (owns a qas_ctx)."""
 
@@ -238,6 +245,8 @@ class Solver:
             self.lib.qas_destroy(self.ctx)
             self.ctx = None
             raise QasError("qas_create: %s (%d)" % (msg, r))
+        if GRAPHS is not None:
+            self._check(self.lib.qas_use_graphs(self.ctx, 1 if GRAPHS else 0), "qas_use_graphs")
 
     def _check(self, r, what):
         if r != QAS_OK:

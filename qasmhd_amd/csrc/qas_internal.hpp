@@ -158,11 +158,13 @@ struct qas_ctx {
   double* d_cflpart = nullptr;        // CFL_MAX_BLOCKS per-block maxima
   int cfl_blocks = 1;                 // blocks of the last CFL launch
   long long* d_hcount = nullptr;      // dt-history slot counter (graph replays)
-  // hipGraph of two consecutive steps of qas_run (see qas_run)
+  // hipGraph of two consecutive steps of qas_run (qas_use_graphs(c, 1));
+  // off by default: host launches measured as fast (the host enqueues
+  // ahead of the GPU, so replay saves nothing)
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_flg = -1, graph_hist = -1, graph_cur = -1;
-  bool use_graph = true;
+  bool use_graph = false;
   double* h_pinned = nullptr;         // small pinned staging
   bool uploaded = false;
   bool bounded = false;

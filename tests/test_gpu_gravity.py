@@ -50,6 +50,7 @@ def test_gravity_steps_and_graph_replay(oracle):
     oracle.bound(cfg, ref)
     h_ref, _ = oracle.run(cfg, ref, 20, 1, dt=dt)
     with capi.solver_for(p) as s:
+        s.lib.qas_use_graphs(s.ctx, 1)
         s.upload(p.state)
         s.set_gravity(phi)
         s.bound()
