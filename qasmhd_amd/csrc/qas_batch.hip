@@ -46,10 +46,8 @@ __global__ __launch_bounds__(256) void kb_bound(const BatchInst* I, const int* o
 {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
   const BatchInst& B = I[i];
-  BoundArgs a;
-  for (int m = 0; m < 8; m++) a.f[m] = stage < 0 ? B.cur[m] : B.st[stage].out[m];
-  a.nvar = 8;
-  bound_at(a, B.g, B.bc, B.nghost, NG, NG, 0, batch_thread(off, i, b));
+  bound_at_f([&](int m) { return stage < 0 ? B.cur[m] : B.st[stage].out[m]; }, 8, nullptr, 0.0, B.g, B.bc, B.nghost,
+             NG, NG, 0, batch_thread(off, i, b));
 }
 
 __global__ __launch_bounds__(256) void kb_cfl(const BatchInst* I, const int* off, int n, double gam)

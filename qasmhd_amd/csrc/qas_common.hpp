@@ -14,10 +14,14 @@ struct BoundArgs {
   double gfac = 0.0;            // (2 - gam) / (gam - 1)
 };
 
-__device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const BcFlags& bc, long long nghost,
-                                         int ox, int oy, int oz, long long t)
+// fv(m): the array of variable m (a kernel argument's entry, or a pointer
+// read from an instance table in device memory: a local copy of the
+// pointer array indexed by m would sit in scratch memory)
+template <typename FV>
+__device__ __forceinline__ void bound_at_f(const FV& fv, int nvar, const double* phi, double gfac, const Geo& g,
+                                           const BcFlags& bc, long long nghost, int ox, int oy, int oz, long long t)
 {
-  if (t >= nghost * a.nvar) return;
+  if (t >= nghost * nvar) return;
   const int m = (int)(t / nghost);
   long long r = t - (long long)m * nghost;
   // decode ghost cell r: z slabs, then y slabs of the inner z range, then
@@ -47,7 +51,7 @@ __device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const
     j = oy + (int)(line % (ny - 2 * oy));
     k = oz + (int)(line / (ny - 2 * oy));
   }
-  if (a.phi && m == 7 && a.nvar == 8 && oz == 0 && (j < oy || j >= ny - oy)) {
+  if (phi && m == 7 && nvar == 8 && oz == 0 && (j < oy || j >= ny - oy)) {
     // with gravity along y the energy's y ghosts (x ghosts included) are
     // the hydrostatic extrapolation from the first / last interior row, as
     // 2D/SERIAL/RTI/mhd2d_class.cpp:10-32 sets them after bc2d; that row's
@@ -55,22 +59,29 @@ __device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const
     const int jb = j < oy ? oy : (int)ny - oy - 1;
     const Src ex = resolve(i, g.nx, bc.st[0][7], bc.dn[0][7]);
     const Src rx = resolve(i, g.nx, bc.st[0][0], bc.dn[0][0]);
-    const double enb = apply(ex, a.f[7][nx * jb + ex.i]);
-    const double rob = apply(rx, a.f[0][nx * jb + rx.i]);
+    double* en = fv(7);
+    const double enb = apply(ex, en[nx * jb + ex.i]);
+    const double rob = apply(rx, fv(0)[nx * jb + rx.i]);
     const long long ss = nx * j + i, sb = nx * jb + i;
-    a.f[7][ss] = enb - a.gfac * rob * (a.phi[ss] - a.phi[sb]);
+    en[ss] = enb - gfac * rob * (phi[ss] - phi[sb]);
     return;
   }
   const Src sx = ox ? resolve(i, g.nx, bc.st[0][m], bc.dn[0][m]) : Src{i, 1.0, false};
   const Src sy = oy ? resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]) : Src{j, 1.0, false};
   const Src sz = oz ? resolve(k, g.nz, bc.st[2][m], bc.dn[2][m]) : Src{k, 1.0, false};
   if (sx.i == i && sy.i == j && sz.i == k) return;  // a kept cell (staggered boundary face)
-  double* f = a.f[m];
+  double* f = fv(m);
   double v = f[(long long)nx * (ny * sz.i + sy.i) + sx.i];
   v = apply(sx, v);
   v = apply(sy, v);
   v = apply(sz, v);
   f[(long long)nx * (ny * k + j) + i] = v;
+}
+
+__device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const BcFlags& bc, long long nghost,
+                                         int ox, int oy, int oz, long long t)
+{
+  bound_at_f([&](int m) { return a.f[m]; }, a.nvar, a.phi, a.gfac, g, bc, nghost, ox, oy, oz, t);
 }
 
 struct CflArgs {

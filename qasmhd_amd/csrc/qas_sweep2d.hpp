@@ -60,8 +60,12 @@ constexpr int YRP = YR + 1;    // padded LDS column length
 // FMA: a tag (0 or 1) naming the build of the instantiation: the FP64
 // sweeps built with FMA contraction (qas_fma64.hip) are the same template
 // under FMA = 1, so their kernel symbols differ from the bit-exact ones.
+// FP32 (mixed) LHLLD and MLAU instances run at 3 waves per SIMD: held to 4
+// (128 VGPRs) with the FP64 primitives of the staging they spilled 36-72
+// bytes per lane, and the MLAU y sweep at 2048^2 took 314 us per launch
+// against 272 us at 3 waves (135 VGPRs, no spill).
 template <int RMN, int ODR, typename T, bool BATCH, bool GRAV = false, int FMA = 0>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : (RMN != 1 ? 2 : 1)) void k_sweep2d_y(
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? (RMN >= 2 ? 3 : 4) : (RMN != 1 ? 2 : 1)) void k_sweep2d_y(
     StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam, int nseg_arg, const BatchInst* I, const int* off,
     int ninst, int stage)
 {
