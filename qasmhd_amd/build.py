@@ -7,7 +7,9 @@ into one shared library exporting the C ABI of include/qasmhd.h.  Flags:
   -ffp-contract=off      no FMA contraction: the FP64 path keeps the
                          reference's rounding sequence (bit-identical results);
                          qas_mixed.hip and qas_fma64.hip (precision 1, 2)
-                         are built with contraction
+                         are built with contraction, and the discrete adjoint
+                         (qas_adjoint2d.hip) with contraction within each
+                         source expression (-ffp-contract=on)
   -Xclang -target-feature -Xclang -packed-fp32-ops
                          no packed FP32 instructions are generated
 """
@@ -30,9 +32,13 @@ COMMON_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                 "-Wno-unused-command-line-argument", "-Wno-unused-result"]
 # per-source flags: every unit is built without FMA contraction except the
 # FP32 sweeps of the mixed path, which also use the fast FP32 divide/sqrt,
-# and the contracted FP64 sweeps (IEEE FP64 divide and sqrt)
+# the contracted FP64 sweeps (IEEE FP64 divide and sqrt), and the discrete
+# adjoint.  The adjoint is checked against finite differences, not bit for
+# bit; "on" fuses a*b+c only inside one source expression, so every kernel
+# that inlines the same code (single-domain and batched) rounds alike
 EXTRA = {"qas_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
-         "qas_fma64.hip": ["-ffp-contract=fast"]}
+         "qas_fma64.hip": ["-ffp-contract=fast"],
+         "qas_adjoint2d.hip": ["-ffp-contract=on"]}
 
 
 def lib_path() -> str:
