@@ -432,15 +432,39 @@ def adjoint_line(n, steps, rmn=1):
             s.adj_backward_device(dg, di)
             t_bwd = time.perf_counter() - t0
             gd = di.download()
+            # the same with the forward keeping every stage's intermediates
+            # (qas_adj_keep_stages): the backward skips the recompute; the
+            # first recorded forward allocates the store, the second is timed
+            s.adj_keep_stages(True)
+            t_keep = []
+            for _ in range(2):
+                s.upload(p.state)
+                s.sync()
+                t0 = time.perf_counter()
+                s.adj_forward(dts)
+                t_keep.append(time.perf_counter() - t0)
+            s.adj_backward_device(dg, di)
+            s.sync()
+            t0 = time.perf_counter()
+            s.adj_backward_device(dg, di)
+            t_bwd_k = time.perf_counter() - t0
+            gk = di.download()
+            s.adj_keep_stages(False)
+    nd = int(np.prod(p.state.shape[1:]))
     return {"workload": "2d_decaying_mhd_%dx%d_mach0.3_%s_wcns4_ct_rk3" % (n, n, "mlau" if rmn == 3 else "hlld"),
             "steps": steps,
             "forward_s": t_fwd, "forward_with_checkpoints_s": t_rec, "backward_s": t_bwd,
             "backward_with_host_copies_s": t_bwd_host, "ratio": t_bwd / t_fwd,
             "gradient_ratio": (t_rec + t_bwd) / t_fwd,
             "finite": bool(np.isfinite(g).all()), "device_equals_host": bool(np.array_equal(g, gd)),
+            "kept_stages": {"forward_s": t_keep[1], "backward_s": t_bwd_k, "ratio": t_bwd_k / t_fwd,
+                            "gradient_ratio": (t_keep[1] + t_bwd_k) / t_fwd,
+                            "bytes_per_step": 8 * 54 * nd, "checkpoint_bytes_per_step": 8 * 8 * nd,
+                            "equals_recompute": bool(np.array_equal(gk, gd))},
             "note": "ratio = backward / forward, gradient buffers resident in HBM (qas_adj_backward_device); "
                     "backward = per step: 3 recomputed forward stages + 3 stage adjoints; gradient_ratio = "
-                    "(checkpointed forward + backward) / forward"}
+                    "(checkpointed forward + backward) / forward; kept_stages: the forward keeps every "
+                    "stage's intermediates (qas_adj_keep_stages) and the backward skips the recompute"}
 
 
 def kernel_roofline(s, lib, sweeps, wname):

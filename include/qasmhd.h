@@ -233,6 +233,14 @@ int qas_batch_adj_backward(qas_batch *b, const double *const *g_final, double *c
 int qas_adj_forward(qas_ctx *c, int nsteps, const double *dts);
 int qas_adj_backward(qas_ctx *c, const double *g_final, double *g_init);
 int qas_adj_backward_device(qas_ctx *c, const double *g_final, double *g_init);
+/* on = 1: the following qas_adj_forward calls keep, besides each step's
+ * input state, every stage's intermediates (the two inner stage states,
+ * the primitives and face EMF parts of all three stages: 46 x nd doubles
+ * more per step), and qas_adj_backward reads them instead of recomputing
+ * the three forward stages of each step.  Same results bit for bit; more
+ * memory, less time.  on = 0 (the default): checkpoints only.  The batched
+ * adjoint (qas_batch_adj_*) always recomputes. */
+int qas_adj_keep_stages(qas_ctx *c, int on);
 
 
 /* ---- instrumentation (no counterpart in the reference) -----------------

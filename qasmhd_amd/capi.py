@@ -42,7 +42,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct", "qas_test_rcp",
            "qas_set_dissipation", "qas_dsptv", "qas_set_gravity", "qas_run_static",
-           "qas_batch_adj_forward", "qas_batch_adj_backward"]
+           "qas_batch_adj_forward", "qas_batch_adj_backward", "qas_adj_keep_stages"]
 
 _lib = None
 
@@ -96,6 +96,7 @@ def load(path: str | None = None):
     l.qas_adj_forward.argtypes = [vp, C.c_int, dp]
     l.qas_adj_backward.argtypes = [vp, dp, dp]
     l.qas_adj_backward_device.argtypes = [vp, vp, vp]
+    l.qas_adj_keep_stages.argtypes = [vp, C.c_int]
     l.qas_test_interp.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     l.qas_test_f4.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
     l.qas_test_lrstate.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
@@ -349,6 +350,12 @@ class Solver:
         d = np.ascontiguousarray(dts, dtype=np.float64)
         self.adj_generation = getattr(self, "adj_generation", 0) + 1
         self._check(self.lib.qas_adj_forward(self.ctx, int(d.size), _p(d)), "qas_adj_forward")
+
+    def adj_keep_stages(self, on=True):
+        """Later adj_forward calls keep every stage's intermediates (46 more
+        arrays of the state's size per step) and adj_backward skips the
+        forward recompute; same results (qas_adj_keep_stages)."""
+        self._check(self.lib.qas_adj_keep_stages(self.ctx, 1 if on else 0), "qas_adj_keep_stages")
 
     def adj_backward(self, g_final):
         """Adjoint of the recorded steps: d(loss)/d(initial state) from

@@ -16,7 +16,10 @@
 // device.  qas_adj_backward walks the steps in reverse; for each step it
 // recomputes the three forward stages from the checkpoint, keeping each
 // stage's primitives and face EMF parts, and then runs the stage adjoints
-// 3, 2, 1.  A stage adjoint is these kernels, all in gather form (every
+// 3, 2, 1.  With qas_adj_keep_stages the forward also writes every stage's
+// input state, primitives and face parts into a per-step store (the same
+// kernels on other buffers) and the backward reads them instead of
+// recomputing.  A stage adjoint is these kernels, all in gather form (every
 // output element is written by one thread; no global atomics):
 //   corner    adjoint of the CT update -> g_ez at corners; forward CUCT
 //             weight and the adjoint of its inputs (once per corner)
@@ -972,6 +975,9 @@ struct qas_adj_work {
   double* tin[NTR] = {};      // transposed inputs of the y pass
   double* tout[9] = {};       // its transposed outputs: gp[0..7], gbf[1]
   double* ck = nullptr;       // checkpoints: nsteps x 8 x stride
+  double* ks = nullptr;       // kept stages (qas_adj_keep_stages): nsteps x KEPT x stride
+  int ks_cap = 0;
+  bool kept = false;          // the recorded forward kept its stages
   double* d_dts = nullptr;    // the recorded step sizes (device), ck_cap + 1 entries
   double* dslot = nullptr;    // d_dts[ck_cap]: the step size a batched backward reads
   double* stage = nullptr;    // qas_adj_backward's device copies of g_final, g_init
@@ -987,6 +993,7 @@ void adj_release(qas_ctx* c)
   if (!c || !c->adj) return;
   if (c->adj->slab) (void)hipFree(c->adj->slab);
   if (c->adj->ck) (void)hipFree(c->adj->ck);
+  if (c->adj->ks) (void)hipFree(c->adj->ks);
   if (c->adj->d_dts) (void)hipFree(c->adj->d_dts);
   if (c->adj->stage) (void)hipFree(c->adj->stage);
   delete c->adj;
@@ -1058,11 +1065,42 @@ static int adj_workspace(qas_ctx* c)
   return QAS_OK;
 }
 
-// forward stage with its primitives and face parts kept in stage slot s;
-// the last stage's output is not needed (only its primitives and face
-// parts are), so its corner EMF, CT and ghost fill are skipped
-static hipError_t fwd_stage(qas_ctx* c, int s, double* const in[8], double* const u0[8], double* const out[8],
-                            int rk)
+// Where a stage's forward intermediates live: the recompute slots of the
+// workspace, or a step's entries of the kept-stage store.
+struct StageSlot {
+  double* prim[6];  // vx vy vz pr cx cy
+  double* emf[8];   // face parts (the 2D sweeps write slots 0, 1, 4, 5)
+};
+static StageSlot work_slot(const qas_adj_work* w, int s)
+{
+  StageSlot r;
+  for (int k = 0; k < 6; k++) r.prim[k] = w->prim[s][k];
+  for (int k = 0; k < 8; k++) r.emf[k] = w->emf[s][k];
+  return r;
+}
+// kept-stage store, per step: the inputs of stages 2 and 3 (8 + 8 arrays),
+// then per stage its 6 primitives and its 4 face parts
+constexpr int KEPT = 16 + 3 * 6 + 3 * 4;
+static double* kept_arr(const qas_adj_work* w, int n, int k)
+{
+  return w->ks + ((size_t)n * KEPT + (size_t)k) * w->stride;
+}
+static StageSlot kept_slot(const qas_adj_work* w, int n, int s)
+{
+  StageSlot r;
+  for (int k = 0; k < 6; k++) r.prim[k] = kept_arr(w, n, 16 + 6 * s + k);
+  for (int k = 0; k < 8; k++) r.emf[k] = nullptr;
+  const int q[4] = {0, 1, 4, 5};
+  for (int k = 0; k < 4; k++) r.emf[q[k]] = kept_arr(w, n, 34 + 4 * s + k);
+  return r;
+}
+
+// forward stage rk with its primitives and face parts in `sl`; when
+// `whole` is false the last stage's output is not needed (only its
+// primitives and face parts are), so its corner EMF, CT and ghost fill are
+// skipped; every other stage runs as ideal(dt) runs it, ghost fill included
+static hipError_t fwd_stage(qas_ctx* c, const StageSlot& sl, double* ez, double* const in[8],
+                            double* const u0[8], double* const out[8], int rk, bool whole)
 {
   qas_adj_work* w = c->adj;
   StagePtrs p;
@@ -1072,25 +1110,25 @@ static hipError_t fwd_stage(qas_ctx* c, int s, double* const in[8], double* cons
     p.out[m] = out[m];
     p.u0[m] = u0[m];
   }
-  p.vx = w->prim[s][0]; p.vy = w->prim[s][1]; p.vz = w->prim[s][2]; p.pr = w->prim[s][3];
-  p.cx = w->prim[s][4]; p.cy = w->prim[s][5];
+  p.vx = sl.prim[0]; p.vy = sl.prim[1]; p.vz = sl.prim[2]; p.pr = sl.prim[3];
+  p.cx = sl.prim[4]; p.cy = sl.prim[5];
   for (int k = 0; k < 6; k++) p.acc[k] = c->acc[k];
-  for (int k = 0; k < 8; k++) p.emf[k] = w->emf[s][k];
-  p.ez = w->ez[s];
+  for (int k = 0; k < 8; k++) p.emf[k] = sl.emf[k];
+  p.ez = ez;
   p.dt = w->cur_dt;
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
   p.b = rb[rk];
-  const bool last = rk == 2;
-  hipError_t e = launch_stage2d(c, p, last);
-  if (e != hipSuccess || last) return e;
+  const bool faces_only = rk == 2 && !whole;
+  hipError_t e = launch_stage2d(c, p, faces_only);
+  if (e != hipSuccess || faces_only) return e;
   return launch_bound(c, out, 8);
 }
 
 // adjoint of stage rk (0..2) whose input is `in` and whose forward
 // intermediates sit in slot rk; reads w->gO, leaves the folded input adjoint
 // in w->gO (plus the step-base adjoint when `last`)
-static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
+static hipError_t adj_stage(qas_ctx* c, int rk, const StageSlot& sl, double* const in[8], bool last)
 {
   qas_adj_work* w = c->adj;
   const Geo& g = c->geo;
@@ -1103,10 +1141,10 @@ static hipError_t adj_stage(qas_ctx* c, int rk, double* const in[8], bool last)
     p.gU0[m] = w->gU0[m];
     p.gI[m] = w->gI[m];
     p.gp[m] = w->gp[m];
-    p.emf[m] = w->emf[rk][m];
+    p.emf[m] = sl.emf[m];
   }
-  p.vx = w->prim[rk][0]; p.vy = w->prim[rk][1]; p.vz = w->prim[rk][2]; p.pr = w->prim[rk][3];
-  p.cx = w->prim[rk][4]; p.cy = w->prim[rk][5];
+  p.vx = sl.prim[0]; p.vy = sl.prim[1]; p.vz = sl.prim[2]; p.pr = sl.prim[3];
+  p.cx = sl.prim[4]; p.cy = sl.prim[5];
   for (int k = 0; k < 2; k++) p.gbf[k] = w->gbf[k];
   for (int k = 0; k < 4; k++) p.gemf[k] = w->gemf[k];
   for (int k = 0; k < 2; k++) p.gdv[k] = w->gdv[k];
@@ -1224,6 +1262,7 @@ int adj_prepare(qas_ctx* c, int nsteps, const double* dts)
   QAS_ACHECK(c, hipMemcpyAsync(w->d_dts, w->dts.data(), (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice,
                                c->stream));
   w->nrec = 0;
+  w->kept = false;
   return QAS_OK;
 }
 
@@ -1400,23 +1439,61 @@ hipError_t adjb_stage(AdjBatchDev* d, int rk, bool last, hipStream_t st)
 }
 }  // namespace qas
 
+extern "C" int qas_adj_keep_stages(qas_ctx* c, int on)
+{
+  if (!c) return QAS_E_ARG;
+  c->adj_keep = on != 0;
+  return QAS_OK;
+}
+
 extern "C" int qas_adj_forward(qas_ctx* c, int nsteps, const double* dts)
 {
   int r = adj_prepare(c, nsteps, dts);
   if (r) return r;
   qas_adj_work* w = c->adj;
   const size_t per = 8 * w->stride;
+  if (c->adj_keep && nsteps > w->ks_cap) {
+    if (w->ks) QAS_ACHECK(c, hipFree(w->ks));
+    w->ks = nullptr;
+    w->ks_cap = 0;
+    hipError_t e = hipMalloc(&w->ks, (size_t)KEPT * w->stride * (size_t)nsteps * sizeof(double));
+    if (e != hipSuccess) return afail(c, QAS_E_NOMEM, std::string("adjoint kept stages: ") + hipGetErrorName(e));
+    w->ks_cap = nsteps;
+  }
   if (!c->bounded) {
     QAS_ACHECK(c, launch_bound(c, c->U[c->cur], 8));
     c->bounded = true;
   }
   const size_t bytes = (size_t)c->geo.nd * sizeof(double);
   for (int n = 0; n < nsteps; n++) {
-    for (int m = 0; m < 8; m++)
-      QAS_ACHECK(c, hipMemcpyAsync(w->ck + per * n + w->stride * m, c->U[c->cur][m], bytes,
-                                   hipMemcpyDeviceToDevice, c->stream));
-    r = qas_step(c, dts[n]);
-    if (r) return r;
+    double* U[8];
+    for (int m = 0; m < 8; m++) {
+      U[m] = w->ck + per * n + w->stride * m;
+      QAS_ACHECK(c, hipMemcpyAsync(U[m], c->U[c->cur][m], bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (!c->adj_keep) {
+      r = qas_step(c, dts[n]);
+      if (r) return r;
+      continue;
+    }
+    // ideal(dt) from the checkpoint, the stage intermediates written into
+    // this step's entries of the store and the step's result into the
+    // current state buffer (the kernels and buffers order of ctx_step)
+    w->cur_dt = w->d_dts + n;
+    double *S0[8], *S1[8];
+    for (int m = 0; m < 8; m++) {
+      S0[m] = kept_arr(w, n, m);
+      S1[m] = kept_arr(w, n, 8 + m);
+    }
+    QAS_ACHECK(c, fwd_stage(c, kept_slot(w, n, 0), c->ez, U, U, S0, 0, true));
+    QAS_ACHECK(c, fwd_stage(c, kept_slot(w, n, 1), c->ez, S0, U, S1, 1, true));
+    QAS_ACHECK(c, fwd_stage(c, kept_slot(w, n, 2), c->ez, S1, U, c->U[c->cur], 2, true));
+  }
+  if (c->adj_keep) {
+    // qas_step leaves the last dt in the context's device dt
+    QAS_ACHECK(c, launch_set_dt(c, dts[nsteps - 1]));
+    c->bounded = true;
+    w->kept = true;
   }
   w->nrec = nsteps;
   QAS_ACHECK(c, hipStreamSynchronize(c->stream));
@@ -1446,13 +1523,25 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
     w->cur_dt = w->d_dts + n;  // the step's dt, read by the recompute and the stage adjoints
     double* U[8];
     for (int m = 0; m < 8; m++) U[m] = w->ck + per * n + w->stride * m;
+    if (w->kept) {  // the recorded forward kept this step's stages
+      double *S0[8], *S1[8];
+      for (int m = 0; m < 8; m++) {
+        S0[m] = kept_arr(w, n, m);
+        S1[m] = kept_arr(w, n, 8 + m);
+      }
+      QAS_ACHECK(c, adj_stage(c, 2, kept_slot(w, n, 2), S1, false));
+      QAS_ACHECK(c, adj_stage(c, 1, kept_slot(w, n, 1), S0, false));
+      QAS_ACHECK(c, adj_stage(c, 0, kept_slot(w, n, 0), U, true));
+      continue;
+    }
     // forward recompute: stage inputs S1, S2 and every stage's intermediates
-    QAS_ACHECK(c, fwd_stage(c, 0, U, U, w->S[0], 0));
-    QAS_ACHECK(c, fwd_stage(c, 1, w->S[0], U, w->S[1], 1));
-    QAS_ACHECK(c, fwd_stage(c, 2, w->S[1], U, w->S[2], 2));
-    QAS_ACHECK(c, adj_stage(c, 2, w->S[1], false));
-    QAS_ACHECK(c, adj_stage(c, 1, w->S[0], false));
-    QAS_ACHECK(c, adj_stage(c, 0, U, true));
+    const StageSlot s0 = work_slot(w, 0), s1 = work_slot(w, 1), s2 = work_slot(w, 2);
+    QAS_ACHECK(c, fwd_stage(c, s0, w->ez[0], U, U, w->S[0], 0, false));
+    QAS_ACHECK(c, fwd_stage(c, s1, w->ez[1], w->S[0], U, w->S[1], 1, false));
+    QAS_ACHECK(c, fwd_stage(c, s2, w->ez[2], w->S[1], U, w->S[2], 2, false));
+    QAS_ACHECK(c, adj_stage(c, 2, s2, w->S[1], false));
+    QAS_ACHECK(c, adj_stage(c, 1, s1, w->S[0], false));
+    QAS_ACHECK(c, adj_stage(c, 0, s0, U, true));
   }
   for (int m = 0; m < 8; m++)
     QAS_ACHECK(c, hipMemcpyAsync(g_init + (size_t)m * g.nd, w->gO[m], (size_t)g.nd * sizeof(double),

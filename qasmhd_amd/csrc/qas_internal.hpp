@@ -175,8 +175,10 @@ struct qas_ctx {
   std::vector<hipEvent_t> prof_ev;    // start/stop pairs
   std::vector<int> prof_kid;
   size_t prof_n = 0;
-  // discrete-adjoint workspace (qas_adjoint2d.hip), created on first use
+  // discrete-adjoint workspace (qas_adjoint2d.hip), created on first use;
+  // adj_keep: the next qas_adj_forward keeps every stage's intermediates
   struct qas_adj_work* adj = nullptr;
+  int adj_keep = 0;
   // dissipative step (qas_dissipative.hip): per-cell viscosity and
   // resistivity (one allocation, eta follows nu) and max(2 nu, eta)
   double *d_nu = nullptr, *d_eta = nullptr;

@@ -99,6 +99,44 @@ def test_adjoint_vs_fd(rmn, ctw, nx, ny):
     assert max(errs) < 1e-5, errs
 
 
+@pytest.mark.parametrize("rmn,ctw,nx,ny", [(1, 1, 40, 24), (3, 1, 32, 32), (1, 0, 24, 40)])
+def test_kept_stages_equal_recompute(rmn, ctw, nx, ny):
+    """qas_adj_keep_stages: the recorded forward keeps every stage's
+    intermediates and the backward reads them instead of recomputing the
+    three stages of each step.  The state after the recorded forward equals
+    qas_step's, and the gradients equal those of the checkpoint-and-
+    recompute path bit for bit, for a repeated backward and for a second,
+    shorter recorded forward on the same store."""
+    print("case: kept stages vs recompute, rmn %d ctw %d, %dx%d" % (rmn, ctw, nx, ny), flush=True)
+    p, s, dts = _setup(n=nx, ny=ny, ctw=ctw, rmn=rmn)
+    dts = np.concatenate([dts, dts[::-1]])
+    W = np.random.default_rng(12).standard_normal(p.state.shape)
+    s.upload(p.state)
+    s.adj_forward(dts)
+    fin_a = s.download()
+    g_a = s.adj_backward(W)
+    s.upload(p.state)
+    s.adj_forward(dts[:3])
+    g_short_a = s.adj_backward(W)
+    fin_step = _forward(s, p.state, dts)
+    s.adj_keep_stages(True)
+    s.upload(p.state)
+    s.adj_forward(dts)
+    fin_b = s.download()
+    g_b = s.adj_backward(W)
+    g_b2 = s.adj_backward(W)
+    s.upload(p.state)
+    s.adj_forward(dts[:3])
+    g_short_b = s.adj_backward(W)
+    s.close()
+    assert np.array_equal(fin_a, fin_step)
+    assert np.array_equal(fin_b, fin_step)
+    assert np.array_equal(g_a, g_b)
+    assert np.array_equal(g_b, g_b2)
+    assert np.array_equal(g_short_a, g_short_b)
+    assert not np.array_equal(g_a, g_short_a)
+
+
 def test_adjoint_linear_in_seed():
     # the adjoint is linear in the final-state weight
     print("case: adjoint linearity", flush=True)
