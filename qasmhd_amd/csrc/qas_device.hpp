@@ -209,7 +209,10 @@ QAS_HD T dfc(T fm1, T f0, T f1, T f2)
 // ------------------------------------------------------------------ characteristic reconstruction
 // s[k][m]: k = (ro, vn, vt, vu, bt, bu, pr), m = stencil i-2..i+2.
 // bn: cell-centred normal B.  vl: face i+1/2 (left state), vr: face i-1/2.
-// MODE 0: both projections formed, selected per lane (the kernels);
+// MODE 0: both projections formed, selected per lane (the kernels); in the
+// FP64 WCNS-4 instances the fast-only terms are skipped by a wavefront none
+// of whose lanes takes the fast branch (the selects would discard them:
+// same results; elsewhere the extra branch cost a wave per SIMD);
 // MODE 1 / 2: only the fast/Alfven resp. Alfven/entropy projection is
 // evaluated (used by the flop model to count one branch).  fbc, when given,
 // receives the positivity-fallback flags (1: left state, 2: right state).
@@ -225,6 +228,15 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
                 (s[n][4] == T(0.0));
   // branch of mhd_func.c:143 (strict >)
   const bool fast = MODE == 1 ? true : (MODE == 2 ? false : (s[0][2] * (s[1][2] * s[1][2] + s[2][2] * s[2][2] + s[3][2] * s[3][2]) > gam * s[6][2]));
+  // whether the fast-only terms are formed: in the FP64 WCNS-4 MODE 0 on
+  // the device, when any lane of the wavefront takes the fast branch (a
+  // uniform value)
+#if defined(__HIP_DEVICE_COMPILE__)
+  constexpr bool WAVE_BRANCH = MODE == 0 && ODR == 4 && std::is_same<T, double>::value;
+  const bool anyf = WAVE_BRANCH ? (__any(fast) != 0) : MODE != 2;
+#else
+  const bool anyf = MODE != 2;
+#endif
   // shared eigen quantities (mhd_eigen_mi / mhd_eigen_ae)
   const T iro = T(1.0) / ro0;
   const T cs2 = gam * pr0 * iro;
@@ -233,14 +245,14 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
   const T cb2 = cs2 + ca2 + ct2;
   const T cf2 = T(0.5) * (cb2 + dsqrt(dabs(cb2 * cb2 - T(4.0) * cs2 * ca2)));
   T rcf = T(0.0), icf2h = T(0.0), ircfh = T(0.0);
-  if (MODE != 2) {
+  if (MODE != 2 && anyf) {
     const T cf = dsqrt(cf2);
     rcf = ro0 * cf;
   }
   const T sro = dsqrt(ro0);
   const T isroh = T(0.5) / sro;
   const T icf2 = T(1.0) / cf2;
-  if (MODE != 2) {
+  if (MODE != 2 && anyf) {
     icf2h = T(0.5) * icf2;
     ircfh = T(0.5) / rcf;
   }
@@ -251,11 +263,14 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
   // k = 0: fast -> -rcf vn + bt0 bt + bu0 bu + pr ; Alfven/entropy -> vn
 #pragma unroll
   for (int m = 0; m < 5; m++) {
-    if (MODE == 2) w[m] = s[1][m];
-    else if (MODE == 1) w[m] = -rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-    else {
+    if (MODE == 1) w[m] = -rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+    else w[m] = s[1][m];
+  }
+  if (MODE == 0 && anyf) {
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
       const T wf = -rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-      w[m] = fast ? wf : s[1][m];
+      w[m] = fast ? wf : w[m];
     }
   }
   interp5<ODR, T, INRANGE>(w, wl[0], wr[0]);
@@ -267,11 +282,13 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
 #pragma unroll
   for (int m = 0; m < 5; m++) {
     if (MODE == 1) w[m] = rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-    else if (MODE == 2) w[m] = bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-    else {
+    else w[m] = bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
+  }
+  if (MODE == 0 && anyf) {
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
       const T wf = rcf * s[1][m] + bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-      const T wa = bt0 * s[4][m] + bu0 * s[5][m] + s[6][m];
-      w[m] = fast ? wf : wa;
+      w[m] = fast ? wf : w[m];
     }
   }
   interp5<ODR, T, INRANGE>(w, wl[2], wr[2]);
@@ -303,14 +320,17 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
       u[1] = q[0];
       u[6] = q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
     } else {
-      const T rof = icf2h * q[0] + icf2 * q[1] + icf2h * q[2];
-      const T roa = icf2 * q[1] + icf2 * q[2];
-      const T vnf = -ircfh * q[0] + ircfh * q[2];
-      const T prf = T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
-      const T pra = q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
-      u[0] = fast ? rof : roa;
-      u[1] = fast ? vnf : q[0];
-      u[6] = fast ? prf : pra;
+      u[0] = icf2 * q[1] + icf2 * q[2];
+      u[1] = q[0];
+      u[6] = q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
+      if (anyf) {
+        const T rof = icf2h * q[0] + icf2 * q[1] + icf2h * q[2];
+        const T vnf = -ircfh * q[0] + ircfh * q[2];
+        const T prf = T(0.5) * q[0] + T(0.5) * q[2] - byh * q[3] + byh * q[4] - bzh * q[5] + bzh * q[6];
+        u[0] = fast ? rof : u[0];
+        u[1] = fast ? vnf : u[1];
+        u[6] = fast ? prf : u[6];
+      }
     }
     u[2] = isroh * q[3] + isroh * q[4];
     u[3] = isroh * q[5] + isroh * q[6];
