@@ -832,6 +832,7 @@ def main():
     s.sync()
     el = time.perf_counter() - t0
     finite = state_finite(s, p)
+    shape = dict(zip(("x_waves_per_workgroup", "elementwise_workgroup"), s.shape_now()))
     value = zones * args.steps / el
     _log("bench: %.3f s for %d steps -> %.4g zone-cycles/s (finite=%s)" % (el, args.steps, value, finite))
     # a second pass of K steps, continuing the run, with the sweep kernels
@@ -964,7 +965,8 @@ def main():
                    "odr": p.odr, "r_k": p.r_k, "ctw": p.ctw, "cfl": p.cfl,
                    "precision": {0: "fp64", 1: "mixed", 2: "fp64_fma"}[args.precision],
                    "dt_final": dt_now, "finite": finite,
-                   "launch": {None: "host (library default)", False: "host", True: "graph replay"}[capi.GRAPHS]},
+                   "launch": {None: "host (library default)", False: "host", True: "graph replay"}[capi.GRAPHS],
+                   "launch_shape": shape},
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -119,6 +119,18 @@ int qas_reset_counter(qas_ctx *c);
  * faster at 256^2 and 512^2 (the host enqueues ahead of the GPU). */
 int qas_use_graphs(qas_ctx *c, int on);
 
+/* 2D launch shape (no counterpart in the reference: its loops have no
+ * launch geometry): x-sweep wavefronts per workgroup (xwpb 1, 2, 4) and the
+ * workgroup size of the element-wise kernels (ewb 64, 128, 256).  Every
+ * shape gives the same results.  qas_tune times RK stage 1 of the uploaded
+ * state in each of the 9 shapes and keeps the fastest (the state is not
+ * changed); the first qas_run of a 2D context calls it unless a shape was
+ * set.  qas_set_shape fixes a shape, qas_shape reports the current one
+ * (default 4, 256).  1D and 3D have one shape. */
+int qas_tune(qas_ctx *c);
+int qas_set_shape(qas_ctx *c, int xwpb, int ewb);
+int qas_shape(const qas_ctx *c, int *xwpb, int *ewb);
+
 /* Block until all work queued on the context has finished. */
 int qas_sync(qas_ctx *c);
 

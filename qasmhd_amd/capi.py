@@ -42,7 +42,8 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_adj_forward", "qas_adj_backward", "qas_adj_backward_device",
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct", "qas_test_rcp",
            "qas_set_dissipation", "qas_dsptv", "qas_set_gravity", "qas_run_static",
-           "qas_batch_adj_forward", "qas_batch_adj_backward", "qas_adj_keep_stages"]
+           "qas_batch_adj_forward", "qas_batch_adj_backward", "qas_adj_keep_stages", "qas_tune", "qas_set_shape",
+           "qas_shape"]
 
 _lib = None
 
@@ -97,6 +98,9 @@ def load(path: str | None = None):
     l.qas_adj_backward.argtypes = [vp, dp, dp]
     l.qas_adj_backward_device.argtypes = [vp, vp, vp]
     l.qas_adj_keep_stages.argtypes = [vp, C.c_int]
+    l.qas_tune.argtypes = [vp]
+    l.qas_set_shape.argtypes = [vp, C.c_int, C.c_int]
+    l.qas_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     l.qas_test_interp.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     l.qas_test_f4.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
     l.qas_test_lrstate.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]
@@ -350,6 +354,22 @@ class Solver:
         d = np.ascontiguousarray(dts, dtype=np.float64)
         self.adj_generation = getattr(self, "adj_generation", 0) + 1
         self._check(self.lib.qas_adj_forward(self.ctx, int(d.size), _p(d)), "qas_adj_forward")
+
+    def tune(self):
+        """Pick the 2D launch shape by timing (qas_tune); returns it."""
+        self._check(self.lib.qas_tune(self.ctx), "qas_tune")
+        return self.shape_now()
+
+    def set_shape(self, xwpb, ewb):
+        """Fix the 2D launch shape: x-sweep wavefronts per workgroup (1, 2, 4),
+        element-wise workgroup size (64, 128, 256) (qas_set_shape)."""
+        self._check(self.lib.qas_set_shape(self.ctx, int(xwpb), int(ewb)), "qas_set_shape")
+
+    def shape_now(self):
+        """The current 2D launch shape (xwpb, ewb) (qas_shape)."""
+        a, b = C.c_int(), C.c_int()
+        self._check(self.lib.qas_shape(self.ctx, C.byref(a), C.byref(b)), "qas_shape")
+        return a.value, b.value
 
     def adj_keep_stages(self, on=True):
         """Later adj_forward calls keep every stage's intermediates (46 more

@@ -302,6 +302,29 @@ static void rk_coef(int r_k, int rk, double* a, double* b)
   else { *a = 1. / 3.; *b = 2. / 3.; }
 }
 
+// the pointers of RK stage rk reading buffer `in` and writing `out` (the
+// step's input in buffer A), with the context's scratch and device dt
+static StagePtrs stage_ptrs(const qas_ctx* c, int rk, int in, int out, int A)
+{
+  StagePtrs p;
+  std::memset(&p, 0, sizeof(p));
+  for (int m = 0; m < 8; m++) {
+    p.in[m] = c->U[in][m];
+    p.out[m] = c->U[out][m];
+    p.u0[m] = c->U[A][m];
+  }
+  p.vx = c->vx; p.vy = c->vy; p.vz = c->vz; p.pr = c->pr;
+  p.cx = c->cx; p.cy = c->cy; p.cz = c->cz;
+  for (int k = 0; k < NACC; k++) p.acc[k] = c->acc[k];
+  for (int k = 0; k < 12; k++) p.emf[k] = c->emf[k];
+  p.ex = c->ex; p.ey = c->ey; p.ez = c->ez;
+  p.dt = c->d_dt;
+  p.phi = c->d_phi;
+  p.enrt = c->d_enrt;
+  rk_coef(c->s.r_k, rk, &p.a, &p.b);
+  return p;
+}
+
 // ideal(dt) with dt already in device memory
 int qas::ctx_step(qas_ctx* c)
 {
@@ -312,22 +335,7 @@ int qas::ctx_step(qas_ctx* c)
   else if (R == 2) { seq_in[0] = A; seq_out[0] = B; seq_in[1] = B; seq_out[1] = A; }
   else { seq_in[0] = A; seq_out[0] = B; seq_in[1] = B; seq_out[1] = C; seq_in[2] = C; seq_out[2] = A; }
   for (int rk = 0; rk < R; rk++) {
-    StagePtrs p;
-    std::memset(&p, 0, sizeof(p));
-    for (int m = 0; m < 8; m++) {
-      p.in[m] = c->U[seq_in[rk]][m];
-      p.out[m] = c->U[seq_out[rk]][m];
-      p.u0[m] = c->U[A][m];
-    }
-    p.vx = c->vx; p.vy = c->vy; p.vz = c->vz; p.pr = c->pr;
-    p.cx = c->cx; p.cy = c->cy; p.cz = c->cz;
-    for (int k = 0; k < NACC; k++) p.acc[k] = c->acc[k];
-    for (int k = 0; k < 12; k++) p.emf[k] = c->emf[k];
-    p.ex = c->ex; p.ey = c->ey; p.ez = c->ez;
-    p.dt = c->d_dt;
-    p.phi = c->d_phi;
-    p.enrt = c->d_enrt;
-    rk_coef(R, rk, &p.a, &p.b);
+    const StagePtrs p = stage_ptrs(c, rk, seq_in[rk], seq_out[rk], A);
     hipError_t e;
     if (c->g.ndim == 1) e = launch_stage1d(c, p);
     else if (c->g.ndim == 2) e = launch_stage2d(c, p);
@@ -359,6 +367,97 @@ extern "C" int qas_cfl_dt(qas_ctx* c, double* dt_out)
   QAS_CHECK(c, hipMemcpyAsync(c->h_pinned, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   QAS_CHECK(c, hipStreamSynchronize(c->stream));
   if (dt_out) *dt_out = c->h_pinned[0];
+  return QAS_OK;
+}
+
+// 2D launch-shape autotuner: x-sweep wavefronts per workgroup (1, 2, 4) and
+// element-wise workgroup size (64, 128, 256).  Every shape gives the same
+// arithmetic per cell (bit-identical results); they differ in how the
+// workgroups fill the GPU, by about 1 %, so each timing brackets 3 steps'
+// worth of RK stages with their ghost fills, the shapes are visited in turn
+// twice after a warm-up, and the shape with the lowest time is kept.  The
+// stages read the current state A (ghosts filled, dt on the device) and
+// run A -> B -> C -> B: the last stage writes the buffer of the first
+// instead of A, so the state is not changed (the run overwrites B, C and
+// the scratch).  Timing stage 1 alone ranked the shapes differently from
+// whole runs.
+int qas::ctx_tune(qas_ctx* c)
+{
+  c->tuned = true;
+  if (c->g.ndim != 2) return QAS_OK;  // one shape in 1D and 3D
+  const int cand[9][2] = {{4, 256}, {2, 256}, {1, 256}, {4, 128}, {2, 128}, {1, 128}, {4, 64}, {2, 64}, {1, 64}};
+  const int A = c->cur, B = (c->cur + 1) % 3;
+  constexpr int NSHAPE = 9, NREP = 3, NROUND = 2;
+  const int R = c->s.r_k;
+  const int C = (c->cur + 2) % 3;
+  // stage rk reads s_in[rk] and writes s_out[rk] (R_K 1: A -> B; 2: A -> B -> C;
+  // 3: A -> B -> C -> B)
+  const int s_in[3] = {A, B, C}, s_out[3] = {B, C, B};
+  StagePtrs ps[3];
+  for (int rk = 0; rk < R; rk++) ps[rk] = stage_ptrs(c, rk, s_in[rk], s_out[rk], A);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  QAS_CHECK(c, hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  float tmin[NSHAPE];
+  for (int k = 0; k < NSHAPE; k++) tmin[k] = -1.0f;
+  for (int round = -1; round < NROUND && e == hipSuccess; round++) {  // round -1: warm-up
+    for (int k = 0; k < NSHAPE && e == hipSuccess; k++) {
+      c->xwpb = cand[k][0];
+      c->ewb = cand[k][1];
+      e = hipEventRecord(e0, c->stream);
+      for (int rep = 0; rep < NREP && e == hipSuccess; rep++)
+        for (int rk = 0; rk < R && e == hipSuccess; rk++) {
+          e = launch_stage2d(c, ps[rk]);
+          if (e == hipSuccess) e = launch_bound(c, c->U[s_out[rk]], 8);
+        }
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess && round >= 0 && (tmin[k] < 0.0f || ms < tmin[k])) tmin[k] = ms;
+    }
+  }
+  int bi = 0;
+  for (int k = 1; k < NSHAPE; k++)
+    if (tmin[k] >= 0.0f && tmin[k] < tmin[bi]) bi = k;
+  (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  c->xwpb = cand[bi][0];
+  c->ewb = cand[bi][1];
+  c->graph_flg = -1;  // a captured graph has the old grid sizes
+  if (e != hipSuccess) return hipfail(c, e, "qas_tune");
+  return QAS_OK;
+}
+
+extern "C" int qas_tune(qas_ctx* c)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_tune: null context");
+  if (!c->uploaded) return fail(c, QAS_E_STATE, "qas_tune: no state uploaded");
+  if (c->prof_mask) return fail(c, QAS_E_STATE, "qas_tune: timing hooks are on");
+  if (!c->bounded) {
+    QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+    c->bounded = true;
+  }
+  return ctx_tune(c);
+}
+
+extern "C" int qas_set_shape(qas_ctx* c, int xwpb, int ewb)
+{
+  if (!ready(c)) return fail(c, QAS_E_ARG, "qas_set_shape: null context");
+  if ((xwpb != 1 && xwpb != 2 && xwpb != 4) || (ewb != 64 && ewb != 128 && ewb != 256))
+    return fail(c, QAS_E_ARG, "qas_set_shape: xwpb 1/2/4, ewb 64/128/256");
+  c->xwpb = xwpb;
+  c->ewb = ewb;
+  c->tuned = true;
+  c->graph_flg = -1;
+  return QAS_OK;
+}
+
+extern "C" int qas_shape(const qas_ctx* c, int* xwpb, int* ewb)
+{
+  if (!c || !xwpb || !ewb) return QAS_E_ARG;
+  *xwpb = c->xwpb;
+  *ewb = c->ewb;
   return QAS_OK;
 }
 
@@ -434,6 +533,15 @@ extern "C" int qas_run(qas_ctx* c, int nsteps, int flg, double* dt_inout, double
   } else {
     QAS_CHECK(c, launch_cfl(c, c->U[c->cur]));
     QAS_CHECK(c, launch_cfl_finalize(c, nullptr));
+  }
+  // the first run of a 2D context picks its launch shape (qas_tune)
+  if (!c->tuned && c->g.ndim == 2 && c->prof_mask == 0 && nsteps > 0) {
+    if (!c->bounded) {
+      QAS_CHECK(c, launch_bound(c, c->U[c->cur], 8));
+      c->bounded = true;
+    }
+    const int rt = ctx_tune(c);
+    if (rt) return rt;
   }
   int n = 0;
   // bring the running counter to an even count before pairing steps

@@ -165,6 +165,11 @@ struct qas_ctx {
   hipGraph_t graph = nullptr;
   int graph_flg = -1, graph_hist = -1, graph_cur = -1;
   bool use_graph = false;
+  // 2D launch shape (qas_tune): x-sweep wavefronts per workgroup and the
+  // workgroup size of the element-wise kernels (EMF, CT); every shape gives
+  // the same results
+  int xwpb = 4, ewb = 256;
+  bool tuned = false;
   double* h_pinned = nullptr;         // small pinned staging
   bool uploaded = false;
   bool bounded = false;
@@ -236,6 +241,7 @@ std::string validate_config(const qas_grid* g, const qas_scheme* s);
 void ctx_init(qas_ctx* c, const qas_grid* g, const qas_scheme* s);
 size_t ctx_layout(qas_ctx* c, char* base, double* state);
 int ctx_step(qas_ctx* c);
+int ctx_tune(qas_ctx* c);  // picks the 2D launch shape (qas_tune)
 // dissipative step (qas_dissipative.hip)
 void dsp_release(qas_ctx* c);
 // gravity (qas_gravity.hip)

@@ -74,17 +74,20 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
   // x sweep applies the stage sum
   QAS_LAUNCH(c, QAS_K_SWEEP_Y, fy, (unsigned)(((g.nx + YC - 1) / YC) * nsy), 256, 0, st, p, g, c->bc, gam, nsy,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
-  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, 256), 256, 0, st, ps, g, c->bc, gam, nsx,
+  // x sweep: one wavefront per (line, segment), c->xwpb per workgroup
+  const unsigned xb = (unsigned)(WAVE * c->xwpb);
+  QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, (int)xb), xb, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   if (faces_only) return hipGetLastError();
   // 3) corner EMF, 4) CT
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
   // (the face parts are stored in the sweep precision)
+  const int eb = c->ewb;
 #define QAS_EMF_CT(O)                                                                                      \
-  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);      \
-  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, 256), 256, 0, st, p, g, c->s.ctw);          \
-  QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<O>), nblk(nc, 256), 256, 0, st, p, g);
+  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);        \
+  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);            \
+  QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<O>), nblk(nc, eb), eb, 0, st, p, g);
   switch (odr) {
     case 1: QAS_EMF_CT(1) break;
     case 2: QAS_EMF_CT(2) break;

@@ -307,6 +307,41 @@ def test_cpp_driver_mpi_files_and_resume(oracle, tmp_path):
             assert np.array_equal(m, st[:, 4:-4, 4:-4].astype(np.float32)), c
 
 
+def test_launch_shapes_bit_identical(qas, oracle):
+    """Every 2D launch shape (x-sweep wavefronts per workgroup, element-wise
+    workgroup size) gives the oracle's result bit for bit; qas_tune picks
+    one of them without changing the state, and qas_run tunes on its first
+    call unless a shape was set."""
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=40, ny=56, seed=6)
+    _say("launch shapes 40x56")
+    cfg = oracle.make_cfg(p.state.shape[1:], p.dx, p.dy, gam=p.gam)
+    ref = p.state.copy()
+    href, _ = oracle.run(cfg, ref, 6, 1)
+    for xwpb, ewb in ((1, 64), (2, 128), (4, 256), (1, 256), (4, 64)):
+        with qas.solver_for(p) as s:
+            s.set_shape(xwpb, ewb)
+            s.upload(p.state)
+            h, _ = s.run(6, flg=1)
+            assert s.shape_now() == (xwpb, ewb)  # a set shape is not re-tuned
+            assert np.array_equal(s.download(), ref), (xwpb, ewb)
+            assert np.array_equal(h, href)
+    with qas.solver_for(p) as s:
+        s.upload(p.state)
+        shp = s.tune()
+        assert shp[0] in (1, 2, 4) and shp[1] in (64, 128, 256)
+        bounded = p.state.copy()
+        oracle.bound(cfg, bounded)  # the tuner fills the ghosts, nothing else
+        assert np.array_equal(s.download(), bounded)
+        h, _ = s.run(6, flg=1)
+        assert np.array_equal(s.download(), ref)
+    with qas.solver_for(p) as s:
+        s.upload(p.state)
+        h, _ = s.run(6, flg=1)  # tunes first
+        assert np.array_equal(s.download(), ref)
+        assert np.array_equal(h, href)
+
+
 @pytest.mark.parametrize("nsteps", [7, 10])
 def test_graph_replay_matches_direct_launches(qas, oracle, nsteps):
     """qas_run with captured 2-step graphs == host-launched steps == oracle,
