@@ -765,9 +765,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed-precision and adjoint lines")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="qas_run launches every kernel from the host in every line (the library default; the "
-                         "batch line keeps its graph)")
-    ap.add_argument("--graphs", action="store_true", help="qas_run replays captured step-pair graphs in every line")
+                    help="every solver and batch launches every kernel from the host (qas_run's default)")
+    ap.add_argument("--graphs", action="store_true", help="every solver and batch replays captured step-pair graphs")
     ap.add_argument("--precision", type=int, default=0,
                     help="0 FP64 (default), 1 mixed, 2 FP64 with FMA contraction, for the main line")
     ap.add_argument("--calibrate-copy", action="store_true",

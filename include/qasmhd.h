@@ -209,6 +209,10 @@ int qas_batch_run(qas_batch *b, int nsteps, int flg, double *dt_inout, double *d
 int qas_batch_tune(qas_batch *b);
 int qas_batch_set_shape(qas_batch *b, int order, int xwpb, int ewb);
 int qas_batch_shape(const qas_batch *b, int *order, int *xwpb, int *ewb);
+/* qas_batch_run replays pairs of steps of all domains as one captured HIP
+ * graph (on = 1, the default) or launches every kernel from the host
+ * (on = 0); the same results either way. */
+int qas_batch_use_graphs(qas_batch *b, int on);
 const char *qas_batch_last_error(const qas_batch *b);
 void qas_batch_destroy(qas_batch *b);
 /* Batched discrete adjoint (no counterpart in the reference): the adjoint

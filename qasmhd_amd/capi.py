@@ -43,7 +43,7 @@ EXPORTS = ["qas_scheme_default", "qas_create", "qas_destroy", "qas_last_error", 
            "qas_test_interp", "qas_test_f4", "qas_test_lrstate", "qas_test_flux", "qas_test_cuct", "qas_test_rcp",
            "qas_set_dissipation", "qas_dsptv", "qas_set_gravity", "qas_run_static",
            "qas_batch_adj_forward", "qas_batch_adj_backward", "qas_adj_keep_stages", "qas_tune", "qas_set_shape",
-           "qas_shape"]
+           "qas_shape", "qas_batch_use_graphs"]
 
 _lib = None
 
@@ -99,6 +99,7 @@ def load(path: str | None = None):
     l.qas_adj_backward_device.argtypes = [vp, vp, vp]
     l.qas_adj_keep_stages.argtypes = [vp, C.c_int]
     l.qas_tune.argtypes = [vp]
+    l.qas_batch_use_graphs.argtypes = [vp, C.c_int]
     l.qas_set_shape.argtypes = [vp, C.c_int, C.c_int]
     l.qas_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     l.qas_test_interp.argtypes = [C.c_int, C.c_int, dp, dp, dp]
@@ -228,10 +229,11 @@ def make_grid(shape, dx, dy=1.0, dz=1.0):
     return g
 
 
-# qas_run's launch mode for the solvers created from here on: None keeps
-# the library's default (every kernel launched from the host); True replays
-# pairs of steps as captured HIP graphs, False sets host launches
-# (qas_use_graphs)
+# launch mode of the solvers and batches created from here on: None keeps
+# the library's defaults (qas_run launches every kernel from the host,
+# qas_batch_run replays captured step pairs); True replays pairs of steps as
+# captured HIP graphs, False launches from the host (qas_use_graphs,
+# qas_batch_use_graphs)
 GRAPHS = None
 
 
@@ -468,6 +470,15 @@ class Batch:
             self.lib.qas_batch_destroy(self.b)
             self.b = None
             raise QasError("qas_batch_create: %s (%d)" % (msg, r))
+        if GRAPHS is not None:
+            self.use_graphs(GRAPHS)
+
+    def use_graphs(self, on=True):
+        """Replay captured step pairs (True) or launch from the host (False)
+        (qas_batch_use_graphs)."""
+        r = self.lib.qas_batch_use_graphs(self.b, 1 if on else 0)
+        if r != QAS_OK:
+            raise QasError("qas_batch_use_graphs (%d)" % r)
 
     def run(self, nsteps, flg=1, dts=None, history=True):
         """qas_run on every domain; returns (dt_history[n, nsteps], dt_now[n])."""

@@ -158,6 +158,7 @@ struct qas_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_flg = -1, graph_hist = -1;
+  bool use_graph = true;  // qas_batch_use_graphs
   long long nstep = 0;
   // launch shape (qas_batch_tune): block order of the instances (0 as given,
   // 1 largest first), wavefronts per x-sweep workgroup, threads per
@@ -452,7 +453,7 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
     QAS_BCHECK(b, batch_step(b, flg, hist, true));
     k++;
   }
-  if (nsteps - k >= 2) {
+  if (b->use_graph && nsteps - k >= 2) {
     if (!b->gexec || b->graph_flg != flg || b->graph_hist != (hist ? 1 : 0)) {
       r = build_batch_graph(b, flg, hist);
       if (r) return r;
@@ -537,6 +538,13 @@ extern "C" int qas_batch_tune(qas_batch* b)
   b->tuned = true;
   b->graph_flg = -1;  // the captured graph has the old grid sizes
   return fill_table(b);
+}
+
+extern "C" int qas_batch_use_graphs(qas_batch* b, int on)
+{
+  if (!b) return QAS_E_ARG;
+  b->use_graph = on != 0;
+  return QAS_OK;
 }
 
 extern "C" int qas_batch_set_shape(qas_batch* b, int order, int xwpb, int ewb)

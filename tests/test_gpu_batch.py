@@ -76,6 +76,30 @@ def test_batch_shapes_are_bit_identical(shape):
         s.close()
 
 
+def test_batch_host_launches_equal_graph_replay():
+    # qas_batch_use_graphs(b, 0): every kernel launched from the host gives
+    # the graph replay's states and dt histories (and the single runs')
+    from qasmhd_amd import capi
+    print("case: batch host launches vs graph replay", flush=True)
+    ps = _problems()
+    outs = {}
+    for graphs in (True, False):
+        solvers = [capi.solver_for(p) for p in ps]
+        for s, p in zip(solvers, ps):
+            s.upload(p.state)
+        with capi.Batch(solvers) as b:
+            b.use_graphs(graphs)
+            hist, dts = b.run(7, flg=1)
+            hist2, _ = b.run(4, flg=1)
+        outs[graphs] = (hist, dts, hist2, [s.download() for s in solvers])
+        for s in solvers:
+            s.close()
+    a, h = outs[True], outs[False]
+    assert np.array_equal(a[0], h[0]) and np.array_equal(a[1], h[1]) and np.array_equal(a[2], h[2])
+    for i in range(len(ps)):
+        assert np.array_equal(a[3][i], h[3][i]), i
+
+
 def test_batch_autotune_picks_a_candidate():
     from qasmhd_amd import capi
     print("case: batch autotune", flush=True)
