@@ -362,7 +362,8 @@ def batch_line(ninst, steps, seed=5):
     for s, p in zip(solvers, ps):
         s.upload(p.state)
     with capi.Batch(solvers) as b:
-        b.run(2, flg=1, history=False)  # launch-shape tuning, graph capture, warm-up
+        # launch-shape tuning, capture of the whole run as one graph, warm-up
+        b.run(steps, flg=1, history=False)
         shape = b.shape()
         for s, p in zip(solvers, ps):
             s.upload(p.state)

@@ -188,8 +188,9 @@ int qas_set_gravity(qas_ctx *c, const double *phi);
 
 /* ---- batched domains (no counterpart in the reference) -----------------
  * Advances many independent 2D domains of uneven sizes with one launch per
- * stage kernel for all of them, two steps per replay of a captured
- * hipGraph.  Each domain is a qas_ctx with its own grid, state and dt; all
+ * stage kernel for all of them; a run of up to 256 steps is one launch of a
+ * captured hipGraph (cached by run length; longer runs replay it).  Each
+ * domain is a qas_ctx with its own grid, state and dt; all
  * must share rmn, odr, r_k (2 or 3), ctw, precision and gam, live on one
  * device, hold an uploaded state and have run the same number of steps.
  * qas_batch_run(b, n, flg, dt, hist) gives every domain i the result of

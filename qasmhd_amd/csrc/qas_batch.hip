@@ -157,7 +157,7 @@ struct qas_batch {
   ct_fn ct = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int graph_flg = -1, graph_hist = -1;
+  int graph_flg = -1, graph_hist = -1, graph_pairs = 0;
   bool use_graph = true;  // qas_batch_use_graphs
   long long nstep = 0;
   // launch shape (qas_batch_tune): block order of the instances (0 as given,
@@ -375,26 +375,34 @@ extern "C" void qas_batch_destroy(qas_batch* b)
 
 extern "C" const char* qas_batch_last_error(const qas_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
-static int build_batch_graph(qas_batch* b, int flg, bool hist)
+// Capture `pairs` consecutive step pairs (odd, even) as one graph: a run of
+// up to 2 * QAS_BATCH_GRAPH_PAIRS steps is one graph launch.  The kernels
+// take dt and the dt-history slot from device memory, so one graph serves
+// every run of the same length.
+constexpr int QAS_BATCH_GRAPH_PAIRS = 128;
+static int build_batch_graph(qas_batch* b, int flg, bool hist, int pairs)
 {
   // wait for replays of the old graph before its executable is destroyed
   if (b->gexec || b->graph) QAS_BCHECK(b, hipStreamSynchronize(b->stream));
   if (b->gexec) { QAS_BCHECK(b, hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
   if (b->graph) { QAS_BCHECK(b, hipGraphDestroy(b->graph)); b->graph = nullptr; }
   QAS_BCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e1 = batch_step(b, flg, hist, false);
-  hipError_t e2 = batch_step(b, flg, hist, true);
+  hipError_t e1 = hipSuccess;
+  for (int q = 0; q < pairs && e1 == hipSuccess; q++) {
+    e1 = batch_step(b, flg, hist, false);
+    if (e1 == hipSuccess) e1 = batch_step(b, flg, hist, true);
+  }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
-  if (e1 != hipSuccess || e2 != hipSuccess || e != hipSuccess) {
+  if (e1 != hipSuccess || e != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
-    return bfail(b, QAS_E_HIP, std::string("batch graph capture: ") +
-                                   hipGetErrorName(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e)));
+    return bfail(b, QAS_E_HIP, std::string("batch graph capture: ") + hipGetErrorName(e1 != hipSuccess ? e1 : e));
   }
   b->graph = g;
   QAS_BCHECK(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
   b->graph_flg = flg;
   b->graph_hist = hist ? 1 : 0;
+  b->graph_pairs = pairs;
   return QAS_OK;
 }
 
@@ -454,13 +462,17 @@ extern "C" int qas_batch_run(qas_batch* b, int nsteps, int flg, double* dt_inout
     k++;
   }
   if (b->use_graph && nsteps - k >= 2) {
-    if (!b->gexec || b->graph_flg != flg || b->graph_hist != (hist ? 1 : 0)) {
-      r = build_batch_graph(b, flg, hist);
+    // the remaining step pairs as one graph launch (runs longer than the
+    // cap: the cap's graph launched repeatedly, the rest step by step)
+    const int pairs = (nsteps - k) / 2;
+    const int gp = pairs < QAS_BATCH_GRAPH_PAIRS ? pairs : QAS_BATCH_GRAPH_PAIRS;
+    if (!b->gexec || b->graph_flg != flg || b->graph_hist != (hist ? 1 : 0) || b->graph_pairs != gp) {
+      r = build_batch_graph(b, flg, hist, gp);
       if (r) return r;
     }
-    for (; k + 2 <= nsteps; k += 2) {
+    for (; k + 2 * gp <= nsteps; k += 2 * gp) {
       QAS_BCHECK(b, hipGraphLaunch(b->gexec, b->stream));
-      ns += 2;
+      ns += 2 * gp;
     }
   }
   for (; k < nsteps; k++) {

@@ -91,7 +91,8 @@ class VyModeInversion:
     def schedule(self, warm=False):
         """Batched forward at a0 (dt from setdt every 2nd step): fixes each
         instance's dt schedule; returns the final states at a0.  With `warm`,
-        two steps run first (graph capture) and the states are re-uploaded;
+        the run goes once first (launch-shape tuning, capture of the whole
+        run as one graph) and the states are re-uploaded;
         `self.batch_seconds` is the wall time of the batched run."""
         solvers = [self._solver(i) for i in range(self.ninst)]
         try:
@@ -99,7 +100,7 @@ class VyModeInversion:
                 s.upload(self.state(i, self.a0[i]))
             with capi.Batch(solvers) as b:
                 if warm:
-                    b.run(2, flg=1, history=True)
+                    b.run(self.steps, flg=1, history=True)
                     for i, s in enumerate(solvers):
                         s.upload(self.state(i, self.a0[i]))
                         s.reset_counter()

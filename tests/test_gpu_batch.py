@@ -100,6 +100,31 @@ def test_batch_host_launches_equal_graph_replay():
         assert np.array_equal(a[3][i], h[3][i]), i
 
 
+def test_batch_run_longer_than_one_graph():
+    # a run is captured as one graph of up to 128 step pairs: 515 steps are
+    # two launches of that graph and three host-launched steps, and give
+    # every domain its own qas_run's state and dt history
+    from qasmhd_amd import capi
+    print("case: batch run of 515 steps (graph cap)", flush=True)
+    ps = _problems()[:3]
+    ref = []
+    for p in ps:
+        with capi.solver_for(p) as s:
+            s.upload(p.state)
+            h, dt = s.run(515, flg=1)
+            ref.append((h, dt, s.download()))
+    solvers = [capi.solver_for(p) for p in ps]
+    for s, p in zip(solvers, ps):
+        s.upload(p.state)
+    with capi.Batch(solvers) as b:
+        hist, dts = b.run(515, flg=1)
+    for i, s in enumerate(solvers):
+        assert np.array_equal(hist[i], ref[i][0]), i
+        assert dts[i] == ref[i][1], i
+        assert np.array_equal(s.download(), ref[i][2]), i
+        s.close()
+
+
 def test_batch_autotune_picks_a_candidate():
     from qasmhd_amd import capi
     print("case: batch autotune", flush=True)
