@@ -157,14 +157,16 @@ def test_batch_rejects_mixed_schemes():
 def test_batch_history_growth_then_single_run():
     """A batch run with history over more steps than a context's history
     buffer reallocates that buffer; the context's own step-pair graph
-    (captured by an earlier qas_run with history) must not keep writing the
-    freed one.  s.run(4) -> batch.run(8) -> s.run(4): the last history and
-    state equal those of a fresh context run over the same 16 steps."""
+    (captured by an earlier qas_run with history, graph replay on) must not
+    keep writing the freed one.  s.run(4) -> batch.run(8) -> s.run(4): the
+    last history and state equal those of a fresh context run over the same
+    16 steps."""
     from qasmhd_amd import capi
     print("case: batch history growth", flush=True)
     ps = _problems()[:2]
     solvers = [capi.solver_for(p) for p in ps]
     for s, p in zip(solvers, ps):
+        s.lib.qas_use_graphs(s.ctx, 1)
         s.upload(p.state)
         s.run(4, flg=1)
     with capi.Batch(solvers) as b:
