@@ -74,6 +74,7 @@ struct AdjPtrs {
                          // four cells), bx, by (each of the two faces)
   const double* dt;
   double a, b;
+  BcFlags bc;            // boundary flags: the adjoint of bound() in the folds
 };
 
 // ---------------------------------------------------------------- helpers
@@ -119,6 +120,75 @@ __device__ __forceinline__ double fold_at(const double* f, int i, int j, const G
 }
 
 __device__ __forceinline__ bool kept(int i, int n) { return i >= NG && i <= n - NG - 1; }
+
+// The bound() map of any boundary kind (common_func.c:60-138): a ghost
+// index holds (factor x) the value at its source index (resolve).  Its
+// adjoint gathers, at a source index q, q itself and every ghost whose
+// source is q, with that ghost's factor (an open boundary copies its edge
+// cell into all NG ghosts).  Candidate c of an axis: c = 0 is q itself,
+// c = 1 + 2 g + s the ghost g (s = 0) or n - 1 - g (s = 1); its index and
+// factor are recomputed on the fly (a per-thread array indexed at run time
+// would live in scratch memory).
+__device__ __forceinline__ bool is_source(int i, int n, int st, int dn)
+{
+  const Src s = resolve(i, n, st, dn);
+  return s.i == i && !s.mul;
+}
+__device__ __forceinline__ double candidate(int c, int q, int n, int st, int dn, int* idx)
+{
+  if (c == 0) {
+    *idx = q;
+    return 1.0;
+  }
+  const int g = (c - 1) >> 1;
+  const int gi = ((c - 1) & 1) ? n - 1 - g : g;
+  *idx = gi;
+  const Src s = resolve(gi, n, st, dn);
+  return ((s.i != gi || s.mul) && s.i == q) ? (s.mul ? s.k : 1.0) : 0.0;
+}
+// f at (i, j) plus (factor x) f at every ghost bound() fills from it, the
+// x map of (stx, dnx) and the y map of (sty, dny) composed as bound()
+// composes them (y ghosts of x ghosts); any boundary kinds.  No ghost maps
+// to an index more than 2 NG + 1 away from both ends.
+template <typename F>
+__device__ __forceinline__ double fold_general(const F& f, int i, int j, const Geo& g, int stx, int dnx, int sty,
+                                               int dny)
+{
+  const long long nx = g.nx;
+  const int ncx = (i > 2 * NG && i < g.nx - 2 * NG - 1) ? 1 : 1 + 2 * NG;
+  const int ncy = (j > 2 * NG && j < g.ny - 2 * NG - 1) ? 1 : 1 + 2 * NG;
+  double v = 0.0;
+#pragma unroll 1
+  for (int b = 0; b < ncy; b++) {
+    int jy;
+    const double kb = candidate(b, j, g.ny, sty, dny, &jy);
+    if (kb == 0.0) continue;
+    double row = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < ncx; a++) {
+      int ix;
+      const double ka = candidate(a, i, g.nx, stx, dnx, &ix);
+      if (ka != 0.0) row += ka * f(nx * jy + ix);
+    }
+    v += kb * row;
+  }
+  return v;
+}
+// variable m's fold: the periodic images (fold_with) when m is periodic on
+// both axes, else the general gather.  cell: a cell-centred quantity under
+// m's boundary kinds (the cell-centred B of bx, by, whose map has no
+// stagger: prim2d_cell resolves it with st = 0)
+template <typename F>
+__device__ __forceinline__ double fold_var(const F& f, int i, int j, const Geo& g, const BcFlags& bc, int m,
+                                           bool cell = false)
+{
+  if (bc.dn[0][m] == 0 && bc.dn[1][m] == 0) return fold_with(f, i, j, g);
+  return fold_general(f, i, j, g, cell ? 0 : bc.st[0][m], bc.dn[0][m], cell ? 0 : bc.st[1][m], bc.dn[1][m]);
+}
+__device__ __forceinline__ bool source_cell(int i, int j, const Geo& g, const BcFlags& bc, int m)
+{
+  return is_source(i, g.nx, bc.st[0][m], bc.dn[0][m]) && is_source(j, g.ny, bc.st[1][m], bc.dn[1][m]);
+}
 
 // inputs of the CUCT weight at corner ss (k_emf2d): ro, vx, vy of the cells
 // (i-1,j-1), (i,j-1), (i-1,j), (i,j); bx(i,j-1), bx(i,j), by(i-1,j), by(i,j)
@@ -699,14 +769,14 @@ __device__ __forceinline__ void gin_face_at(const AdjPtrs& p, const Geo& g, doub
   for (int kk = 0; kk < 4; kk++) {
     const int iq = i - (kk - 1);
     const int iqk = iq < NG ? NG : (iq > g.nx - NG - 1 ? g.nx - NG - 1 : iq);
-    const double v = fold_with(tcx, iqk, jk, g);
+    const double v = fold_var(tcx, iqk, jk, g, p.bc, 4, true);
     gbx += (kept(j, g.ny) && kept(iq, g.nx)) ? fw[kk] * v : 0.0;
   }
 #pragma unroll
   for (int kk = 0; kk < 4; kk++) {
     const int jq = j - (kk - 1);
     const int jqk = jq < NG ? NG : (jq > g.ny - NG - 1 ? g.ny - NG - 1 : jq);
-    const double v = fold_with(tcy, ik, jqk, g);
+    const double v = fold_var(tcy, ik, jqk, g, p.bc, 5, true);
     gby += (kept(i, g.nx) && kept(jq, g.ny)) ? fw[kk] * v : 0.0;
   }
   p.gI[4][s] = gbx;
@@ -741,8 +811,9 @@ __device__ __forceinline__ void foldstage_at(const AdjPtrs& p, const Geo& g, int
 {
   if (t >= g.nd) return;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
-  if (!kept(i, g.nx) || !kept(j, g.ny)) return;
-  double v = fold_at(p.gI[m], i, j, g);
+  if (!source_cell(i, j, g, p.bc, m)) return;
+  const double* gi = p.gI[m];
+  double v = fold_var([gi](long long q) { return gi[q]; }, i, j, g, p.bc, m);
   const double go = p.gO[m][t];
   if (p.a != 0.0) {
     const double gu = first ? 0.0 : p.gU0[m][t];
@@ -762,6 +833,7 @@ struct FoldArgs {
   const double* src[8];
   double* dst[8];
   const double* extra[8];  // added at kept cells when non-null
+  BcFlags bc;
 };
 
 // grid: (cells, 8 variables)
@@ -770,8 +842,9 @@ __device__ __forceinline__ void fold_cell(const FoldArgs& f, const Geo& g, long 
   if (t >= g.nd) return;
   const int i = (int)(t % g.nx), j = (int)(t / g.nx);
   double v = 0.0;
-  if (kept(i, g.nx) && kept(j, g.ny)) {
-    v = fold_at(f.src[m], i, j, g);
+  if (source_cell(i, j, g, f.bc, m)) {
+    const double* src = f.src[m];
+    v = fold_var([src](long long q) { return src[q]; }, i, j, g, f.bc, m);
     if (f.extra[m]) v += f.extra[m][t];
   }
   f.dst[m][t] = v;
@@ -1021,9 +1094,12 @@ static std::string adj_supported(const qas_ctx* c)
   if (c->s.odr != 4) return "the adjoint covers odr 4 only";
   if (c->s.r_k != 3) return "the adjoint covers r_k 3 only";
   if (c->d_phi || c->d_nu) return "the adjoint covers the ideal step without gravity or dissipation";
+  // any boundary kind with HLLD; with LHLLD and MLAU periodic only (the
+  // adjoint of their shock sensor gathers over periodic images)
   for (int a = 0; a < 2; a++)
     for (int m = 0; m < 8; m++)
-      if (c->s.dn[a][m] != 0) return "the adjoint covers periodic boundaries only";
+      if (c->s.dn[a][m] != 0 && c->s.rmn != 1)
+        return "the adjoint covers non-periodic boundaries with HLLD (rmn 1) only";
   return "";
 }
 
@@ -1155,6 +1231,7 @@ static hipError_t adj_stage(qas_ctx* c, int rk, const StageSlot& sl, double* con
   const double ra[3] = {0.0, 0.75, 1.0 / 3.0}, rb[3] = {1.0, 0.25, 2.0 / 3.0};
   p.a = ra[rk];
   p.b = rb[rk];
+  p.bc = c->bc;
   const unsigned nb = nblk(g.nd, 256);
   const bool ctw = c->s.ctw != 0;
   const int csx = (g.nx + SEGA - 1) / SEGA, csy = (g.ny + SEGA - 1) / SEGA;
@@ -1343,6 +1420,7 @@ int adjb_create(AdjBatchDev** out, qas_ctx* const* ctxs, double* const* ub, int 
       p.dt = w->dslot;
       p.a = ra[rk];
       p.b = rb[rk];
+      p.bc = c->bc;
       // the y pass's view on the transposed copies (as adj_stage)
       AdjPtrs pt = p;
       pt.in[0] = w->tin[0];
@@ -1379,6 +1457,7 @@ int adjb_create(AdjBatchDev** out, qas_ctx* const* ctxs, double* const* ub, int 
       A.fin.dst[m] = w->gO[m];
       A.fin.extra[m] = nullptr;
     }
+    A.fin.bc = c->bc;
     A.nb = (int)nblk(g.nd, 256);
     A.csx = (g.nx + SEGA - 1) / SEGA;
     A.csy = (g.ny + SEGA - 1) / SEGA;
@@ -1516,6 +1595,7 @@ extern "C" int qas_adj_backward_device(qas_ctx* c, const double* g_final, double
       f.dst[m] = w->gO[m];
       f.extra[m] = nullptr;
     }
+    f.bc = c->bc;
     hipLaunchKernelGGL(k_adj_fold, dim3(nb, 8), 256, 0, c->stream, f, g);
     QAS_ACHECK(c, hipGetLastError());
   }

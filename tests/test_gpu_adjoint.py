@@ -39,9 +39,11 @@ def _with_vz_bz(p, a=0.1):
     return p
 
 
-def _setup(n=32, seed=5, ctw=1, ny=None, rmn=1):
+def _setup(n=32, seed=5, ctw=1, ny=None, rmn=1, dn=None):
     from qasmhd_amd import capi, problems
     p = _with_vz_bz(problems.turbulence2d(n=n, seed=seed, vrms=0.3, brms=0.3, ny=ny))
+    if dn is not None:
+        p.dn = np.array([dn[0], dn[1], [0] * 8])
     s = capi.solver_for(p, ctw=ctw, rmn=rmn)
     s.upload(p.state)
     hist, _ = s.run(3, flg=1)          # a realistic dt schedule
@@ -69,6 +71,52 @@ def _dir_deriv(s, u0, dts, W, d, h):
 # (MLAU) and 2 (LHLLD) add the shock sensor: dv0 along the line, dv1 from
 # the transverse velocity differences (gathered onto cells of the
 # neighbouring lines).
+# boundary flags (dnxs, dnys of the reference drivers): 2 open, +-1 mirror
+# (-1: the normal velocity and normal B odd), 0 periodic; bx is staggered
+# along x and by along y (the default st flags)
+WALLS = {"open_x_mirror_y": ([2] * 8, [1, 1, -1, 1, 1, -1, 1, 1]),
+         "mirror_x_periodic_y": ([1, -1, 1, 1, -1, 1, 1, 1], [0] * 8)}
+
+
+@pytest.mark.parametrize("walls,ctw,nx,ny", [("open_x_mirror_y", 1, 32, 32), ("open_x_mirror_y", 0, 40, 24),
+                                             ("mirror_x_periodic_y", 1, 24, 40)])
+def test_adjoint_vs_fd_walls(walls, ctw, nx, ny):
+    """Non-periodic boundaries (HLLD): the adjoint of bound() gathers every
+    ghost onto its source with the ghost's factor (open boundaries copy the
+    edge cell into all four ghosts, mirrors flip the odd components)."""
+    print("case: adjoint vs FD, %s, %dx%d, 3 steps, ctw %d" % (walls, nx, ny, ctw), flush=True)
+    p, s, dts = _setup(n=nx, ny=ny, ctw=ctw, rmn=1, dn=WALLS[walls])
+    rng = np.random.default_rng(13)
+    u0 = p.state.copy()
+    W = rng.standard_normal(u0.shape)
+    s.upload(u0)
+    s.adj_forward(dts)
+    g = s.adj_backward(W)
+    errs = []
+    for m in range(8):
+        # directions on the interior (the forward overwrites ghosts)
+        d = np.zeros_like(u0)
+        d[m][NG:-NG, NG:-NG] = rng.standard_normal((u0.shape[1] - 2 * NG, u0.shape[2] - 2 * NG))
+        scale = np.abs(u0[m][NG:-NG, NG:-NG]).max() + 1e-3
+        ad = float(np.sum(g * d))
+        fd = _dir_deriv(s, u0, dts, W, d, 1e-7 * scale)
+        err = abs(ad - fd) / max(abs(ad), abs(fd), 1e-300)
+        print("  var %d: adjoint %.12e  fd %.12e  rel %.2e" % (m, ad, fd, err), flush=True)
+        errs.append(err)
+    s.close()
+    assert max(errs) < 1e-5, errs
+
+
+def test_adjoint_walls_need_hlld():
+    # LHLLD and MLAU keep periodic boundaries only (their shock sensor's adjoint)
+    from qasmhd_amd import capi
+    p, s, dts = _setup(n=24, rmn=3, dn=WALLS["open_x_mirror_y"])
+    s.upload(p.state)
+    with pytest.raises(capi.QasError, match="HLLD"):
+        s.adj_forward(dts)
+    s.close()
+
+
 @pytest.mark.parametrize("rmn,ctw,nx,ny", [(1, 1, 32, 32), (1, 0, 32, 32), (1, 1, 40, 24), (1, 1, 24, 40),
                                            (3, 1, 32, 32), (3, 0, 32, 32), (3, 1, 40, 24), (3, 1, 24, 40),
                                            (2, 1, 32, 32), (2, 1, 40, 24)])
