@@ -232,8 +232,9 @@ int qas_batch_adj_backward(qas_batch *b, const double *const *g_final, double *c
 /* ---- discrete adjoint (no counterpart in the reference) ----------------
  * Reverse-mode derivative of `nsteps` steps of ideal(dt) (with the bound()
  * before the first step) at frozen step sizes.  Scope: 2D, FP64, HLLD,
- * LHLLD or MLAU (rmn 1-3), odr 4, r_k 3, ctw 0 or 1, periodic on both axes;
- * other schemes are rejected with QAS_E_ARG.
+ * LHLLD or MLAU (rmn 1-3), odr 4, r_k 3, ctw 0 or 1; HLLD with any boundary
+ * kinds (periodic, open, mirror, staggered flags), LHLLD and MLAU periodic
+ * on both axes; other schemes are rejected with QAS_E_ARG.
  * qas_adj_forward   advances the current state by nsteps steps with the
  *                   given dt values (as qas_step would), keeping each step's
  *                   input state on the device (8 x nd doubles per step).
