@@ -253,18 +253,21 @@ def _ref_binary(rmn, n):
     return path if os.access(path, os.X_OK) else None
 
 
-@pytest.mark.parametrize("rmn", [1, 3])
-def test_adjoint_vs_reference_fd(rmn):
+@pytest.mark.parametrize("rmn,walls", [(1, None), (3, None), (1, "open_x_mirror_y")])
+def test_adjoint_vs_reference_fd(rmn, walls):
     """The adjoint gradient against central finite differences of the
     reference's own CPU code: its 2D driver run at a fixed dt (exec_ with
-    flg = 0: bound, then MHD2D::ideal(dt), every step) on U0 +- h d."""
+    flg = 0: bound, then MHD2D::ideal(dt), every step) on U0 +- h d, with
+    periodic boundaries or the reference's bound() of open x / mirror y."""
     from oracle import build as obuild
     n = 32
     binary = _ref_binary(rmn, n)
     if binary is None:
         pytest.skip("reference 2D driver %dx%d rmn %d not built" % (n, n, rmn))
-    print("case: adjoint vs FD of the reference driver, rmn %d, %dx%d, 3 steps" % (rmn, n, n), flush=True)
-    p, s, hist = _setup(n=n, rmn=rmn)
+    print("case: adjoint vs FD of the reference driver, rmn %d, %dx%d, 3 steps, walls %s"
+          % (rmn, n, n, walls), flush=True)
+    dn = WALLS[walls] if walls else ([0] * 8, [0] * 8)
+    p, s, hist = _setup(n=n, rmn=rmn, dn=None if walls is None else dn)
     dts = [float(hist[0])] * 3
     rng = np.random.default_rng(17)
     u0 = p.state.copy()
@@ -277,7 +280,8 @@ def test_adjoint_vs_reference_fd(rmn):
     inner[NG:-NG, NG:-NG] = True
 
     def ref_final(u):
-        out, _ = obuild.run_ref(binary, u, p.dx, p.gam, len(dts), 0, dt=dts[0], dy=p.dy, threads=1)
+        out, _ = obuild.run_ref(binary, u, p.dx, p.gam, len(dts), 0, dt=dts[0], dnx=dn[0], dny=dn[1],
+                                dy=p.dy, threads=1)
         return out
 
     def D(d, hh):
