@@ -1196,9 +1196,7 @@ static hipError_t fwd_stage(qas_ctx* c, const StageSlot& sl, double* ez, double*
   p.a = ra[rk];
   p.b = rb[rk];
   const bool faces_only = rk == 2 && !whole;
-  hipError_t e = launch_stage2d(c, p, faces_only);
-  if (e != hipSuccess || faces_only) return e;
-  return launch_bound(c, out, 8);
+  return launch_stage2d(c, p, faces_only, true);
 }
 
 // adjoint of stage rk (0..2) whose input is `in` and whose forward

@@ -6,6 +6,7 @@
 #pragma once
 #include "qas_device.hpp"
 #include "qas_internal.hpp"
+#include "qas_common.hpp"
 
 namespace qas {
 
@@ -138,6 +139,24 @@ __device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int c
 }
 
 // ------------------------------------------------------------------ CT
+// the new bx (comp 0, faces i in [4, nx-4], rows j in [4, ny-5]) or by
+// (comp 1, i in [4, nx-5], j in [4, ny-4]) at face ss
+template <int ODR>
+__device__ __forceinline__ double ct2d_face(const StagePtrs& p, const Geo& g, long long ss, int comp)
+{
+  const long long nx = g.nx;
+  const double dtv = *p.dt;
+  const double* ez = p.ez;
+  if (comp == 0) {
+    const double dtdy = dtv / g.dy;
+    const double du = dfc<ODR>(ez[ss - nx], ez[ss], ez[ss + nx], ez[ss + 2 * nx]);
+    return rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du, p.a, p.b);
+  }
+  const double mdtdx = -(dtv / g.dx);  // mhd_updt2d_ctb is called with -dtdx
+  const double du = dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
+  return rk_upd(p.in[5][ss], p.u0[5][ss], -mdtdx * du, p.a, p.b);
+}
+
 template <int ODR>
 __device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long long t)
 {
@@ -146,20 +165,49 @@ __device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long l
   if (t >= (long long)ni * nj) return;
   const int i = 4 + (int)(t % ni);
   const int j = 4 + (int)(t / ni);
+  const long long ss = (long long)g.nx * j + i;
+  if (j <= g.ny - 5) p.out[4][ss] = ct2d_face<ODR>(p, g, ss, 0);
+  if (i <= g.nx - 5) p.out[5][ss] = ct2d_face<ODR>(p, g, ss, 1);
+}
+
+// true where bound() writes face (i, j) of variable m (4: bx, 5: by)
+__device__ __forceinline__ bool bound_writes(const Geo& g, const BcFlags& bc, int m, int i, int j)
+{
+  return resolve(i, g.nx, bc.st[0][m], bc.dn[0][m]).i != i || resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]).i != j;
+}
+
+// CT fused with the ghost fill of the stage output (one launch instead of
+// ct2d + bound): thread t < nc does ct2d_at's faces, except those bound()
+// overwrites (the periodic image of the last face); thread nc + r does
+// ghost r of bound_at_f, and where that ghost's source is a bx or by face
+// this launch updates, it evaluates the CT update of the source face
+// itself (same operations, so the same value: the launch must not write
+// the stage input or the step base, which that update reads).  Cell-centred
+// sources were written by the x sweep, an earlier launch.
+template <int ODR>
+__device__ __forceinline__ void ctb2d_at(const StagePtrs& p, const Geo& g, const BcFlags& bc, const double* phi,
+                                         double gfac, long long nghost, long long t, bool ct_part)
+{
+  if (ct_part) {
+    const int ni = g.nx - 7, nj = g.ny - 7;
+    if (t >= (long long)ni * nj) return;
+    const int i = 4 + (int)(t % ni);
+    const int j = 4 + (int)(t / ni);
+    const long long ss = (long long)g.nx * j + i;
+    if (j <= g.ny - 5 && !bound_writes(g, bc, 4, i, j)) p.out[4][ss] = ct2d_face<ODR>(p, g, ss, 0);
+    if (i <= g.nx - 5 && !bound_writes(g, bc, 5, i, j)) p.out[5][ss] = ct2d_face<ODR>(p, g, ss, 1);
+    return;
+  }
   const long long nx = g.nx;
-  const long long ss = nx * j + i;
-  const double dtv = *p.dt;
-  const double* ez = p.ez;
-  if (j <= g.ny - 5) {
-    const double dtdy = dtv / g.dy;
-    const double du = dfc<ODR>(ez[ss - nx], ez[ss], ez[ss + nx], ez[ss + 2 * nx]);
-    p.out[4][ss] = rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du, p.a, p.b);
-  }
-  if (i <= g.nx - 5) {
-    const double mdtdx = -(dtv / g.dx);  // mhd_updt2d_ctb is called with -dtdx
-    const double du = dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
-    p.out[5][ss] = rk_upd(p.in[5][ss], p.u0[5][ss], -mdtdx * du, p.a, p.b);
-  }
+  auto rd = [&](int m, long long s) -> double {
+    if (m == 4 || m == 5) {
+      const int i = (int)(s % nx), j = (int)(s / nx);
+      const bool in_ct = i >= 4 && j >= 4 && i <= g.nx - 4 - (m == 5) && j <= g.ny - 4 - (m == 4);
+      if (in_ct) return ct2d_face<ODR>(p, g, s, m - 4);
+    }
+    return p.out[m][s];
+  };
+  bound_at_f([&](int m) { return p.out[m]; }, rd, 8, phi, gfac, g, bc, nghost, NG, NG, 0, t);
 }
 
 }  // namespace qas
