@@ -1196,7 +1196,7 @@ static hipError_t fwd_stage(qas_ctx* c, const StageSlot& sl, double* ez, double*
   p.a = ra[rk];
   p.b = rb[rk];
   const bool faces_only = rk == 2 && !whole;
-  return launch_stage2d(c, p, faces_only, true);
+  return launch_stage2d(c, p, faces_only);
 }
 
 // adjoint of stage rk (0..2) whose input is `in` and whose forward

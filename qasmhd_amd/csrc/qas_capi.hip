@@ -338,7 +338,7 @@ int qas::ctx_step(qas_ctx* c)
     const StagePtrs p = stage_ptrs(c, rk, seq_in[rk], seq_out[rk], A);
     hipError_t e;
     if (c->g.ndim == 1) e = launch_stage1d(c, p);
-    else if (c->g.ndim == 2) e = launch_stage2d(c, p, false, true);  // ghost fill inside the CT launch
+    else if (c->g.ndim == 2) e = launch_stage2d(c, p);  // ghost fill inside the CT launch
     else e = launch_stage3d(c, p);
     if (e != hipSuccess) return hipfail(c, e, "stage launch");
     if (c->g.ndim != 2) {
@@ -409,7 +409,7 @@ int qas::ctx_tune(qas_ctx* c)
       e = hipEventRecord(e0, c->stream);
       for (int rep = 0; rep < NREP && e == hipSuccess; rep++)
         for (int rk = 0; rk < R && e == hipSuccess; rk++) {
-          e = launch_stage2d(c, ps[rk], false, true);
+          e = launch_stage2d(c, ps[rk]);
         }
       if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
       if (e == hipSuccess) e = hipEventSynchronize(e1);

@@ -16,12 +16,12 @@ struct BoundArgs {
 
 // fv(m): the array of variable m (a kernel argument's entry, or a pointer
 // read from an instance table in device memory: a local copy of the
-// pointer array indexed by m would sit in scratch memory); rd(m, s): the
-// value of the source cell s of variable m that the ghost copies
-template <typename FV, typename RD>
-__device__ __forceinline__ void bound_at_f(const FV& fv, const RD& rd, int nvar, const double* phi, double gfac,
-                                           const Geo& g, const BcFlags& bc, long long nghost, int ox, int oy, int oz,
-                                           long long t)
+// pointer array indexed by m would sit in scratch memory); skip(m, s):
+// the ghosts of variable m whose source is cell s are written elsewhere
+template <typename FV, typename SKIP>
+__device__ __forceinline__ void bound_at_f(const FV& fv, const SKIP& skip, int nvar, const double* phi,
+                                           double gfac, const Geo& g, const BcFlags& bc, long long nghost, int ox,
+                                           int oy, int oz, long long t)
 {
   if (t >= nghost * nvar) return;
   const int m = (int)(t / nghost);
@@ -72,7 +72,9 @@ __device__ __forceinline__ void bound_at_f(const FV& fv, const RD& rd, int nvar,
   const Src sy = oy ? resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]) : Src{j, 1.0, false};
   const Src sz = oz ? resolve(k, g.nz, bc.st[2][m], bc.dn[2][m]) : Src{k, 1.0, false};
   if (sx.i == i && sy.i == j && sz.i == k) return;  // a kept cell (staggered boundary face)
-  double v = rd(m, (long long)nx * (ny * sz.i + sy.i) + sx.i);
+  const long long s = (long long)nx * (ny * sz.i + sy.i) + sx.i;
+  if (skip(m, s)) return;
+  double v = fv(m)[s];
   v = apply(sx, v);
   v = apply(sy, v);
   v = apply(sz, v);
@@ -83,7 +85,7 @@ template <typename FV>
 __device__ __forceinline__ void bound_at_f(const FV& fv, int nvar, const double* phi, double gfac, const Geo& g,
                                            const BcFlags& bc, long long nghost, int ox, int oy, int oz, long long t)
 {
-  bound_at_f(fv, [&](int m, long long s) { return fv(m)[s]; }, nvar, phi, gfac, g, bc, nghost, ox, oy, oz, t);
+  bound_at_f(fv, [](int, long long) { return false; }, nvar, phi, gfac, g, bc, nghost, ox, oy, oz, t);
 }
 
 __device__ __forceinline__ void bound_at(const BoundArgs& a, const Geo& g, const BcFlags& bc, long long nghost,

@@ -231,10 +231,9 @@ hipError_t launch_zero_counter(qas_ctx* c);
 hipError_t launch_stage1d(qas_ctx* c, const StagePtrs& p);
 // faces_only: primitives and both sweeps (face parts, stage sum) but no
 // corner EMF / CT (the adjoint's recompute of a step's last stage)
-// with_bound: also fills the ghosts of p.out (bound() of the stage output;
-// inside the CT launch unless p.out is p.in or p.u0), so the caller
-// launches no bound()
-hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only = false, bool with_bound = false);
+// (the CT launch also fills the ghosts of p.out: bound() of the stage
+// output, so no launch_bound follows)
+hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only = false);
 hipError_t launch_stage3d(qas_ctx* c, const StagePtrs& p);
 void adj_release(qas_ctx* c);  // frees the adjoint workspace (qas_adjoint2d.hip)
 // context set-up shared by qas_create and the allocation-free step

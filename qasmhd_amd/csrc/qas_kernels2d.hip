@@ -13,7 +13,7 @@
 //            en (the reference's expression)               (:127-199, 341-358)
 //   emf2d    CUCT corner EMF from both sweeps' face parts   (:91-103, 201-232, 308-339)
 //   ct2d     staggered bx, by update                        (:359-380)
-// followed by the bound() kernel (:384-385).
+//            and the bound() of the stage output             (:384-385)
 // A sweep assigns one 64-lane wavefront to 64 consecutive cells of a line;
 // lane l reconstructs cell f0-2+l, neighbour states and fluxes move between
 // lanes by cross-lane shuffles, and the wavefront owns the 60 faces/cells
@@ -36,12 +36,6 @@ __global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
   emf2d_at<ODR, FT>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
 }
 
-template <int ODR>
-__global__ __launch_bounds__(256) void k_ct2d(StagePtrs p, Geo g)
-{
-  ct2d_at<ODR>(p, g, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
-}
-
 // CT and the stage output's ghost fill in one launch: blocks [0, nbct) do
 // the CT faces (renumbered XCD-aware among themselves), the rest the ghosts
 template <int ODR>
@@ -58,7 +52,7 @@ __global__ __launch_bounds__(256) void k_ctb2d(StagePtrs p, Geo g, BcFlags bc, c
 // ------------------------------------------------------------------ launch
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only, bool with_bound)
+hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
 {
   const Geo& g = c->geo;
   const int odr = c->s.odr;
@@ -92,8 +86,7 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only, bool 
   QAS_LAUNCH(c, QAS_K_SWEEP_X, fx, nblk((long long)nsx * g.ny * WAVE, (int)xb), xb, 0, st, ps, g, c->bc, gam, nsx,
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   if (faces_only) return hipGetLastError();
-  // 3) corner EMF, 4) CT (with the ghost fill of the stage output when
-  // with_bound: the caller then launches no bound())
+  // 3) corner EMF, 4) CT with the ghost fill of the stage output
   const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
   const long long nghost = (long long)g.nx * g.ny - (long long)(g.nx - 2 * NG) * (g.ny - 2 * NG);
@@ -101,19 +94,12 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only, bool 
   // (the face parts are stored in the sweep precision)
   const int eb = c->ewb;
   const unsigned nbct = nblk(nc, eb);
-  // a ghost of bx or by evaluates the CT update of its source face from the
-  // stage input and the step base, so the fused launch needs both to stay
-  // unwritten: the last RK stage writes its output over the step base (the
-  // rotating buffers), and there CT and bound() stay two launches
-  const bool fuse = with_bound && p.out[4] != p.in[4] && p.out[4] != p.u0[4] && p.out[5] != p.in[5] &&
-                    p.out[5] != p.u0[5];
+
 #define QAS_EMF_CT(O)                                                                                      \
   if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);        \
   else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);            \
-  if (fuse)                                                                                                \
-    QAS_LAUNCH(c, QAS_K_CT, (k_ctb2d<O>), nbct + nblk(nghost * 8, eb), eb, 0, st, p, g, c->bc, c->d_phi,   \
-               gfac, nghost, (int)nbct);                                                                   \
-  else QAS_LAUNCH(c, QAS_K_CT, (k_ct2d<O>), nbct, eb, 0, st, p, g);
+  QAS_LAUNCH(c, QAS_K_CT, (k_ctb2d<O>), nbct + nblk(nghost * 8, eb), eb, 0, st, p, g, c->bc, c->d_phi, gfac,   \
+             nghost, (int)nbct);
   switch (odr) {
     case 1: QAS_EMF_CT(1) break;
     case 2: QAS_EMF_CT(2) break;
@@ -121,7 +107,6 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only, bool 
     default: QAS_EMF_CT(4) break;
   }
 #undef QAS_EMF_CT
-  if (with_bound && !fuse) return launch_bound(c, p.out, 8);
   return hipGetLastError();
 }
 

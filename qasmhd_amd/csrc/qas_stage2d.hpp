@@ -157,6 +157,7 @@ __device__ __forceinline__ double ct2d_face(const StagePtrs& p, const Geo& g, lo
   return rk_upd(p.in[5][ss], p.u0[5][ss], -mdtdx * du, p.a, p.b);
 }
 
+// CT alone (the batched stages, whose ghost fill is its own launch)
 template <int ODR>
 __device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long long t)
 {
@@ -176,38 +177,110 @@ __device__ __forceinline__ bool bound_writes(const Geo& g, const BcFlags& bc, in
   return resolve(i, g.nx, bc.st[0][m], bc.dn[0][m]).i != i || resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]).i != j;
 }
 
+// the CT face range of variable m (4: bx, 5: by) holds cell (i, j)
+__device__ __forceinline__ bool ct_range(const Geo& g, int m, int i, int j)
+{
+  return i >= 4 && j >= 4 && i <= g.nx - 4 - (m == 5) && j <= g.ny - 4 - (m == 4);
+}
+
+// The ghost columns (or rows) of one axis that bound() fills from the
+// kept cell i, as a mask over the 2 NG ghost slots: slot a < NG is column
+// a, slot a >= NG column n - 2 NG + a (resolve() inverted: periodic images,
+// the mirrored cell, the edge cell of an open boundary for all its ghosts)
+__device__ __forceinline__ unsigned ghost_images(int i, int n, int st, int dn)
+{
+  unsigned msk = 0;
+  if (dn == 0) {
+    if (i >= NG && i < 2 * NG) msk |= 1u << i;
+    const int l = i - (n - 2 * NG);
+    if (l >= 0 && l < NG) msk |= 1u << l;
+  } else if (dn == 1 || dn == -1) {
+    const int l = (2 * NG - 1 + st) - i;
+    if (l >= 0 && l < NG) msk |= 1u << l;
+    const int r = (n - 2 * NG + st) + (n - 1 - i);
+    if (r >= n - NG + st && r < n) msk |= 1u << (r - (n - 2 * NG));
+  } else if (dn == 2 || dn == -2) {
+    if (i == NG) msk |= (1u << NG) - 1;
+    if (i == n - 1 - NG + st) msk |= ((1u << (NG - st)) - 1) << (NG + st);
+  }
+  return msk;
+}
+
+// writes v (the new value of a kept face (i, j)) into every ghost whose
+// bound() source it is: mx, my are its ghost_images on the two axes and
+// fx, fy the factors bound() applies there (the same on both sides of an
+// axis), applied in bound_at_f's order (x, then y)
+__device__ __forceinline__ void push_ghosts(double* f, int nx, int ny, int i, int j, unsigned mx, unsigned my,
+                                            const Src& fx, const Src& fy, double v)
+{
+  for (unsigned rx = mx; rx; rx &= rx - 1) {
+    const int a = __builtin_ctz(rx);
+    const int gx = a < NG ? a : nx - 2 * NG + a;
+    const double vx = apply(fx, v);
+    f[(long long)nx * j + gx] = vx;
+    for (unsigned ry = my; ry; ry &= ry - 1) {
+      const int b = __builtin_ctz(ry);
+      f[(long long)nx * (b < NG ? b : ny - 2 * NG + b) + gx] = apply(fy, vx);
+    }
+  }
+  for (unsigned ry = my; ry; ry &= ry - 1) {
+    const int b = __builtin_ctz(ry);
+    f[(long long)nx * (b < NG ? b : ny - 2 * NG + b) + i] = apply(fy, v);
+  }
+}
+
 // CT fused with the ghost fill of the stage output (one launch instead of
-// ct2d + bound): thread t < nc does ct2d_at's faces, except those bound()
-// overwrites (the periodic image of the last face); thread nc + r does
-// ghost r of bound_at_f, and where that ghost's source is a bx or by face
-// this launch updates, it evaluates the CT update of the source face
-// itself (same operations, so the same value: the launch must not write
-// the stage input or the step base, which that update reads).  Cell-centred
-// sources were written by the x sweep, an earlier launch.
+// ct2d + bound).  Thread t < nc updates the faces (i, j) with i, j in
+// [4, n-4] (bx for j <= ny-5, by for i <= nx-5), except those that bound()
+// overwrites (the periodic image of the last face), and writes each
+// updated face's ghost images itself; thread nc + r fills ghost r as
+// bound_at_f does, except the bx / by ghosts whose source a CT thread
+// updates.  Cell-centred sources were written by the x sweep (an earlier
+// launch); no thread reads a cell another thread of this launch writes.
 template <int ODR>
 __device__ __forceinline__ void ctb2d_at(const StagePtrs& p, const Geo& g, const BcFlags& bc, const double* phi,
                                          double gfac, long long nghost, long long t, bool ct_part)
 {
   if (ct_part) {
-    const int ni = g.nx - 7, nj = g.ny - 7;
+    const int nx = g.nx, ny = g.ny;
+    const int ni = nx - 7, nj = ny - 7;
     if (t >= (long long)ni * nj) return;
     const int i = 4 + (int)(t % ni);
     const int j = 4 + (int)(t / ni);
-    const long long ss = (long long)g.nx * j + i;
-    if (j <= g.ny - 5 && !bound_writes(g, bc, 4, i, j)) p.out[4][ss] = ct2d_face<ODR>(p, g, ss, 0);
-    if (i <= g.nx - 5 && !bound_writes(g, bc, 5, i, j)) p.out[5][ss] = ct2d_face<ODR>(p, g, ss, 1);
+    const long long ss = (long long)nx * j + i;
+    // everything read from the arguments before the first store (a batched
+    // launch reads them from its instance table, which the compiler must
+    // otherwise reload after each store)
+    // only faces within 2 NG + 1 of an edge are ghost sources or ghosts
+    bool w4 = j <= ny - 5, w5 = i <= nx - 5;
+    unsigned mx4 = 0, my4 = 0, mx5 = 0, my5 = 0;
+    if (i <= 2 * NG || i >= nx - 2 * NG - 1 || j <= 2 * NG || j >= ny - 2 * NG - 1) {
+      w4 = w4 && !bound_writes(g, bc, 4, i, j);
+      w5 = w5 && !bound_writes(g, bc, 5, i, j);
+      mx4 = ghost_images(i, nx, bc.st[0][4], bc.dn[0][4]);
+      my4 = ghost_images(j, ny, bc.st[1][4], bc.dn[1][4]);
+      mx5 = ghost_images(i, nx, bc.st[0][5], bc.dn[0][5]);
+      my5 = ghost_images(j, ny, bc.st[1][5], bc.dn[1][5]);
+    }
+    const Src fx4 = resolve(0, nx, bc.st[0][4], bc.dn[0][4]), fy4 = resolve(0, ny, bc.st[1][4], bc.dn[1][4]);
+    const Src fx5 = resolve(0, nx, bc.st[0][5], bc.dn[0][5]), fy5 = resolve(0, ny, bc.st[1][5], bc.dn[1][5]);
+    double* o4 = p.out[4];
+    double* o5 = p.out[5];
+    const double v4 = w4 ? ct2d_face<ODR>(p, g, ss, 0) : 0.0;
+    const double v5 = w5 ? ct2d_face<ODR>(p, g, ss, 1) : 0.0;
+    if (w4) o4[ss] = v4;
+    if (w5) o5[ss] = v5;
+    if ((mx4 | my4 | mx5 | my5) != 0) {
+      if (w4) push_ghosts(o4, nx, ny, i, j, mx4, my4, fx4, fy4, v4);
+      if (w5) push_ghosts(o5, nx, ny, i, j, mx5, my5, fx5, fy5, v5);
+    }
     return;
   }
   const long long nx = g.nx;
-  auto rd = [&](int m, long long s) -> double {
-    if (m == 4 || m == 5) {
-      const int i = (int)(s % nx), j = (int)(s / nx);
-      const bool in_ct = i >= 4 && j >= 4 && i <= g.nx - 4 - (m == 5) && j <= g.ny - 4 - (m == 4);
-      if (in_ct) return ct2d_face<ODR>(p, g, s, m - 4);
-    }
-    return p.out[m][s];
+  auto pushed = [&](int m, long long s) {
+    return (m == 4 || m == 5) && ct_range(g, m, (int)(s % nx), (int)(s / nx));
   };
-  bound_at_f([&](int m) { return p.out[m]; }, rd, 8, phi, gfac, g, bc, nghost, NG, NG, 0, t);
+  bound_at_f([&](int m) { return p.out[m]; }, pushed, 8, phi, gfac, g, bc, nghost, NG, NG, 0, t);
 }
 
 }  // namespace qas
