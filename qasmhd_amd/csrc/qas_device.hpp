@@ -210,13 +210,14 @@ QAS_HD T dfc(T fm1, T f0, T f1, T f2)
 // s[k][m]: k = (ro, vn, vt, vu, bt, bu, pr), m = stencil i-2..i+2.
 // bn: cell-centred normal B.  vl: face i+1/2 (left state), vr: face i-1/2.
 // MODE 0: both projections formed, selected per lane (the kernels); in the
-// FP64 WCNS-4 instances the fast-only terms are skipped by a wavefront none
-// of whose lanes takes the fast branch (the selects would discard them:
-// same results; elsewhere the extra branch cost a wave per SIMD);
+// WCNS-4 instances with WAVEB (FP64, and the FP32 LHLLD / MLAU y sweeps)
+// the fast-only terms are skipped by a wavefront none of whose lanes takes
+// the fast branch (the selects would discard them: same results; in the
+// other FP32 sweeps the extra branch cost a wave per SIMD);
 // MODE 1 / 2: only the fast/Alfven resp. Alfven/entropy projection is
 // evaluated (used by the flop model to count one branch).  fbc, when given,
 // receives the positivity-fallback flags (1: left state, 2: right state).
-template <int ODR, typename T, int MODE = 0, bool INRANGE = true>
+template <int ODR, typename T, int MODE = 0, bool INRANGE = true, bool WAVEB = std::is_same<T, double>::value>
 QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int* fbc = nullptr)
 {
   const T ro0 = s[0][2], bt0 = s[4][2], bu0 = s[5][2], pr0 = s[6][2];
@@ -228,11 +229,11 @@ QAS_HD void lrstate(const T (&s)[7][5], T bn, T gam, T (&vl)[7], T (&vr)[7], int
                 (s[n][4] == T(0.0));
   // branch of mhd_func.c:143 (strict >)
   const bool fast = MODE == 1 ? true : (MODE == 2 ? false : (s[0][2] * (s[1][2] * s[1][2] + s[2][2] * s[2][2] + s[3][2] * s[3][2]) > gam * s[6][2]));
-  // whether the fast-only terms are formed: in the FP64 WCNS-4 MODE 0 on
+  // whether the fast-only terms are formed: in WCNS-4 MODE 0 with WAVEB on
   // the device, when any lane of the wavefront takes the fast branch (a
   // uniform value)
 #if defined(__HIP_DEVICE_COMPILE__)
-  constexpr bool WAVE_BRANCH = MODE == 0 && ODR == 4 && std::is_same<T, double>::value;
+  constexpr bool WAVE_BRANCH = MODE == 0 && ODR == 4 && WAVEB;
   const bool anyf = WAVE_BRANCH ? (__any(fast) != 0) : MODE != 2;
 #else
   const bool anyf = MODE != 2;

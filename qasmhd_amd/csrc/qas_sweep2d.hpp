@@ -60,10 +60,12 @@ constexpr int YRP = YR + 1;    // padded LDS column length
 // FMA: a tag (0 or 1) naming the build of the instantiation: the FP64
 // sweeps built with FMA contraction (qas_fma64.hip) are the same template
 // under FMA = 1, so their kernel symbols differ from the bit-exact ones.
-// FP32 (mixed) LHLLD and MLAU instances run at 3 waves per SIMD: held to 4
-// (128 VGPRs) with the FP64 primitives of the staging they spilled 36-72
-// bytes per lane, and the MLAU y sweep at 2048^2 took 314 us per launch
-// against 272 us at 3 waves (135 VGPRs, no spill).
+// FP32 (mixed) LHLLD and MLAU instances may take up to 170 VGPRs (3 waves
+// per SIMD): held to 128 with the FP64 primitives of the staging they
+// spilled 36-72 bytes per lane (the MLAU y sweep at 2048^2: 314 us per
+// launch against 272 us at 3 waves).  With the wavefront-uniform skip of
+// the fast-only reconstruction terms they compile to 128 VGPRs without a
+// spill (4 waves per SIMD): 276.6 -> about 265 us.
 template <int RMN, int ODR, typename T, bool BATCH, bool GRAV = false, int FMA = 0>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? (RMN >= 2 ? 3 : 4) : (RMN != 1 ? 2 : 1)) void k_sweep2d_y(
     StagePtrs p_arg, Geo g_arg, BcFlags bc_arg, double gam, int nseg_arg, const BatchInst* I, const int* off,
@@ -170,7 +172,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (RMN >= 2 ? 3 : 4) : (RMN != 
     lr_fb<ODR>(pp, qq, (T)sh[8][w][rb], (T)sh[8][w][rb + 1], ql, qr);
   }
   T vl[7], vr[7];
-  lrstate<ODR>(s, (T)sh[7][w][rb], (T)gam, vl, vr);
+  // (FP32 LHLLD / MLAU: the wavefront-uniform skip of the fast-only terms
+  // brings the kernel to 128 VGPRs, 4 waves per SIMD)
+  lrstate<ODR, T, 0, true, (sizeof(T) == 8 || RMN >= 2)>(s, (T)sh[7][w][rb], (T)gam, vl, vr);
   T Lf[7], Rf[7], F[7];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
