@@ -9,7 +9,9 @@ into one shared library exporting the C ABI of include/qasmhd.h.  Flags:
                          qas_mixed.hip and qas_fma64.hip (precision 1, 2)
                          are built with contraction, and the discrete adjoint
                          (qas_adjoint2d.hip) with contraction within each
-                         source expression (-ffp-contract=on)
+                         source expression (-ffp-contract=on) and FP64
+                         division as reciprocal + Newton steps
+                         (-freciprocal-math -fapprox-func)
   -Xclang -target-feature -Xclang -packed-fp32-ops
                          no packed FP32 instructions are generated
 """
@@ -35,10 +37,14 @@ COMMON_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
 # the contracted FP64 sweeps (IEEE FP64 divide and sqrt), and the discrete
 # adjoint.  The adjoint is checked against finite differences, not bit for
 # bit; "on" fuses a*b+c only inside one source expression, so every kernel
-# that inlines the same code (single-domain and batched) rounds alike
+# that inlines the same code (single-domain and batched) rounds alike.  Its
+# divisions are a reciprocal estimate refined by Newton steps and a
+# corrected quotient, without the operand scaling and special-case fix-up
+# of the IEEE sequence (its operands are densities, pressures, wave speeds
+# and smoothness sums, far from the exponent range's ends)
 EXTRA = {"qas_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
          "qas_fma64.hip": ["-ffp-contract=fast"],
-         "qas_adjoint2d.hip": ["-ffp-contract=on"]}
+         "qas_adjoint2d.hip": ["-ffp-contract=on", "-freciprocal-math", "-fapprox-func"]}
 
 
 def lib_path() -> str:

@@ -7,10 +7,11 @@
 // which reproduce MHD2D::ideal (2D/SERIAL/OTvortex/mhd2d_solve.cpp:3-400)
 // bit for bit; non-smooth operations are differentiated at the branch the
 // forward takes (qas_adjoint.hpp).  This unit is built with FMA contraction
-// inside each source expression: the forward quantities it recomputes (face
-// states, branch predicates) round differently from the forward kernels at
-// the last bit, so a branch can differ only within one rounding of its
-// threshold.
+// inside each source expression and with FP64 division as a refined
+// reciprocal (no operand scaling or special-case fix-up): the forward
+// quantities it recomputes (face states, branch predicates) round
+// differently from the forward kernels at the last bit, so a branch can
+// differ only within one rounding of its threshold.
 //
 // Checkpointing: qas_adj_forward keeps every step's input state on the
 // device.  qas_adj_backward walks the steps in reverse; for each step it
