@@ -30,10 +30,18 @@ namespace qas {
 // bodies in qas_stage2d.hpp (shared with the batched kernels).  Blocks are
 // renumbered XCD-aware (xcd_remap): the stencils of neighbouring rows then
 // meet in the same L2.
-template <int ODR, typename FT>
+// corner EMF: CJ corners of a column per thread (CJ = 1: one corner).
+// Measured at 1024^2: 20.7 us with one corner per thread, 18.3 with 2, 18.8
+// with 4 (156 VGPRs); at 256^2 the 2-corner form made the step 0.8 % slower
+// (half the wavefronts for a domain that fills the GPU once), so it is
+// used from EMF_COL_MIN corners on.
+constexpr long long EMF_COL_MIN = 512LL * 512;
+template <int ODR, typename FT, int CJ>
 __global__ __launch_bounds__(256) void k_emf2d(StagePtrs p, Geo g, int ctw)
 {
-  emf2d_at<ODR, FT>(p, g, ctw, (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x);
+  const long long t = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+  if constexpr (CJ == 1) emf2d_at<ODR, FT>(p, g, ctw, t);
+  else emf2d_col<ODR, FT, CJ>(p, g, ctw, t);
 }
 
 // CT and the stage output's ghost fill in one launch: blocks [0, nbct) do
@@ -87,7 +95,8 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
              (const BatchInst*)nullptr, (const int*)nullptr, 0, 0);
   if (faces_only) return hipGetLastError();
   // 3) corner EMF, 4) CT with the ghost fill of the stage output
-  const long long ne = (long long)(g.nx - 5) * (g.ny - 5);
+  const bool col = (long long)(g.nx - 5) * (g.ny - 5) >= EMF_COL_MIN;
+  const long long ne = (long long)(g.nx - 5) * (col ? (g.ny - 5 + 1) / 2 : g.ny - 5);
   const long long nc = (long long)(g.nx - 7) * (g.ny - 7);
   const long long nghost = (long long)g.nx * g.ny - (long long)(g.nx - 2 * NG) * (g.ny - 2 * NG);
   const double gfac = c->d_phi ? (2.0 - gam) / (gam - 1.0) : 0.0;
@@ -96,8 +105,10 @@ hipError_t launch_stage2d(qas_ctx* c, const StagePtrs& p, bool faces_only)
   const unsigned nbct = nblk(nc, eb);
 
 #define QAS_EMF_CT(O)                                                                                      \
-  if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);        \
-  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);            \
+  if (f32 && col) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float, 2>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw); \
+  else if (f32) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, float, 1>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);  \
+  else if (col) QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double, 2>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw); \
+  else QAS_LAUNCH(c, QAS_K_EMF, (k_emf2d<O, double, 1>), nblk(ne, eb), eb, 0, st, p, g, c->s.ctw);         \
   QAS_LAUNCH(c, QAS_K_CT, (k_ctb2d<O>), nbct + nblk(nghost * 8, eb), eb, 0, st, p, g, c->bc, c->d_phi, gfac,   \
              nghost, (int)nbct);
   switch (odr) {

@@ -138,6 +138,96 @@ __device__ __forceinline__ void emf2d_at(const StagePtrs& p, const Geo& g, int c
   p.ez[ss] = ez;
 }
 
+// CJ corners (i, j0 .. j0 + CJ - 1) of one column per thread, each with
+// emf2d_at's arithmetic: the x-sweep face parts along y and the cell and
+// face values of the CUCT weight are loaded once into register windows
+// that the column's corners share (per corner 40 loads in emf2d_at).
+// Window rows past the last corner's stencil are clamped to the array and
+// not used.
+template <int ODR, typename FT, int CJ>
+__device__ __forceinline__ void emf2d_col(const StagePtrs& p, const Geo& g, int ctw, long long t)
+{
+  const int ni = g.nx - 5, nj = g.ny - 5;   // corners [3, n-3]
+  const int ncol = (nj + CJ - 1) / CJ;
+  if (t >= (long long)ni * ncol) return;
+  const int i = 3 + (int)(t % ni);
+  const int j0 = 3 + CJ * (int)(t / ni);
+  const int jend = 3 + nj;                  // one past the last corner row
+  const long long nx = g.nx;
+  auto at = [&](int jj, int di) -> long long { return nx * (jj < g.ny - 1 ? jj : g.ny - 1) + i + di; };
+  // x-sweep face parts at rows j0-3 .. j0+CJ+1 of column i
+  double fu[CJ + 5], fc[CJ + 5];
+#pragma unroll
+  for (int r = 0; r < CJ + 5; r++) {
+    fu[r] = load_as<FT>(p.emf[0], at(j0 - 3 + r, 0));
+    fc[r] = load_as<FT>(p.emf[1], at(j0 - 3 + r, 0));
+  }
+  // CUCT cells (i-1, i) x rows j0-1 .. j0+CJ-1, bx at (i, row), by at (i-1, i) x rows j0 .. j0+CJ-1
+  double ro[CJ + 1][2], va[CJ + 1][2], vb[CJ + 1][2], bx[CJ + 1], by[CJ][2];
+  if (ctw) {
+#pragma unroll
+    for (int r = 0; r < CJ + 1; r++) {
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        const long long c = at(j0 - 1 + r, d - 1);
+        ro[r][d] = p.in[0][c];
+        if constexpr (sizeof(FT) == 8) {
+          va[r][d] = p.vx[c];
+          vb[r][d] = p.vy[c];
+        } else {
+          va[r][d] = vel(p, 0, c);
+          vb[r][d] = vel(p, 1, c);
+        }
+      }
+      bx[r] = p.in[4][at(j0 - 1 + r, 0)];
+    }
+#pragma unroll
+    for (int r = 0; r < CJ; r++) {
+      by[r][0] = p.in[5][at(j0 + r, -1)];
+      by[r][1] = p.in[5][at(j0 + r, 0)];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CJ; c++) {
+    const int j = j0 + c;
+    if (j >= jend) break;
+    const long long ss = nx * j + i;
+    double ct = 0.5;
+    if (ctw)
+      ct = cuct_weight(ro[c][0], ro[c][1], ro[c + 1][0], ro[c + 1][1], va[c][0], va[c][1], va[c + 1][0],
+                       va[c + 1][1], vb[c][0], vb[c][1], vb[c + 1][0], vb[c + 1][1], bx[c], bx[c + 1], by[c][0],
+                       by[c][1]);
+    double ez = 0.0;
+    {  // from the x sweep: By flux interpolated along y to (i-1/2, j-1/2)
+      double a[5], b[5], ul0, ur0, ul1, ur1, dum;
+#pragma unroll
+      for (int m = 0; m < 5; m++) { a[m] = fu[c + m]; b[m] = fc[c + m]; }
+      lin5<ODR>(a, ul0, dum);
+      lin5<ODR>(b, ul1, dum);
+#pragma unroll
+      for (int m = 0; m < 5; m++) { a[m] = fu[c + 1 + m]; b[m] = fc[c + 1 + m]; }
+      lin5<ODR>(a, dum, ur0);
+      lin5<ODR>(b, dum, ur1);
+      ez += -0.5 * ((ul0 + ur0) + (1.0 - ct) * (ul1 + ur1));
+    }
+    {  // from the y sweep: Bx flux interpolated along x
+      const double* gu = p.emf[4];
+      const double* gc = p.emf[5];
+      double a[5], b[5], ul0, ur0, ul1, ur1, dum;
+#pragma unroll
+      for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(gu, ss + (m - 3)); b[m] = load_as<FT>(gc, ss + (m - 3)); }
+      lin5<ODR>(a, ul0, dum);
+      lin5<ODR>(b, ul1, dum);
+#pragma unroll
+      for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(gu, ss + (m - 2)); b[m] = load_as<FT>(gc, ss + (m - 2)); }
+      lin5<ODR>(a, dum, ur0);
+      lin5<ODR>(b, dum, ur1);
+      ez += +0.5 * ((ul0 + ur0) + ct * (ul1 + ur1));
+    }
+    p.ez[ss] = ez;
+  }
+}
+
 // ------------------------------------------------------------------ CT
 // the new bx (comp 0, faces i in [4, nx-4], rows j in [4, ny-5]) or by
 // (comp 1, i in [4, nx-5], j in [4, ny-4]) at face ss
