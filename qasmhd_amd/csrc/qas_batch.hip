@@ -27,11 +27,13 @@ __device__ __forceinline__ long long batch_thread(const int* off, int i, int b)
   return (long long)(b - off[i]) * blockDim.x + threadIdx.x;
 }
 
+// corner EMF, two corners of a column per thread (the batch's launch is
+// one large grid however small its domains)
 template <int ODR, typename FT>
 __global__ __launch_bounds__(256) void kb_emf2d(const BatchInst* I, const int* off, int n, int stage, int ctw)
 {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x), i = batch_find(off, n, b);
-  emf2d_at<ODR, FT>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
+  emf2d_col<ODR, FT, 2>(I[i].st[stage], I[i].g, ctw, batch_thread(off, i, b));
 }
 
 template <int ODR>
@@ -242,7 +244,7 @@ static int fill_table(qas_batch* b)
     B.cflblocks = std::min(cdiv(ninter, 256), CFL_MAX_BLOCKS);
     const int cnt[BK_COUNT] = {cdiv((long long)B.nsx * g.ny * WAVE, (long long)WAVE * b->xwpb),
                                ((g.nx + YC - 1) / YC) * B.nsy,
-                               cdiv((long long)(g.nx - 5) * (g.ny - 5), b->ewb),
+                               cdiv((long long)(g.nx - 5) * ((g.ny - 5 + 1) / 2), b->ewb),
                                cdiv((long long)(g.nx - 7) * (g.ny - 7), b->ewb),
                                cdiv(B.nghost * 8, 256),
                                B.cflblocks};
