@@ -268,8 +268,9 @@ int qas_adj_keep_stages(qas_ctx *c, int on);
 #define QAS_K_SWEEP_Y 2
 #define QAS_K_SWEEP_Z 3
 #define QAS_K_EMF 4       /* CUCT edge EMFs                                 */
-#define QAS_K_CT 5        /* staggered-B update                             */
-#define QAS_K_BOUND 6     /* ghost fill                                     */
+#define QAS_K_CT 5        /* staggered-B update (2D: with the ghost fill of  */
+                          /* the stage output)                              */
+#define QAS_K_BOUND 6     /* ghost fill (2D: before the first step only)    */
 #define QAS_K_CFL 7       /* CFL reduction                                  */
 #define QAS_K_COUNT 8
 
