@@ -333,6 +333,21 @@ int qas_test_flux(int rmn, int n, const double *ql, const double *qr, const doub
 int qas_test_cuct(int n, const double *cells, const double *faces, double *out);
 int qas_test_rcp(int n, const double *d, double *fast, double *div);
 
+/* ---- host runs of the 2D ghost fill (tests; no GPU) ----------------------
+ * The same __host__ __device__ functions the kernels use, run on the host
+ * one thread after another; dn, st = 2 x 8 boundary flags (x, y), arrays
+ * 8 x (nx * ny) doubles with ghosts, x fastest, nx, ny >= 16.
+ *   qas_test_bound_host  bound() of f in place (common/common_func.c:34-138
+ *                        via the k_bound body)
+ *   qas_test_ctb2d_host  the fused CT + ghost-fill launch of one stage with
+ *                        zero EMF (each CT face becomes a u0 + b in; u0 is
+ *                        out itself when alias, as in the last RK stage);
+ *                        the CT threads run first, or the ghost threads
+ *                        (ghosts_first) */
+int qas_test_bound_host(int nx, int ny, const int *dn, const int *st, double *f);
+int qas_test_ctb2d_host(int nx, int ny, const int *dn, const int *st, const double *in, const double *u0,
+                        double *out, double a, double b, int alias, int ghosts_first);
+
 #ifdef __cplusplus
 }
 #endif

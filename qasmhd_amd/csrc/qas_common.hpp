@@ -19,7 +19,7 @@ struct BoundArgs {
 // pointer array indexed by m would sit in scratch memory); skip(m, s):
 // the ghosts of variable m whose source is cell s are written elsewhere
 template <typename FV, typename SKIP>
-__device__ __forceinline__ void bound_at_f(const FV& fv, const SKIP& skip, int nvar, const double* phi,
+__host__ __device__ __forceinline__ void bound_at_f(const FV& fv, const SKIP& skip, int nvar, const double* phi,
                                            double gfac, const Geo& g, const BcFlags& bc, long long nghost, int ox,
                                            int oy, int oz, long long t)
 {
@@ -82,7 +82,7 @@ __device__ __forceinline__ void bound_at_f(const FV& fv, const SKIP& skip, int n
 }
 
 template <typename FV>
-__device__ __forceinline__ void bound_at_f(const FV& fv, int nvar, const double* phi, double gfac, const Geo& g,
+__host__ __device__ __forceinline__ void bound_at_f(const FV& fv, int nvar, const double* phi, double gfac, const Geo& g,
                                            const BcFlags& bc, long long nghost, int ox, int oy, int oz, long long t)
 {
   bound_at_f(fv, [](int, long long) { return false; }, nvar, phi, gfac, g, bc, nghost, ox, oy, oz, t);

@@ -102,7 +102,7 @@ struct Src {
   bool mul;
 };
 
-__device__ __forceinline__ Src resolve(int i, int n, int st, int dn)
+__host__ __device__ __forceinline__ Src resolve(int i, int n, int st, int dn)
 {
   Src r{i, 1.0, false};
   if (dn == 0) {  // periodic: plain copy (common_func.c:67-74)
@@ -132,7 +132,7 @@ __device__ __forceinline__ Src resolve(int i, int n, int st, int dn)
   return r;
 }
 
-__device__ __forceinline__ double apply(const Src& s, double v) { return s.mul ? s.k * v : v; }
+__host__ __device__ __forceinline__ double apply(const Src& s, double v) { return s.mul ? s.k * v : v; }
 
 }  // namespace qas
 

@@ -232,7 +232,7 @@ __device__ __forceinline__ void emf2d_col(const StagePtrs& p, const Geo& g, int 
 // the new bx (comp 0, faces i in [4, nx-4], rows j in [4, ny-5]) or by
 // (comp 1, i in [4, nx-5], j in [4, ny-4]) at face ss
 template <int ODR>
-__device__ __forceinline__ double ct2d_face(const StagePtrs& p, const Geo& g, long long ss, int comp)
+__host__ __device__ __forceinline__ double ct2d_face(const StagePtrs& p, const Geo& g, long long ss, int comp)
 {
   const long long nx = g.nx;
   const double dtv = *p.dt;
@@ -262,13 +262,13 @@ __device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long l
 }
 
 // true where bound() writes face (i, j) of variable m (4: bx, 5: by)
-__device__ __forceinline__ bool bound_writes(const Geo& g, const BcFlags& bc, int m, int i, int j)
+__host__ __device__ __forceinline__ bool bound_writes(const Geo& g, const BcFlags& bc, int m, int i, int j)
 {
   return resolve(i, g.nx, bc.st[0][m], bc.dn[0][m]).i != i || resolve(j, g.ny, bc.st[1][m], bc.dn[1][m]).i != j;
 }
 
 // the CT face range of variable m (4: bx, 5: by) holds cell (i, j)
-__device__ __forceinline__ bool ct_range(const Geo& g, int m, int i, int j)
+__host__ __device__ __forceinline__ bool ct_range(const Geo& g, int m, int i, int j)
 {
   return i >= 4 && j >= 4 && i <= g.nx - 4 - (m == 5) && j <= g.ny - 4 - (m == 4);
 }
@@ -277,7 +277,7 @@ __device__ __forceinline__ bool ct_range(const Geo& g, int m, int i, int j)
 // kept cell i, as a mask over the 2 NG ghost slots: slot a < NG is column
 // a, slot a >= NG column n - 2 NG + a (resolve() inverted: periodic images,
 // the mirrored cell, the edge cell of an open boundary for all its ghosts)
-__device__ __forceinline__ unsigned ghost_images(int i, int n, int st, int dn)
+__host__ __device__ __forceinline__ unsigned ghost_images(int i, int n, int st, int dn)
 {
   unsigned msk = 0;
   if (dn == 0) {
@@ -300,7 +300,7 @@ __device__ __forceinline__ unsigned ghost_images(int i, int n, int st, int dn)
 // bound() source it is: mx, my are its ghost_images on the two axes and
 // fx, fy the factors bound() applies there (the same on both sides of an
 // axis), applied in bound_at_f's order (x, then y)
-__device__ __forceinline__ void push_ghosts(double* f, int nx, int ny, int i, int j, unsigned mx, unsigned my,
+__host__ __device__ __forceinline__ void push_ghosts(double* f, int nx, int ny, int i, int j, unsigned mx, unsigned my,
                                             const Src& fx, const Src& fy, double v)
 {
   for (unsigned rx = mx; rx; rx &= rx - 1) {
@@ -328,7 +328,7 @@ __device__ __forceinline__ void push_ghosts(double* f, int nx, int ny, int i, in
 // updates.  Cell-centred sources were written by the x sweep (an earlier
 // launch); no thread reads a cell another thread of this launch writes.
 template <int ODR>
-__device__ __forceinline__ void ctb2d_at(const StagePtrs& p, const Geo& g, const BcFlags& bc, const double* phi,
+__host__ __device__ __forceinline__ void ctb2d_at(const StagePtrs& p, const Geo& g, const BcFlags& bc, const double* phi,
                                          double gfac, long long nghost, long long t, bool ct_part)
 {
   if (ct_part) {

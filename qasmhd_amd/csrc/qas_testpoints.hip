@@ -11,6 +11,7 @@
 
 #include "../../include/qasmhd.h"
 #include "qas_device.hpp"
+#include "qas_stage2d.hpp"
 
 namespace qas {
 
@@ -241,4 +242,83 @@ extern "C" int qas_test_rcp(int n, const double* d, double* fast, double* div)
   b.get(fast, df, n);
   b.get(div, dv, n);
   return b.finish();
+}
+
+// ---------------------------------------------------------------- host runs
+// The 2D boundary fill and the CT launch with the fused ghost fill, run on
+// the host through the same __host__ __device__ functions the kernels use,
+// one "thread" after another.  dn, st: 2 x 8 boundary flags (x, y); the
+// arrays are 8 x (nx * ny) doubles, x fastest.
+namespace {
+qas::Geo host_geo(int nx, int ny)
+{
+  qas::Geo g{};
+  g.ndim = 2;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = 1;
+  g.sy = nx;
+  g.sz = (long long)nx * ny;
+  g.nd = (long long)nx * ny;
+  g.dx = g.dy = g.dz = 1.0;
+  return g;
+}
+qas::BcFlags host_bc(const int* dn, const int* st)
+{
+  qas::BcFlags bc{};
+  for (int a = 0; a < 2; a++)
+    for (int m = 0; m < 8; m++) {
+      bc.dn[a][m] = (signed char)dn[8 * a + m];
+      bc.st[a][m] = (signed char)st[8 * a + m];
+    }
+  return bc;
+}
+bool host_args_ok(int nx, int ny, const int* dn, const int* st)
+{
+  if (nx < 2 * qas::NG + 8 || ny < 2 * qas::NG + 8 || !dn || !st) return false;
+  for (int k = 0; k < 16; k++)
+    if (dn[k] < -2 || dn[k] > 2 || st[k] < 0 || st[k] > 1) return false;
+  return true;
+}
+}  // namespace
+
+extern "C" int qas_test_bound_host(int nx, int ny, const int* dn, const int* st, double* f)
+{
+  if (!host_args_ok(nx, ny, dn, st) || !f) return QAS_E_ARG;
+  const qas::Geo g = host_geo(nx, ny);
+  const qas::BcFlags bc = host_bc(dn, st);
+  const long long nd = g.nd;
+  const long long nghost = nd - (long long)(nx - 2 * qas::NG) * (ny - 2 * qas::NG);
+  for (long long t = 0; t < 8 * nghost; t++)
+    qas::bound_at_f([&](int m) { return f + m * nd; }, 8, nullptr, 0.0, g, bc, nghost, qas::NG, qas::NG, 0, t);
+  return QAS_OK;
+}
+
+extern "C" int qas_test_ctb2d_host(int nx, int ny, const int* dn, const int* st, const double* in, const double* u0,
+                                   double* out, double a, double b, int alias, int ghosts_first)
+{
+  if (!host_args_ok(nx, ny, dn, st) || !in || !out || (!u0 && !alias)) return QAS_E_ARG;
+  const qas::Geo g = host_geo(nx, ny);
+  const qas::BcFlags bc = host_bc(dn, st);
+  const long long nd = g.nd;
+  std::vector<double> ez((size_t)nd, 0.0);
+  const double dt = 0.5;
+  qas::StagePtrs p{};
+  for (int m = 0; m < 8; m++) {
+    p.in[m] = in + m * nd;
+    p.out[m] = out + m * nd;
+    p.u0[m] = alias ? out + m * nd : u0 + m * nd;
+  }
+  p.ez = ez.data();
+  p.dt = &dt;
+  p.a = a;
+  p.b = b;
+  const long long nc = (long long)(nx - 7) * (ny - 7);
+  const long long nghost = nd - (long long)(nx - 2 * qas::NG) * (ny - 2 * qas::NG);
+  for (int part = 0; part < 2; part++) {
+    const bool ct = (part == 0) != (ghosts_first != 0);
+    const long long n = ct ? nc : 8 * nghost;
+    for (long long t = 0; t < n; t++) qas::ctb2d_at<4>(p, g, bc, nullptr, 0.0, nghost, t, ct);
+  }
+  return QAS_OK;
 }
