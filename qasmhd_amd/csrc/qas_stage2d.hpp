@@ -187,11 +187,15 @@ __device__ __forceinline__ void emf2d_col(const StagePtrs& p, const Geo& g, int 
       by[r][1] = p.in[5][at(j0 + r, 0)];
     }
   }
+  const double* gu = p.emf[4];
+  const double* gc = p.emf[5];
+  double* ezo = p.ez;
+  double ezv[CJ];
 #pragma unroll
   for (int c = 0; c < CJ; c++) {
     const int j = j0 + c;
-    if (j >= jend) break;
-    const long long ss = nx * j + i;
+    // (a corner past the last row computes on clamped rows, never stored)
+    const long long ss = nx * (j < jend ? j : jend - 1) + i;
     double ct = 0.5;
     if (ctw)
       ct = cuct_weight(ro[c][0], ro[c][1], ro[c + 1][0], ro[c + 1][1], va[c][0], va[c][1], va[c + 1][0],
@@ -211,8 +215,6 @@ __device__ __forceinline__ void emf2d_col(const StagePtrs& p, const Geo& g, int 
       ez += -0.5 * ((ul0 + ur0) + (1.0 - ct) * (ul1 + ur1));
     }
     {  // from the y sweep: Bx flux interpolated along x
-      const double* gu = p.emf[4];
-      const double* gc = p.emf[5];
       double a[5], b[5], ul0, ur0, ul1, ur1, dum;
 #pragma unroll
       for (int m = 0; m < 5; m++) { a[m] = load_as<FT>(gu, ss + (m - 3)); b[m] = load_as<FT>(gc, ss + (m - 3)); }
@@ -224,8 +226,13 @@ __device__ __forceinline__ void emf2d_col(const StagePtrs& p, const Geo& g, int 
       lin5<ODR>(b, dum, ur1);
       ez += +0.5 * ((ul0 + ur0) + ct * (ul1 + ur1));
     }
-    p.ez[ss] = ez;
+    ezv[c] = ez;
   }
+  // the stores after every load (a batched launch reads its pointers from
+  // the instance table, which the compiler must otherwise reload)
+#pragma unroll
+  for (int c = 0; c < CJ; c++)
+    if (j0 + c < jend) ezo[nx * (j0 + c) + i] = ezv[c];
 }
 
 // ------------------------------------------------------------------ CT
@@ -257,8 +264,14 @@ __device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long l
   const int i = 4 + (int)(t % ni);
   const int j = 4 + (int)(t / ni);
   const long long ss = (long long)g.nx * j + i;
-  if (j <= g.ny - 5) p.out[4][ss] = ct2d_face<ODR>(p, g, ss, 0);
-  if (i <= g.nx - 5) p.out[5][ss] = ct2d_face<ODR>(p, g, ss, 1);
+  // both updates before the first store: a batched launch reads the stage
+  // pointers from its instance table, which the compiler must otherwise
+  // reload after a store
+  const bool wx = j <= g.ny - 5, wy = i <= g.nx - 5;
+  const double vx = wx ? ct2d_face<ODR>(p, g, ss, 0) : 0.0;
+  const double vy = wy ? ct2d_face<ODR>(p, g, ss, 1) : 0.0;
+  if (wx) p.out[4][ss] = vx;
+  if (wy) p.out[5][ss] = vy;
 }
 
 // true where bound() writes face (i, j) of variable m (4: bx, 5: by)
