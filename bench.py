@@ -565,9 +565,18 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
         grp[groups] = (time.perf_counter() - t0, t_first)
         inv.close()
     t_bat, t_grad_first = min(grp.values())
+    # batched recorded forward, then each instance's own backward (4 groups
+    # on 4 host threads), second call timed
+    inv.loss_and_grad_batched(inv.a0, groups=4, backward="per_instance")
+    t0 = time.perf_counter()
+    Lh, Gh = inv.loss_and_grad_batched(inv.a0, groups=4, backward="per_instance")
+    t_hyb = time.perf_counter() - t0
+    inv.close()
     t0 = time.perf_counter()
     L4, G4 = inv.loss_and_grad(inv.a0)
     t_grad_thr = time.perf_counter() - t0
+    paths = {"batched": t_bat, "batched_forward_per_instance_backward": t_hyb, "per_instance_threads": t_grad_thr}
+    best = min(paths, key=paths.get)
     errs = []
     scale = float(np.abs(G[0]).max())
     for m in range(nfd):
@@ -576,20 +585,22 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
     return {"workload": "%d_instances_2d_vy_mode_inversion_sizes_128-512_mach0.3_hlld_wcns4_ct_rk3" % ninst,
             "steps": steps, "zones": zones, "params": int(G.size), "loss": L,
             "batched_forward_s": t_fwd, "batched_forward_value": zones * steps / t_fwd,
-            "gradient_s": min(t_bat, t_grad_thr), "ratio": min(t_bat, t_grad_thr) / t_fwd,
-            "gradient_path": "batched" if t_bat <= t_grad_thr else "per_instance_threads",
+            "gradient_s": paths[best], "ratio": paths[best] / t_fwd, "gradient_path": best,
             "fd_max_rel_err": max(errs) if errs else None, "fd_checked": nfd, "finite": bool(np.isfinite(G).all()),
             "gradient_batched_s": t_bat, "gradient_batched_first_call_s": t_grad_first,
             "gradient_batched_s_by_groups": {str(k): v[0] for k, v in grp.items()},
             "gradient_per_instance_threads_s": t_grad_thr,
-            "batched_equals_per_instance": bool(L == L4 and np.array_equal(G, G4)),
-            "note": "gradient_s = the faster of two paths with bit-identical results: batched adjoint "
+            "gradient_batched_forward_per_instance_backward_s": t_hyb,
+            "batched_equals_per_instance": bool(L == L4 and np.array_equal(G, G4) and Lh == L4
+                                                and np.array_equal(Gh, G4)),
+            "note": "gradient_s = the fastest of three paths with bit-identical results: batched adjoint "
                     "(second call, contexts and checkpoint room kept from the first; instances in 1 or 4 "
                     "size-balanced batches on as many host threads: qas_batch_adj_forward records the forward "
                     "of a batch in one batched run, qas_batch_adj_backward runs per step the 3 recomputed "
-                    "stages and 3 stage adjoints with one launch per kernel for the batch) and per instance "
-                    "(own context and stream, 4 host threads); both include host copies of states and "
-                    "gradients; forward = one batched run of all instances (graph replay)"}
+                    "stages and 3 stage adjoints with one launch per kernel for the batch), the batched "
+                    "recorded forward followed by each instance's own backward (4 batches on 4 host threads, "
+                    "second call), and per instance (own context and stream, 4 host threads); all include host "
+                    "copies of states and gradients; forward = one batched run of all instances (graph replay)"}
 
 
 def gravity_line(nx, ny, steps, cpu=True):

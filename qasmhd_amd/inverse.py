@@ -194,7 +194,7 @@ class VyModeInversion:
             G[i] = gi
         return L, G
 
-    def loss_and_grad_batched(self, a, groups=4, workers=4):
+    def loss_and_grad_batched(self, a, groups=4, workers=4, backward="batched"):
         """Loss and gradient of all instances with the batched adjoint:
         the instances split into `groups` batches of about equal zone count,
         driven by `workers` host threads, each batch on its own stream, so
@@ -202,10 +202,14 @@ class VyModeInversion:
         wave per SIMD; other kernels fill the rest of each SIMD).  Per batch:
         the recorded forward of its instances in one batched run
         (qas_batch_adj_forward) and the backward with one launch per kernel
-        for all of them (qas_batch_adj_backward).  Same per-instance
-        arithmetic as instance_grad.  The contexts, their checkpoint room
-        and the batches are kept for the next call (an iterative inversion
-        evaluates many gradients): close() frees them."""
+        for all of them (qas_batch_adj_backward), or, with backward
+        "per_instance", each instance's own qas_adj_backward on the
+        checkpoints the batched forward left in its context.  Same
+        per-instance arithmetic as instance_grad.  The contexts, their
+        checkpoint room and the batches are kept for the next call (an
+        iterative inversion evaluates many gradients): close() frees them."""
+        if backward not in ("batched", "per_instance"):
+            raise ValueError("backward: 'batched' or 'per_instance'")
         if self.dts is None:
             self.schedule()
         tgt = self.targets()
@@ -242,8 +246,12 @@ class VyModeInversion:
                 g[0, NG:-NG, NG:-NG] = 2.0 * r[NG:-NG, NG:-NG]
                 gfin.append(g)
                 Ls[i] = float(np.sum(r[NG:-NG, NG:-NG] ** 2))
-            for i, g in zip(mem, b.adj_backward(gfin)):
-                gi[i] = g
+            if backward == "per_instance":
+                for i, s, g in zip(mem, solvers, gfin):
+                    gi[i] = s.adj_backward(g)
+            else:
+                for i, g in zip(mem, b.adj_backward(gfin)):
+                    gi[i] = g
 
         if len(pool) > 1 and workers > 1:
             from concurrent.futures import ThreadPoolExecutor

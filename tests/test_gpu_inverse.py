@@ -95,8 +95,14 @@ def test_batched_gradient_equals_per_instance(inv):
     Lb2, Gb2 = inv.loss_and_grad_batched(inv.a0, groups=1)  # kept contexts, second call
     Lg, Gg = inv.loss_and_grad_batched(inv.a0, groups=2)    # two batches on two host threads
     inv.close()
-    assert L1 == Lb == Lb2 == Lg
+    # batched recorded forward, then each instance's own backward on the
+    # checkpoints the batch left in its context (twice: kept contexts)
+    Lh, Gh = inv.loss_and_grad_batched(inv.a0, groups=2, backward="per_instance")
+    Lh2, Gh2 = inv.loss_and_grad_batched(inv.a0, groups=2, backward="per_instance")
+    inv.close()
+    assert L1 == Lb == Lb2 == Lg == Lh == Lh2
     assert np.array_equal(G1, Gb) and np.array_equal(G1, Gb2) and np.array_equal(G1, Gg)
+    assert np.array_equal(G1, Gh) and np.array_equal(G1, Gh2)
 
 
 def test_config_sizes_batched_gradient_vs_fd():
