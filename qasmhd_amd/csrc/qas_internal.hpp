@@ -76,15 +76,24 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
 // divergences (acc) in element type ST = the sweep precision: double on the
 // FP64 path; float on the mixed path, where the face parts are FP32 values
 // already and the divergences are rounded once on store.
+// G(p): p as a global-memory pointer.  Every array of a stage is in global
+// memory, but a pointer read from device memory (a batch's instance table)
+// is generic to the compiler, whose flat loads and stores also count
+// against the LDS wait counter; accesses through G(p) are global ones.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
+{
+  return (__attribute__((address_space(1))) T*)p;
+}
 template <typename ST>
 __device__ __forceinline__ void store_as(double* a, long long i, double v)
 {
-  reinterpret_cast<ST*>(a)[i] = (ST)v;
+  G(reinterpret_cast<ST*>(a))[i] = (ST)v;
 }
 template <typename ST>
 __device__ __forceinline__ double load_as(const double* a, long long i)
 {
-  return (double)reinterpret_cast<const ST*>(a)[i];
+  return (double)G(reinterpret_cast<const ST*>(a))[i];
 }
 
 template <typename PT>

@@ -33,8 +33,8 @@ __device__ __forceinline__ Prim2d prim2d_cell(const StagePtrs& p, const Geo& g, 
 {
 #pragma clang fp contract(off)
   const long long nx = g.nx;
-  const double* bx = p.in[4];
-  const double* by = p.in[5];
+  const auto bx = G(p.in[4]);
+  const auto by = G(p.in[5]);
   Prim2d r;
   if constexpr (INNER) {
     r.cx = fcen<ODR>(bx[t - 1], bx[t], bx[t + 1], bx[t + 2]);
@@ -55,15 +55,15 @@ __device__ __forceinline__ Prim2d prim2d_cell(const StagePtrs& p, const Geo& g, 
     r.cy = apply(sy, apply(sx, v));
   }
   }
-  const double ro = p.in[0][t];
+  const double ro = G(p.in[0])[t];
   const double iro = 1.0 / ro;
-  r.vx = p.in[1][t] * iro;
-  r.vy = p.in[2][t] * iro;
-  r.vz = p.in[3][t] * iro;
-  const double cz = p.in[6][t];
+  r.vx = G(p.in[1])[t] * iro;
+  r.vy = G(p.in[2])[t] * iro;
+  r.vz = G(p.in[3])[t] * iro;
+  const double cz = G(p.in[6])[t];
   const double v2 = r.vx * r.vx + r.vy * r.vy + r.vz * r.vz;
   const double b2 = r.cx * r.cx + r.cy * r.cy + cz * cz;
-  double en = p.in[7][t];
+  double en = G(p.in[7])[t];
   r.enrt = en;
   if constexpr (GRAV) {
     const double rp = ro * p.phi[t];

@@ -120,22 +120,22 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (RMN >= 2 ? 3 : 4) : (RMN != 
       const Prim2d q = inner ? prim2d_cell<ODR, GRAV, true>(p, g, bc, gam, col, row, a)
                              : prim2d_cell<ODR, GRAV, false>(p, g, bc, gam, col, row, a);
       const T vx = (T)q.vx, vy = (T)q.vy, vz = (T)q.vz, pr = (T)q.pr, cx = (T)q.cx, cy = (T)q.cy;
-      sh[0][c][r] = p.in[0][a];
+      sh[0][c][r] = G(p.in[0])[a];
       sh[1][c][r] = (double)vy;
       sh[2][c][r] = (double)vz;
       sh[3][c][r] = (double)vx;
-      sh[4][c][r] = p.in[6][a];
+      sh[4][c][r] = G(p.in[6])[a];
       sh[5][c][r] = (double)cx;
       sh[6][c][r] = (double)pr;
       sh[7][c][r] = (double)cy;
-      sh[8][c][r] = p.in[5][a];
+      sh[8][c][r] = G(p.in[5])[a];
       if (colu < g.nx && rowu >= lo && rowu < hi) {
-        reinterpret_cast<T*>(p.vx)[a] = vx;
-        reinterpret_cast<T*>(p.vy)[a] = vy;
-        reinterpret_cast<T*>(p.vz)[a] = vz;
-        reinterpret_cast<T*>(p.pr)[a] = pr;
-        reinterpret_cast<T*>(p.cx)[a] = cx;
-        reinterpret_cast<T*>(p.cy)[a] = cy;
+        G(reinterpret_cast<T*>(p.vx))[a] = vx;
+        G(reinterpret_cast<T*>(p.vy))[a] = vy;
+        G(reinterpret_cast<T*>(p.vz))[a] = vz;
+        G(reinterpret_cast<T*>(p.pr))[a] = pr;
+        G(reinterpret_cast<T*>(p.cx))[a] = cx;
+        G(reinterpret_cast<T*>(p.cy))[a] = cy;
         if constexpr (GRAV) p.enrt[a] = q.enrt;
       }
     }
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (RMN >= 2 ? 3 : 4) : (RMN != 
       if (upd) {
         // dt/dy * Dy of (ro, mx, my, mz, bz, en); the x sweep, which runs
         // next, subtracts it in the stage sum
-        const double dtdy = *p.dt / g.dy;
+        const double dtdy = G(p.dt)[0] / g.dy;
 #pragma unroll
         for (int v = 0; v < 6; v++) store_as<T>(p.acc[v], a, dtdy * so[v][c][r]);
       }
@@ -287,23 +287,26 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
 
   // primitive stencil, cyclic order (vn, vt, vu), (bt, bu)
   const PrimView<T> pv = prim_view<T>(p);
-  const T* vn = DIR == 0 ? pv.vx : pv.vy;
-  const T* vt = DIR == 0 ? pv.vy : pv.vz;
-  const T* vu = DIR == 0 ? pv.vz : pv.vx;
-  const T* cn = DIR == 0 ? pv.cx : pv.cy;
-  const T* cb = DIR == 0 ? pv.cy : pv.cx;  // the cell-centred tangential B (the other one is bz)
-  const double* bf = DIR == 0 ? p.in[4] : p.in[5];  // staggered normal B
+  const auto vn = G(DIR == 0 ? pv.vx : pv.vy);
+  const auto vt = G(DIR == 0 ? pv.vy : pv.vz);
+  const auto vu = G(DIR == 0 ? pv.vz : pv.vx);
+  const auto cn = G(DIR == 0 ? pv.cx : pv.cy);
+  const auto cb = G(DIR == 0 ? pv.cy : pv.cx);  // the cell-centred tangential B (the other one is bz)
+  const auto bf = G(DIR == 0 ? p.in[4] : p.in[5]);  // staggered normal B
+  const auto ro = G(p.in[0]);
+  const auto bz = G(p.in[6]);
+  const auto pr = G(pv.pr);
   T s[7][5];
 #pragma unroll
   for (int m = 0; m < 5; m++) {
     const long long a = sc + (m - 2) * o;
-    s[0][m] = (T)p.in[0][a];
+    s[0][m] = (T)ro[a];
     s[1][m] = vn[a];
     s[2][m] = vt[a];
     s[3][m] = vu[a];
-    s[4][m] = DIR == 0 ? cb[a] : (T)p.in[6][a];
-    s[5][m] = DIR == 0 ? (T)p.in[6][a] : cb[a];
-    s[6][m] = pv.pr[a];
+    s[4][m] = DIR == 0 ? cb[a] : (T)bz[a];
+    s[5][m] = DIR == 0 ? (T)bz[a] : cb[a];
+    s[6][m] = pr[a];
   }
   T vl[7], vr[7];
   lrstate<ODR>(s, (T)cn[sc], (T)gam, vl, vr);
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
   // component that pairs with this sweep (x: By, y: Bx)
   T ql, qr;
   {
-    const T* vv = DIR == 0 ? pv.vy : pv.vx;
+    const auto vv = G(DIR == 0 ? pv.vy : pv.vx);
     T pp[5], qq[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
@@ -382,13 +385,13 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
   // expression in its order (mhd2d_solve.cpp:199, 341-358)
   static_assert(DIR == 0, "the per-lane sweep applies the stage sum along x");
   const long long c = base + o * q;
-  const double dtdx = *p.dt / g.dx;
+  const double dtdx = G(p.dt)[0] / g.dx;
   const int vid[6] = {0, 1, 2, 3, 6, 7};
   double un[6];
 #pragma unroll
   for (int v = 0; v < 6; v++) {
     const int m = vid[v];
-    un[v] = rk_upd(p.in[m][c], p.u0[m][c], (-dtdx * du[v]) - load_as<T>(p.acc[v], c), p.a, p.b);
+    un[v] = rk_upd(G(p.in[m])[c], G(p.u0[m])[c], (-dtdx * du[v]) - load_as<T>(p.acc[v], c), p.a, p.b);
   }
   if constexpr (GRAV) {
     // gravity on the momentum with the updated density (RTI/mhd2d_solve.cpp:361-362)
@@ -397,7 +400,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && RMN != 1) ? 2 : 1) void k_s
     un[2] += -p.b * un[0] * 0.5 * (p.phi[c + g.sy] - p.phi[c - g.sy]) * dtdy;
   }
 #pragma unroll
-  for (int v = 0; v < 6; v++) p.out[vid[v]][c] = un[v];
+  for (int v = 0; v < 6; v++) G(p.out[vid[v]])[c] = un[v];
 }
 
 // kernel tables: y sweep and x sweep (single domain or batch)
