@@ -81,9 +81,13 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
 // is generic to the compiler, whose flat loads and stores also count
 // against the LDS wait counter; accesses through G(p) are global ones.
 template <typename T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p)
+__host__ __device__ __forceinline__ auto G(T* p)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
   return (__attribute__((address_space(1))) T*)p;
+#else
+  return p;  // host runs of the same bodies (tests)
+#endif
 }
 template <typename ST>
 __device__ __forceinline__ void store_as(double* a, long long i, double v)

@@ -170,26 +170,26 @@ __device__ __forceinline__ void emf2d_col(const StagePtrs& p, const Geo& g, int 
 #pragma unroll
       for (int d = 0; d < 2; d++) {
         const long long c = at(j0 - 1 + r, d - 1);
-        ro[r][d] = p.in[0][c];
+        ro[r][d] = G(p.in[0])[c];
         if constexpr (sizeof(FT) == 8) {
-          va[r][d] = p.vx[c];
-          vb[r][d] = p.vy[c];
+          va[r][d] = G(p.vx)[c];
+          vb[r][d] = G(p.vy)[c];
         } else {
           va[r][d] = vel(p, 0, c);
           vb[r][d] = vel(p, 1, c);
         }
       }
-      bx[r] = p.in[4][at(j0 - 1 + r, 0)];
+      bx[r] = G(p.in[4])[at(j0 - 1 + r, 0)];
     }
 #pragma unroll
     for (int r = 0; r < CJ; r++) {
-      by[r][0] = p.in[5][at(j0 + r, -1)];
-      by[r][1] = p.in[5][at(j0 + r, 0)];
+      by[r][0] = G(p.in[5])[at(j0 + r, -1)];
+      by[r][1] = G(p.in[5])[at(j0 + r, 0)];
     }
   }
   const double* gu = p.emf[4];
   const double* gc = p.emf[5];
-  double* ezo = p.ez;
+  const auto ezo = G(p.ez);
   double ezv[CJ];
 #pragma unroll
   for (int c = 0; c < CJ; c++) {
@@ -242,16 +242,16 @@ template <int ODR>
 __host__ __device__ __forceinline__ double ct2d_face(const StagePtrs& p, const Geo& g, long long ss, int comp)
 {
   const long long nx = g.nx;
-  const double dtv = *p.dt;
-  const double* ez = p.ez;
+  const double dtv = G(p.dt)[0];
+  const auto ez = G(p.ez);
   if (comp == 0) {
     const double dtdy = dtv / g.dy;
     const double du = dfc<ODR>(ez[ss - nx], ez[ss], ez[ss + nx], ez[ss + 2 * nx]);
-    return rk_upd(p.in[4][ss], p.u0[4][ss], -dtdy * du, p.a, p.b);
+    return rk_upd(G(p.in[4])[ss], G(p.u0[4])[ss], -dtdy * du, p.a, p.b);
   }
   const double mdtdx = -(dtv / g.dx);  // mhd_updt2d_ctb is called with -dtdx
   const double du = dfc<ODR>(ez[ss - 1], ez[ss], ez[ss + 1], ez[ss + 2]);
-  return rk_upd(p.in[5][ss], p.u0[5][ss], -mdtdx * du, p.a, p.b);
+  return rk_upd(G(p.in[5])[ss], G(p.u0[5])[ss], -mdtdx * du, p.a, p.b);
 }
 
 // CT alone (the batched stages, whose ghost fill is its own launch)
@@ -270,8 +270,8 @@ __device__ __forceinline__ void ct2d_at(const StagePtrs& p, const Geo& g, long l
   const bool wx = j <= g.ny - 5, wy = i <= g.nx - 5;
   const double vx = wx ? ct2d_face<ODR>(p, g, ss, 0) : 0.0;
   const double vy = wy ? ct2d_face<ODR>(p, g, ss, 1) : 0.0;
-  if (wx) p.out[4][ss] = vx;
-  if (wy) p.out[5][ss] = vy;
+  if (wx) G(p.out[4])[ss] = vx;
+  if (wy) G(p.out[5])[ss] = vy;
 }
 
 // true where bound() writes face (i, j) of variable m (4: bx, 5: by)
