@@ -44,12 +44,18 @@ def _gpu_run(p, nsteps):
         return s.download(), hist
 
 
-@pytest.mark.parametrize("name,nsteps", [("turb2d_1024", 20), ("khi_2048_mlau", 10)])
+@pytest.mark.parametrize("name,nsteps", [("turb2d_1024", 20), ("khi_2048_mlau", 10), ("turb2d_1024_walls", 6)])
 def test_2d_full_size_bit_exact_vs_reference(name, nsteps):
+    """(The walls case: open x, mirror y with the odd components flipped,
+    bx / by staggered: the ghost fill inside the CT launch and the
+    two-corner EMF at full size against the reference's bound().)"""
     from oracle import build as obuild
     from qasmhd_amd import problems
     if name == "turb2d_1024":
         p = problems.turbulence2d(n=1024, seed=2)
+    elif name == "turb2d_1024_walls":
+        p = problems.turbulence2d(n=1024, seed=2)
+        p.dn = np.array([[2] * 8, [1, 1, -1, 1, 1, -1, 1, 1], [0] * 8])
     else:
         p = problems.kelvin_helmholtz(n=2048)
         assert p.rmn == 3
