@@ -598,7 +598,8 @@ __global__ __launch_bounds__(256, 1) void k_adj_sweep_xy(AdjPtrs px, AdjPtrs py,
 // ---------------------------------------------------------------- cell
 // Adjoint of the stage input at cell s before the ghost fold (gI):
 //   gin_prim_at: ro, mx, my, mz, bz, en from the primitive recovery
-//     (k_prim2d) with the CUCT cell terms, plus the RK pass-through b * g_out;
+//     (prim2d_cell, the y sweep's staging) with the CUCT cell terms, plus the
+//     RK pass-through b * g_out;
 //   gin_face_at: bx, by from the face-B terms of the sweeps and the CUCT
 //     corners, the adjoint of fcen (cx, cy of the kept cells, with their
 //     ghost copies folded on), plus b * g_out.
@@ -708,7 +709,7 @@ __device__ __forceinline__ void gin_prim_at(const AdjPtrs& p, const Geo& g, doub
     }
     double gbz = p.gp[6][s];
     const double gpr = p.gp[7][s];
-    // forward of k_prim2d
+    // forward of the primitive recovery (prim2d_cell)
     const double ro = p.in[0][s];
     const double iro = 1.0 / ro;
     const double mx = p.in[1][s], my = p.in[2][s], mz = p.in[3][s];
