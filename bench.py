@@ -574,7 +574,13 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
     inv.close()
     t0 = time.perf_counter()
     L4, G4 = inv.loss_and_grad(inv.a0)
+    t_grad_thr_fresh = time.perf_counter() - t0
+    # per instance with kept contexts (second call timed, as for the batches)
+    inv.loss_and_grad(inv.a0, keep=True)
+    t0 = time.perf_counter()
+    Lk, Gk = inv.loss_and_grad(inv.a0, keep=True)
     t_grad_thr = time.perf_counter() - t0
+    inv.close()
     paths = {"batched": t_bat, "batched_forward_per_instance_backward": t_hyb, "per_instance_threads": t_grad_thr}
     best = min(paths, key=paths.get)
     errs = []
@@ -590,16 +596,18 @@ def inverse_line(ninst, steps, seed=5, nfd=2):
             "gradient_batched_s": t_bat, "gradient_batched_first_call_s": t_grad_first,
             "gradient_batched_s_by_groups": {str(k): v[0] for k, v in grp.items()},
             "gradient_per_instance_threads_s": t_grad_thr,
+            "gradient_per_instance_threads_fresh_contexts_s": t_grad_thr_fresh,
             "gradient_batched_forward_per_instance_backward_s": t_hyb,
             "batched_equals_per_instance": bool(L == L4 and np.array_equal(G, G4) and Lh == L4
-                                                and np.array_equal(Gh, G4)),
+                                                and np.array_equal(Gh, G4) and Lk == L4 and np.array_equal(Gk, G4)),
             "note": "gradient_s = the fastest of three paths with bit-identical results: batched adjoint "
                     "(second call, contexts and checkpoint room kept from the first; instances in 1 or 4 "
                     "size-balanced batches on as many host threads: qas_batch_adj_forward records the forward "
                     "of a batch in one batched run, qas_batch_adj_backward runs per step the 3 recomputed "
                     "stages and 3 stage adjoints with one launch per kernel for the batch), the batched "
                     "recorded forward followed by each instance's own backward (4 batches on 4 host threads, "
-                    "second call), and per instance (own context and stream, 4 host threads); all include host "
+                    "second call), and per instance (own context and stream, 4 host threads; contexts kept from a "
+                    "first call, the fresh-context time reported beside it); all include host "
                     "copies of states and gradients; forward = one batched run of all instances (graph replay)"}
 
 

@@ -165,18 +165,24 @@ class VyModeInversion:
         grad = np.tensordot(self.psi[i], w, axes=([1, 2], [0, 1]))
         return float(np.sum(r[NG:-NG, NG:-NG] ** 2)), grad
 
-    def loss_and_grad(self, a, workers=4):
+    def loss_and_grad(self, a, workers=4, keep=False):
         """Loss and gradient of all instances.  Each instance has its own
         context (its own HIP stream); `workers` host threads drive them, so
         up to that many instances' kernels run on the GPU side by side
         (small domains alone leave most of it idle).  The contexts stay open
         until every instance is done: freeing device memory waits for the
-        whole device.  Memory: each instance keeps 8 x nd doubles per step
-        of checkpoints (about 100 GB for the 64-instance workload)."""
+        whole device.  With `keep` they stay open for the next call (with
+        their checkpoint room), until close().  Memory: each instance keeps
+        8 x nd doubles per step of checkpoints (about 100 GB for the
+        64-instance workload)."""
         if self.dts is None:
             self.schedule()
         self.targets()
-        solvers = [self._solver(i) for i in range(self.ninst)]
+        solvers = getattr(self, "_ipool", None) if keep else None
+        if solvers is None:
+            solvers = [self._solver(i) for i in range(self.ninst)]
+            if keep:
+                self._ipool = solvers
         try:
             if workers and workers > 1:
                 from concurrent.futures import ThreadPoolExecutor
@@ -185,8 +191,9 @@ class VyModeInversion:
             else:
                 res = [self.instance_grad(i, a[i], solvers[i]) for i in range(self.ninst)]
         finally:
-            for s in solvers:
-                s.close()
+            if not keep:
+                for s in solvers:
+                    s.close()
         L = 0.0
         G = np.zeros_like(np.asarray(a, dtype=np.float64))
         for i, (li, gi) in enumerate(res):
@@ -267,13 +274,18 @@ class VyModeInversion:
         return L, G
 
     def close(self):
-        """Free the contexts loss_and_grad_batched keeps."""
+        """Free the contexts loss_and_grad_batched and loss_and_grad(keep)
+        keep."""
         pool = getattr(self, "_gpool", None)
         self._gpool = None
         for _, solvers, b in pool or []:
             b.close()
             for s in solvers:
                 s.close()
+        ipool = getattr(self, "_ipool", None)
+        self._ipool = None
+        for s in ipool or []:
+            s.close()
 
     def fd_grad(self, i, a, m, h=1e-6, richardson=True):
         """Central difference of instance i's loss in a_m over the frozen

@@ -100,9 +100,14 @@ def test_batched_gradient_equals_per_instance(inv):
     Lh, Gh = inv.loss_and_grad_batched(inv.a0, groups=2, backward="per_instance")
     Lh2, Gh2 = inv.loss_and_grad_batched(inv.a0, groups=2, backward="per_instance")
     inv.close()
-    assert L1 == Lb == Lb2 == Lg == Lh == Lh2
+    # per instance with the contexts kept for the next call
+    Lk, Gk = inv.loss_and_grad(inv.a0, workers=2, keep=True)
+    Lk2, Gk2 = inv.loss_and_grad(inv.a0, workers=2, keep=True)
+    inv.close()
+    assert L1 == Lb == Lb2 == Lg == Lh == Lh2 == Lk == Lk2
     assert np.array_equal(G1, Gb) and np.array_equal(G1, Gb2) and np.array_equal(G1, Gg)
     assert np.array_equal(G1, Gh) and np.array_equal(G1, Gh2)
+    assert np.array_equal(G1, Gk) and np.array_equal(G1, Gk2)
 
 
 def test_config_sizes_batched_gradient_vs_fd():
