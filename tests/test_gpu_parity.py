@@ -94,6 +94,33 @@ def test_2d_vs_oracle(qas, oracle, seed, n, ny, rmn, odr):
     assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("rmn,odr,walls", [(1, 2, False), (3, 3, True), (2, 1, False)])
+def test_2d_column_emf_vs_oracle(qas, oracle, rmn, odr, walls):
+    """Domains of 512^2 corners and more compute two EMF corners of a column
+    per thread (k_emf2d<ODR, FT, 2>): the same arithmetic per corner as the
+    one-corner form the small cases run, for every ODR's interpolation; a
+    ragged last column (an odd corner count along y) and walls."""
+    from qasmhd_amd import problems
+    p = problems.turbulence2d(n=516, ny=518, seed=7, vrms=0.8)
+    dn = np.zeros((3, 8), dtype=int)
+    if walls:
+        dn[0] = [2] * 8
+        dn[1] = [1, 1, -1, 1, 1, -1, 1, 1]
+    ny_g, nx_g = p.state.shape[1:]
+    assert (nx_g - 5) * (ny_g - 5) >= 512 * 512 and (ny_g - 5) % 2 == 1
+    _say("2d column EMF vs oracle", rmn, odr, walls)
+    cfg = oracle.make_cfg(p.state.shape[1:], p.dx, p.dy, gam=p.gam, rmn=rmn, odr=odr, dn=dn)
+    ref = p.state.copy()
+    h_ref, dt_ref = oracle.run(cfg, ref, 2, 1)
+    with qas.Solver(p.state.shape[1:], p.dx, p.dy, gam=p.gam, rmn=rmn, odr=odr, dn=dn) as s:
+        s.upload(p.state)
+        hist, dt_now = s.run(2, flg=1)
+        out = s.download()
+    assert np.isfinite(ref).all()
+    assert np.array_equal(hist, h_ref) and dt_now == dt_ref
+    assert np.array_equal(out, ref)
+
+
 def test_step_and_bound_entry_points(qas, oracle):
     """qas_bound / qas_step / qas_cfl_dt individually == bound / ideal / setdt."""
     from qasmhd_amd import problems
