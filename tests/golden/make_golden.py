@@ -13,7 +13,7 @@ time.  Files:
   unit_vectors.npz  per-function vectors: interpolation (interpol/l_interp
                     tables, ODR 1..4), fcen/df, mhd_lrstate (ODR 1..4),
                     calc_flux_{roe,hlld,lhlld,mlau}, mhd_cuct_weight
-  steps_1d.npz      Brio-Wu 1024 (config 1), all four solvers
+  steps_1d.npz      Brio-Wu 1024 (the 1D shock-tube workload), all four solvers
   steps_2d.npz      2D decaying-turbulence states after N steps: every
                     RMN x ODR pair, R_K 1/2, CTW 0, four boundary families
 """
@@ -146,7 +146,7 @@ def make_steps_1d(path):
     cfg = po.make_cfg(p.state.shape[1:], p.dx, gam=p.gam, dn=p.dn)
     dt0 = po.cfl_dt(cfg, p.state.copy())
     out["dt0"] = np.array(dt0)
-    # config 1: setdt(1) then exec_(0) -> fixed dt, nrec = (int)(0.1/dt + 0.5) steps
+    # the 1D shock tube: setdt(1) then exec_(0) -> fixed dt, nrec = (int)(0.1/dt + 0.5) steps
     nsteps = int(0.1 / dt0 + 0.5)
     out["nsteps"] = np.array(nsteps)
     for rmn in (0, 1, 2, 3):

@@ -107,7 +107,7 @@ def test_fma64_full_size_vs_exact(name):
     the distance is set against the bit-exact path's own response to a
     one-ulp change of the initial density (rho (1 + 2^-52) in every cell).
     The contracted path must stay within 10x of that round-off response.
-    The north star's 1e-12 is met only by the bit-exact path here: measured
+    A 1e-12 relative L1 bar is met only by the bit-exact path here: measured
     one-ulp responses are 6e-11 (1024^2 turbulence, 1000 steps) and 1.7e-6
     (Kelvin-Helmholtz at Mach 0.01, 2000 steps), and the contracted path
     lands at 2.4x and 1.0x of them."""
