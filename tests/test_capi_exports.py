@@ -30,3 +30,11 @@ def test_library_exports_declared_symbols():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     assert set(capi.EXPORTS) <= set(declared())
+
+
+def test_integration_guide_names_every_entry_point():
+    """Each declared entry point has a line in the integration guide (what
+    it replaces, or that it has no counterpart in the reference)."""
+    guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    missing = [n for n in declared() if not re.search(r"\b%s\b" % n, guide)]
+    assert not missing, missing
