@@ -174,10 +174,12 @@ class VyModeInversion:
         whole device.  With `keep` they stay open for the next call (with
         their checkpoint room), until close().  Memory: each instance keeps
         8 x nd doubles per step of checkpoints (about 100 GB for the
-        64-instance workload)."""
+        64-instance workload), so the contexts loss_and_grad_batched kept
+        are freed first."""
         if self.dts is None:
             self.schedule()
         self.targets()
+        self._free_batched()
         solvers = getattr(self, "_ipool", None) if keep else None
         if solvers is None:
             solvers = [self._solver(i) for i in range(self.ninst)]
@@ -214,16 +216,18 @@ class VyModeInversion:
         checkpoints the batched forward left in its context.  Same
         per-instance arithmetic as instance_grad.  The contexts, their
         checkpoint room and the batches are kept for the next call (an
-        iterative inversion evaluates many gradients): close() frees them."""
+        iterative inversion evaluates many gradients): close() frees them.
+        Contexts loss_and_grad(keep=True) kept are freed first."""
         if backward not in ("batched", "per_instance"):
             raise ValueError("backward: 'batched' or 'per_instance'")
         if self.dts is None:
             self.schedule()
         tgt = self.targets()
+        self._free_instances()
         groups = max(1, min(int(groups), self.ninst))
         pool = getattr(self, "_gpool", None)
         if pool is None or len(pool) != groups:
-            self.close()
+            self._free_batched()
             order = sorted(range(self.ninst), key=lambda i: -self.sizes[i] ** 2)
             members, load = [[] for _ in range(groups)], [0] * groups
             for i in order:  # largest first onto the lightest group
@@ -273,19 +277,25 @@ class VyModeInversion:
             G[i] = np.tensordot(self.psi[i], w, axes=([1, 2], [0, 1]))
         return L, G
 
-    def close(self):
-        """Free the contexts loss_and_grad_batched and loss_and_grad(keep)
-        keep."""
+    def _free_batched(self):
         pool = getattr(self, "_gpool", None)
         self._gpool = None
         for _, solvers, b in pool or []:
             b.close()
             for s in solvers:
                 s.close()
+
+    def _free_instances(self):
         ipool = getattr(self, "_ipool", None)
         self._ipool = None
         for s in ipool or []:
             s.close()
+
+    def close(self):
+        """Free the contexts loss_and_grad_batched and loss_and_grad(keep)
+        keep (at most one of the two sets is held at a time)."""
+        self._free_batched()
+        self._free_instances()
 
     def fd_grad(self, i, a, m, h=1e-6, richardson=True):
         """Central difference of instance i's loss in a_m over the frozen
