@@ -66,7 +66,7 @@ def _ct_faces(nx, ny):
 
 
 @pytest.mark.parametrize("bcname,stname,nx,ny", [(b, s, nx, ny) for b in BCS for s in ("default", "none")
-                                                 for nx, ny in ((24, 20), (25, 31))])
+                                                 for nx, ny in ((24, 20), (25, 31), (16, 16), (16, 19))])
 def test_fused_ct_ghost_fill_matches_ct_then_bound(lib, bcname, stname, nx, ny):
     dn = np.array(BCS[bcname], dtype=np.int32).ravel()
     st = np.array(ST_DEFAULT if stname == "default" else ST_NONE, dtype=np.int32).ravel()
